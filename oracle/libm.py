@@ -1,0 +1,35 @@
+"""The oracle's exp: the fdlibm method in C (oracle/libm.c), built into oracle/_build/ by
+oracle/Makefile, applied element by element through a C loop."""
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "_build" / "liboracle_libm.so"
+_LIB = []
+
+
+def build():
+    """Compile oracle/libm.c (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", str(LIB_PATH.parent.parent)], check=True)
+
+
+def _lib():
+    if not _LIB:
+        if not LIB_PATH.exists():
+            build()
+        lib = ctypes.CDLL(str(LIB_PATH))
+        lib.oracle_exp_map.restype = None
+        lib.oracle_exp_map.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        _LIB.append(lib)
+    return _LIB[0]
+
+
+def exp(x):
+    """fdlibm exp of every element of x (float64)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    y = np.empty_like(x)
+    _lib().oracle_exp_map(x.ctypes.data, y.ctypes.data, x.size)
+    return y
