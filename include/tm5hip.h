@@ -1,0 +1,157 @@
+/* libtm5hip: the TM5 transport step on AMD Instinct MI355X (gfx950), FP64, bit-identical to
+   pydyn's NumPy reference.
+
+   This is the C ABI that replaces pydyn.gpu's backend boundary (Backend.launch of the PD_GLOBAL
+   kernels, pydyn/gpu/backend.py:317-338, driven by the operator layer pydyn/gpu/kernels.py and
+   the body table pydyn/gpu/zoom.py:38-74).  One entry point per operator; every call is
+   asynchronous on the context's stream and returns 0 or an error code (a hipError_t value, or
+   TM5_EINVAL); nothing throws across the ABI.
+
+   Arrays are caller-owned device memory described by tm5_view: the address of element (0,0,0,0)
+   and the element strides of the axes (i, j, l, t) -- the halo-free pydyn layout
+   (pydyn/kernels.py:7-16): cells (im, jm, lm[, nt]), am (im+1, jm, lm), bm (im, jm+1, lm),
+   cm (im, jm, lm+1).  Any strides work; i-fastest strides give coalesced sweeps.  The four
+   tracer arrays rm, rxm, rym, rzm share one stride set (tm5_tracers), as in the reference
+   (pydyn/gpu/kernels.py:48-52).  Kernels write in place, as the reference's do.
+
+   Errors found on the device (a sub-cycle cap exceeded, non-zero outer faces) are recorded in
+   the context's flag word, like the reference's deferred device flags (pydyn/gpu/backend.py:
+   173-181), and read with tm5_ctx_flags.  Once an error bit is set, later operators on the
+   context do nothing until the flags are cleared. */
+#ifndef TM5HIP_H
+#define TM5HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tm5_ctx tm5_ctx;
+
+typedef struct tm5_view {
+    double *data;      /* device address of element (0, 0, 0, 0) */
+    int64_t stride[4]; /* element strides of i, j, l, t (t unused for 3-D arrays) */
+} tm5_view;
+
+typedef struct tm5_tracers {
+    double *rm, *rxm, *rym, *rzm; /* tracer mass and its x, y, z slopes, (im, jm, lm, nt) */
+    int64_t stride[4];            /* shared strides of i, j, l, t */
+    int nt;                       /* number of tracers */
+} tm5_tracers;
+
+typedef struct tm5_iview {
+    int32_t *data;
+    int64_t stride[2]; /* i, j */
+} tm5_iview;
+
+/* flag word bits */
+#define TM5_FAIL_X 1     /* dynamu: nloop too high (CFLError) */
+#define TM5_FAIL_Y 2     /* dynamv: nloop too high (CFLError) */
+#define TM5_FAIL_Z 4     /* dynamw: too many CFL violations (CFLError) */
+#define TM5_BAD_Y 8      /* non-zero flux through the polar outer faces (ValueError) */
+#define TM5_BAD_Z 16     /* non-zero halo interfaces in cm (ValueError) */
+#define TM5_ERRORS 31    /* any of the above */
+#define TM5_REHEARSAL 32 /* a Check_CFL mass-only operator exceeded its cap (the verdict) */
+
+#define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
+#define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
+
+/* ------------------------------------------------------------------ context */
+/* A context: the stream every call is queued on (0 = the default stream) and the device scratch
+   the operators use (flag word, search results).  Not thread-safe; one context per stream.
+   Replaces pydyn/gpu/backend.py:460-541 (stream, flags, scratch). */
+int tm5_ctx_create(uint64_t stream, tm5_ctx **out);
+int tm5_ctx_destroy(tm5_ctx *ctx);
+/* Wait for the stream, then the flag word (and clear it when clear != 0).  The reference's
+   Backend.check (pydyn/gpu/backend.py:173-181, 674-689). */
+int tm5_ctx_flags(tm5_ctx *ctx, int clear, int *flags);
+/* The last sub-cycle counts found on the device (valid after tm5_ctx_flags or a stream sync):
+   x: max over rows; y: per layer into nloop_layers[lm] (may be NULL); z: n_advz. */
+int tm5_ctx_nloop(tm5_ctx *ctx, int *x_max, int *y_max, int *nloop_layers, int lm, int *z_nadv);
+
+/* ------------------------------------------------------------------ advection */
+/* dynamu: periodic x rows, each (j, l) row with its own nloop < max_nloop; am left scaled by
+   the search and multiplied back by nloop.  pydyn/gpu/kernels.py:64 advect_x_rows
+   (pydyn/kernels.py:129-219). */
+int tm5_advect_x(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                 const tm5_view *am, int limits, int max_nloop);
+/* dynamv: y columns with polar caps, one nloop < max_nloop per layer, bm of the layer scaled
+   by the search and multiplied back.  pydyn/gpu/kernels.py:103 advect_y (pydyn/kernels.py:
+   223-291). */
+int tm5_advect_y(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                 const tm5_view *bm, int limits, int max_nloop);
+/* dynamw: z columns, one n_advz <= iter_limit for the region, cm / n_advz fresh, cm unchanged.
+   pydyn/gpu/kernels.py:227 advect_z (pydyn/kernels.py:295-346). */
+int tm5_advect_z(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                 const tm5_view *cm, int limits, int iter_limit);
+
+/* The mass-only operators of Check_CFL's rehearsal: a cap exceeded sets TM5_REHEARSAL (not an
+   error), and they do nothing once it is set.  pydyn/gpu/kernels.py:420 mass_x_rows, :447
+   mass_y, :497 mass_z (pydyn/cfl.py:140-194, pydyn/zoom/check_cfl.py:206-216). */
+int tm5_mass_x(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_view *am, int max_nloop);
+int tm5_mass_y(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_view *bm, int max_nloop);
+int tm5_mass_z(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_view *cm, int iter_limit);
+
+/* ------------------------------------------------------------------ mass fluxes */
+/* dynam0: am = dt (a pu), bm = dt (a pv), cm from the convergence with bt (device, lm+1), the
+   periodic face am(0) = am(im); a is the meteo amplitude (1.0: pu, pv as given).
+   pydyn/gpu/kernels.py:300 fluxes_from_meteo + pydyn/zoom/massflow.py:446-461 dynam0. */
+int tm5_dynam0(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *pu, const tm5_view *pv, double a,
+               const double *bt, double dt, const tm5_view *am, const tm5_view *bm, const tm5_view *cm);
+/* m2phlb1: sp = seqsum(m) g / dxyp, phlb = at + bt sp (dxyp (jm), at, bt (lm+1) on the device).
+   pydyn/zoom/massflow.py:282-290 (device pd_m2phlb1, pydyn/gpu/csrc/mass.cuh:44-79). */
+int tm5_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const double *dxyp, const double *at,
+                const double *bt, const tm5_view *sp, const tm5_view *phlb);
+
+/* ------------------------------------------------------------------ physics */
+/* convec on every column: the convection + Kzz diffusion matrix, its Crout inverse, slopes
+   times diag, rm = C rm.  pydyn/gpu/kernels.py:351 convec (pydyn/kernels.py:350-457). */
+int tm5_convec(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+               const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
+               const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax);
+/* dkg (im, jm, lmax) from kvh (im, jm, lm-1), m and gph (im, jm, lm+1).
+   pydyn/diffusion.py:256-263 (device part of pd_calc_kzz, pydyn/gpu/csrc/kzz.cuh:91-242). */
+int tm5_dkg(tm5_ctx *ctx, int im, int jm, const tm5_view *kvh, const tm5_view *m, const tm5_view *gph, int lmax,
+            const tm5_view *dkg);
+/* do_add_2d of x = ((emis factor) ratio) efrac at the 1-based level and tracer, every cell.
+   pydyn/zoom/physics.py:516-525 sources_region with pydyn/emis.py:52-54 inject. */
+int tm5_add_2d(tm5_ctx *ctx, int im, int jm, const tm5_view *emis, double factor, double ratio, double efrac,
+               const tm5_tracers *q, int level, int tracer);
+/* CH4 decay: factor = exp(-(((L1 + L2) + L3) + L4) dt) (fdlibm exp), rm..rzm of tracers
+   [t0, t0 + count) times factor.  loss: (im, jm, lm, 4), stride[3] the reaction axis.
+   pydyn/chem.py:168-178 + pydyn/zoom/physics.py:528-539 (device pd_scale_owned_all). */
+int tm5_decay(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *loss, double dt, const tm5_tracers *q, int t0,
+              int count);
+/* a = a * factor on a 3-D view of n0 x n1 x n2.  pydyn/zoom/host.py:226-230 scale. */
+int tm5_scale(tm5_ctx *ctx, int n0, int n1, int n2, const tm5_view *a, double factor);
+
+/* ------------------------------------------------------------------ the region driver */
+/* One global region resident on the device, run interval by interval by a native driver:
+   Check_CFL (rehearsal on device scratch, the verdict read once per attempt), then the n steps
+   of ndyn, each Setup_MassFlow (dynam0 of a(t) pu0, a(t) pv0 at the middle of the step) and
+   the operators "xyzvsc" m2phlb1 "csvzyx" m2phlb1 with sources (level 1 of tracer 1) and
+   chemistry (decay of every tracer).  pydyn/zoom/steps.py:259-318, pydyn/zoom/check_cfl.py:
+   258-290, pydyn/forward.py:388-451; the meteo amplitude a(t) is the triangle wave of
+   pydyn_amd.meteo (mod_period, mod_amplitude). */
+typedef struct tm5_region_desc {
+    int im, jm, lm, lmax, limits;
+    tm5_view m, am, bm, cm, pu0, pv0, sp, phlb;
+    tm5_tracers q;
+    tm5_view entu, detu, entd, detd, dkg, emis, loss;
+    tm5_iview cloud_top, cloud_lfs;
+    const double *at, *bt, *dxyp; /* device, lm+1, lm+1, jm */
+    double mod_period, mod_amplitude, sec_month, mol_ratio;
+} tm5_region_desc;
+
+typedef struct tm5_region tm5_region;
+
+int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *desc, tm5_region **out);
+int tm5_region_destroy(tm5_region *region);
+/* One interval of nread seconds from t0: returns the (ndyn, n) Check_CFL chose. */
+int tm5_region_interval(tm5_region *region, double t0, int nread, int ndyn_max, int *ndyn, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

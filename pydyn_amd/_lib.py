@@ -1,0 +1,104 @@
+"""ctypes bindings of libtm5hip.so (include/tm5hip.h).
+
+The library is built in-tree by pydyn_amd.build (or __graft_entry__.build()).  There is no
+fallback: without the library, or without a GPU, every operator raises.
+"""
+
+import ctypes
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libtm5hip.so"
+
+FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, ERRORS, REHEARSAL = 1, 2, 4, 8, 16, 31, 32
+EINVAL, EFLAGS = -1, -2
+
+
+class View(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int64 * 4)]
+
+
+class Tracers(ctypes.Structure):
+    _fields_ = [
+        ("rm", ctypes.c_void_p),
+        ("rxm", ctypes.c_void_p),
+        ("rym", ctypes.c_void_p),
+        ("rzm", ctypes.c_void_p),
+        ("stride", ctypes.c_int64 * 4),
+        ("nt", ctypes.c_int),
+    ]
+
+
+class IView(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int64 * 2)]
+
+
+class RegionDesc(ctypes.Structure):
+    _fields_ = [
+        ("im", ctypes.c_int),
+        ("jm", ctypes.c_int),
+        ("lm", ctypes.c_int),
+        ("lmax", ctypes.c_int),
+        ("limits", ctypes.c_int),
+        *((name, View) for name in ("m", "am", "bm", "cm", "pu0", "pv0", "sp", "phlb")),
+        ("q", Tracers),
+        *((name, View) for name in ("entu", "detu", "entd", "detd", "dkg", "emis", "loss")),
+        ("cloud_top", IView),
+        ("cloud_lfs", IView),
+        ("at", ctypes.c_void_p),
+        ("bt", ctypes.c_void_p),
+        ("dxyp", ctypes.c_void_p),
+        ("mod_period", ctypes.c_double),
+        ("mod_amplitude", ctypes.c_double),
+        ("sec_month", ctypes.c_double),
+        ("mol_ratio", ctypes.c_double),
+    ]
+
+
+_P = ctypes.POINTER
+_I, _D, _VP = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+_SIGNATURES = {
+    "tm5_ctx_create": [ctypes.c_uint64, _P(_VP)],
+    "tm5_ctx_destroy": [_VP],
+    "tm5_ctx_flags": [_VP, _I, _P(_I)],
+    "tm5_ctx_nloop": [_VP, _P(_I), _P(_I), _P(_I), _I, _P(_I)],
+    "tm5_advect_x": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],
+    "tm5_advect_y": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],
+    "tm5_advect_z": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],
+    "tm5_mass_x": [_VP, _I, _I, _I, _P(View), _P(View), _I],
+    "tm5_mass_y": [_VP, _I, _I, _I, _P(View), _P(View), _I],
+    "tm5_mass_z": [_VP, _I, _I, _I, _P(View), _P(View), _I],
+    "tm5_dynam0": [_VP, _I, _I, _I, _P(View), _P(View), _D, _VP, _D, _P(View), _P(View), _P(View)],
+    "tm5_m2phlb1": [_VP, _I, _I, _I, _P(View), _VP, _VP, _VP, _P(View), _P(View)],
+    "tm5_convec": [_VP, _I, _I, _I, _P(View), _P(Tracers), *([_P(View)] * 4), _P(IView), _P(IView), _P(View), _D, _I],
+    "tm5_dkg": [_VP, _I, _I, _P(View), _P(View), _P(View), _I, _P(View)],
+    "tm5_add_2d": [_VP, _I, _I, _P(View), _D, _D, _D, _P(Tracers), _I, _I],
+    "tm5_decay": [_VP, _I, _I, _I, _P(View), _D, _P(Tracers), _I, _I],
+    "tm5_scale": [_VP, _I, _I, _I, _P(View), _D],
+    "tm5_region_create": [_VP, _P(RegionDesc), _P(_VP)],
+    "tm5_region_destroy": [_VP],
+    "tm5_region_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
+}
+
+_STATE = {}
+
+
+class LibraryMissing(RuntimeError):
+    """libtm5hip.so is not built (run pydyn_amd.build or __graft_entry__.build())."""
+
+
+def lib():
+    """The loaded library, its prototypes set; raises LibraryMissing when it is not built."""
+    if "lib" not in _STATE:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(f"{LIB_PATH} is not built: run `python -m pydyn_amd.build`")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, args in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype, fn.argtypes = ctypes.c_int, args
+        _STATE["lib"] = handle
+    return _STATE["lib"]
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"libtm5hip: {what} failed with code {rc}")

@@ -1,0 +1,395 @@
+// The extern "C" boundary of libtm5hip (include/tm5hip.h): the context, one entry point per
+// operator, and the native driver of a single global region.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tm5_lib.h"
+
+namespace tm5 {
+int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 entu, View3 detu, View3 entd,
+                  View3 detd, IView2 top, IView2 lfs, View3 dkg, double dt, int lmax);
+int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
+                  View3 am, View3 bm, View3 cm, int skip_mask);
+int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
+                   const double *bt, View3 sp, View3 phlb);
+int launch_dkg(tm5_ctx *ctx, int im, int jm, View3 kvh, View3 m, View3 gph, int lmax, View3 dkg);
+int launch_add_2d(tm5_ctx *ctx, int im, int jm, View3 emis, double factor, double ratio, double efrac, Tracers q,
+                  int level, int tracer);
+int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr, double dt, Tracers q, int t0,
+                 int count);
+int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor);
+
+// a word of the context's scratch set from the stream (a kernel, so that it can be captured)
+__global__ void k_set_word(int *words, int index, int value, int mask)
+{
+    if (threadIdx.x == 0) words[index] = (words[index] & ~mask) | (value & mask);
+}
+
+__global__ void __launch_bounds__(256) k_copy3(int n0, int n1, int n2, View3 src, View3 dst)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n0 * n1 * n2) return;
+    const int i = (int)(idx % n0), j = (int)((idx / n0) % n1), l = (int)(idx / ((long long)n0 * n1));
+    dst(i, j, l) = src(i, j, l);
+}
+
+static int copy3(tm5_ctx *ctx, int n0, int n1, int n2, View3 src, View3 dst)
+{
+    const long long n = (long long)n0 * n1 * n2;
+    hipLaunchKernelGGL(k_copy3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n0, n1, n2, src, dst);
+    return status(hipGetLastError());
+}
+
+static int set_word(tm5_ctx *ctx, int index, int value, int mask)
+{
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, ctx->stream, ctx->words, index, value, mask);
+    return status(hipGetLastError());
+}
+
+static int read_words(tm5_ctx *ctx)
+{
+    hipError_t e = hipMemcpyAsync(ctx->hwords, ctx->words, W_WORDS * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return status(e);
+}
+
+static bool bad_view(const tm5_view *v) { return v == nullptr || v->data == nullptr; }
+static bool bad_tracers(const tm5_tracers *q)
+{
+    return q == nullptr || q->nt < 0 || (q->nt > 0 && (!q->rm || !q->rxm || !q->rym || !q->rzm));
+}
+
+}  // namespace tm5
+
+using namespace tm5;
+
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------------ context
+extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
+{
+    if (!out) return TM5_EINVAL;
+    tm5_ctx *c = (tm5_ctx *)calloc(1, sizeof(tm5_ctx));
+    if (!c) return TM5_EINVAL;
+    c->stream = (hipStream_t)stream;
+    hipError_t e = hipGetDevice(&c->device);
+    c->y_cap = 256;
+    if (e == hipSuccess) e = hipMalloc((void **)&c->words, W_WORDS * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, c->y_cap * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->hwords, W_WORDS * sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        tm5_ctx_destroy(c);
+        return (int)e;
+    }
+    memset(c->hwords, 0, W_WORDS * sizeof(int));
+    e = hipMemcpyAsync(c->words, c->hwords, W_WORDS * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        tm5_ctx_destroy(c);
+        return (int)e;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int tm5_ctx_destroy(tm5_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream || c->words) hipStreamSynchronize(c->stream);
+    if (c->words) hipFree(c->words);
+    if (c->y_nloop) hipFree(c->y_nloop);
+    if (c->hwords) hipHostFree(c->hwords);
+    free(c);
+    return 0;
+}
+
+extern "C" int tm5_ctx_flags(tm5_ctx *c, int clear, int *flags)
+{
+    if (!c) return TM5_EINVAL;
+    TRY(read_words(c));
+    if (flags) *flags = c->hwords[W_FLAGS];
+    if (clear) TRY(set_word(c, W_FLAGS, 0, ~0));
+    return 0;
+}
+
+extern "C" int tm5_ctx_nloop(tm5_ctx *c, int *x_max, int *y_max, int *nloop_layers, int lm, int *z_nadv)
+{
+    if (!c || lm < 0 || lm > c->y_cap) return TM5_EINVAL;
+    TRY(read_words(c));
+    if (x_max) *x_max = c->hwords[W_X_NLOOP];
+    if (z_nadv) *z_nadv = c->hwords[W_Z_NADV];
+    if (lm > 0 && (nloop_layers || y_max)) {
+        int buf[256];
+        hipError_t e = hipMemcpy(buf, c->y_nloop, lm * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return (int)e;
+        int mx = 0;
+        for (int l = 0; l < lm; l++) mx = buf[l] > mx ? buf[l] : mx;
+        if (nloop_layers) memcpy(nloop_layers, buf, lm * sizeof(int));
+        if (y_max) *y_max = mx;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ operators
+extern "C" int tm5_advect_x(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                            const tm5_view *am, int limits, int max_nloop)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(am)) return TM5_EINVAL;
+    TRY(set_word(c, W_X_NLOOP, 0, ~0));
+    return launch_x(c, im, jm, lm, view3(m), tracers(q), view3(am), limits, max_nloop - 1, TM5_FAIL_X, TM5_ERRORS);
+}
+
+extern "C" int tm5_advect_y(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                            const tm5_view *bm, int limits, int max_nloop)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(bm)) return TM5_EINVAL;
+    return launch_y(c, im, jm, lm, view3(m), tracers(q), view3(bm), limits, max_nloop - 1, TM5_FAIL_Y, TM5_ERRORS);
+}
+
+extern "C" int tm5_advect_z(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                            const tm5_view *cm, int limits, int iter_limit)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(cm)) return TM5_EINVAL;
+    return launch_z(c, im, jm, lm, view3(m), tracers(q), view3(cm), limits, iter_limit, TM5_FAIL_Z, TM5_ERRORS);
+}
+
+static const int REHEARSAL_SKIP = TM5_ERRORS | TM5_REHEARSAL;
+
+extern "C" int tm5_mass_x(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_view *am, int max_nloop)
+{
+    if (!c || bad_view(m) || bad_view(am)) return TM5_EINVAL;
+    TRY(set_word(c, W_X_NLOOP, 0, ~0));
+    return launch_x(c, im, jm, lm, view3(m), no_tracers(), view3(am), 0, max_nloop - 1, TM5_REHEARSAL, REHEARSAL_SKIP);
+}
+
+extern "C" int tm5_mass_y(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_view *bm, int max_nloop)
+{
+    if (!c || bad_view(m) || bad_view(bm)) return TM5_EINVAL;
+    return launch_y(c, im, jm, lm, view3(m), no_tracers(), view3(bm), 0, max_nloop, TM5_REHEARSAL, REHEARSAL_SKIP);
+}
+
+extern "C" int tm5_mass_z(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_view *cm, int iter_limit)
+{
+    if (!c || bad_view(m) || bad_view(cm)) return TM5_EINVAL;
+    return launch_z(c, im, jm, lm, view3(m), no_tracers(), view3(cm), 0, iter_limit, TM5_REHEARSAL, REHEARSAL_SKIP);
+}
+
+extern "C" int tm5_dynam0(tm5_ctx *c, int im, int jm, int lm, const tm5_view *pu, const tm5_view *pv, double a,
+                          const double *bt, double dt, const tm5_view *am, const tm5_view *bm, const tm5_view *cm)
+{
+    if (!c || bad_view(pu) || bad_view(pv) || bad_view(am) || bad_view(bm) || bad_view(cm)) return TM5_EINVAL;
+    return launch_dynam0(c, im, jm, lm, view3(pu), view3(pv), a, bt, dt, view3(am), view3(bm), view3(cm), TM5_ERRORS);
+}
+
+extern "C" int tm5_m2phlb1(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const double *dxyp,
+                           const double *at, const double *bt, const tm5_view *sp, const tm5_view *phlb)
+{
+    if (!c || bad_view(m) || bad_view(sp) || bad_view(phlb)) return TM5_EINVAL;
+    return launch_m2phlb1(c, im, jm, lm, view3(m), dxyp, at, bt, view3(sp), view3(phlb));
+}
+
+extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                          const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
+                          const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt,
+                          int lmax)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(entu) || bad_view(detu) || bad_view(entd) || bad_view(detd) ||
+        bad_view(dkg) || !cloud_top || !cloud_lfs || !cloud_top->data || !cloud_lfs->data)
+        return TM5_EINVAL;
+    return launch_convec(c, im, jm, lm, view3(m), tracers(q), view3(entu), view3(detu), view3(entd), view3(detd),
+                         iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
+}
+
+extern "C" int tm5_dkg(tm5_ctx *c, int im, int jm, const tm5_view *kvh, const tm5_view *m, const tm5_view *gph,
+                       int lmax, const tm5_view *dkg)
+{
+    if (!c || bad_view(kvh) || bad_view(m) || bad_view(gph) || bad_view(dkg)) return TM5_EINVAL;
+    return launch_dkg(c, im, jm, view3(kvh), view3(m), view3(gph), lmax, view3(dkg));
+}
+
+extern "C" int tm5_add_2d(tm5_ctx *c, int im, int jm, const tm5_view *emis, double factor, double ratio, double efrac,
+                          const tm5_tracers *q, int level, int tracer)
+{
+    if (!c || bad_view(emis) || bad_tracers(q)) return TM5_EINVAL;
+    return launch_add_2d(c, im, jm, view3(emis), factor, ratio, efrac, tracers(q), level, tracer);
+}
+
+extern "C" int tm5_decay(tm5_ctx *c, int im, int jm, int lm, const tm5_view *loss, double dt, const tm5_tracers *q,
+                         int t0, int count)
+{
+    if (!c || bad_view(loss) || bad_tracers(q)) return TM5_EINVAL;
+    return launch_decay(c, im, jm, lm, view3(loss), loss->stride[3], dt, tracers(q), t0, count);
+}
+
+extern "C" int tm5_scale(tm5_ctx *c, int n0, int n1, int n2, const tm5_view *a, double factor)
+{
+    if (!c || bad_view(a)) return TM5_EINVAL;
+    return launch_scale(c, n0, n1, n2, view3(a), factor);
+}
+
+// ------------------------------------------------------------------ the region driver
+struct tm5_region {
+    tm5_ctx *ctx;
+    tm5_region_desc d;
+    double *scratch;  // the rehearsal's m, am, bm, cm (i-fastest, contiguous)
+    View3 ms, ams, bms, cms;
+};
+
+static View3 packed(double *p, int n0, int n1) { return View3{p, 1, n0, (long long)n0 * n1}; }
+
+extern "C" int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *d, tm5_region **out)
+{
+    if (!ctx || !d || !out || d->im < 1 || d->jm < 2 || d->lm < 2 || d->lmax < 1 || d->lmax > d->lm) return TM5_EINVAL;
+    tm5_region *r = (tm5_region *)calloc(1, sizeof(tm5_region));
+    if (!r) return TM5_EINVAL;
+    r->ctx = ctx;
+    r->d = *d;
+    const int im = d->im, jm = d->jm, lm = d->lm;
+    const size_t nm = (size_t)im * jm * lm, na = (size_t)(im + 1) * jm * lm, nb = (size_t)im * (jm + 1) * lm,
+                 nc = (size_t)im * jm * (lm + 1);
+    hipError_t e = hipMalloc((void **)&r->scratch, (nm + na + nb + nc) * sizeof(double));
+    if (e != hipSuccess) {
+        free(r);
+        return (int)e;
+    }
+    r->ms = packed(r->scratch, im, jm);
+    r->ams = packed(r->scratch + nm, im + 1, jm);
+    r->bms = packed(r->scratch + nm + na, im, jm + 1);
+    r->cms = packed(r->scratch + nm + na + nb, im, jm);
+    *out = r;
+    return 0;
+}
+
+extern "C" int tm5_region_destroy(tm5_region *r)
+{
+    if (!r) return 0;
+    hipStreamSynchronize(r->ctx->stream);
+    hipFree(r->scratch);
+    free(r);
+    return 0;
+}
+
+// pydyn_amd.meteo's amplitude(t): the same operations, so the same double
+static double amplitude(const tm5_region_desc &d, double t)
+{
+    const double x = t / d.mod_period;
+    const double frac = x - floor(x);
+    return 1.0 + d.mod_amplitude * (1.0 - 4.0 * fabs(frac - 0.5));
+}
+
+// pydyn/cfl.py:208-213
+static int new_valid_timestep(int dtime, int nread, int cfl_outputstep, int tref_max)
+{
+    for (int dt = dtime - 1; dt > 0; dt--)
+        if (nread % dt == 0 && cfl_outputstep % dt == 0 && dt % (2 * tref_max) == 0) return dt;
+    return -1;
+}
+
+// Fortran nint (pydyn/ops.py:78-80)
+static int nint(double x) { return (int)(floor(fabs(x) + 0.5) * (x >= 0 ? 1.0 : -1.0)); }
+
+static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View3 cm)
+{
+    const tm5_region_desc &d = r->d;
+    const double a = amplitude(d, t);
+    return launch_dynam0(r->ctx, d.im, d.jm, d.lm, view3(&d.pu0), view3(&d.pv0), a, d.bt, (double)ndyn / 2.0, am, bm,
+                         cm, TM5_ERRORS);
+}
+
+// Check_CFL's rehearsal of the n steps of ndyn on the scratch copy (pydyn/zoom/check_cfl.py:
+// 258-290 on one region): the verdict comes back in the rehearsal bit.
+static int rehearse(tm5_region *r, double t0, int ndyn, int n, bool *ok)
+{
+    tm5_ctx *c = r->ctx;
+    const tm5_region_desc &d = r->d;
+    const int im = d.im, jm = d.jm, lm = d.lm;
+    TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL));
+    TRY(copy3(c, im, jm, lm, view3(&d.m), r->ms));
+    for (int i = 1; i <= n; i++) {
+        TRY(dynam0_at(r, t0 + ((double)i - 0.5) * (double)ndyn, ndyn, r->ams, r->bms, r->cms));
+        const char *order = "xyzzyx";
+        for (const char *op = order; *op; op++) {
+            if (*op == 'x')
+                TRY(launch_x(c, im, jm, lm, r->ms, no_tracers(), r->ams, 0, 10 - 1, TM5_REHEARSAL, REHEARSAL_SKIP));
+            else if (*op == 'y')
+                TRY(launch_y(c, im, jm, lm, r->ms, no_tracers(), r->bms, 0, 1, TM5_REHEARSAL, REHEARSAL_SKIP));
+            else
+                TRY(launch_z(c, im, jm, lm, r->ms, no_tracers(), r->cms, 0, 1, TM5_REHEARSAL, REHEARSAL_SKIP));
+        }
+    }
+    TRY(read_words(c));
+    if (c->hwords[W_FLAGS] & TM5_ERRORS) return TM5_EFLAGS;
+    *ok = (c->hwords[W_FLAGS] & TM5_REHEARSAL) == 0;
+    return 0;
+}
+
+static int operator_(tm5_region *r, char op, int ndyn)
+{
+    tm5_ctx *c = r->ctx;
+    const tm5_region_desc &d = r->d;
+    const int im = d.im, jm = d.jm, lm = d.lm;
+    const Tracers q = tracers(&d.q);
+    switch (op) {
+    case 'x':
+        return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, TM5_ERRORS);
+    case 'y':
+        return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, TM5_ERRORS);
+    case 'z':
+        return launch_z(c, im, jm, lm, view3(&d.m), q, view3(&d.cm), d.limits, 15, TM5_FAIL_Z, TM5_ERRORS);
+    case 'v':
+        return launch_convec(c, im, jm, lm, view3(&d.m), q, view3(&d.entu), view3(&d.detu), view3(&d.entd),
+                             view3(&d.detd), iview2(&d.cloud_top), iview2(&d.cloud_lfs), view3(&d.dkg),
+                             (double)ndyn / 2.0, d.lmax);
+    case 's': {
+        const double factor = ((double)ndyn / 2.0) / d.sec_month;  // month2dt, pydyn/emis.py:65-68
+        return launch_add_2d(c, im, jm, view3(&d.emis), factor, d.mol_ratio, 1.0, q, 1, 1);
+    }
+    case 'c':
+        return launch_decay(c, im, jm, lm, view3(&d.loss), d.loss.stride[3], (double)ndyn / 2.0, q, 0, q.nt);
+    }
+    return TM5_EINVAL;
+}
+
+static int step(tm5_region *r, int ndyn)
+{
+    const tm5_region_desc &d = r->d;
+    for (const char *half : {"xyzvsc", "csvzyx"}) {
+        for (const char *op = half; *op; op++) TRY(operator_(r, *op, ndyn));
+        TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
+    }
+    return 0;
+}
+
+extern "C" int tm5_region_interval(tm5_region *r, double t0, int nread, int ndyn_max, int *ndyn_out, int *n_out)
+{
+    if (!r || nread < 1 || ndyn_max < 1 || nread % ndyn_max != 0) return TM5_EINVAL;
+    const tm5_region_desc &d = r->d;
+    int ndyn = ndyn_max, n = nread / ndyn_max;
+    bool ok = false;
+    for (int iteration = 1; iteration <= 100 && !ok; iteration++) {
+        TRY(rehearse(r, t0, ndyn, n, &ok));
+        if (ok) break;
+        const int ndyn_old = ndyn;
+        ndyn = new_valid_timestep(ndyn, nread, nread, 1);
+        if (ndyn < 1) return TM5_EINVAL;
+        n = nint((double)(n * ndyn_old) / (double)ndyn);
+        const double fraction = (double)ndyn / (double)ndyn_old;
+        TRY(launch_scale(r->ctx, d.im + 1, d.jm, d.lm, view3(&d.am), fraction));
+        TRY(launch_scale(r->ctx, d.im, d.jm + 1, d.lm, view3(&d.bm), fraction));
+        TRY(launch_scale(r->ctx, d.im, d.jm, d.lm + 1, view3(&d.cm), fraction));
+    }
+    if (!ok) return TM5_EINVAL;
+    for (int s = 0; s < n; s++) {
+        TRY(dynam0_at(r, (t0 + (double)(s * ndyn)) + (double)ndyn / 2.0, ndyn, view3(&d.am), view3(&d.bm),
+                      view3(&d.cm)));
+        TRY(step(r, ndyn));
+    }
+    if (ndyn_out) *ndyn_out = ndyn;
+    if (n_out) *n_out = n;
+    return 0;
+}

@@ -1,0 +1,174 @@
+// dynam0, m2phlb1, dkg, the CH4 injection and decay, and a scale: per-column or per-cell
+// kernels, one thread per column/cell with i the fastest index (coalesced for i-fastest views).
+#include "tm5_lib.h"
+
+namespace tm5 {
+
+__device__ __forceinline__ bool skipped_p(const int *words, int mask)
+{
+    return (__hip_atomic_load(words + W_FLAGS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != 0;
+}
+
+// ------------------------------------------------------------------ dynam0
+// pydyn/kernels.py:102-125 + pydyn/zoom/massflow.py:446-461 for column (i, j): the fluxes are
+// the meteo's times the amplitude a (rounded, as the interpolated meteo the reference stores),
+// then am = dt pu, bm = dt pv, cm = -dt sd with sd accumulated from the top; am(0) = am(im).
+__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
+                                                const double *bt, double dt, View3 am, View3 bm, View3 cm, int *words,
+                                                int skip_mask)
+{
+    if (skipped_p(words, skip_mask)) return;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= im * jm) return;
+    const int i = col % im, j = col / im;
+    auto conv = [&](int l) {
+        return ((a * pu(i, j, l) - a * pu(i + 1, j, l)) + a * pv(i, j, l)) - a * pv(i, j + 1, l);
+    };
+    double pit = conv(0);
+    for (int l = 1; l < lm; l++) pit = pit + conv(l);
+    cm(i, j, 0) = 0.0;
+    cm(i, j, lm) = 0.0;
+    if (lm > 1) {
+        double sd = conv(lm - 1) - (bt[lm - 1] - bt[lm]) * pit;
+        cm(i, j, lm - 1) = (-dt) * sd;
+        for (int l = lm - 2; l >= 1; l--) {
+            sd = (sd + conv(l)) - (bt[l] - bt[l + 1]) * pit;
+            cm(i, j, l) = (-dt) * sd;
+        }
+    }
+    for (int l = 0; l < lm; l++) {
+        am(i, j, l) = dt * (a * pu(i == 0 ? im : i, j, l));  // am(0) = am(im)
+        if (i == im - 1) am(im, j, l) = dt * (a * pu(im, j, l));
+        bm(i, j, l) = dt * (a * pv(i, j, l));
+        if (j == jm - 1) bm(i, jm, l) = dt * (a * pv(i, jm, l));
+    }
+}
+
+// ------------------------------------------------------------------ m2phlb1
+// pydyn/zoom/massflow.py:282-290: sp = seqsum(m over l) g / dxyp(j), phlb = at + bt sp.
+__global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m, const double *dxyp,
+                                                 const double *at, const double *bt, View3 sp, View3 phlb, int *words,
+                                                 int skip_mask)
+{
+    if (skipped_p(words, skip_mask)) return;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= im * jm) return;
+    const int i = col % im, j = col / im;
+    double p = m(i, j, 0);
+    for (int l = 1; l < lm; l++) p = p + m(i, j, l);
+    p = p * 9.80665 / dxyp[j];
+    sp(i, j, 0) = p;
+    for (int l = 0; l <= lm; l++) phlb(i, j, l) = at[l] + bt[l] * p;
+}
+
+// ------------------------------------------------------------------ dkg
+// pydyn/diffusion.py:256-263: dkg(l) = max(0, kvh(l)) 2 (m(l+1) + m(l)) / dz^2, dz = gph(l+2) -
+// gph(l), for l < lmax - 1; zero at lmax - 1.
+__global__ void __launch_bounds__(256) k_dkg(int im, int jm, int lmax, View3 kvh, View3 m, View3 gph, View3 dkg)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= im * jm * lmax) return;
+    const int i = cell % im, j = (cell / im) % jm, l = cell / (im * jm);
+    if (l == lmax - 1) {
+        dkg(i, j, l) = 0.0;
+        return;
+    }
+    const double dz = gph(i, j, l + 2) - gph(i, j, l);
+    dkg(i, j, l) = max0(kvh(i, j, l)) * 2.0 * (m(i, j, l + 1) + m(i, j, l)) / (dz * dz);
+}
+
+// ------------------------------------------------------------------ sources: do_add_2d
+// pydyn/emis.py:52-54 inject, pydyn/zoom/physics.py:516-525: rm += x, rzm -= x.
+__global__ void __launch_bounds__(256) k_add_2d(int im, int jm, View3 emis, double factor, double ratio,
+                                                double efrac, Tracers q, int lev, int t, int *words, int skip_mask)
+{
+    if (skipped_p(words, skip_mask)) return;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= im * jm) return;
+    const int i = col % im, j = col / im;
+    const double x = ((emis(i, j, 0) * factor) * ratio) * efrac;
+    const long long o = q.off(i, j, lev, t);
+    q.q[0][o] = q.q[0][o] + x;
+    q.q[3][o] = q.q[3][o] - x;
+}
+
+// ------------------------------------------------------------------ chemistry: decay
+// pydyn/chem.py:168-178: factor = exp(-(((L1 + L2) + L3) + L4) dt), rm..rzm of the tracers times it.
+__global__ void __launch_bounds__(256) k_decay(int im, int jm, int lm, View3 loss, long long sr, double dt, Tracers q,
+                                               int t0, int count, int *words, int skip_mask)
+{
+    if (skipped_p(words, skip_mask)) return;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= im * jm * lm) return;
+    const int i = cell % im, j = (cell / im) % jm, l = cell / (im * jm);
+    const double *L = &loss(i, j, l);
+    const double total = ((L[0] + L[sr]) + L[2 * sr]) + L[3 * sr];
+    const double f = fdlibm_exp(-(total * dt));
+    for (int t = t0; t < t0 + count; t++) {
+        const long long o = q.off(i, j, l, t);
+        for (int a = 0; a < 4; a++) q.q[a][o] = q.q[a][o] * f;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scale(int n0, int n1, int n2, View3 a, double factor)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n0 * n1 * n2) return;
+    const int i = (int)(idx % n0), j = (int)((idx / n0) % n1), l = (int)(idx / ((long long)n0 * n1));
+    a(i, j, l) = a(i, j, l) * factor;
+}
+
+static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
+                  View3 am, View3 bm, View3 cm, int skip_mask)
+{
+    if (im < 1 || jm < 1 || lm < 1 || !bt) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_dynam0, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, lm, pu, pv, a, bt, dt,
+                       am, bm, cm, ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
+                   const double *bt, View3 sp, View3 phlb)
+{
+    if (im < 1 || jm < 1 || lm < 1 || !dxyp || !at || !bt) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_m2phlb1, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, lm, m, dxyp, at, bt,
+                       sp, phlb, ctx->words, TM5_ERRORS);
+    return status(hipGetLastError());
+}
+
+int launch_dkg(tm5_ctx *ctx, int im, int jm, View3 kvh, View3 m, View3 gph, int lmax, View3 dkg)
+{
+    if (im < 1 || jm < 1 || lmax < 1) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_dkg, blocks((long long)im * jm * lmax), dim3(256), 0, ctx->stream, im, jm, lmax, kvh, m, gph,
+                       dkg);
+    return status(hipGetLastError());
+}
+
+int launch_add_2d(tm5_ctx *ctx, int im, int jm, View3 emis, double factor, double ratio, double efrac, Tracers q,
+                  int level, int tracer)
+{
+    if (im < 1 || jm < 1 || level < 1 || tracer < 1 || tracer > q.nt) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_add_2d, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, emis, factor, ratio,
+                       efrac, q, level - 1, tracer - 1, ctx->words, TM5_ERRORS);
+    return status(hipGetLastError());
+}
+
+int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr, double dt, Tracers q, int t0,
+                 int count)
+{
+    if (im < 1 || jm < 1 || lm < 1 || t0 < 0 || count < 0 || t0 + count > q.nt) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_decay, blocks((long long)im * jm * lm), dim3(256), 0, ctx->stream, im, jm, lm, loss, sr, dt,
+                       q, t0, count, ctx->words, TM5_ERRORS);
+    return status(hipGetLastError());
+}
+
+int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor)
+{
+    if (n0 < 1 || n1 < 1 || n2 < 1) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_scale, blocks((long long)n0 * n1 * n2), dim3(256), 0, ctx->stream, n0, n1, n2, a, factor);
+    return status(hipGetLastError());
+}
+
+}  // namespace tm5

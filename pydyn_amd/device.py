@@ -1,0 +1,142 @@
+"""Device arrays and the library context.
+
+Arrays live in HBM as torch tensors (torch is the allocator and stream provider here, nothing
+else): a logical pydyn-layout shape (im, jm, lm[, nt]) stored with i fastest -- the physical
+order is the reversed axes -- so that every sweep of the kernels reads consecutive doubles
+across lanes.  `upload` / `download` convert from / to C-ordered NumPy arrays in the pydyn
+layout.  Every operator runs on the context's stream (torch's current stream when the context
+was made), so torch copies and library kernels are ordered on it.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class CFLError(RuntimeError):
+    """A sub-cycle count exceeded its cap (the reference's pydyn.kernels.CFLError)."""
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("pydyn_amd needs an AMD Instinct GPU with ROCm: torch.cuda.is_available() is False")
+    return torch
+
+
+class Context:
+    """A tm5_ctx bound to one stream: the flag word and the search scratch of the operators."""
+
+    def __init__(self, stream=None):
+        torch = _torch()
+        lib = _lib.lib()
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        ptr = ctypes.c_void_p()
+        _lib.check(lib.tm5_ctx_create(self.stream.cuda_stream, ctypes.byref(ptr)), "tm5_ctx_create")
+        self.ptr = ptr
+
+    def flags(self, clear=True):
+        """Wait for the stream and return the flag word (cleared when `clear`)."""
+        out = ctypes.c_int()
+        _lib.check(_lib.lib().tm5_ctx_flags(self.ptr, int(clear), ctypes.byref(out)), "tm5_ctx_flags")
+        return out.value
+
+    def raise_errors(self):
+        """The reference's exceptions for the error bits of the flag word (clearing it)."""
+        f = self.flags(clear=True)
+        if f & _lib.BAD_Y:
+            raise ValueError("advect_y: non-zero flux through the polar outer faces")
+        if f & _lib.BAD_Z:
+            raise ValueError("dynamw: non-zero halo interfaces in cm")
+        if f & _lib.FAIL_X:
+            raise CFLError("dynamu: nloop too high")
+        if f & _lib.FAIL_Y:
+            raise CFLError("dynamv: nloop too high")
+        if f & _lib.FAIL_Z:
+            raise CFLError("dynamw: too many CFL violations!")
+        return f
+
+    def nloop(self, lm):
+        """(x max nloop, y per-layer nloop (lm,), z n_advz) of the last operators."""
+        x, y, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        layers = (ctypes.c_int * max(lm, 1))()
+        _lib.check(_lib.lib().tm5_ctx_nloop(self.ptr, ctypes.byref(x), ctypes.byref(y), layers, lm, ctypes.byref(z)),
+                   "tm5_ctx_nloop")  # fmt: skip
+        return x.value, np.array(layers[:lm]), z.value
+
+    def close(self):
+        if self.ptr:
+            _lib.lib().tm5_ctx_destroy(self.ptr)
+            self.ptr = None
+
+
+_CONTEXTS = {}
+
+
+def context():
+    """This process's context on the current device."""
+    torch = _torch()
+    dev = torch.cuda.current_device()
+    if dev not in _CONTEXTS:
+        _CONTEXTS[dev] = Context()
+    return _CONTEXTS[dev]
+
+
+# ------------------------------------------------------------------ arrays
+def empty(shape, dtype=None):
+    """An uninitialised device array of logical shape `shape`, i fastest."""
+    torch = _torch()
+    dtype = torch.float64 if dtype is None else dtype
+    t = torch.empty(tuple(reversed(shape)), dtype=dtype, device="cuda")
+    return t.permute(*reversed(range(len(shape))))
+
+
+def zeros(shape, dtype=None):
+    t = empty(shape, dtype)
+    t.zero_()
+    return t
+
+
+def upload(a, dtype=None):
+    """A C-ordered NumPy array (pydyn layout) as a device array, i fastest."""
+    torch = _torch()
+    a = np.asarray(a)
+    host = np.ascontiguousarray(np.transpose(a))
+    t = torch.from_numpy(host).to("cuda")
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.permute(*reversed(range(a.ndim)))
+
+
+def download(t):
+    """A device array as a C-ordered NumPy array in its logical shape."""
+    host = t.permute(*reversed(range(t.dim()))).contiguous().cpu().numpy()
+    return np.ascontiguousarray(np.transpose(host))
+
+
+def view(t):
+    """tm5_view of a float64 device array (up to 4 axes)."""
+    if t.dtype.is_floating_point is False or t.element_size() != 8:
+        raise TypeError("view: float64 device arrays only")
+    strides = list(t.stride()) + [0] * (4 - t.dim())
+    return _lib.View(t.data_ptr(), (ctypes.c_int64 * 4)(*strides))
+
+
+def iview(t):
+    if t.element_size() != 4 or t.dtype.is_floating_point:
+        raise TypeError("iview: int32 device arrays only")
+    return _lib.IView(t.data_ptr(), (ctypes.c_int64 * 2)(*t.stride()[:2]))
+
+
+def tracers(rm, rxm, rym, rzm):
+    """tm5_tracers of four (im, jm, lm, nt) arrays with one stride set."""
+    for q in (rxm, rym, rzm):
+        if q.shape != rm.shape or q.stride() != rm.stride():
+            raise ValueError("tracers: rm, rxm, rym, rzm must share shape and strides")
+    strides = list(rm.stride()) + [0] * (4 - rm.dim())
+    nt = rm.shape[3] if rm.dim() == 4 else 1
+    return _lib.Tracers(rm.data_ptr(), rxm.data_ptr(), rym.data_ptr(), rzm.data_ptr(),
+                        (ctypes.c_int64 * 4)(*strides), nt)  # fmt: skip
