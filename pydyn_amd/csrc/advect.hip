@@ -16,7 +16,7 @@ namespace tm5 {
 
 __device__ __forceinline__ bool skipped(const int *words, int mask)
 {
-    return (__hip_atomic_load(words + W_FLAGS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != 0;
+    return (words[W_FLAGS] & mask) != 0;  // written by earlier kernels of the stream
 }
 
 // ------------------------------------------------------------------ x: dynamu
@@ -26,7 +26,7 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // every face is scaled by (nloop-1)/nloop in place.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
-                                          int nmax, int *words, int fail_bit, int skip_mask)
+                                          int nmax, int *words, int *rows_out, int fail_bit, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
@@ -66,11 +66,14 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         for (int k = lane; k <= n; k += 64) A[k] = shrink(A[k], nl);
         __syncthreads();
         if (nl > nmax) {
-            if (lane == 0) atomicOr(words + W_FLAGS, fail_bit);
+            if (lane == 0) {
+                atomicOr(words + W_FLAGS, fail_bit);
+                rows_out[row] = nl;
+            }
             return;
         }
     }
-    if (lane == 0) atomicMax(words + W_X_NLOOP, nl);
+    if (lane == 0) rows_out[row] = nl;  // one plain store per row: no contended atomic
 
     if (MASS_ONLY) {  // each cell reads only itself and its two faces: in place
         for (int s = 0; s < nl; s++)
@@ -137,15 +140,16 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask)
 {
-    if (im < 1 || im > 512 || jm < 1 || lm < 1) return TM5_EINVAL;
+    if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
     const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
+    ctx->x_nrows = jm * lm;
     if (q.nt > 0)
         hipLaunchKernelGGL(k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                           fail_bit, skip_mask);
+                           ctx->x_rows, fail_bit, skip_mask);
     else
         hipLaunchKernelGGL(k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                           fail_bit, skip_mask);
+                           ctx->x_rows, fail_bit, skip_mask);
     return status(hipGetLastError());
 }
 
@@ -154,8 +158,8 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
 // until every interior face of the layer passes its trial; on a failure the whole layer's
 // faces are scaled by (nloop-1)/nloop in place (the reference scales bm(:,:,l) whole).  The
 // polar outer faces must be zero (pydyn/kernels.py:233-234).
-__global__ void __launch_bounds__(256) k_y_search(int im, int jm, int lm, View3 m, View3 bm, int nmax, int *nloop,
-                                                  int *words, int fail_bit, int skip_mask)
+__global__ void __launch_bounds__(1024) k_y_search(int im, int jm, int lm, View3 m, View3 bm, int nmax, int *nloop,
+                                                   int *words, int fail_bit, int skip_mask)
 {
     const int l = blockIdx.x;
     if (skipped(words, skip_mask)) return;
@@ -170,196 +174,162 @@ __global__ void __launch_bounds__(256) k_y_search(int im, int jm, int lm, View3 
         return;
     }
     const int nface = im * (jm - 1);
+    // nloop = 1: one Courant number per interior face, the loads independent of each other
+    bool ok = true;
+#pragma unroll 4
+    for (int idx = tid; idx < nface; idx += nth) {
+        const int i = idx % im, k = idx / im + 1;
+        const double bk = bm(i, k, l), mlo = m(i, k - 1, l), mhi = m(i, k, l);
+        ok &= fabs(bk / (bk >= 0.0 ? mlo : mhi)) < 1.0;
+    }
     int nl = 1;
-    for (;;) {
-        bool ok = true;
-        for (int idx = tid; idx < nface; idx += nth) {
-            const int i = idx % im, k = idx / im + 1;
-            const double bk = bm(i, k, l);
-            const int u = bk >= 0.0 ? k - 1 : k;
-            const double blo = bm(i, u, l), bhi = bm(i, u + 1, l);
-            double mx = m(i, u, l);
-            for (int s = 0; s < nl; s++) {
-                if (!(fabs(bk / mx) < 1.0)) {
-                    ok = false;
-                    break;
+    if (!__syncthreads_and(ok)) {
+        // sub-cycling: the sequential search, the layer's faces scaled in place per retry
+        for (;;) {
+            nl += 1;
+            for (int idx = tid; idx < im * (jm + 1); idx += nth) {
+                double &b = bm(idx % im, idx / im, l);
+                b = shrink(b, nl);
+            }
+            __threadfence_block();
+            __syncthreads();
+            if (nl > nmax) {
+                if (tid == 0) {
+                    atomicOr(words + W_FLAGS, fail_bit);
+                    nloop[l] = nl;
                 }
-                mx = (mx + blo) - bhi;
+                return;
             }
-        }
-        if (__syncthreads_and(ok)) break;
-        nl += 1;
-        for (int idx = tid; idx < im * (jm + 1); idx += nth) {
-            double &b = bm(idx % im, idx / im, l);
-            b = shrink(b, nl);
-        }
-        __threadfence_block();
-        __syncthreads();
-        if (nl > nmax) {
-            if (tid == 0) {
-                atomicOr(words + W_FLAGS, fail_bit);
-                nloop[l] = nl;
+            ok = true;
+            for (int idx = tid; idx < nface; idx += nth) {
+                const int i = idx % im, k = idx / im + 1;
+                const double bk = bm(i, k, l);
+                const int u = bk >= 0.0 ? k - 1 : k;
+                const double blo = bm(i, u, l), bhi = bm(i, u + 1, l);
+                double mx = m(i, u, l);
+                for (int s = 0; s < nl; s++) {
+                    if (!(fabs(bk / mx) < 1.0)) {
+                        ok = false;
+                        break;
+                    }
+                    mx = (mx + blo) - bhi;
+                }
             }
-            return;
+            if (__syncthreads_and(ok)) break;
         }
     }
     if (tid == 0) nloop[l] = nl;
 }
 
 // ------------------------------------------------------------------ y: dynamv's sub-steps
-// pydyn/kernels.py:246-291 on a tile of TI columns of layer l (blockIdx.y), all jm cells.
-// Along-flow slope rym; carried cross slopes rxm (ft) and rzm (fw).  Flow out of a pole cell
-// carries no slope; the pole cells keep their rxm, rym.
+// pydyn/kernels.py:246-291 with the layer's nloop: one thread per cell of TI whole columns of
+// layer l (blockIdx.y).  Each sub-step every thread reads its cell's neighbourhood (cells j-1,
+// j, j+1 and faces j, j+1; independent loads), computes the fluxes of its two faces and its
+// update, then -- after the workgroup's barrier, since the neighbours read the old values --
+// writes it.  Along-flow slope rym; carried cross slopes rxm (ft) and rzm (fw).  Flow out of a
+// pole cell carries no slope; the pole cells keep their rxm, rym.  The limiter of the first
+// sub-step is taken on the values as they are read (it is the reference's initial limiter).
+__device__ __forceinline__ FaceFlux y_face(const View3 &m, const Tracers &q, const View3 &bm, int i, int k, int l, int t,
+                                           int jm, bool lim0)
+{
+    FaceFlux ff{0.0, 0.0, 0.0, 0.0};
+    if (k <= 0 || k >= jm) return ff;  // the closed polar faces
+    const double bf = bm(i, k, l);
+    const int u = bf >= 0.0 ? k - 1 : k;
+    const double beta = bf / m(i, u, l);
+    const long long o = q.off(i, u, l, t);
+    const double rm = q.q[0][o], rx = q.q[1][o], rz = q.q[3][o];
+    const double ry = lim0 ? limit(q.q[2][o], rm) : q.q[2][o];
+    ff = face_flux(bf, beta, rm, ry, rx, rz);
+    if ((k == 1 && bf >= 0.0) || (k == jm - 1 && bf < 0.0)) {  // leaving a pole cell
+        ff.f = beta * rm;
+        ff.pf = (-3.0 * bf) * ff.f;
+        ff.ft = 0.0;
+        ff.fw = beta * rz;
+    }
+    return ff;
+}
+
 template <bool MASS_ONLY>
-__global__ void __launch_bounds__(256) k_y_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 bm,
-                                                 const int *nloop, int limits, int *words, int skip_mask)
+__global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 bm,
+                                                  const int *nloop, int limits, int *words, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    extern __shared__ double lds[];
     const int l = blockIdx.y, i0 = blockIdx.x * TI;
-    const int ni = min(TI, im - i0);
+    const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
+    const bool mine = j < jm && i0 + ti < im;
+    const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
     const int nl = nloop[l];
-    const int tid = threadIdx.x, nth = blockDim.x;
-    const int ncell = jm * TI, nfc = (jm + 1) * TI;
-    double *M = lds, *B = M + ncell;
-    double *M0 = B + nfc, *R = M0 + ncell, *F = R + 4 * ncell;  // tracer part
-    // index: cell (ti, j) at j * TI + ti; face (ti, k) at k * TI + ti
-    for (int idx = tid; idx < ncell; idx += nth) {
-        const int ti = idx % TI, j = idx / TI;
-        if (ti < ni) M[idx] = m(i0 + ti, j, l);
-    }
-    for (int idx = tid; idx < nfc; idx += nth) {
-        const int ti = idx % TI, k = idx / TI;
-        if (ti < ni) B[idx] = bm(i0 + ti, k, l);
-    }
-    __syncthreads();
-
-    if (MASS_ONLY) {
-        for (int s = 0; s < nl; s++)
-            for (int idx = tid; idx < ncell; idx += nth) M[idx] = (M[idx] + B[idx]) - B[idx + TI];
-        __syncthreads();
-    } else {
-        for (int idx = tid; idx < ncell; idx += nth) M0[idx] = M[idx];
-        for (int ti = tid; ti < TI; ti += nth) {  // the closed outer faces
-            for (int a = 0; a < 4; a++) {
-                F[a * nfc + ti] = 0.0;
-                F[a * nfc + jm * TI + ti] = 0.0;
-            }
-        }
-        for (int t = 0; t < q.nt; t++) {
-            for (int idx = tid; idx < ncell; idx += nth) {
-                const int ti = idx % TI, j = idx / TI;
-                if (ti >= ni) continue;
-                const long long o = q.off(i0 + ti, j, l, t);
+    for (int s = 0; s < nl; s++) {
+        const double m_old = m(i, jj, l), b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
+        const double m_new = (m_old + b_lo) - b_hi;
+        if (!MASS_ONLY) {
+            const double inv_mnew = 1.0 / m_new;
+            const double flow_conv = b_lo - b_hi;
+            const double flow_mean = 3.0 * (b_lo + b_hi);
+            const double mass_x3 = 3.0 * m_old;
+            const bool lim0 = limits && s == 0;
+            const bool pole = jj == 0 || jj == jm - 1;
+            for (int t = 0; t < q.nt; t++) {
+                const FaceFlux lo = y_face(m, q, bm, i, jj, l, t, jm, lim0);
+                const FaceFlux hi = y_face(m, q, bm, i, jj + 1, l, t, jm, lim0);
+                const long long o = q.off(i, jj, l, t);
                 const double rm = q.q[0][o];
-                R[idx] = rm;                                                    // rm
-                R[ncell + idx] = limits ? limit(q.q[2][o], rm) : q.q[2][o];     // rym (along)
-                R[2 * ncell + idx] = q.q[1][o];                                 // rxm
-                R[3 * ncell + idx] = q.q[3][o];                                 // rzm
-                M[idx] = M0[idx];
-            }
-            __syncthreads();
-            for (int s = 0; s < nl; s++) {
-                for (int idx = tid; idx < (jm - 1) * TI; idx += nth) {
-                    const int ti = idx % TI, k = idx / TI + 1;
-                    if (ti >= ni) continue;
-                    const double bf = B[k * TI + ti];
-                    const int u = (bf >= 0.0 ? k - 1 : k) * TI + ti;
-                    const double beta = bf / M[u];
-                    FaceFlux ff = face_flux(bf, beta, R[u], R[ncell + u], R[2 * ncell + u], R[3 * ncell + u]);
-                    if (k == 1 && bf >= 0.0) {  // leaving the south pole cell
-                        ff.f = beta * R[u];
-                        ff.pf = (-3.0 * bf) * ff.f;
-                        ff.ft = 0.0;
-                        ff.fw = beta * R[3 * ncell + u];
-                    } else if (k == jm - 1 && bf < 0.0) {  // leaving the north pole cell
-                        ff.f = beta * R[u];
-                        ff.pf = (-3.0 * bf) * ff.f;
-                        ff.ft = 0.0;
-                        ff.fw = beta * R[3 * ncell + u];
-                    }
-                    const int fo = k * TI + ti;
-                    F[fo] = ff.f;
-                    F[nfc + fo] = ff.pf;
-                    F[2 * nfc + fo] = ff.ft;
-                    F[3 * nfc + fo] = ff.fw;
-                }
+                double rym = q.q[2][o];
+                if (lim0) rym = limit(rym, rm);
+                const double rm_new = rm - (hi.f - lo.f);
+                double ry = rym + ((((lo.pf - hi.pf) - flow_conv * rym) + flow_mean * rm_new) - mass_x3 * (lo.f + hi.f)) *
+                                      inv_mnew;
+                if (limits) ry = limit(ry, rm_new);
+                const double rx = q.q[1][o] - (hi.ft - lo.ft);
+                const double rz = q.q[3][o] - (hi.fw - lo.fw);
                 __syncthreads();
-                for (int idx = tid; idx < ncell; idx += nth) {
-                    const int ti = idx % TI, j = idx / TI;
-                    if (ti >= ni) continue;
-                    const double b_lo = B[idx], b_hi = B[idx + TI];
-                    const double m_old = M[idx];
-                    const double m_new = (m_old + b_lo) - b_hi;
-                    const double inv_mnew = 1.0 / m_new;
-                    const double flow_conv = b_lo - b_hi;
-                    const double flow_mean = 3.0 * (b_lo + b_hi);
-                    const double mass_x3 = 3.0 * m_old;
-                    const double *f = F, *pf = F + nfc, *fx = F + 2 * nfc, *fz = F + 3 * nfc;
-                    const double rm_new = R[idx] - (f[idx + TI] - f[idx]);
-                    const double rym = R[ncell + idx];
-                    double ry = rym + ((((pf[idx] - pf[idx + TI]) - flow_conv * rym) + flow_mean * rm_new) -
-                                       mass_x3 * (f[idx] + f[idx + TI])) *
-                                          inv_mnew;
-                    if (limits) ry = limit(ry, rm_new);
-                    const double rx = R[2 * ncell + idx] - (fx[idx + TI] - fx[idx]);
-                    R[idx] = rm_new;
-                    R[3 * ncell + idx] = R[3 * ncell + idx] - (fz[idx + TI] - fz[idx]);
-                    if (j != 0 && j != jm - 1) {
-                        R[ncell + idx] = ry;
-                        R[2 * ncell + idx] = rx;
+                if (mine) {
+                    q.q[0][o] = rm_new;
+                    q.q[3][o] = rz;
+                    if (!pole) {
+                        q.q[2][o] = ry;
+                        q.q[1][o] = rx;
+                    } else if (lim0) {
+                        q.q[2][o] = rym;  // the pole keeps its (limited) slope
                     }
-                    M[idx] = m_new;
                 }
+                __threadfence_block();
                 __syncthreads();
             }
-            for (int idx = tid; idx < ncell; idx += nth) {
-                const int ti = idx % TI, j = idx / TI;
-                if (ti >= ni) continue;
-                const long long o = q.off(i0 + ti, j, l, t);
-                q.q[0][o] = R[idx];
-                q.q[2][o] = R[ncell + idx];
-                q.q[1][o] = R[2 * ncell + idx];
-                q.q[3][o] = R[3 * ncell + idx];
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        if (mine) m(i, jj, l) = m_new;
+        __threadfence_block();
+        __syncthreads();
     }
-    for (int idx = tid; idx < ncell; idx += nth) {
-        const int ti = idx % TI, j = idx / TI;
-        if (ti < ni) m(i0 + ti, j, l) = M[idx];
-    }
-    for (int idx = tid; idx < nfc; idx += nth) {
-        const int ti = idx % TI, k = idx / TI;
-        if (ti < ni) bm(i0 + ti, k, l) = B[idx] * (double)nl;  // "restore old bm"
+    if (nl > 1 && mine) {  // "restore old bm": the search left the faces scaled
+        bm(i, jj, l) = bm(i, jj, l) * (double)nl;
+        if (jj == jm - 1) bm(i, jm, l) = bm(i, jm, l) * (double)nl;
     }
 }
 
-static int tile_width(int cells, int faces, int per_cell, int per_face, int cap)
-{
-    const long long per_line = (long long)cells * per_cell + (long long)faces * per_face;
-    long long ti = (60 * 1024 / 8) / per_line;
-    if (ti > cap) ti = cap;
-    return (int)ti;
-}
+// columns (or z lines) of `len` cells per workgroup of at most 1024 threads
+static int lines_per_block(int len, int cap) { return len > 1024 ? 0 : (1024 / len < cap ? 1024 / len : cap); }
+
+static dim3 line_block(int TI, int len) { return dim3((unsigned)(((TI * len) + 63) / 64 * 64)); }
 
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
              int skip_mask)
 {
     if (im < 1 || jm < 2 || lm < 1 || lm > ctx->y_cap) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = mass_only ? tile_width(jm, jm + 1, 1, 1, 16) : tile_width(jm, jm + 1, 6, 5, 8);
+    const int TI = lines_per_block(jm, 16);
     if (TI < 1) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_y_search, dim3(lm), dim3(256), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
+    hipLaunchKernelGGL(k_y_search, dim3(lm), dim3(1024), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
                        ctx->words, fail_bit, skip_mask);
-    const size_t lds = (size_t)(mass_only ? (2 * jm + 1) * TI : (11 * jm + 5) * TI) * sizeof(double);
-    const dim3 grid((im + TI - 1) / TI, lm);
+    const dim3 grid((im + TI - 1) / TI, lm), block = line_block(TI, jm);
     if (mass_only)
-        hipLaunchKernelGGL(k_y_steps<true>, grid, dim3(256), lds, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop,
-                           limits, ctx->words, skip_mask);
+        hipLaunchKernelGGL(k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop, limits,
+                           ctx->words, skip_mask);
     else
-        hipLaunchKernelGGL(k_y_steps<false>, grid, dim3(256), lds, ctx->stream, im, jm, lm, TI, m, q, bm,
-                           ctx->y_nloop, limits, ctx->words, skip_mask);
+        hipLaunchKernelGGL(k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop,
+                           limits, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 
@@ -393,47 +363,63 @@ __device__ int z_column_next(const View3 &m, const View3 &cm, int i, int j, int 
     return nmax + 1;
 }
 
-__global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 m, View3 cm, int nmax, int *words,
-                                                  int fail_bit, int skip_mask)
+constexpr int Z_FPT = 16;  // interior faces per thread in the z search
+
+__global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 m, View3 cm, int nmax, int *ok_blocks,
+                                                  int *words, int fail_bit, int skip_mask)
 {
-    // every workgroup takes its ticket, skipping or not, so that the last one always resets the
-    // reduction words for the next launch
+    // every workgroup takes its ticket, skipping or not, so that the last one always resets it
     const bool skip = skipped(words, skip_mask);
-    const int tid = threadIdx.x;
-    const int col = blockIdx.x * blockDim.x + tid;
-    const int ncol = im * jm;
-    int mine = 1;
-    if (!skip && col < ncol) {
-        const int i = col % im, j = col / im;
-        if (cm(i, j, 0) != 0.0 || cm(i, j, lm) != 0.0) atomicOr(words + W_FLAGS, TM5_BAD_Z);
-        mine = z_column_next(m, cm, i, j, lm, 1, nmax);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    bool ok = true;
+    if (!skip) {
+        // n = 1 (cm itself): one Courant number per interior face, the loads independent
+        const long long nface = (long long)im * jm * (lm - 1);
+        const long long base = (long long)blockIdx.x * nth * Z_FPT + tid;
+#pragma unroll
+        for (int r = 0; r < Z_FPT; r++) {
+            const long long idx = base + (long long)r * nth;
+            if (idx < nface) {
+                const int i = (int)(idx % im);
+                const long long rest = idx / im;
+                const int j = (int)(rest % jm), k = (int)(rest / jm) + 1;
+                const double ck = cm(i, j, k), mlo = m(i, j, k - 1), mhi = m(i, j, k);
+                ok &= fabs(ck / (ck >= 0.0 ? mlo : mhi)) < 1.0;
+            }
+        }
+        const int col = blockIdx.x * nth + tid;  // the closed ends (pydyn/kernels.py:301-302)
+        if (col < im * jm) {
+            const int i = col % im, j = col / im;
+            if (cm(i, j, 0) != 0.0 || cm(i, j, lm) != 0.0) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+        }
     }
-    __shared__ int red[256];
+    ok = __syncthreads_and(ok);
     __shared__ bool last;
-    red[tid] = mine;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = max(red[tid], red[tid + s]);
-        __syncthreads();
-    }
     if (tid == 0) {
-        atomicMax(words + W_Z_CAND, red[0]);
+        __hip_atomic_store(ok_blocks + blockIdx.x, ok ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         last = atomicAdd(words + W_Z_TICKET, 1) == (int)gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
     __threadfence();
-    int cand = __hip_atomic_load(words + W_Z_CAND, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (!skip && cand > 1 && cand <= nmax) {
+    bool all = true;
+    for (int b = tid; b < (int)gridDim.x; b += nth)
+        all &= __hip_atomic_load(ok_blocks + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    all = __syncthreads_and(all);
+    // n = 1 failed somewhere: the sequential search as a fixpoint over the columns (rare)
+    __shared__ int red[256];
+    int cand = all ? 1 : 2;
+    const int ncol = im * jm;
+    while (!skip && !all && cand <= nmax) {
         int need = cand;
-        for (int c = tid; c < ncol; c += blockDim.x) {
+        for (int c = tid; c < ncol; c += nth) {
             const int i = c % im, j = c / im;
             if (!z_column_passes(m, cm, i, j, lm, cand)) need = max(need, z_column_next(m, cm, i, j, lm, cand + 1, nmax));
         }
         red[tid] = need;
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
+        for (int s = nth / 2; s > 0; s >>= 1) {
             if (tid < s) red[tid] = max(red[tid], red[tid + s]);
             __syncthreads();
         }
@@ -448,111 +434,71 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
             words[W_Z_NADV] = cand;
         }
         words[W_Z_TICKET] = 0;
-        words[W_Z_CAND] = 0;
     }
 }
 
 // ------------------------------------------------------------------ z: dynamw's sub-steps
-// pydyn/kernels.py:324-346 on a tile of TI columns (i0.., j = blockIdx.y), all lm levels, with
-// c = cm / n_advz; cm itself is not written.  Along-flow slope rzm; cross slopes rxm (ft), rym (fw).
+// pydyn/kernels.py:324-346 with c = cm / n_advz (cm itself not written): one thread per cell of
+// TI whole columns (i0.., j = blockIdx.y), the neighbourhood read into registers, the update
+// written after the workgroup's barrier.  Along-flow slope rzm; cross slopes rxm (ft), rym (fw).
+__device__ __forceinline__ FaceFlux z_face(const View3 &m, const Tracers &q, const View3 &cm, int i, int j, int k,
+                                           int t, int lm, double dn, bool lim0)
+{
+    FaceFlux ff{0.0, 0.0, 0.0, 0.0};
+    if (k <= 0 || k >= lm) return ff;  // the closed ends
+    const double cf = cm(i, j, k) / dn;
+    const int u = cf >= 0.0 ? k - 1 : k;
+    const double gamma = cf / m(i, j, u);
+    const long long o = q.off(i, j, u, t);
+    const double rm = q.q[0][o];
+    const double rz = lim0 ? limit(q.q[3][o], rm) : q.q[3][o];
+    return face_flux(cf, gamma, rm, rz, q.q[1][o], q.q[2][o]);
+}
+
 template <bool MASS_ONLY>
-__global__ void __launch_bounds__(256) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
-                                                 int limits, int *words, int skip_mask)
+__global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
+                                                  int limits, int *words, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    extern __shared__ double lds[];
     const int j = blockIdx.y, i0 = blockIdx.x * TI;
-    const int ni = min(TI, im - i0);
+    const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
+    const bool mine = l < lm && i0 + ti < im;
+    const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;
     const int nz = words[W_Z_NADV];
     const double dn = (double)nz;
-    const int tid = threadIdx.x, nth = blockDim.x;
-    const int ncell = lm * TI, nfc = (lm + 1) * TI;
-    double *M = lds, *C = M + ncell;
-    double *M0 = C + nfc, *R = M0 + ncell, *F = R + 4 * ncell;
-    for (int idx = tid; idx < ncell; idx += nth) {
-        const int ti = idx % TI, l = idx / TI;
-        if (ti < ni) M[idx] = m(i0 + ti, j, l);
-    }
-    for (int idx = tid; idx < nfc; idx += nth) {
-        const int ti = idx % TI, k = idx / TI;
-        if (ti < ni) C[idx] = cm(i0 + ti, j, k) / dn;
-    }
-    __syncthreads();
-
-    if (MASS_ONLY) {
-        for (int s = 0; s < nz; s++)
-            for (int idx = tid; idx < ncell; idx += nth) M[idx] = (M[idx] + C[idx]) - C[idx + TI];
-        __syncthreads();
-    } else {
-        for (int idx = tid; idx < ncell; idx += nth) M0[idx] = M[idx];
-        for (int ti = tid; ti < TI; ti += nth) {
-            for (int a = 0; a < 4; a++) {
-                F[a * nfc + ti] = 0.0;
-                F[a * nfc + lm * TI + ti] = 0.0;
-            }
-        }
-        for (int t = 0; t < q.nt; t++) {
-            for (int idx = tid; idx < ncell; idx += nth) {
-                const int ti = idx % TI, l = idx / TI;
-                if (ti >= ni) continue;
-                const long long o = q.off(i0 + ti, j, l, t);
+    const double c_lo = cm(i, j, ll) / dn, c_hi = cm(i, j, ll + 1) / dn;
+    for (int s = 0; s < nz; s++) {
+        const double m_old = m(i, j, ll);
+        const double m_new = (m_old + c_lo) - c_hi;
+        if (!MASS_ONLY) {
+            const bool lim0 = limits && s == 0;
+            for (int t = 0; t < q.nt; t++) {
+                const FaceFlux lo = z_face(m, q, cm, i, j, ll, t, lm, dn, lim0);
+                const FaceFlux hi = z_face(m, q, cm, i, j, ll + 1, t, lm, dn, lim0);
+                const long long o = q.off(i, j, ll, t);
                 const double rm = q.q[0][o];
-                R[idx] = rm;                                                  // rm
-                R[ncell + idx] = limits ? limit(q.q[3][o], rm) : q.q[3][o];   // rzm (along)
-                R[2 * ncell + idx] = q.q[1][o];                               // rxm
-                R[3 * ncell + idx] = q.q[2][o];                               // rym
-                M[idx] = M0[idx];
-            }
-            __syncthreads();
-            for (int s = 0; s < nz; s++) {
-                for (int idx = tid; idx < (lm - 1) * TI; idx += nth) {
-                    const int ti = idx % TI, k = idx / TI + 1;
-                    if (ti >= ni) continue;
-                    const double cf = C[k * TI + ti];
-                    const int u = (cf >= 0.0 ? k - 1 : k) * TI + ti;
-                    const double gamma = cf / M[u];
-                    const FaceFlux ff = face_flux(cf, gamma, R[u], R[ncell + u], R[2 * ncell + u], R[3 * ncell + u]);
-                    const int fo = k * TI + ti;
-                    F[fo] = ff.f;
-                    F[nfc + fo] = ff.pf;
-                    F[2 * nfc + fo] = ff.ft;
-                    F[3 * nfc + fo] = ff.fw;
-                }
+                double rzm = q.q[3][o];
+                if (lim0) rzm = limit(rzm, rm);
+                const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
+                double rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
+                if (limits) rz = limit(rz, rm_new);
+                const double rx = (q.q[1][o] + lo.ft) - hi.ft;
+                const double ry = (q.q[2][o] + lo.fw) - hi.fw;
                 __syncthreads();
-                for (int idx = tid; idx < ncell; idx += nth) {
-                    const int ti = idx % TI;
-                    if (ti >= ni) continue;
-                    const double c_lo = C[idx], c_hi = C[idx + TI];
-                    const double m_old = M[idx];
-                    const double m_new = (m_old + c_lo) - c_hi;
-                    const double *f = F, *pf = F + nfc, *fx = F + 2 * nfc, *fy = F + 3 * nfc;
-                    const double rm_new = (R[idx] + f[idx]) - f[idx + TI];  // advance: rm + f(l-1) - f(l)
-                    double rz = slope_update(R[ncell + idx], pf[idx], pf[idx + TI], c_lo, c_hi, rm_new, f[idx],
-                                             f[idx + TI], m_old, m_new);
-                    if (limits) rz = limit(rz, rm_new);
-                    R[idx] = rm_new;
-                    R[ncell + idx] = rz;
-                    R[2 * ncell + idx] = (R[2 * ncell + idx] + fx[idx]) - fx[idx + TI];
-                    R[3 * ncell + idx] = (R[3 * ncell + idx] + fy[idx]) - fy[idx + TI];
-                    M[idx] = m_new;
+                if (mine) {
+                    q.q[0][o] = rm_new;
+                    q.q[3][o] = rz;
+                    q.q[1][o] = rx;
+                    q.q[2][o] = ry;
                 }
+                __threadfence_block();
                 __syncthreads();
             }
-            for (int idx = tid; idx < ncell; idx += nth) {
-                const int ti = idx % TI, l = idx / TI;
-                if (ti >= ni) continue;
-                const long long o = q.off(i0 + ti, j, l, t);
-                q.q[0][o] = R[idx];
-                q.q[3][o] = R[ncell + idx];
-                q.q[1][o] = R[2 * ncell + idx];
-                q.q[2][o] = R[3 * ncell + idx];
-            }
-            __syncthreads();
         }
-    }
-    for (int idx = tid; idx < ncell; idx += nth) {
-        const int ti = idx % TI, l = idx / TI;
-        if (ti < ni) m(i0 + ti, j, l) = M[idx];
+        __syncthreads();
+        if (mine) m(i, j, ll) = m_new;
+        __threadfence_block();
+        __syncthreads();
     }
 }
 
@@ -561,18 +507,21 @@ int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm,
 {
     if (im < 1 || jm < 1 || lm < 2) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = mass_only ? tile_width(lm, lm + 1, 1, 1, 32) : tile_width(lm, lm + 1, 6, 5, 16);
+    const int TI = lines_per_block(lm, 32);
     if (TI < 1) return TM5_EINVAL;
-    const int ncol = im * jm;
-    hipLaunchKernelGGL(k_z_search, dim3((ncol + 255) / 256), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, nmax,
+    const long long nface = (long long)im * jm * (lm - 1);
+    long long nb = (nface + 256LL * Z_FPT - 1) / (256LL * Z_FPT);
+    const long long nb_cols = ((long long)im * jm + 255) / 256;
+    if (nb < nb_cols) nb = nb_cols;
+    if (nb > ctx->z_cap) return TM5_EINVAL;
+    hipLaunchKernelGGL(k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, nmax, ctx->z_ok,
                        ctx->words, fail_bit, skip_mask);
-    const size_t lds = (size_t)(mass_only ? (2 * lm + 1) * TI : (11 * lm + 5) * TI) * sizeof(double);
-    const dim3 grid((im + TI - 1) / TI, jm);
+    const dim3 grid((im + TI - 1) / TI, jm), block = line_block(TI, lm);
     if (mass_only)
-        hipLaunchKernelGGL(k_z_steps<true>, grid, dim3(256), lds, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
-                           ctx->words, skip_mask);
+        hipLaunchKernelGGL(k_z_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits, ctx->words,
+                           skip_mask);
     else
-        hipLaunchKernelGGL(k_z_steps<false>, grid, dim3(256), lds, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
+        hipLaunchKernelGGL(k_z_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
                            ctx->words, skip_mask);
     return status(hipGetLastError());
 }

@@ -20,6 +20,26 @@ int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr,
                  int count);
 int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor);
 
+struct ColumnArgs {
+    int im, jm, lm;
+    View3 m;
+    Tracers q;
+    View3 entu, detu, entd, detd, dkg;
+    IView2 top, lfs;
+    double dt;
+    int fused;
+    View3 emis;
+    double factor, ratio, efrac;
+    int level, tracer;
+    View3 loss;
+    long long loss_sr;
+    double dt_chem;
+    const double *dxyp, *at, *bt;
+    View3 sp, phlb;
+};
+bool column_supported(int n);
+int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+
 // a word of the context's scratch set from the stream (a kernel, so that it can be captured)
 __global__ void k_set_word(int *words, int index, int value, int mask)
 {
@@ -79,8 +99,12 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     c->stream = (hipStream_t)stream;
     hipError_t e = hipGetDevice(&c->device);
     c->y_cap = 256;
+    c->x_cap = 1 << 17;
+    c->z_cap = 1 << 13;
     if (e == hipSuccess) e = hipMalloc((void **)&c->words, W_WORDS * sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, c->y_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->x_rows, c->x_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->z_ok, c->z_cap * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hwords, W_WORDS * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         tm5_ctx_destroy(c);
@@ -103,6 +127,8 @@ extern "C" int tm5_ctx_destroy(tm5_ctx *c)
     if (c->stream || c->words) hipStreamSynchronize(c->stream);
     if (c->words) hipFree(c->words);
     if (c->y_nloop) hipFree(c->y_nloop);
+    if (c->x_rows) hipFree(c->x_rows);
+    if (c->z_ok) hipFree(c->z_ok);
     if (c->hwords) hipHostFree(c->hwords);
     free(c);
     return 0;
@@ -121,7 +147,18 @@ extern "C" int tm5_ctx_nloop(tm5_ctx *c, int *x_max, int *y_max, int *nloop_laye
 {
     if (!c || lm < 0 || lm > c->y_cap) return TM5_EINVAL;
     TRY(read_words(c));
-    if (x_max) *x_max = c->hwords[W_X_NLOOP];
+    if (x_max) {
+        *x_max = 0;
+        if (c->x_nrows > 0) {
+            int *rows = (int *)malloc(c->x_nrows * sizeof(int));
+            if (!rows) return TM5_EINVAL;
+            hipError_t e = hipMemcpy(rows, c->x_rows, c->x_nrows * sizeof(int), hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                for (int r = 0; r < c->x_nrows; r++) *x_max = rows[r] > *x_max ? rows[r] : *x_max;
+            free(rows);
+            if (e != hipSuccess) return (int)e;
+        }
+    }
     if (z_nadv) *z_nadv = c->hwords[W_Z_NADV];
     if (lm > 0 && (nloop_layers || y_max)) {
         int buf[256];
@@ -140,7 +177,6 @@ extern "C" int tm5_advect_x(tm5_ctx *c, int im, int jm, int lm, const tm5_view *
                             const tm5_view *am, int limits, int max_nloop)
 {
     if (!c || bad_view(m) || bad_tracers(q) || bad_view(am)) return TM5_EINVAL;
-    TRY(set_word(c, W_X_NLOOP, 0, ~0));
     return launch_x(c, im, jm, lm, view3(m), tracers(q), view3(am), limits, max_nloop - 1, TM5_FAIL_X, TM5_ERRORS);
 }
 
@@ -163,7 +199,6 @@ static const int REHEARSAL_SKIP = TM5_ERRORS | TM5_REHEARSAL;
 extern "C" int tm5_mass_x(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_view *am, int max_nloop)
 {
     if (!c || bad_view(m) || bad_view(am)) return TM5_EINVAL;
-    TRY(set_word(c, W_X_NLOOP, 0, ~0));
     return launch_x(c, im, jm, lm, view3(m), no_tracers(), view3(am), 0, max_nloop - 1, TM5_REHEARSAL, REHEARSAL_SKIP);
 }
 
@@ -201,6 +236,24 @@ extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m,
     if (!c || bad_view(m) || bad_tracers(q) || bad_view(entu) || bad_view(detu) || bad_view(entd) || bad_view(detd) ||
         bad_view(dkg) || !cloud_top || !cloud_lfs || !cloud_top->data || !cloud_lfs->data)
         return TM5_EINVAL;
+    if (column_supported(lmax) && lm <= 64) {
+        ColumnArgs a{};
+        a.im = im;
+        a.jm = jm;
+        a.lm = lm;
+        a.m = view3(m);
+        a.q = tracers(q);
+        a.entu = view3(entu);
+        a.detu = view3(detu);
+        a.entd = view3(entd);
+        a.detd = view3(detd);
+        a.dkg = view3(dkg);
+        a.top = iview2(cloud_top);
+        a.lfs = iview2(cloud_lfs);
+        a.dt = dt;
+        a.fused = 0;
+        return launch_column(c, a, lmax, TM5_ERRORS);
+    }
     return launch_convec(c, im, jm, lm, view3(m), tracers(q), view3(entu), view3(detu), view3(entd), view3(detd),
                          iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
 }
@@ -355,13 +408,56 @@ static int operator_(tm5_region *r, char op, int ndyn)
     return TM5_EINVAL;
 }
 
+// "v s c m2phlb1 c s v" in one kernel when lmax has a register form (column.hip), else the
+// operators one by one
+static int middle(tm5_region *r, int ndyn)
+{
+    const tm5_region_desc &d = r->d;
+    if (!column_supported(d.lmax) || d.lm > 64) {
+        for (const char *op = "vsc"; *op; op++) TRY(operator_(r, *op, ndyn));
+        TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
+        for (const char *op = "csv"; *op; op++) TRY(operator_(r, *op, ndyn));
+        return 0;
+    }
+    ColumnArgs a{};
+    a.im = d.im;
+    a.jm = d.jm;
+    a.lm = d.lm;
+    a.m = view3(&d.m);
+    a.q = tracers(&d.q);
+    a.entu = view3(&d.entu);
+    a.detu = view3(&d.detu);
+    a.entd = view3(&d.entd);
+    a.detd = view3(&d.detd);
+    a.dkg = view3(&d.dkg);
+    a.top = iview2(&d.cloud_top);
+    a.lfs = iview2(&d.cloud_lfs);
+    a.dt = (double)ndyn / 2.0;
+    a.fused = 1;
+    a.emis = view3(&d.emis);
+    a.factor = ((double)ndyn / 2.0) / d.sec_month;  // month2dt, pydyn/emis.py:65-68
+    a.ratio = d.mol_ratio;
+    a.efrac = 1.0;
+    a.level = 0;
+    a.tracer = 0;
+    a.loss = view3(&d.loss);
+    a.loss_sr = d.loss.stride[3];
+    a.dt_chem = (double)ndyn / 2.0;
+    a.dxyp = d.dxyp;
+    a.at = d.at;
+    a.bt = d.bt;
+    a.sp = view3(&d.sp);
+    a.phlb = view3(&d.phlb);
+    return launch_column(r->ctx, a, d.lmax, TM5_ERRORS);
+}
+
 static int step(tm5_region *r, int ndyn)
 {
     const tm5_region_desc &d = r->d;
-    for (const char *half : {"xyzvsc", "csvzyx"}) {
-        for (const char *op = half; *op; op++) TRY(operator_(r, *op, ndyn));
-        TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
-    }
+    for (const char *op = "xyz"; *op; op++) TRY(operator_(r, *op, ndyn));
+    TRY(middle(r, ndyn));
+    for (const char *op = "zyx"; *op; op++) TRY(operator_(r, *op, ndyn));
+    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
     return 0;
 }
 

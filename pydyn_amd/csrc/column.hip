@@ -1,0 +1,337 @@
+// Column physics in registers: convection (convec) and the fused middle of a dynamic step.
+//
+// Between the two 'v' operators of a step ("xyz v s c | m2phlb1 | c s v zyx") only the sources,
+// the chemistry and m2phlb1 run; none of them changes the air mass, so both convection calls
+// build the same matrix and invert it to the same bits.  k_column<N> builds conv1 once per
+// column, inverts it, and applies "v s c m2phlb1 c s v" to every tracer of the column in one
+// pass (fused mode), or just "v" (the convec operator alone).
+//
+// One column per 64-lane wavefront, N = lmax known at compile time so that the matrix lives in
+// registers:
+//   build      lane c owns matrix column c (the up/downdraft recursions, pydyn/kernels.py:350-396),
+//              through LDS;
+//   LU         lane r holds row r; right-looking Crout without pivoting, each element updated in
+//              increasing k as in the reference's left-looking loops (pydyn/kernels.py:404-407);
+//   inverse    lane c solves for the right-hand side e_c (forward, then back substitution in the
+//              reference's order, pydyn/kernels.py:409-419);
+//   apply      lane l holds row l of C: rm(l) = sum_k C(l,k) rm(k) from zero in increasing k,
+//              slopes times C(l,l) (pydyn/kernels.py:448-457).
+// Uniform values (a pivot, an L or U entry) come from the owning lane by readlane.
+//
+// The matrix of a column with cloud top T and level of free sinking F is dense in its leading
+// DB x DB block (DB = max(T, F) + 1) and tridiagonal outside it; the entries outside are exact
+// zeros (the reference's where(act, .., 0.0) and 0 / m).  LU fill-in stays inside that envelope,
+// so the products outside it all have a +-0 factor and are skipped: subtracting +-0 leaves
+// every non-zero value as it is and can only change the sign of a zero, and no zero's sign
+// reaches a result (zeros only ever enter products, sums with non-zero terms, or numerators),
+// so the outputs are the reference's bits for any non-singular matrix of finite entries.
+#include "tm5_lib.h"
+
+namespace tm5 {
+
+__device__ __forceinline__ bool skipped_k(const int *words, int mask)
+{
+    return (words[W_FLAGS] & mask) != 0;  // written by earlier kernels of the stream
+}
+
+__device__ __forceinline__ double bcast(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+struct ColumnArgs {
+    int im, jm, lm;
+    View3 m;
+    Tracers q;
+    View3 entu, detu, entd, detd, dkg;
+    IView2 top, lfs;
+    double dt;  // convection's dt = ndyn / (2 tref)
+    // fused mode: sources, chemistry and m2phlb1
+    int fused;
+    View3 emis;
+    double factor, ratio, efrac;
+    int level, tracer;  // 0-based
+    View3 loss;
+    long long loss_sr;
+    double dt_chem;
+    const double *dxyp, *at, *bt;
+    View3 sp, phlb;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
+{
+    if (skipped_k(words, skip_mask)) return;
+    constexpr int LD = (N % 2 == 0) ? N + 1 : N;
+    extern __shared__ double lds[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ncol = a.im * a.jm;
+    int col = blockIdx.x * 4 + wave;
+    const bool active = col < ncol;
+    if (!active) col = ncol - 1;
+    const int i = col % a.im, j = col / a.im;
+    const int lm = a.lm;
+    // per wave: fu / conv1 / C^T (N+1)*LD, fd (N+1)*LD, vectors
+    double *U = lds + (size_t)wave * (2 * (N + 1) * LD + 8 * 64);
+    double *D = U + (N + 1) * LD;
+    double *vec = D + (N + 1) * LD;
+    double *mc = vec, *eu = vec + 64, *du = vec + 128, *ed = vec + 192, *dd = vec + 256, *dk = vec + 320;
+    double *amu = vec + 384, *amd = vec + 448;
+
+    const double m_l = lane < lm ? a.m(i, j, lane) : 0.0;
+    mc[lane] = m_l;
+    if (lane < N) {
+        eu[lane] = a.entu(i, j, lane);
+        du[lane] = a.detu(i, j, lane);
+        ed[lane] = a.entd(i, j, lane);
+        dd[lane] = a.detd(i, j, lane);
+        dk[lane] = a.dkg(i, j, lane);
+    }
+    for (int x = lane; x < (N + 1) * LD; x += 64) U[x] = D[x] = 0.0;
+    amu[lane] = amd[lane] = 0.0;
+    __syncthreads();
+
+    // ---- build conv1: lane c owns column c (pydyn/kernels.py:366-395)
+    const int ctop = a.top(i, j), clfs = a.lfs(i, j);
+    // levels above the cloud top (updraft) or the level of free sinking (downdraft) contribute
+    // exact zeros (the reference's where(act, .., 0.0)): their iterations are skipped
+    const int kup = ctop < N ? ctop : N, kdn = clfs < N ? clfs : N;
+    // the dense leading block of conv1; outside it the matrix is tridiagonal
+    const int DB = (kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N;
+    {
+        double run = 0.0, prev = 0.0;
+        for (int k = 1; k <= kup; k++) {
+            const bool on = ctop >= k;
+            const double inflow = prev + eu[k - 1];
+            const double aa = inflow - du[k - 1];
+            const bool rising = aa > 0.0;
+            const double keep = rising ? max0(1.0 - du[k - 1] / inflow) : 0.0;
+            const double amu_k = (on && rising) ? aa : 0.0;
+            if (lane < N) {
+                double v = 0.0;
+                if (lane < k - 1)
+                    v = on ? run * keep : 0.0;
+                else if (lane == k - 1)
+                    v = on ? eu[k - 1] / mc[k - 1] * keep : 0.0;
+                U[k * LD + lane] = v;
+                run = v;
+            }
+            if (lane == 0) amu[k] = amu_k;
+            prev = amu_k;
+        }
+    }
+    {
+        double run = 0.0, prev = 0.0;
+        for (int k = kdn; k >= 2; k--) {
+            const bool on = clfs >= k;
+            const double inflow = prev - ed[k - 1];
+            const double aa = inflow + dd[k - 1];
+            const bool sinking = aa < 0.0;
+            const double keep = sinking ? max0(1.0 + dd[k - 1] / inflow) : 0.0;
+            const double amd_k = (on && sinking) ? aa : 0.0;
+            if (lane < N) {
+                double v = 0.0;
+                if (lane >= k)
+                    v = on ? run * keep : 0.0;
+                else if (lane == k - 1)
+                    v = on ? -ed[k - 1] / mc[k - 1] * keep : 0.0;
+                D[(k - 1) * LD + lane] = v;
+                run = v;
+            }
+            if (lane == 0) amd[k - 1] = amd_k;
+            prev = amd_k;
+        }
+    }
+    __syncthreads();
+    if (lane < N) {
+        for (int k = 1; k < N; k++) {
+            double v = U[k * LD + lane] + D[k * LD + lane];
+            // amu, amd are +0 beyond the cloud levels, and x - (+0) is x for every x
+            if (k <= kup && lane == k) v = v - amu[k] / mc[k];
+            if (k < kdn && lane == k - 1) v = v - amd[k] / mc[k - 1];
+            if (lane == k - 1) v = v + dk[k - 1] / mc[k - 1];
+            if (lane == k) v = v - dk[k - 1] / mc[k];
+            D[k * LD + lane] = v;
+        }
+    }
+    __syncthreads();
+    // conv1 row r into lane r's registers
+    double A[N];
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+        const int r = lane < N ? lane : N - 1;
+        double v = (-a.dt) * (D[r * LD + c] - D[(r + 1) * LD + c]);
+        if (r == c) v = v + 1.0;
+        A[c] = v;
+    }
+
+    // ---- LU, lanes over rows.  Entry (r, c) is non-zero only inside the dense block (r, c < DB)
+    // or on the three diagonals, and LU fill-in stays inside that envelope; the products skipped
+    // outside it have a factor that is exactly +-0.
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const double piv = bcast(A[k], k);
+        if (lane > k) A[k] = A[k] / piv;
+        const double lrk = A[k];
+#pragma unroll
+        for (int c = k + 1; c < N; c++) {
+            if (c < DB || c == k + 1) {
+                const double u = bcast(A[c], k);
+                if (lane > k) A[c] = A[c] - lrk * u;
+            }
+        }
+    }
+
+    // ---- inverse: lane c solves L U x = e_c
+    double y[N];
+#pragma unroll
+    for (int r = 0; r < N; r++) y[r] = (lane == r) ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = 1; r < N; r++) {
+#pragma unroll
+        for (int s = 0; s < r; s++) {
+            if (r < DB || s == r - 1) {
+                const double l = bcast(A[s], r);
+                y[r] = y[r] - l * y[s];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = N - 1; r >= 0; r--) {
+        double acc = y[r];
+#pragma unroll
+        for (int s = r + 1; s < N; s++) {
+            if (s < DB || s == r + 1) {
+                const double u = bcast(A[s], r);
+                acc = acc - u * y[s];
+            }
+        }
+        y[r] = acc / bcast(A[r], r);
+    }
+    // C^T through LDS: lane c wrote column c; lane l reads row l
+    __syncthreads();
+    if (lane < N) {
+#pragma unroll
+        for (int r = 0; r < N; r++) U[r * LD + lane] = y[r];
+    }
+    __syncthreads();
+    double crow[N];
+    {
+        const int r = lane < N ? lane : N - 1;
+#pragma unroll
+        for (int k = 0; k < N; k++) crow[k] = U[r * LD + k];
+    }
+    const double diag = lane < N ? U[lane * LD + lane] : 0.0;
+
+    // ---- apply
+    double decay = 1.0, x = 0.0;
+    if (a.fused) {
+        if (lane < lm) {
+            const double *L = &a.loss(i, j, lane);
+            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
+            decay = fdlibm_exp(-(total * a.dt_chem));
+        }
+        x = ((a.emis(i, j, 0) * a.factor) * a.ratio) * a.efrac;
+    }
+    for (int t = 0; t < a.q.nt; t++) {
+        const long long o = a.q.off(i, j, lane < lm ? lane : 0, t);
+        double rm = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+        if (lane < lm) {
+            rm = a.q.q[0][o];
+            rx = a.q.q[1][o];
+            ry = a.q.q[2][o];
+            rz = a.q.q[3][o];
+        }
+        const int passes = a.fused ? 2 : 1;
+        for (int p = 0; p < passes; p++) {
+            if (p == 1) {  // c, s (second half's order), then v
+                rm = rm * decay;
+                rx = rx * decay;
+                ry = ry * decay;
+                rz = rz * decay;
+                if (lane == a.level && t == a.tracer) {
+                    rm = rm + x;
+                    rz = rz - x;
+                }
+            }
+            // v: rm <- C rm over levels < N, slopes times diag(C)
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; k++) acc = acc + crow[k] * bcast(rm, k);
+            if (lane < N) {
+                rm = acc;
+                rx = diag * rx;
+                ry = diag * ry;
+                rz = diag * rz;
+            }
+            if (p == 0 && a.fused) {  // s, c (first half's order)
+                if (lane == a.level && t == a.tracer) {
+                    rm = rm + x;
+                    rz = rz - x;
+                }
+                rm = rm * decay;
+                rx = rx * decay;
+                ry = ry * decay;
+                rz = rz * decay;
+            }
+        }
+        if (active && lane < lm) {
+            a.q.q[0][o] = rm;
+            a.q.q[1][o] = rx;
+            a.q.q[2][o] = ry;
+            a.q.q[3][o] = rz;
+        }
+    }
+    // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290)
+    if (a.fused && active) {
+        double p = bcast(m_l, 0);
+        for (int l = 1; l < lm; l++) p = p + mc[l];
+        p = p * 9.80665 / a.dxyp[j];
+        if (lane == 0) a.sp(i, j, 0) = p;
+        for (int l = lane; l <= lm; l += 64) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
+    }
+}
+
+template <int N>
+static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
+{
+    constexpr int LD = (N % 2 == 0) ? N + 1 : N;
+    const size_t per_wave = (size_t)(2 * (N + 1) * LD + 8 * 64) * sizeof(double);
+    const int ncol = args.im * args.jm;
+    hipLaunchKernelGGL(k_column<N>, dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
+                       skip_mask);
+    return status(hipGetLastError());
+}
+
+// the lmax values compiled in register form (a wave's LDS stays under 16 KB, four waves a
+// workgroup); others go through the LDS kernel (convec.hip)
+bool column_supported(int n)
+{
+    switch (n) {
+    case 4: case 5: case 8: case 10: case 15: case 20: case 25: case 26:
+        return true;
+    default:
+        return false;
+    }
+}
+
+int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask)
+{
+    if (args.lm > 64 || lmax > args.lm) return TM5_EINVAL;
+    switch (lmax) {
+    case 4: return launch_column_n<4>(ctx, args, skip_mask);
+    case 5: return launch_column_n<5>(ctx, args, skip_mask);
+    case 8: return launch_column_n<8>(ctx, args, skip_mask);
+    case 10: return launch_column_n<10>(ctx, args, skip_mask);
+    case 15: return launch_column_n<15>(ctx, args, skip_mask);
+    case 20: return launch_column_n<20>(ctx, args, skip_mask);
+    case 25: return launch_column_n<25>(ctx, args, skip_mask);
+    case 26: return launch_column_n<26>(ctx, args, skip_mask);
+    default: return TM5_EINVAL;
+    }
+}
+
+}  // namespace tm5

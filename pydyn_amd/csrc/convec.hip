@@ -14,7 +14,7 @@ namespace tm5 {
 
 __device__ __forceinline__ bool skipped_c(const int *words, int mask)
 {
-    return (__hip_atomic_load(words + W_FLAGS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != 0;
+    return (words[W_FLAGS] & mask) != 0;  // written by earlier kernels of the stream
 }
 
 static inline int lead(int n) { return (n % 2 == 0) ? n + 1 : n; }

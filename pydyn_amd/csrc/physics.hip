@@ -6,59 +6,92 @@ namespace tm5 {
 
 __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 {
-    return (__hip_atomic_load(words + W_FLAGS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != 0;
+    return (words[W_FLAGS] & mask) != 0;  // written by earlier kernels of the stream
 }
 
 // ------------------------------------------------------------------ dynam0
-// pydyn/kernels.py:102-125 + pydyn/zoom/massflow.py:446-461 for column (i, j): the fluxes are
-// the meteo's times the amplitude a (rounded, as the interpolated meteo the reference stores),
-// then am = dt pu, bm = dt pv, cm = -dt sd with sd accumulated from the top; am(0) = am(im).
+// pydyn/kernels.py:102-125 + pydyn/zoom/massflow.py:446-461 on a tile of TI columns (i0.., j =
+// blockIdx.y), all levels: the fluxes are the meteo's times the amplitude a (rounded, as the
+// interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
+// convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
+// pit summed from the surface and sd accumulated from the top, in the reference's order.
+constexpr int D0_TI = 32;
+
 __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int *words,
                                                 int skip_mask)
 {
     if (skipped_p(words, skip_mask)) return;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= im * jm) return;
-    const int i = col % im, j = col / im;
-    auto conv = [&](int l) {
-        return ((a * pu(i, j, l) - a * pu(i + 1, j, l)) + a * pv(i, j, l)) - a * pv(i, j + 1, l);
-    };
-    double pit = conv(0);
-    for (int l = 1; l < lm; l++) pit = pit + conv(l);
-    cm(i, j, 0) = 0.0;
-    cm(i, j, lm) = 0.0;
-    if (lm > 1) {
-        double sd = conv(lm - 1) - (bt[lm - 1] - bt[lm]) * pit;
-        cm(i, j, lm - 1) = (-dt) * sd;
-        for (int l = lm - 2; l >= 1; l--) {
-            sd = (sd + conv(l)) - (bt[l] - bt[l + 1]) * pit;
-            cm(i, j, l) = (-dt) * sd;
-        }
+    extern __shared__ double conv[];  // [lm][TI]
+    const int TI = D0_TI;
+    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    const int ni = min(TI, im - i0);
+    const int tid = threadIdx.x, nth = blockDim.x;
+#pragma unroll 4
+    for (int idx = tid; idx < lm * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti >= ni) continue;
+        const int i = i0 + ti;
+        const double p0 = a * pu(i, j, l), p1 = a * pu(i + 1, j, l);
+        const double v0 = a * pv(i, j, l), v1 = a * pv(i, j + 1, l);
+        conv[idx] = ((p0 - p1) + v0) - v1;
+        am(i, j, l) = dt * (i == 0 ? a * pu(im, j, l) : p0);  // am(0) = am(im)
+        if (i == im - 1) am(im, j, l) = dt * p1;
+        bm(i, j, l) = dt * v0;
+        if (j == jm - 1) bm(i, jm, l) = dt * v1;
     }
-    for (int l = 0; l < lm; l++) {
-        am(i, j, l) = dt * (a * pu(i == 0 ? im : i, j, l));  // am(0) = am(im)
-        if (i == im - 1) am(im, j, l) = dt * (a * pu(im, j, l));
-        bm(i, j, l) = dt * (a * pv(i, j, l));
-        if (j == jm - 1) bm(i, jm, l) = dt * (a * pv(i, jm, l));
+    __syncthreads();
+    for (int ti = tid; ti < ni; ti += nth) {
+        const int i = i0 + ti;
+        double pit = conv[ti];
+        for (int l = 1; l < lm; l++) pit = pit + conv[l * TI + ti];
+        cm(i, j, 0) = 0.0;
+        cm(i, j, lm) = 0.0;
+        if (lm > 1) {
+            double sd = conv[(lm - 1) * TI + ti] - (bt[lm - 1] - bt[lm]) * pit;
+            cm(i, j, lm - 1) = (-dt) * sd;
+            for (int l = lm - 2; l >= 1; l--) {
+                sd = (sd + conv[l * TI + ti]) - (bt[l] - bt[l + 1]) * pit;
+                cm(i, j, l) = (-dt) * sd;
+            }
+        }
     }
 }
 
 // ------------------------------------------------------------------ m2phlb1
-// pydyn/zoom/massflow.py:282-290: sp = seqsum(m over l) g / dxyp(j), phlb = at + bt sp.
+// pydyn/zoom/massflow.py:282-290 on a tile of TI columns: the column masses staged in LDS,
+// sp = seqsum(m over l) g / dxyp(j) per column, then phlb = at + bt sp in parallel.
+constexpr int MP_TI = 32;
+
 __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m, const double *dxyp,
                                                  const double *at, const double *bt, View3 sp, View3 phlb, int *words,
                                                  int skip_mask)
 {
     if (skipped_p(words, skip_mask)) return;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= im * jm) return;
-    const int i = col % im, j = col / im;
-    double p = m(i, j, 0);
-    for (int l = 1; l < lm; l++) p = p + m(i, j, l);
-    p = p * 9.80665 / dxyp[j];
-    sp(i, j, 0) = p;
-    for (int l = 0; l <= lm; l++) phlb(i, j, l) = at[l] + bt[l] * p;
+    extern __shared__ double sh[];  // M [lm][TI], P [TI]
+    const int TI = MP_TI;
+    double *M = sh, *P = sh + lm * TI;
+    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    const int ni = min(TI, im - i0);
+    const int tid = threadIdx.x, nth = blockDim.x;
+#pragma unroll 8
+    for (int idx = tid; idx < lm * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti < ni) M[idx] = m(i0 + ti, j, l);
+    }
+    __syncthreads();
+    for (int ti = tid; ti < ni; ti += nth) {
+        double p = M[ti];
+        for (int l = 1; l < lm; l++) p = p + M[l * TI + ti];
+        p = p * 9.80665 / dxyp[j];
+        P[ti] = p;
+        sp(i0 + ti, j, 0) = p;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < (lm + 1) * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti < ni) phlb(i0 + ti, j, l) = at[l] + bt[l] * P[ti];
+    }
 }
 
 // ------------------------------------------------------------------ dkg
@@ -123,18 +156,20 @@ static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask)
 {
-    if (im < 1 || jm < 1 || lm < 1 || !bt) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_dynam0, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, lm, pu, pv, a, bt, dt,
-                       am, bm, cm, ctx->words, skip_mask);
+    if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
+    const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
+    hipLaunchKernelGGL(k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
+                       pv, a, bt, dt, am, bm, cm, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb)
 {
-    if (im < 1 || jm < 1 || lm < 1 || !dxyp || !at || !bt) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_m2phlb1, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, lm, m, dxyp, at, bt,
-                       sp, phlb, ctx->words, TM5_ERRORS);
+    if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !dxyp || !at || !bt) return TM5_EINVAL;
+    const dim3 grid((im + MP_TI - 1) / MP_TI, jm);
+    hipLaunchKernelGGL(k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * MP_TI * sizeof(double), ctx->stream, im, jm,
+                       lm, m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS);
     return status(hipGetLastError());
 }
 

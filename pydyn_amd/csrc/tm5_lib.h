@@ -24,6 +24,11 @@ struct tm5_ctx {
     int *hwords;    // pinned host mirror
     int *y_nloop;   // device, per-layer nloop of the last y operator
     int y_cap;      // its length
+    int *x_rows;    // device, per-row nloop of the last x operator
+    int x_cap;      // its length
+    int x_nrows;    // rows the last x operator wrote
+    int *z_ok;      // device, per-block verdicts of the z check at n = 1
+    int z_cap;      // its length
 };
 
 namespace tm5 {
