@@ -61,13 +61,16 @@ struct ColumnArgs {
     View3 sp, phlb;
 };
 
+constexpr int VEC = 64 + 13 * 32;  // per-level vectors of a wave's LDS
+
 template <int N>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
     constexpr int LD = (N % 2 == 0) ? N + 1 : N;
     extern __shared__ double lds[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the column is uniform over the wavefront: said so, the branches on it are scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ncol = a.im * a.jm;
     int col = blockIdx.x * 4 + wave;
     const bool active = col < ncol;
@@ -75,32 +78,39 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     const int i = col % a.im, j = col / a.im;
     const int lm = a.lm;
     // per wave: fu / conv1 / C^T (N+1)*LD, fd (N+1)*LD, vectors
-    double *U = lds + (size_t)wave * (2 * (N + 1) * LD + 8 * 64);
+    static_assert(N < 32, "the per-level vectors hold 32 levels");
+    double *U = lds + (size_t)wave * (2 * (N + 1) * LD + VEC);
     double *D = U + (N + 1) * LD;
     double *vec = D + (N + 1) * LD;
-    double *mc = vec, *eu = vec + 64, *du = vec + 128, *ed = vec + 192, *dd = vec + 256, *dk = vec + 320;
-    double *amu = vec + 384, *amd = vec + 448;
+    double *mc = vec;  // all lm levels (m2phlb1)
+    double *eu = vec + 64, *du = eu + 32, *ed = du + 32, *dd = ed + 32, *dk = dd + 32;
+    double *amu = dk + 32, *amd = amu + 32, *eum = amd + 32, *edm = eum + 32;
+    double *cu = edm + 32, *cd = cu + 32, *cdl = cd + 32, *cdk = cdl + 32;
 
     const double m_l = lane < lm ? a.m(i, j, lane) : 0.0;
     mc[lane] = m_l;
+    for (int x = lane; x < (N + 1) * LD; x += 64) U[x] = D[x] = 0.0;
+    if (lane < 32) amu[lane] = amd[lane] = 0.0;
+    __syncthreads();
     if (lane < N) {
-        eu[lane] = a.entu(i, j, lane);
+        const double e_u = a.entu(i, j, lane), e_d = a.entd(i, j, lane);
+        eu[lane] = e_u;
         du[lane] = a.detu(i, j, lane);
-        ed[lane] = a.entd(i, j, lane);
+        ed[lane] = e_d;
         dd[lane] = a.detd(i, j, lane);
         dk[lane] = a.dkg(i, j, lane);
+        eum[lane] = e_u / m_l;   // the fu(k, k-1) and fd(k-1, k-1) quotients, every level at once
+        edm[lane] = -e_d / m_l;
     }
-    for (int x = lane; x < (N + 1) * LD; x += 64) U[x] = D[x] = 0.0;
-    amu[lane] = amd[lane] = 0.0;
     __syncthreads();
 
     // ---- build conv1: lane c owns column c (pydyn/kernels.py:366-395)
-    const int ctop = a.top(i, j), clfs = a.lfs(i, j);
+    const int ctop = __builtin_amdgcn_readfirstlane(a.top(i, j)), clfs = __builtin_amdgcn_readfirstlane(a.lfs(i, j));
     // levels above the cloud top (updraft) or the level of free sinking (downdraft) contribute
     // exact zeros (the reference's where(act, .., 0.0)): their iterations are skipped
     const int kup = ctop < N ? ctop : N, kdn = clfs < N ? clfs : N;
     // the dense leading block of conv1; outside it the matrix is tridiagonal
-    const int DB = (kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N;
+    const int DB = __builtin_amdgcn_readfirstlane((kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N);
     {
         double run = 0.0, prev = 0.0;
         for (int k = 1; k <= kup; k++) {
@@ -115,7 +125,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 if (lane < k - 1)
                     v = on ? run * keep : 0.0;
                 else if (lane == k - 1)
-                    v = on ? eu[k - 1] / mc[k - 1] * keep : 0.0;
+                    v = on ? eum[k - 1] * keep : 0.0;
                 U[k * LD + lane] = v;
                 run = v;
             }
@@ -137,7 +147,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 if (lane >= k)
                     v = on ? run * keep : 0.0;
                 else if (lane == k - 1)
-                    v = on ? -ed[k - 1] / mc[k - 1] * keep : 0.0;
+                    v = on ? edm[k - 1] * keep : 0.0;
                 D[(k - 1) * LD + lane] = v;
                 run = v;
             }
@@ -146,14 +156,30 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         }
     }
     __syncthreads();
+    // the four corrections of interface k (pydyn/kernels.py:388-391), every k at once in lane k
+    {
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        if (lane >= 1 && lane < N) {
+            q0 = amu[lane] / mc[lane];
+            q1 = amd[lane] / mc[lane - 1];
+            q2 = dk[lane - 1] / mc[lane - 1];
+            q3 = dk[lane - 1] / mc[lane];
+        }
+        if (lane >= 1 && lane < N) {
+            cu[lane] = q0;
+            cd[lane] = q1;
+            cdl[lane] = q2;
+            cdk[lane] = q3;
+        }
+        __syncthreads();
+    }
     if (lane < N) {
         for (int k = 1; k < N; k++) {
             double v = U[k * LD + lane] + D[k * LD + lane];
-            // amu, amd are +0 beyond the cloud levels, and x - (+0) is x for every x
-            if (k <= kup && lane == k) v = v - amu[k] / mc[k];
-            if (k < kdn && lane == k - 1) v = v - amd[k] / mc[k - 1];
-            if (lane == k - 1) v = v + dk[k - 1] / mc[k - 1];
-            if (lane == k) v = v - dk[k - 1] / mc[k];
+            if (lane == k) v = v - cu[k];
+            if (lane == k - 1) v = v - cd[k];
+            if (lane == k - 1) v = v + cdl[k];
+            if (lane == k) v = v - cdk[k];
             D[k * LD + lane] = v;
         }
     }
@@ -171,17 +197,17 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     // ---- LU, lanes over rows.  Entry (r, c) is non-zero only inside the dense block (r, c < DB)
     // or on the three diagonals, and LU fill-in stays inside that envelope; the products skipped
     // outside it have a factor that is exactly +-0.
+    // Rows r <= k take the update with a zero multiplier: r - (+-0) is r up to a zero's sign.
 #pragma unroll
     for (int k = 0; k < N; k++) {
         const double piv = bcast(A[k], k);
-        if (lane > k) A[k] = A[k] / piv;
-        const double lrk = A[k];
+        const double q = A[k] / piv;
+        A[k] = lane > k ? q : A[k];
+        const double lrk = lane > k ? q : 0.0;
+        if (k + 1 < N) A[k + 1] = A[k + 1] - lrk * bcast(A[k + 1], k);
 #pragma unroll
-        for (int c = k + 1; c < N; c++) {
-            if (c < DB || c == k + 1) {
-                const double u = bcast(A[c], k);
-                if (lane > k) A[c] = A[c] - lrk * u;
-            }
+        for (int c = k + 2; c < N; c++) {
+            if (c < DB) A[c] = A[c] - lrk * bcast(A[c], k);
         }
     }
 
@@ -191,23 +217,19 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     for (int r = 0; r < N; r++) y[r] = (lane == r) ? 1.0 : 0.0;
 #pragma unroll
     for (int r = 1; r < N; r++) {
+        if (r < DB) {
 #pragma unroll
-        for (int s = 0; s < r; s++) {
-            if (r < DB || s == r - 1) {
-                const double l = bcast(A[s], r);
-                y[r] = y[r] - l * y[s];
-            }
+            for (int s = 0; s < r - 1; s++) y[r] = y[r] - bcast(A[s], r) * y[s];
         }
+        y[r] = y[r] - bcast(A[r - 1], r) * y[r - 1];
     }
 #pragma unroll
     for (int r = N - 1; r >= 0; r--) {
         double acc = y[r];
+        if (r + 1 < N) acc = acc - bcast(A[r + 1], r) * y[r + 1];
 #pragma unroll
-        for (int s = r + 1; s < N; s++) {
-            if (s < DB || s == r + 1) {
-                const double u = bcast(A[s], r);
-                acc = acc - u * y[s];
-            }
+        for (int s = r + 2; s < N; s++) {
+            if (s < DB) acc = acc - bcast(A[s], r) * y[s];
         }
         y[r] = acc / bcast(A[r], r);
     }
@@ -299,7 +321,7 @@ template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
 {
     constexpr int LD = (N % 2 == 0) ? N + 1 : N;
-    const size_t per_wave = (size_t)(2 * (N + 1) * LD + 8 * 64) * sizeof(double);
+    const size_t per_wave = (size_t)(2 * (N + 1) * LD + VEC) * sizeof(double);
     const int ncol = args.im * args.jm;
     hipLaunchKernelGGL(k_column<N>, dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
                        skip_mask);
