@@ -11,9 +11,10 @@ pydyn_amd.meteo).  Every interval is bit-identical to the reference (tests/test_
 value = simulated seconds advanced / wall seconds x 3600 / (365 x 86400), the reference's
 years/hour (pydyn/gpu/bench.py:26, 262) with the ndyn Check_CFL actually chose.
 
-Also reported: the dominant kernel's roofline (its algorithmic bytes per launch over its
-average launch time, measured here with HIP events on the stream it runs on) and the oracle
-(NumPy, one core) timed on a bounded sample of the same workload on this host.
+Also reported: the dominant kernel's roofline (its algorithmic bytes or FP64 operations per
+launch over its average launch time, from HIP events the library records around each launch on
+its stream, over intervals run after the timed ones) and the oracle (NumPy, one core) timed on a
+bounded sample of the same workload on this host.
 """
 
 import argparse
@@ -36,65 +37,102 @@ def parse():
     p.add_argument("--warmup", type=int, default=4)
     p.add_argument("--grid", default="glb3x2")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-intervals", type=int, default=4)
+    p.add_argument("--cpu-intervals", type=int, default=24)
+    p.add_argument("--profile-intervals", type=int, default=12, help="intervals timed per kernel after the run")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="end with three plain device copies of MIB MiB (a known size for rocprofv3 --pmc)")
     return p.parse_args()
 
 
-def time_kernel(fn, reps, torch):
-    """Average duration (s) of fn() over reps launches, with events on the current stream."""
-    fn()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(reps):
-        fn()
-    stop.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / 1e3 / reps
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
-def roofline(reg, torch):
-    """The x advection (dynamu) at glb3x2 on copies of the region's state: algorithmic bytes
-    per launch = read and write of m, rm, rxm, rym, rzm (im jm lm each) and am ((im+1) jm lm),
-    divided by the average launch time."""
-    from pydyn_amd import device as D
-    from pydyn_amd.gpu import kernels as K
+def column_flops(n, nt):
+    """Algorithmic FP64 operations of the fused column kernel per column: the dense Crout LU
+    of the n x n convection matrix (n-1-k divisions and (n-1-k)^2 multiply-subtracts at step
+    k), its explicit inverse by forward and back substitution of the n unit vectors, and per
+    tracer two applications (rm <- C rm, three slopes times diag C).  The matrix build is not
+    counted."""
+    lu = sum(m + 2 * m * m for m in range(n))
+    inverse = n * (2 * n * (n - 1) + n)
+    apply = nt * 2 * (2 * n * n + 3 * n)
+    return lu + inverse + apply
 
-    g = reg.g
-    m, am = reg.m.clone(), reg.am.clone()
-    q = [t.clone() for t in (reg.rm, reg.rxm, reg.rym, reg.rzm)]
-    saved = [t.clone() for t in (m, am, *q)]
 
-    def once():
-        K.advect_x(m, *q, am, deferred=True)
-
-    def restore():
-        for dst, src in zip((m, am, *q), saved):
-            dst.copy_(src)
-
+def kernel_units(g, nt):
+    """Algorithmic work per launch of the step's kernels at grid g: (bound, amount, unit).
+    Bytes are one read of every input and one write of every output array the kernel touches
+    (sub-cycles stay on chip)."""
     cells = g.im * g.jm * g.lm
-    faces = (g.im + 1) * g.jm * g.lm
-    nt = q[0].shape[3]
-    traffic = 2 * 8 * (cells * (1 + 4 * nt) + faces)
-    # time single launches with their state restored in between (the restore is not timed)
-    durations = []
-    for _ in range(20):
-        restore()
-        durations.append(time_kernel(once, 1, torch))
-    D.context().raise_errors()
-    dt = float(np.median(durations))
-    achieved = traffic / dt / 1e9
+    fx, fy, fz = (g.im + 1) * g.jm * g.lm, g.im * (g.jm + 1) * g.lm, g.im * g.jm * (g.lm + 1)
+    cols = g.im * g.jm
+    q = 4 * nt
     return {
-        "kernel": "k_x (dynamu, x advection with sub-cycling)",
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "bytes_per_launch": traffic,
-        "launch_us": round(dt * 1e6, 2),
+        "k_column<%d>" % g.lmax: ("mfma", cols * column_flops(g.lmax, nt), "flop"),
+        "k_x<false>": ("hbm", 8 * (2 * cells * (1 + q) + 2 * fx), "B"),
+        "k_y_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fy), "B"),
+        "k_z_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fz), "B"),
+        "k_x<true>": ("hbm", 8 * (2 * cells + 2 * fx), "B"),
+        "k_y_steps<true>": ("hbm", 8 * (2 * cells + fy), "B"),
+        "k_z_steps<true>": ("hbm", 8 * (2 * cells + fz), "B"),
+        "k_z_search": ("hbm", 8 * (cells + fz), "B"),
+        "k_y_search": ("hbm", 8 * (cells + fy), "B"),
+        "k_dynam0": ("hbm", 8 * (fx + fy + fx + fy + fz), "B"),
     }
+
+
+def traffic_of(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy kernel of known size), as recorded in profiles/traffic.json; None if not recorded."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def roofline(reg, intervals):
+    """Per-kernel GPU time of `intervals` more intervals, from events the library records around
+    every launch on its stream; the roofline of the kernel with the largest total."""
+    from pydyn_amd import device as D
+
+    ctx = D.context()
+    ctx.profile_begin()
+    for _ in range(intervals):
+        reg.interval()
+    prof = ctx.profile_end()
+    total = sum(s for _, s in prof.values())
+    kernel, (count, secs) = max(prof.items(), key=lambda kv: kv[1][1])
+    units = kernel_units(reg.g, reg.rm.shape[3])
+    if kernel not in units:
+        return {"kernel": kernel, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+                "traffic": None, "launch_us": round(secs / count * 1e6, 2)}
+    bound, amount, unit = units[kernel]
+    avg = secs / count
+    if bound == "hbm":
+        achieved, peak, runit = amount / avg / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved, peak, runit = amount / avg / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
+    out = {
+        "kernel": kernel,
+        "bound": bound,
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": runit,
+        "frac": round(achieved / peak, 4),
+        "traffic": traffic_of(kernel),
+        "algorithmic_per_launch": amount,
+        "algorithmic_unit": unit,
+        "launch_us": round(avg * 1e6, 2),
+        "launches": count,
+        "share_of_gpu_time": round(secs / total, 3),
+    }
+    if bound == "mfma":
+        out["note"] = ("FP64 vector adds and multiplies, no FMA (bit-exactness); peak is the published FP64 "
+                       "figure, which counts an FMA as two operations")
+    return out
 
 
 def cpu_baseline(met, intervals):
@@ -164,11 +202,17 @@ def main():
             "interval_s": 3600,
             "simulated_s": sim,
         },
-        "roofline": roofline(reg, torch),
+        "roofline": roofline(reg, a.profile_intervals),
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(met, a.cpu_intervals)
     print(json.dumps(line), flush=True)
+    if a.copy_calibration:
+        src = torch.ones(a.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     return 0
 
 

@@ -69,6 +69,13 @@ int tm5_ctx_flags(tm5_ctx *ctx, int clear, int *flags);
 /* The last sub-cycle counts found on the device (valid after tm5_ctx_flags or a stream sync):
    x: max over rows; y: per layer into nloop_layers[lm] (may be NULL); z: n_advz. */
 int tm5_ctx_nloop(tm5_ctx *ctx, int *x_max, int *y_max, int *nloop_layers, int lm, int *z_nadv);
+/* Per-kernel GPU time: from tm5_profile_begin every launch on the context is bracketed by
+   events on its stream; tm5_profile_end waits for the stream and totals them per kernel, and
+   tm5_profile_entry reads the idx-th total (TM5_EINVAL past the last).  The reference's
+   Backend.profile_begin / profile_end (pydyn/gpu/backend.py:633-649). */
+int tm5_profile_begin(tm5_ctx *ctx);
+int tm5_profile_end(tm5_ctx *ctx);
+int tm5_profile_entry(tm5_ctx *ctx, int idx, const char **kernel, long long *count, double *seconds);
 
 /* ------------------------------------------------------------------ advection */
 /* dynamu: periodic x rows, each (j, l) row with its own nloop < max_nloop; am left scaled by

@@ -61,6 +61,9 @@ _SIGNATURES = {
     "tm5_ctx_destroy": [_VP],
     "tm5_ctx_flags": [_VP, _I, _P(_I)],
     "tm5_ctx_nloop": [_VP, _P(_I), _P(_I), _P(_I), _I, _P(_I)],
+    "tm5_profile_begin": [_VP],
+    "tm5_profile_end": [_VP],
+    "tm5_profile_entry": [_VP, _I, _P(ctypes.c_char_p), _P(ctypes.c_longlong), _P(_D)],
     "tm5_advect_x": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],
     "tm5_advect_y": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],
     "tm5_advect_z": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I],

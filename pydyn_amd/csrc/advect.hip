@@ -145,46 +145,75 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
     if (q.nt > 0)
-        hipLaunchKernelGGL(k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
+        TM5_LAUNCH(ctx, k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
                            ctx->x_rows, fail_bit, skip_mask);
     else
-        hipLaunchKernelGGL(k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
+        TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
                            ctx->x_rows, fail_bit, skip_mask);
     return status(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ y: dynamv's search
-// pydyn/ops.py:315-341 with progressive scaling, one layer per workgroup: nloop = 1, 2, ...
-// until every interior face of the layer passes its trial; on a failure the whole layer's
-// faces are scaled by (nloop-1)/nloop in place (the reference scales bm(:,:,l) whole).  The
-// polar outer faces must be zero (pydyn/kernels.py:233-234).
+// pydyn/ops.py:315-341 with progressive scaling, per layer: nloop = 1, 2, ... until every
+// interior face of the layer passes its trial; on a failure the whole layer's faces are scaled
+// by (nloop-1)/nloop in place (the reference scales bm(:,:,l) whole).  The polar outer faces
+// must be zero (pydyn/kernels.py:233-234).
+constexpr int Y_FPT = 2;  // interior faces per thread in the y check at nloop = 1
+
+// grid (chunks, lm): every workgroup checks its chunk of layer l's faces at nloop = 1 (chunk 0
+// also the polar faces) and or-s its verdict into the layer's word; the last workgroup of the
+// layer to take its ticket reads the word, resets it and the ticket, and -- only where some
+// face failed -- runs the sequential search for the layer.  ticket and verdict: lm ints each.
 __global__ void __launch_bounds__(1024) k_y_search(int im, int jm, int lm, View3 m, View3 bm, int nmax, int *nloop,
-                                                   int *words, int fail_bit, int skip_mask)
+                                                   int *ticket, int *verdict, int *words, int fail_bit, int skip_mask)
 {
-    const int l = blockIdx.x;
-    if (skipped(words, skip_mask)) return;
+    const int l = blockIdx.y, chunk = blockIdx.x, chunks = gridDim.x;
+    const bool skip = skipped(words, skip_mask);
     const int tid = threadIdx.x, nth = blockDim.x;
-    bool bad = false;
-    for (int i = tid; i < im; i += nth) bad |= (bm(i, 0, l) != 0.0) || (bm(i, jm, l) != 0.0);
-    if (__syncthreads_or(bad)) {
+    const int nface = im * (jm - 1);
+    bool ok = true, bad = false;
+    if (!skip) {
+        if (chunk == 0)
+            for (int i = tid; i < im; i += nth) bad |= (bm(i, 0, l) != 0.0) || (bm(i, jm, l) != 0.0);
+        // nloop = 1: one Courant number per interior face, the loads independent of each other
+#pragma unroll
+        for (int r = 0; r < Y_FPT; r++) {
+            const int idx = (chunk * Y_FPT + r) * nth + tid;
+            if (idx < nface) {
+                const int i = idx % im, k = idx / im + 1;
+                const double bk = bm(i, k, l), mlo = m(i, k - 1, l), mhi = m(i, k, l);
+                ok &= fabs(bk / (bk >= 0.0 ? mlo : mhi)) < 1.0;
+            }
+        }
+    }
+    ok = __syncthreads_and(ok);
+    bad = __syncthreads_or(bad);
+    __shared__ bool last;
+    __shared__ int word;
+    if (tid == 0) {
+        if (!ok || bad) atomicOr(verdict + l, (ok ? 0 : 1) | (bad ? 2 : 0));
+        __threadfence();
+        last = atomicAdd(ticket + l, 1) == chunks - 1;
+        if (last) {
+            __threadfence();
+            word = __hip_atomic_load(verdict + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            verdict[l] = 0;
+            ticket[l] = 0;
+        }
+    }
+    __syncthreads();
+    if (!last || skip) return;
+    if (word & 2) {
         if (tid == 0) {
             atomicOr(words + W_FLAGS, TM5_BAD_Y);
             nloop[l] = 0;
         }
         return;
     }
-    const int nface = im * (jm - 1);
-    // nloop = 1: one Courant number per interior face, the loads independent of each other
-    bool ok = true;
-#pragma unroll 4
-    for (int idx = tid; idx < nface; idx += nth) {
-        const int i = idx % im, k = idx / im + 1;
-        const double bk = bm(i, k, l), mlo = m(i, k - 1, l), mhi = m(i, k, l);
-        ok &= fabs(bk / (bk >= 0.0 ? mlo : mhi)) < 1.0;
-    }
     int nl = 1;
-    if (!__syncthreads_and(ok)) {
+    if (word & 1) {
         // sub-cycling: the sequential search, the layer's faces scaled in place per retry
+        __threadfence();
         for (;;) {
             nl += 1;
             for (int idx = tid; idx < im * (jm + 1); idx += nth) {
@@ -321,14 +350,15 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
     const bool mass_only = q.nt == 0;
     const int TI = lines_per_block(jm, 16);
     if (TI < 1) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_y_search, dim3(lm), dim3(1024), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
-                       ctx->words, fail_bit, skip_mask);
+    const int chunks = (im * (jm - 1) + 1024 * Y_FPT - 1) / (1024 * Y_FPT);
+    TM5_LAUNCH(ctx, k_y_search, dim3(chunks, lm), dim3(1024), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
+               ctx->y_nloop + ctx->y_cap, ctx->y_nloop + 2 * ctx->y_cap, ctx->words, fail_bit, skip_mask);
     const dim3 grid((im + TI - 1) / TI, lm), block = line_block(TI, jm);
     if (mass_only)
-        hipLaunchKernelGGL(k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop, limits,
+        TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop, limits,
                            ctx->words, skip_mask);
     else
-        hipLaunchKernelGGL(k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop,
+        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop,
                            limits, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
@@ -363,7 +393,7 @@ __device__ int z_column_next(const View3 &m, const View3 &cm, int i, int j, int 
     return nmax + 1;
 }
 
-constexpr int Z_FPT = 16;  // interior faces per thread in the z search
+constexpr int Z_FPT = 8;  // interior faces per thread in the z search
 
 __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 m, View3 cm, int nmax, int *ok_blocks,
                                                   int *words, int fail_bit, int skip_mask)
@@ -374,15 +404,14 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
     bool ok = true;
     if (!skip) {
         // n = 1 (cm itself): one Courant number per interior face, the loads independent
-        const long long nface = (long long)im * jm * (lm - 1);
-        const long long base = (long long)blockIdx.x * nth * Z_FPT + tid;
+        const int nface = im * jm * (lm - 1);  // < 2^31 (launch_z)
+        const int base = blockIdx.x * nth * Z_FPT + tid;
 #pragma unroll
         for (int r = 0; r < Z_FPT; r++) {
-            const long long idx = base + (long long)r * nth;
+            const int idx = base + r * nth;
             if (idx < nface) {
-                const int i = (int)(idx % im);
-                const long long rest = idx / im;
-                const int j = (int)(rest % jm), k = (int)(rest / jm) + 1;
+                const int i = idx % im, rest = idx / im;
+                const int j = rest % jm, k = rest / jm + 1;
                 const double ck = cm(i, j, k), mlo = m(i, j, k - 1), mhi = m(i, j, k);
                 ok &= fabs(ck / (ck >= 0.0 ? mlo : mhi)) < 1.0;
             }
@@ -505,7 +534,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
              int skip_mask)
 {
-    if (im < 1 || jm < 1 || lm < 2) return TM5_EINVAL;
+    if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
     const int TI = lines_per_block(lm, 32);
     if (TI < 1) return TM5_EINVAL;
@@ -514,14 +543,14 @@ int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm,
     const long long nb_cols = ((long long)im * jm + 255) / 256;
     if (nb < nb_cols) nb = nb_cols;
     if (nb > ctx->z_cap) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, nmax, ctx->z_ok,
+    TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, nmax, ctx->z_ok,
                        ctx->words, fail_bit, skip_mask);
     const dim3 grid((im + TI - 1) / TI, jm), block = line_block(TI, lm);
     if (mass_only)
-        hipLaunchKernelGGL(k_z_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits, ctx->words,
+        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits, ctx->words,
                            skip_mask);
     else
-        hipLaunchKernelGGL(k_z_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
+        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
                            ctx->words, skip_mask);
     return status(hipGetLastError());
 }

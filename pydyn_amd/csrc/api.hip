@@ -4,9 +4,34 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 #include "tm5_lib.h"
 
+// events of the launches timed while profiling, and the totals of the last profile_end
+struct tm5_prof {
+    std::vector<hipEvent_t> events;                  // created on demand, reused
+    size_t used = 0;                                 // events recorded so far
+    std::vector<const char *> names;                 // per span: its kernel (spans use events 2s, 2s+1)
+    std::vector<const char *> total_names;
+    std::vector<long long> counts;
+    std::vector<double> seconds;
+};
+
 namespace tm5 {
+
+void prof_mark(tm5_ctx *ctx, const char *name, bool stop)
+{
+    tm5_prof *p = ctx->prof;
+    if (p->used == p->events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        p->events.push_back(e);
+    }
+    if (!stop) p->names.push_back(name);
+    hipEventRecord(p->events[p->used++], ctx->stream);
+}
+
 int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 entu, View3 detu, View3 entd,
                   View3 detd, IView2 top, IView2 lfs, View3 dkg, double dt, int lmax);
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
@@ -57,13 +82,13 @@ __global__ void __launch_bounds__(256) k_copy3(int n0, int n1, int n2, View3 src
 static int copy3(tm5_ctx *ctx, int n0, int n1, int n2, View3 src, View3 dst)
 {
     const long long n = (long long)n0 * n1 * n2;
-    hipLaunchKernelGGL(k_copy3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n0, n1, n2, src, dst);
+    TM5_LAUNCH(ctx, k_copy3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n0, n1, n2, src, dst);
     return status(hipGetLastError());
 }
 
 static int set_word(tm5_ctx *ctx, int index, int value, int mask)
 {
-    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, ctx->stream, ctx->words, index, value, mask);
+    TM5_LAUNCH(ctx, k_set_word, dim3(1), dim3(64), 0, ctx->stream, ctx->words, index, value, mask);
     return status(hipGetLastError());
 }
 
@@ -102,7 +127,9 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     c->x_cap = 1 << 17;
     c->z_cap = 1 << 13;
     if (e == hipSuccess) e = hipMalloc((void **)&c->words, W_WORDS * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, c->y_cap * sizeof(int));
+    // per layer: nloop, the y check's tickets and verdicts (zero between launches)
+    if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, 3 * c->y_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->y_nloop, 0, 3 * c->y_cap * sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&c->x_rows, c->x_cap * sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&c->z_ok, c->z_cap * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hwords, W_WORDS * sizeof(int), hipHostMallocDefault);
@@ -125,6 +152,10 @@ extern "C" int tm5_ctx_destroy(tm5_ctx *c)
 {
     if (!c) return 0;
     if (c->stream || c->words) hipStreamSynchronize(c->stream);
+    if (c->prof_store) {
+        for (hipEvent_t e : c->prof_store->events) hipEventDestroy(e);
+        delete c->prof_store;
+    }
     if (c->words) hipFree(c->words);
     if (c->y_nloop) hipFree(c->y_nloop);
     if (c->x_rows) hipFree(c->x_rows);
@@ -140,6 +171,53 @@ extern "C" int tm5_ctx_flags(tm5_ctx *c, int clear, int *flags)
     TRY(read_words(c));
     if (flags) *flags = c->hwords[W_FLAGS];
     if (clear) TRY(set_word(c, W_FLAGS, 0, ~0));
+    return 0;
+}
+
+extern "C" int tm5_profile_begin(tm5_ctx *c)
+{
+    if (!c) return TM5_EINVAL;
+    TRY(status(hipStreamSynchronize(c->stream)));
+    if (!c->prof_store) c->prof_store = new tm5_prof();
+    c->prof = c->prof_store;
+    c->prof->used = 0;
+    c->prof->names.clear();
+    return 0;
+}
+
+extern "C" int tm5_profile_end(tm5_ctx *c)
+{
+    if (!c || !c->prof) return TM5_EINVAL;
+    tm5_prof *p = c->prof;
+    TRY(status(hipStreamSynchronize(c->stream)));
+    p->total_names.clear();
+    p->counts.clear();
+    p->seconds.clear();
+    for (size_t s = 0; s < p->names.size() && 2 * s + 1 < p->used; s++) {
+        float ms = 0.0f;
+        TRY(status(hipEventElapsedTime(&ms, p->events[2 * s], p->events[2 * s + 1])));
+        size_t k = 0;
+        while (k < p->total_names.size() && strcmp(p->total_names[k], p->names[s]) != 0) k++;
+        if (k == p->total_names.size()) {
+            p->total_names.push_back(p->names[s]);
+            p->counts.push_back(0);
+            p->seconds.push_back(0.0);
+        }
+        p->counts[k] += 1;
+        p->seconds[k] += ms * 1e-3;
+    }
+    c->prof = nullptr;  // stop recording; the totals stay until the next begin
+    return 0;
+}
+
+extern "C" int tm5_profile_entry(tm5_ctx *c, int idx, const char **kernel, long long *count, double *seconds)
+{
+    if (!c) return TM5_EINVAL;
+    tm5_prof *p = c->prof_store;
+    if (!p || idx < 0 || (size_t)idx >= p->total_names.size()) return TM5_EINVAL;
+    if (kernel) *kernel = p->total_names[idx];
+    if (count) *count = p->counts[idx];
+    if (seconds) *seconds = p->seconds[idx];
     return 0;
 }
 

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     // the column is uniform over the wavefront: said so, the branches on it are scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ncol = a.im * a.jm;
-    int col = blockIdx.x * 4 + wave;
+    int col = xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
     const bool active = col < ncol;
     if (!active) col = ncol - 1;
     const int i = col % a.im, j = col / a.im;
@@ -323,7 +323,14 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     constexpr int LD = (N % 2 == 0) ? N + 1 : N;
     const size_t per_wave = (size_t)(2 * (N + 1) * LD + VEC) * sizeof(double);
     const int ncol = args.im * args.jm;
-    hipLaunchKernelGGL(k_column<N>, dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
+    static const char *name[] = {"k_column<0>", "k_column<1>", "k_column<2>", "k_column<3>", "k_column<4>",
+                                 "k_column<5>", "k_column<6>", "k_column<7>", "k_column<8>", "k_column<9>",
+                                 "k_column<10>", "k_column<11>", "k_column<12>", "k_column<13>", "k_column<14>",
+                                 "k_column<15>", "k_column<16>", "k_column<17>", "k_column<18>", "k_column<19>",
+                                 "k_column<20>", "k_column<21>", "k_column<22>", "k_column<23>", "k_column<24>",
+                                 "k_column<25>", "k_column<26>", "k_column<27>", "k_column<28>", "k_column<29>",
+                                 "k_column<30>", "k_column<31>"};
+    TM5_LAUNCH_NAMED(ctx, name[N], k_column<N>, dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
                        skip_mask);
     return status(hipGetLastError());
 }

@@ -176,7 +176,7 @@ int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View
     if (wpb < 1) return TM5_EINVAL;
     const int ncol = im * jm;
     const dim3 grid((ncol + wpb - 1) / wpb), block(64 * wpb);
-    hipLaunchKernelGGL(k_convec, grid, block, per * wpb, ctx->stream, im, jm, lmax, wpb, m, q, entu, detu, entd, detd,
+    TM5_LAUNCH(ctx, k_convec, grid, block, per * wpb, ctx->stream, im, jm, lmax, wpb, m, q, entu, detu, entd, detd,
                        top, lfs, dkg, dt, ctx->words, TM5_ERRORS);
     return status(hipGetLastError());
 }

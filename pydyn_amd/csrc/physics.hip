@@ -158,7 +158,7 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
     const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
-    hipLaunchKernelGGL(k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
+    TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
                        pv, a, bt, dt, am, bm, cm, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
@@ -168,7 +168,7 @@ int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !dxyp || !at || !bt) return TM5_EINVAL;
     const dim3 grid((im + MP_TI - 1) / MP_TI, jm);
-    hipLaunchKernelGGL(k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * MP_TI * sizeof(double), ctx->stream, im, jm,
+    TM5_LAUNCH(ctx, k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * MP_TI * sizeof(double), ctx->stream, im, jm,
                        lm, m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS);
     return status(hipGetLastError());
 }
@@ -176,7 +176,7 @@ int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *
 int launch_dkg(tm5_ctx *ctx, int im, int jm, View3 kvh, View3 m, View3 gph, int lmax, View3 dkg)
 {
     if (im < 1 || jm < 1 || lmax < 1) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_dkg, blocks((long long)im * jm * lmax), dim3(256), 0, ctx->stream, im, jm, lmax, kvh, m, gph,
+    TM5_LAUNCH(ctx, k_dkg, blocks((long long)im * jm * lmax), dim3(256), 0, ctx->stream, im, jm, lmax, kvh, m, gph,
                        dkg);
     return status(hipGetLastError());
 }
@@ -185,7 +185,7 @@ int launch_add_2d(tm5_ctx *ctx, int im, int jm, View3 emis, double factor, doubl
                   int level, int tracer)
 {
     if (im < 1 || jm < 1 || level < 1 || tracer < 1 || tracer > q.nt) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_add_2d, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, emis, factor, ratio,
+    TM5_LAUNCH(ctx, k_add_2d, blocks((long long)im * jm), dim3(256), 0, ctx->stream, im, jm, emis, factor, ratio,
                        efrac, q, level - 1, tracer - 1, ctx->words, TM5_ERRORS);
     return status(hipGetLastError());
 }
@@ -194,7 +194,7 @@ int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr,
                  int count)
 {
     if (im < 1 || jm < 1 || lm < 1 || t0 < 0 || count < 0 || t0 + count > q.nt) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_decay, blocks((long long)im * jm * lm), dim3(256), 0, ctx->stream, im, jm, lm, loss, sr, dt,
+    TM5_LAUNCH(ctx, k_decay, blocks((long long)im * jm * lm), dim3(256), 0, ctx->stream, im, jm, lm, loss, sr, dt,
                        q, t0, count, ctx->words, TM5_ERRORS);
     return status(hipGetLastError());
 }
@@ -202,7 +202,7 @@ int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr,
 int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor)
 {
     if (n0 < 1 || n1 < 1 || n2 < 1) return TM5_EINVAL;
-    hipLaunchKernelGGL(k_scale, blocks((long long)n0 * n1 * n2), dim3(256), 0, ctx->stream, n0, n1, n2, a, factor);
+    TM5_LAUNCH(ctx, k_scale, blocks((long long)n0 * n1 * n2), dim3(256), 0, ctx->stream, n0, n1, n2, a, factor);
     return status(hipGetLastError());
 }
 

@@ -50,6 +50,17 @@ __device__ __forceinline__ double slope_update(double s, double pf_lo, double pf
 }
 
 // pydyn/ops.py:304-307: dynamu/dynamv's retry scaling of the current flux.
+// Workgroups go to the eight XCDs round-robin (workgroup b to XCD b % 8), each XCD with its
+// own L2.  Renumbered by this bijection of [0, nb), each XCD takes a contiguous range of tiles,
+// so neighbouring tiles, which share cache lines, meet in one L2 instead of fetching the lines
+// once per XCD.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    constexpr int X = 8;
+    const int per = nb / X, rem = nb % X, x = b % X, k = b / X;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 __device__ __forceinline__ double shrink(double f, int n) { return f * (double)(n - 1) / (double)n; }
 
 // ------------------------------------------------------------------ the fdlibm exp

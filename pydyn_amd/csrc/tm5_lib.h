@@ -17,21 +17,48 @@ enum {
     W_WORDS = 8,
 };
 
+struct tm5_prof;  // the launches timed between tm5_profile_begin and tm5_profile_end (api.hip)
+
 struct tm5_ctx {
     hipStream_t stream;
     int device;
     int *words;     // device, W_WORDS ints
     int *hwords;    // pinned host mirror
-    int *y_nloop;   // device, per-layer nloop of the last y operator
+    int *y_nloop;   // device, per-layer nloop of the last y operator, then y_cap tickets, y_cap verdicts
     int y_cap;      // its length
     int *x_rows;    // device, per-row nloop of the last x operator
     int x_cap;      // its length
     int x_nrows;    // rows the last x operator wrote
     int *z_ok;      // device, per-block verdicts of the z check at n = 1
     int z_cap;      // its length
+    tm5_prof *prof; // non-null while profiling (then prof == prof_store)
+    tm5_prof *prof_store;  // owned: the events and the last totals
 };
 
 namespace tm5 {
+
+// Profiling: events recorded on the context's stream around a launch while profiling is on.
+void prof_mark(tm5_ctx *ctx, const char *name, bool stop);
+struct Span {
+    tm5_ctx *ctx;
+    const char *name;
+    Span(tm5_ctx *c, const char *n) : ctx(c), name(n)
+    {
+        if (ctx->prof) prof_mark(ctx, name, false);
+    }
+    ~Span()
+    {
+        if (ctx->prof) prof_mark(ctx, name, true);
+    }
+};
+
+// every kernel of the library is launched through these, on the context's stream
+#define TM5_LAUNCH_NAMED(ctx_, name, kernel, ...)    \
+    do {                                             \
+        ::tm5::Span span_((ctx_), (name));           \
+        hipLaunchKernelGGL(kernel, __VA_ARGS__);     \
+    } while (0)
+#define TM5_LAUNCH(ctx_, kernel, ...) TM5_LAUNCH_NAMED(ctx_, #kernel, kernel, __VA_ARGS__)
 
 struct View3 {
     double *p;

@@ -67,6 +67,26 @@ class Context:
                    "tm5_ctx_nloop")  # fmt: skip
         return x.value, np.array(layers[:lm]), z.value
 
+    def profile_begin(self):
+        """Time every launch on the context's stream until profile_end (pydyn/gpu/backend.py:633)."""
+        _lib.check(_lib.lib().tm5_profile_begin(self.ptr), "tm5_profile_begin")
+
+    def profile_end(self):
+        """{kernel: [launches, seconds]} since profile_begin, from events on the stream
+        (pydyn/gpu/backend.py:639-649)."""
+        _lib.check(_lib.lib().tm5_profile_end(self.ptr), "tm5_profile_end")
+        return self.profile_entries()
+
+    def profile_entries(self):
+        """The totals of the last profile_end."""
+        lib = _lib.lib()
+        out, idx = {}, 0
+        name, count, secs = ctypes.c_char_p(), ctypes.c_longlong(), ctypes.c_double()
+        while lib.tm5_profile_entry(self.ptr, idx, ctypes.byref(name), ctypes.byref(count), ctypes.byref(secs)) == 0:
+            out[name.value.decode()] = [count.value, secs.value]
+            idx += 1
+        return out
+
     def close(self):
         if self.ptr:
             _lib.lib().tm5_ctx_destroy(self.ptr)

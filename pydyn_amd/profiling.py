@@ -1,0 +1,92 @@
+"""Summaries of rocprofv3 CSV output for this library's kernels.
+
+    python -m pydyn_amd.profiling stats DIR
+        the kernel statistics of a `rocprofv3 --kernel-trace --stats --output-format csv` run
+    python -m pydyn_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_BYTES [--json OUT]
+        HBM bytes per launch of every kernel from a `--pmc FETCH_SIZE` run and a `--pmc
+        WRITE_SIZE` run of the same command, each scaled by what the counter reads for a plain
+        device copy of COPY_BYTES bytes in that run (the largest __amd_rocclr_copyBuffer
+        dispatch): bytes = FETCH_KiB * 1024 * (COPY_BYTES / copy FETCH bytes) + likewise WRITE.
+"""
+
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import sys
+
+
+def _csv(d, suffix):
+    found = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not found:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    with open(found[0], newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def stats(d):
+    """[(kernel, calls, total_ns, average_ns, percent)] from kernel_stats.csv, largest first."""
+    rows = []
+    for r in _csv(d, "kernel_stats.csv"):
+        rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                     float(r["Percentage"])))  # fmt: skip
+    return sorted(rows, key=lambda x: -x[2])
+
+
+def counter(d, name):
+    """{kernel: [dispatches, mean value]} of one counter from counter_collection.csv."""
+    acc = collections.defaultdict(list)
+    for r in _csv(d, "counter_collection.csv"):
+        if r["Counter_Name"] == name:
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: [len(v), sum(v) / len(v), max(v)] for k, v in acc.items()}
+
+
+def traffic(fetch_dir, write_dir, copy_bytes):
+    fetch, write = counter(fetch_dir, "FETCH_SIZE"), counter(write_dir, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise ValueError("no copy kernel in the FETCH_SIZE run")
+    kf = copy_bytes / (max(fetch[k][2] for k in copy) * 1024.0)
+    copy_w = [k for k in write if "copyBuffer" in k]
+    kw = copy_bytes / (max(write[k][2] for k in copy_w) * 1024.0)
+    out = {"_calibration": {"copy_bytes": copy_bytes, "fetch_factor": round(kf, 4), "write_factor": round(kw, 4)}}
+    for k in fetch:
+        if k in write:
+            b = fetch[k][1] * 1024.0 * kf + write[k][1] * 1024.0 * kw
+            short = k.split("(")[0].replace("void ", "").replace("tm5::", "")
+            out[short] = {"bytes_per_launch": round(b), "fetch_kib": round(fetch[k][1], 1),
+                          "write_kib": round(write[k][1], 1), "dispatches": fetch[k][0]}  # fmt: skip
+    return out
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    sub = p.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("copy_bytes", type=int)
+    t.add_argument("--json")
+    a = p.parse_args(argv)
+    if a.cmd == "stats":
+        print(f"{'kernel':64s} {'calls':>6s} {'avg_us':>9s} {'total_us':>10s} {'pct':>6s}")
+        for name, calls, tot, avg, pct in stats(a.dir):
+            short = name.split("(")[0].replace("void ", "")
+            print(f"{short[:64]:64s} {calls:6d} {avg / 1e3:9.2f} {tot / 1e3:10.1f} {pct:6.2f}")
+    else:
+        out = traffic(a.fetch_dir, a.write_dir, a.copy_bytes)
+        text = json.dumps(out, indent=1, sort_keys=True)
+        if a.json:
+            with open(a.json, "w") as f:
+                f.write(text + "\n")
+        print(text)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -220,3 +220,23 @@ def test_run_glb3x2_matches_oracle_and_reference(dev):
             for k, v in st.items():
                 assert sha(v) == want[k], f"glb3x2 interval {it}: {k} differs from the reference"
             print(f"glb3x2: interval {it + 1}/{len(ref)} bitwise equal to the reference", flush=True)
+
+
+def test_profile_times_every_launch(dev):
+    """profile_begin/profile_end total the GPU time per kernel from events around each launch."""
+    D, _ = dev
+    from pydyn_amd import meteo
+    from pydyn_amd.region import Region
+
+    reg = Region(meteo.Meteo(meteo.grid("tiny"), seed=3))
+    ctx = D.context()
+    ctx.profile_begin()
+    reg.interval()
+    prof = ctx.profile_end()
+    for k in ("k_x<false>", "k_y_search", "k_y_steps<false>", "k_z_search", "k_z_steps<false>", "k_dynam0"):
+        assert k in prof, (k, sorted(prof))
+        count, secs = prof[k]
+        assert count >= 2 and 0.0 < secs < 1.0, (k, prof[k])
+    assert any(k.startswith("k_column<") for k in prof), sorted(prof)
+    reg.interval()  # not recorded
+    assert ctx.profile_entries() == prof
