@@ -40,6 +40,15 @@ typedef struct tm5_tracers {
     int nt;                       /* number of tracers */
 } tm5_tracers;
 
+/* A halo'd array with its Fortran bounds (the zoom tree's layout, pydyn/layout.py:26-36):
+   element (i, j, l, t) at data + sum((index - lower) * stride), data the element at the lower
+   bounds. */
+typedef struct tm5_hview {
+    double *data;
+    int64_t stride[4];
+    int32_t lower[4];
+} tm5_hview;
+
 typedef struct tm5_iview {
     int32_t *data;
     int64_t stride[2]; /* i, j */
@@ -51,8 +60,10 @@ typedef struct tm5_iview {
 #define TM5_FAIL_Z 4     /* dynamw: too many CFL violations (CFLError) */
 #define TM5_BAD_Y 8      /* non-zero flux through the polar outer faces (ValueError) */
 #define TM5_BAD_Z 16     /* non-zero halo interfaces in cm (ValueError) */
-#define TM5_ERRORS 31    /* any of the above */
 #define TM5_REHEARSAL 32 /* a Check_CFL mass-only operator exceeded its cap (the verdict) */
+#define TM5_FAIL_XC 64   /* a child's dynamu: nloop too high (CFLError "dynuamu: nloop too high") */
+#define TM5_FAIL_YC 128  /* a child's dynamv: nloop too high (CFLError "nloop too high in dynamv") */
+#define TM5_ERRORS 223   /* any error bit (all but TM5_REHEARSAL) */
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
@@ -69,6 +80,8 @@ int tm5_ctx_flags(tm5_ctx *ctx, int clear, int *flags);
 /* The last sub-cycle counts found on the device (valid after tm5_ctx_flags or a stream sync):
    x: max over rows; y: per layer into nloop_layers[lm] (may be NULL); z: n_advz. */
 int tm5_ctx_nloop(tm5_ctx *ctx, int *x_max, int *y_max, int *nloop_layers, int lm, int *z_nadv);
+/* Clear the bits of `mask` in the flag word from the stream (a kernel: no wait, capturable). */
+int tm5_ctx_clear(tm5_ctx *ctx, int mask);
 /* Per-kernel GPU time: from tm5_profile_begin every launch on the context is bracketed by
    events on its stream; tm5_profile_end waits for the stream and totals them per kernel, and
    tm5_profile_entry reads the idx-th total (TM5_EINVAL past the last).  The reference's
@@ -157,6 +170,96 @@ int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *desc, tm5_region **ou
 int tm5_region_destroy(tm5_region *region);
 /* One interval of nread seconds from t0: returns the (ndyn, n) Check_CFL chose. */
 int tm5_region_interval(tm5_region *region, double t0, int nread, int ndyn_max, int *ndyn, int *n);
+
+
+/* ------------------------------------------------------------------ the zoom tree */
+/* The bodies the zoom driver (pydyn/zoom/steps.py, check_cfl.py, massflow.py) runs through
+   ZoomState.lib for a state on a device: pydyn.gpu.zoom's names (pydyn/gpu/zoom.py:38-74, the
+   list pydyn/zoom/host.py:42-48).  A region is described by tm5_zregion: its sizes, its
+   refinement relative to its parent, its footprint in the parent, its interior scope, and its
+   halo'd arrays (tm5_hview, declared bounds m (-1:im+2, -1:jm+2, 1:lm), rm..rzm (.., 1:nt),
+   am bm cm (0:im+1, 0:jm+1, 0:lm+1), pu (0:im, 1:jm, 1:lm), pv (1:im, 1:jm+1, 1:lm),
+   sp (-1:im+2, -1:jm+2, 1:1), phlb (-1:im+2, -1:jm+2, 1:lm+1)).  `tracers` = 0 is the
+   mass-only form of Check_CFL's rehearsal (putm_xedges, mixm_edges, ...): those kernels skip
+   once TM5_REHEARSAL is set. */
+typedef struct tm5_zregion {
+    int im, jm, lm, nt;
+    int xref, yref, zref, tref;      /* refinement relative to the parent (1 for region 1) */
+    int tref_abs;                    /* time refinement relative to the globe: dt = ndyn / (2 tref_abs) */
+    int ibeg, iend, jbeg, jend;      /* footprint in the parent's cells (children) */
+    int isr, ier, jsr, jer;          /* interior scope */
+    int xcyc, touch_sp, touch_np;    /* region 1: 1 1 1 */
+    int whole;                       /* im / xref spans the globe (no x walls) */
+    tm5_hview m, rm, rxm, rym, rzm, am, bm, cm, pu, pv, sp, phlb;
+    const double *dxyp;              /* device, jm */
+    const int32_t *owned;            /* device (im, jm), i fastest: 1 where convec, sources, chemistry act */
+    const int32_t *edge;             /* device (im, jm), i fastest: blank_zoom_region's edge map */
+} tm5_zregion;
+
+/* put_xedges / put_yedges: the parent's cells at each wall of the child into the child's halo
+   and interface cells (pydyn/zoom/exchange.py:17-105). */
+int tm5_zoom_put_xedges(tm5_ctx *ctx, const tm5_zregion *parent, const tm5_zregion *child, int tracers);
+int tm5_zoom_put_yedges(tm5_ctx *ctx, const tm5_zregion *parent, const tm5_zregion *child, int tracers);
+/* _compress_x/yedges and _uncompress_x/yedges over rows js..je (x) or columns is..ie (y) and
+   levels ls..le (pydyn/zoom/exchange.py:109-177). */
+int tm5_zoom_compress_xedges(tm5_ctx *ctx, const tm5_zregion *r, int js, int je, int ls, int le, int tracers);
+int tm5_zoom_uncompress_xedges(tm5_ctx *ctx, const tm5_zregion *r, int js, int je, int ls, int le, int tracers);
+int tm5_zoom_compress_yedges(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int ls, int le, int tracers);
+int tm5_zoom_uncompress_yedges(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int ls, int le, int tracers);
+/* mix_edges: the child's ring blocks set to their means (pydyn/zoom/exchange.py:181-231). */
+int tm5_zoom_mix_edges(tm5_ctx *ctx, const tm5_zregion *r, int tracers);
+/* update_parent: the parent's m, rm over the footprint from the child's sums
+   (pydyn/zoom/exchange.py:235-258). */
+int tm5_zoom_update_parent(tm5_ctx *ctx, const tm5_zregion *parent, const tm5_zregion *child, int tracers);
+/* _coarsen_block and coarsen_ambm (pydyn/zoom/massflow.py:79-177). */
+int tm5_zoom_coarsen_block(tm5_ctx *ctx, const tm5_zregion *parent, const tm5_zregion *child);
+int tm5_zoom_coarsen_ambm(tm5_ctx *ctx, const tm5_zregion *parent, const tm5_zregion *child);
+/* m2phlb1 (parent = NULL, region 1) or a child's m2phlb (pydyn/zoom/massflow.py:24-58); at, bt
+   device arrays of lm+1. */
+int tm5_zoom_m2phlb(tm5_ctx *ctx, const tm5_zregion *r, const tm5_zregion *parent, const double *at,
+                    const double *bt);
+/* dynam0 of a region from its pu, pv: dt = ndyn / (2 tref_abs), a cyclic region's halo faces
+   (pydyn/zoom/massflow.py:185-203).  The halos of am, bm, cm are never written otherwise, so the
+   reference's zero-fill of the whole arrays leaves them as they are. */
+int tm5_zoom_dynam0(tm5_ctx *ctx, const tm5_zregion *r, const double *bt, int ndyn, int rehearsal);
+/* outer_x/y_faces_zeroed: a child's outer interface faces saved into `save` and zeroed
+   (restore = 0), or put back (restore = 1); axis 0 = am, 1 = bm.  save holds
+   2 (jm+2)(lm+2) (x) or 2 (im+2)(lm+2) (y) doubles (pydyn/zoom/advect.py:121-157). */
+int tm5_zoom_outer_faces(tm5_ctx *ctx, const tm5_zregion *r, int axis, int restore, double *save, int rehearsal);
+/* dynamu / dynamv / dynamw of a region on the block's scope: region 1 on its interior (x then
+   writes its periodic halos on rows js..je), a child with its ends held; mass_only = the
+   rehearsal's dynamum/dynamvm/dynamwm, whose failure sets TM5_REHEARSAL
+   (pydyn/zoom/advect.py:209-472, pydyn/zoom/check_cfl.py:101-226). */
+int tm5_zoom_advect_x(tm5_ctx *ctx, const tm5_zregion *r, int js, int je, int limits, int max_nloop, int mass_only);
+int tm5_zoom_advect_y(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int limits, int max_nloop, int mass_only);
+int tm5_zoom_advect_z(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int js, int je, int limits,
+                      int iter_limit, int mass_only);
+
+/* The column physics of a region on the cells it owns: convec (convection and Kzz diffusion,
+   dt_conv), do_add_2d (x = ((emis factor) ratio) efrac at level/tracer) and the CH4 decay
+   (exp(-(sum of the four loss rates) dt_chem)).  `ops` is the sequence run in one pass over
+   each column: "v", "csv", "vsc", "vsccsv", or "vscMcsv" (M = m2phlb1 of every column, owned
+   or not) -- pydyn/zoom/physics.py:34-78, pydyn/zoom/steps.py:31-69.  Between two v's the air
+   mass does not change, so the convection matrix is built and inverted once. */
+typedef struct tm5_zphysics {
+    tm5_view entu, detu, entd, detd, dkg; /* (im, jm, >= lmax) kg/s; dkg (im, jm, lmax) */
+    tm5_iview cloud_top, cloud_lfs;
+    int lmax;
+    double dt_conv;
+    tm5_view emis;                        /* (im, jm) */
+    double factor, ratio, efrac;
+    int level, tracer;                    /* 1-based */
+    tm5_view loss;                        /* (im, jm, lm) with the 4 rates at stride loss_stride */
+    int64_t loss_stride;
+    double dt_chem;
+    const double *at, *bt;                /* for M */
+} tm5_zphysics;
+
+int tm5_zoom_physics(tm5_ctx *ctx, const tm5_zregion *r, const tm5_zphysics *p, const char *ops);
+
+/* save / restore / scale of whole arrays (pydyn/zoom/host.py:52-71) */
+int tm5_zoom_copy(tm5_ctx *ctx, const double *src, double *dst, int64_t n);
+int tm5_zoom_scale(tm5_ctx *ctx, double *a, int64_t n, double factor);
 
 #ifdef __cplusplus
 }

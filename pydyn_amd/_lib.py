@@ -9,7 +9,8 @@ import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libtm5hip.so"
 
-FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, ERRORS, REHEARSAL = 1, 2, 4, 8, 16, 31, 32
+FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC = 1, 2, 4, 8, 16, 32, 64, 128
+ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC
 EINVAL, EFLAGS = -1, -2
 
 
@@ -30,6 +31,43 @@ class Tracers(ctypes.Structure):
 
 class IView(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int64 * 2)]
+
+
+class HView(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int64 * 4), ("lower", ctypes.c_int32 * 4)]
+
+
+class ZRegion(ctypes.Structure):
+    _fields_ = [
+        *((name, ctypes.c_int) for name in ("im", "jm", "lm", "nt", "xref", "yref", "zref", "tref", "tref_abs",
+                                            "ibeg", "iend", "jbeg", "jend", "isr", "ier", "jsr", "jer",
+                                            "xcyc", "touch_sp", "touch_np", "whole")),
+        *((name, HView) for name in ("m", "rm", "rxm", "rym", "rzm", "am", "bm", "cm", "pu", "pv", "sp", "phlb")),
+        ("dxyp", ctypes.c_void_p),
+        ("owned", ctypes.c_void_p),
+        ("edge", ctypes.c_void_p),
+    ]  # fmt: skip
+
+
+class ZPhysics(ctypes.Structure):
+    _fields_ = [
+        *((name, View) for name in ("entu", "detu", "entd", "detd", "dkg")),
+        ("cloud_top", IView),
+        ("cloud_lfs", IView),
+        ("lmax", ctypes.c_int),
+        ("dt_conv", ctypes.c_double),
+        ("emis", View),
+        ("factor", ctypes.c_double),
+        ("ratio", ctypes.c_double),
+        ("efrac", ctypes.c_double),
+        ("level", ctypes.c_int),
+        ("tracer", ctypes.c_int),
+        ("loss", View),
+        ("loss_stride", ctypes.c_int64),
+        ("dt_chem", ctypes.c_double),
+        ("at", ctypes.c_void_p),
+        ("bt", ctypes.c_void_p),
+    ]
 
 
 class RegionDesc(ctypes.Structure):
@@ -61,6 +99,7 @@ _SIGNATURES = {
     "tm5_ctx_destroy": [_VP],
     "tm5_ctx_flags": [_VP, _I, _P(_I)],
     "tm5_ctx_nloop": [_VP, _P(_I), _P(_I), _P(_I), _I, _P(_I)],
+    "tm5_ctx_clear": [_VP, _I],
     "tm5_profile_begin": [_VP],
     "tm5_profile_end": [_VP],
     "tm5_profile_entry": [_VP, _I, _P(ctypes.c_char_p), _P(ctypes.c_longlong), _P(_D)],
@@ -80,6 +119,25 @@ _SIGNATURES = {
     "tm5_region_create": [_VP, _P(RegionDesc), _P(_VP)],
     "tm5_region_destroy": [_VP],
     "tm5_region_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
+    "tm5_zoom_put_xedges": [_VP, _P(ZRegion), _P(ZRegion), _I],
+    "tm5_zoom_put_yedges": [_VP, _P(ZRegion), _P(ZRegion), _I],
+    "tm5_zoom_compress_xedges": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_uncompress_xedges": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_compress_yedges": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_uncompress_yedges": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_mix_edges": [_VP, _P(ZRegion), _I],
+    "tm5_zoom_update_parent": [_VP, _P(ZRegion), _P(ZRegion), _I],
+    "tm5_zoom_coarsen_block": [_VP, _P(ZRegion), _P(ZRegion)],
+    "tm5_zoom_coarsen_ambm": [_VP, _P(ZRegion), _P(ZRegion)],
+    "tm5_zoom_m2phlb": [_VP, _P(ZRegion), _P(ZRegion), _VP, _VP],
+    "tm5_zoom_dynam0": [_VP, _P(ZRegion), _VP, _I, _I],
+    "tm5_zoom_outer_faces": [_VP, _P(ZRegion), _I, _I, _VP, _I],
+    "tm5_zoom_advect_x": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_advect_y": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
+    "tm5_zoom_advect_z": [_VP, _P(ZRegion), _I, _I, _I, _I, _I, _I, _I],
+    "tm5_zoom_physics": [_VP, _P(ZRegion), _P(ZPhysics), ctypes.c_char_p],
+    "tm5_zoom_copy": [_VP, _VP, _VP, ctypes.c_int64],
+    "tm5_zoom_scale": [_VP, _VP, ctypes.c_int64, _D],
 }
 
 _STATE = {}

@@ -370,7 +370,9 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
 // workgroup to finish checks every column at the candidate and, if one fails there, raises the
 // candidate to that column's next passing n, until all pass (the reference's sequential search,
 // found as a fixpoint).
-__device__ bool z_column_passes(const View3 &m, const View3 &cm, int i, int j, int lm, int n)
+// `moves`: the trial advances the column's mass between sub-steps; a column outside a child's
+// scope keeps its mass through the trial (pydyn/zoom/advect.py:415-441 advances the scope only).
+__device__ bool z_column_passes(const View3 &m, const View3 &cm, int i, int j, int lm, int n, bool moves)
 {
     const double dn = (double)n;
     for (int k = 1; k < lm; k++) {
@@ -380,23 +382,29 @@ __device__ bool z_column_passes(const View3 &m, const View3 &cm, int i, int j, i
         double mx = m(i, j, u);
         for (int s = 0; s < n; s++) {
             if (!(fabs(ck / mx) < 1.0)) return false;
-            mx = (mx + clo) - chi;
+            if (moves) mx = (mx + clo) - chi;
         }
     }
     return true;
 }
 
-__device__ int z_column_next(const View3 &m, const View3 &cm, int i, int j, int lm, int from, int nmax)
+__device__ int z_column_next(const View3 &m, const View3 &cm, int i, int j, int lm, int from, int nmax, bool moves)
 {
     for (int n = from; n <= nmax; n++)
-        if (z_column_passes(m, cm, i, j, lm, n)) return n;
+        if (z_column_passes(m, cm, i, j, lm, n, moves)) return n;
     return nmax + 1;
 }
 
+// the scope the trial advances, 0-based inclusive within the region's interior
+struct ZScope {
+    int i0, i1, j0, j1;
+    __device__ __forceinline__ bool has(int i, int j) const { return i >= i0 && i <= i1 && j >= j0 && j <= j1; }
+};
+
 constexpr int Z_FPT = 8;  // interior faces per thread in the z search
 
-__global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 m, View3 cm, int nmax, int *ok_blocks,
-                                                  int *words, int fail_bit, int skip_mask)
+__global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 m, View3 cm, ZScope scope, int nmax,
+                                                  int *ok_blocks, int *words, int fail_bit, int skip_mask)
 {
     // every workgroup takes its ticket, skipping or not, so that the last one always resets it
     const bool skip = skipped(words, skip_mask);
@@ -444,7 +452,9 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
         int need = cand;
         for (int c = tid; c < ncol; c += nth) {
             const int i = c % im, j = c / im;
-            if (!z_column_passes(m, cm, i, j, lm, cand)) need = max(need, z_column_next(m, cm, i, j, lm, cand + 1, nmax));
+            const bool moves = scope.has(i, j);
+            if (!z_column_passes(m, cm, i, j, lm, cand, moves))
+                need = max(need, z_column_next(m, cm, i, j, lm, cand + 1, nmax, moves));
         }
         red[tid] = need;
         __syncthreads();
@@ -531,10 +541,13 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     }
 }
 
-int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
-             int skip_mask)
+// The search over the whole region (the trial advancing only the scope's columns), then the
+// sub-steps on the scope: region 1's scope is the region, a child's its block's.
+int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
+                    int limits, int nmax, int fail_bit, int skip_mask)
 {
     if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
+    if (i0 < 0 || j0 < 0 || i1 >= im || j1 >= jm || i1 < i0 || j1 < j0) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
     const int TI = lines_per_block(lm, 32);
     if (TI < 1) return TM5_EINVAL;
@@ -543,16 +556,28 @@ int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm,
     const long long nb_cols = ((long long)im * jm + 255) / 256;
     if (nb < nb_cols) nb = nb_cols;
     if (nb > ctx->z_cap) return TM5_EINVAL;
-    TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, nmax, ctx->z_ok,
-                       ctx->words, fail_bit, skip_mask);
-    const dim3 grid((im + TI - 1) / TI, jm), block = line_block(TI, lm);
+    const ZScope scope{i0, i1, j0, j1};
+    TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, scope, nmax,
+               ctx->z_ok, ctx->words, fail_bit, skip_mask);
+    // the scope's views
+    const int si = i1 - i0 + 1, sj = j1 - j0 + 1;
+    View3 ms{m.p + i0 * m.si + j0 * m.sj, m.si, m.sj, m.sl}, cs{cm.p + i0 * cm.si + j0 * cm.sj, cm.si, cm.sj, cm.sl};
+    Tracers qs = q;
+    for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (i0 * q.si + j0 * q.sj);
+    const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
     if (mass_only)
-        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits, ctx->words,
-                           skip_mask);
+        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, ctx->words,
+                   skip_mask);
     else
-        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, cm, limits,
-                           ctx->words, skip_mask);
+        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, ctx->words,
+                   skip_mask);
     return status(hipGetLastError());
+}
+
+int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
+             int skip_mask)
+{
+    return launch_z_scoped(ctx, im, jm, lm, m, q, cm, 0, im - 1, 0, jm - 1, limits, nmax, fail_bit, skip_mask);
 }
 
 }  // namespace tm5

@@ -34,8 +34,6 @@ void prof_mark(tm5_ctx *ctx, const char *name, bool stop)
 
 int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 entu, View3 detu, View3 entd,
                   View3 detd, IView2 top, IView2 lfs, View3 dkg, double dt, int lmax);
-int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask);
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb);
 int launch_dkg(tm5_ctx *ctx, int im, int jm, View3 kvh, View3 m, View3 gph, int lmax, View3 dkg);
@@ -45,25 +43,7 @@ int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr,
                  int count);
 int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor);
 
-struct ColumnArgs {
-    int im, jm, lm;
-    View3 m;
-    Tracers q;
-    View3 entu, detu, entd, detd, dkg;
-    IView2 top, lfs;
-    double dt;
-    int fused;
-    View3 emis;
-    double factor, ratio, efrac;
-    int level, tracer;
-    View3 loss;
-    long long loss_sr;
-    double dt_chem;
-    const double *dxyp, *at, *bt;
-    View3 sp, phlb;
-};
-bool column_supported(int n);
-int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+
 
 // a word of the context's scratch set from the stream (a kernel, so that it can be captured)
 __global__ void k_set_word(int *words, int index, int value, int mask)
@@ -172,6 +152,12 @@ extern "C" int tm5_ctx_flags(tm5_ctx *c, int clear, int *flags)
     if (flags) *flags = c->hwords[W_FLAGS];
     if (clear) TRY(set_word(c, W_FLAGS, 0, ~0));
     return 0;
+}
+
+extern "C" int tm5_ctx_clear(tm5_ctx *c, int mask)
+{
+    if (!c) return TM5_EINVAL;
+    return set_word(c, W_FLAGS, 0, mask);
 }
 
 extern "C" int tm5_profile_begin(tm5_ctx *c)
@@ -329,7 +315,6 @@ extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m,
         a.top = iview2(cloud_top);
         a.lfs = iview2(cloud_lfs);
         a.dt = dt;
-        a.fused = 0;
         return launch_column(c, a, lmax, TM5_ERRORS);
     }
     return launch_convec(c, im, jm, lm, view3(m), tracers(q), view3(entu), view3(detu), view3(entd), view3(detd),
@@ -511,7 +496,8 @@ static int middle(tm5_region *r, int ndyn)
     a.top = iview2(&d.cloud_top);
     a.lfs = iview2(&d.cloud_lfs);
     a.dt = (double)ndyn / 2.0;
-    a.fused = 1;
+    a.twice = 1;
+    a.mid = 1;
     a.emis = view3(&d.emis);
     a.factor = ((double)ndyn / 2.0) / d.sec_month;  // month2dt, pydyn/emis.py:65-68
     a.ratio = d.mol_ratio;

@@ -3,8 +3,9 @@
 // Between the two 'v' operators of a step ("xyz v s c | m2phlb1 | c s v zyx") only the sources,
 // the chemistry and m2phlb1 run; none of them changes the air mass, so both convection calls
 // build the same matrix and invert it to the same bits.  k_column<N> builds conv1 once per
-// column, inverts it, and applies "v s c m2phlb1 c s v" to every tracer of the column in one
-// pass (fused mode), or just "v" (the convec operator alone).
+// column, inverts it, and applies one of the sequences "v", "csv", "vsc", "vsccsv" or
+// "vscMcsv" (M = m2phlb1) to every tracer of the column in one pass; in a zoom tree only on the
+// columns the region owns.
 //
 // One column per 64-lane wavefront, N = lmax known at compile time so that the matrix lives in
 // registers:
@@ -42,26 +43,19 @@ __device__ __forceinline__ double bcast(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-struct ColumnArgs {
-    int im, jm, lm;
-    View3 m;
-    Tracers q;
-    View3 entu, detu, entd, detd, dkg;
-    IView2 top, lfs;
-    double dt;  // convection's dt = ndyn / (2 tref)
-    // fused mode: sources, chemistry and m2phlb1
-    int fused;
-    View3 emis;
-    double factor, ratio, efrac;
-    int level, tracer;  // 0-based
-    View3 loss;
-    long long loss_sr;
-    double dt_chem;
-    const double *dxyp, *at, *bt;
-    View3 sp, phlb;
-};
 
 constexpr int VEC = 64 + 13 * 32;  // per-level vectors of a wave's LDS
+
+// m2phlb1 of one column (pydyn/zoom/massflow.py:24-32): every lane holds m of its level in m_l
+__device__ __forceinline__ void column_m2phlb1(const ColumnArgs &a, int i, int j, double m_l, const double *mc)
+{
+    double p = bcast(m_l, 0);
+    for (int l = 1; l < a.lm; l++) p = p + mc[l];
+    p = p * 9.80665 / a.dxyp[j];
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) a.sp(i, j, 0) = p;
+    for (int l = lane; l <= a.lm; l += 64) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
+}
 
 template <int N>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
@@ -89,6 +83,9 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 
     const double m_l = lane < lm ? a.m(i, j, lane) : 0.0;
     mc[lane] = m_l;
+    // a column the region does not own is computed like the others (every wave reaches every
+    // barrier) but keeps its tracers; the middle m2phlb1 is every column's
+    const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
     for (int x = lane; x < (N + 1) * LD; x += 64) U[x] = D[x] = 0.0;
     if (lane < 32) amu[lane] = amd[lane] = 0.0;
     __syncthreads();
@@ -248,9 +245,10 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     }
     const double diag = lane < N ? U[lane * LD + lane] : 0.0;
 
-    // ---- apply
+    // ---- apply: the flagged sequence around v for each tracer of the column
+    const bool sc = a.pre || a.twice || a.post;
     double decay = 1.0, x = 0.0;
-    if (a.fused) {
+    if (sc) {
         if (lane < lm) {
             const double *L = &a.loss(i, j, lane);
             const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
@@ -267,19 +265,20 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             ry = a.q.q[2][o];
             rz = a.q.q[3][o];
         }
-        const int passes = a.fused ? 2 : 1;
-        for (int p = 0; p < passes; p++) {
-            if (p == 1) {  // c, s (second half's order), then v
-                rm = rm * decay;
-                rx = rx * decay;
-                ry = ry * decay;
-                rz = rz * decay;
-                if (lane == a.level && t == a.tracer) {
-                    rm = rm + x;
-                    rz = rz - x;
-                }
+        const bool here = lane == a.level && t == a.tracer;
+        auto s_op = [&]() {  // do_add_2d: rm + x, rzm - x at the level
+            if (here) {
+                rm = rm + x;
+                rz = rz - x;
             }
-            // v: rm <- C rm over levels < N, slopes times diag(C)
+        };
+        auto c_op = [&]() {  // the decay of every moment
+            rm = rm * decay;
+            rx = rx * decay;
+            ry = ry * decay;
+            rz = rz * decay;
+        };
+        auto v_op = [&]() {  // rm <- C rm over levels < N, slopes times diag(C)
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < N; k++) acc = acc + crow[k] * bcast(rm, k);
@@ -289,18 +288,24 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 ry = diag * ry;
                 rz = diag * rz;
             }
-            if (p == 0 && a.fused) {  // s, c (first half's order)
-                if (lane == a.level && t == a.tracer) {
-                    rm = rm + x;
-                    rz = rz - x;
-                }
-                rm = rm * decay;
-                rx = rx * decay;
-                ry = ry * decay;
-                rz = rz * decay;
-            }
+        };
+        if (a.pre) {
+            c_op();
+            s_op();
         }
-        if (active && lane < lm) {
+        v_op();
+        if (a.twice) {
+            s_op();
+            c_op();
+            c_op();
+            s_op();
+            v_op();
+        }
+        if (a.post) {
+            s_op();
+            c_op();
+        }
+        if (active && own && lane < lm) {
             a.q.q[0][o] = rm;
             a.q.q[1][o] = rx;
             a.q.q[2][o] = ry;
@@ -308,13 +313,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         }
     }
     // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290)
-    if (a.fused && active) {
-        double p = bcast(m_l, 0);
-        for (int l = 1; l < lm; l++) p = p + mc[l];
-        p = p * 9.80665 / a.dxyp[j];
-        if (lane == 0) a.sp(i, j, 0) = p;
-        for (int l = lane; l <= lm; l += 64) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
-    }
+    if (a.mid && active) column_m2phlb1(a, i, j, m_l, mc);
 }
 
 template <int N>

@@ -18,8 +18,8 @@ __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 constexpr int D0_TI = 32;
 
 __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
-                                                const double *bt, double dt, View3 am, View3 bm, View3 cm, int *words,
-                                                int skip_mask)
+                                                const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
+                                                int *words, int skip_mask)
 {
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double conv[];  // [lm][TI]
@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
         const double p0 = a * pu(i, j, l), p1 = a * pu(i + 1, j, l);
         const double v0 = a * pv(i, j, l), v1 = a * pv(i, j + 1, l);
         conv[idx] = ((p0 - p1) + v0) - v1;
-        am(i, j, l) = dt * (i == 0 ? a * pu(im, j, l) : p0);  // am(0) = am(im)
+        am(i, j, l) = dt * (i == 0 && cyclic ? a * pu(im, j, l) : p0);  // a cyclic region: am(0) = am(im)
         if (i == im - 1) am(im, j, l) = dt * p1;
         bm(i, j, l) = dt * v0;
         if (j == jm - 1) bm(i, jm, l) = dt * v1;
@@ -154,12 +154,12 @@ __global__ void __launch_bounds__(256) k_scale(int n0, int n1, int n2, View3 a, 
 static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask)
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
     const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
-                       pv, a, bt, dt, am, bm, cm, ctx->words, skip_mask);
+                       pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 

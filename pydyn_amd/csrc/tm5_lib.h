@@ -85,6 +85,73 @@ struct IView2 {
     __device__ __forceinline__ int operator()(int i, int j) const { return p[(long long)i * si + (long long)j * sj]; }
 };
 
+// A view of a halo'd array addressed by Fortran indices: element (i, j, l[, t]) at
+// p + (i - l0) s0 + (j - l1) s1 + (l - l2) s2 + (t - l3) s3, p the element at the lower bounds.
+struct HV {
+    double *p;
+    long long s0, s1, s2, s3;
+    int l0, l1, l2, l3;
+    __device__ __forceinline__ double &operator()(int i, int j, int l) const
+    {
+        return p[(long long)(i - l0) * s0 + (long long)(j - l1) * s1 + (long long)(l - l2) * s2];
+    }
+    __device__ __forceinline__ double &operator()(int i, int j, int l, int t) const
+    {
+        return p[(long long)(i - l0) * s0 + (long long)(j - l1) * s1 + (long long)(l - l2) * s2 +
+                 (long long)(t - l3) * s3];
+    }
+};
+
+inline HV hv(const tm5_hview &v)
+{
+    return HV{v.data, v.stride[0], v.stride[1], v.stride[2], v.stride[3], v.lower[0], v.lower[1], v.lower[2], v.lower[3]};
+}
+
+// A region of the zoom tree on the device (tm5_zregion with its views as HV).
+struct ZR {
+    int im, jm, lm, nt;
+    int xr, yr, zr, tr, tref;
+    int ibeg, iend, jbeg, jend, isr, ier, jsr, jer;
+    int xcyc, touch_sp, touch_np, whole;
+    HV m, q[4], am, bm, cm, pu, pv, sp, phlb;
+    const double *dxyp;
+    const int *owned, *edge;
+};
+
+inline ZR zr(const tm5_zregion &d)
+{
+    ZR r;
+    r.im = d.im, r.jm = d.jm, r.lm = d.lm, r.nt = d.nt;
+    r.xr = d.xref, r.yr = d.yref, r.zr = d.zref, r.tr = d.tref, r.tref = d.tref_abs;
+    r.ibeg = d.ibeg, r.iend = d.iend, r.jbeg = d.jbeg, r.jend = d.jend;
+    r.isr = d.isr, r.ier = d.ier, r.jsr = d.jsr, r.jer = d.jer;
+    r.xcyc = d.xcyc, r.touch_sp = d.touch_sp, r.touch_np = d.touch_np, r.whole = d.whole;
+    r.m = hv(d.m), r.q[0] = hv(d.rm), r.q[1] = hv(d.rxm), r.q[2] = hv(d.rym), r.q[3] = hv(d.rzm);
+    r.am = hv(d.am), r.bm = hv(d.bm), r.cm = hv(d.cm), r.pu = hv(d.pu), r.pv = hv(d.pv);
+    r.sp = hv(d.sp), r.phlb = hv(d.phlb);
+    r.dxyp = d.dxyp, r.owned = d.owned, r.edge = d.edge;
+    return r;
+}
+
+// the halo-free view of a halo'd array from Fortran (i0, j0, l0)
+inline View3 sub3(const HV &v, int i0, int j0, int l0)
+{
+    return View3{v.p + (long long)(i0 - v.l0) * v.s0 + (long long)(j0 - v.l1) * v.s1 + (long long)(l0 - v.l2) * v.s2,
+                 v.s0, v.s1, v.s2};
+}
+
+inline Tracers subq(const ZR &r, int i0, int j0, int l0)
+{
+    Tracers t;
+    for (int a = 0; a < 4; a++) {
+        const HV &v = r.q[a];
+        t.q[a] = v.p + (long long)(i0 - v.l0) * v.s0 + (long long)(j0 - v.l1) * v.s1 + (long long)(l0 - v.l2) * v.s2;
+    }
+    t.si = r.q[0].s0, t.sj = r.q[0].s1, t.sl = r.q[0].s2, t.st = r.q[0].s3;
+    t.nt = r.nt;
+    return t;
+}
+
 inline View3 view3(const tm5_view *v) { return View3{v->data, v->stride[0], v->stride[1], v->stride[2]}; }
 
 inline Tracers tracers(const tm5_tracers *q)
@@ -120,5 +187,35 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
              int skip_mask);
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
              int skip_mask);
+// The column physics of one launch (column.hip): convec with, around it, the sources (s), the
+// chemistry (c) and m2phlb1 (M) in the sequence the flags give:
+//   plain "v";  pre "csv";  post "vsc";  twice "vsccsv";  twice + mid "vscMcsv".
+struct ColumnArgs {
+    int im, jm, lm;
+    View3 m;
+    Tracers q;
+    View3 entu, detu, entd, detd, dkg;
+    IView2 top, lfs;
+    double dt;  // convection's dt = ndyn / (2 tref)
+    int pre, twice, mid, post;
+    const int *owned;  // null: every column; else (im, jm) i fastest, 1 where the region acts
+    View3 emis;
+    double factor, ratio, efrac;
+    int level, tracer;  // 0-based
+    View3 loss;
+    long long loss_sr;
+    double dt_chem;
+    const double *dxyp, *at, *bt;
+    View3 sp, phlb;
+};
+bool column_supported(int n);
+int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1);
+int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
+                   const double *bt, View3 sp, View3 phlb);
+int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+
+int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
+                    int limits, int nmax, int fail_bit, int skip_mask);
 
 }  // namespace tm5

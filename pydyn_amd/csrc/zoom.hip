@@ -1,0 +1,1031 @@
+// The zoom tree on the device: a global region with nested children (pydyn/zoom/*.py).
+//
+// A region's arrays keep the Fortran bounds the model declares (m (-1:im+2, -1:jm+2, 1:lm),
+// am bm cm (0:im+1, 0:jm+1, 0:lm+1), ...): the exchange between a parent and a child is halo
+// arithmetic, so the kernels index with Fortran indices through HV, a view that knows its lower
+// bounds.  Region 1 runs the single-region kernels on its interior views; this file has what a
+// tree adds:
+//   exchange   put_x/yedges (parent cells into the child's halo and interface cells),
+//              compress/uncompress_x/yedges, mix_edges (the child's ring of parent-cell blocks
+//              set to their means), update_parent (the parent's footprint from the child's sums)
+//   mass flow  coarsen_block (the parent's fluxes and mass over the footprint from the
+//              child's), coarsen_ambm (the child's outer-interface fluxes from the parent's),
+//              m2phlb for a child (the ring's pressure rescaled to the parent cell's area)
+//   advection  the child's x rows and y columns with their end cells held (they are the
+//              parent's), their sub-cycle searches, and the halos region 1's x writes
+//   helpers    the outer interface faces zeroed and restored, whole-array copy and scale.
+// Every kernel is one thread per output element (or per line for the advection), each doing
+// the reference's operations in its order; the sums that the reference accumulates from zero
+// start from 0.0 here too (0.0 + x can change the sign of a zero).
+#include <string.h>
+
+#include "tm5_lib.h"
+
+namespace tm5 {
+
+__device__ __forceinline__ bool zskip(const int *words, int mask) { return (words[W_FLAGS] & mask) != 0; }
+
+static dim3 zgrid(long long n, int per = 256) { return dim3((unsigned)((n + per - 1) / per)); }
+
+// ------------------------------------------------------------------ exchange: put_x/yedges
+// pydyn/zoom/exchange.py:17-60: for every child row jc and level lc, the parent cells ib-1, ib,
+// ie, ie+1 of the row jbeg + (jc-1)/yref into the child's cells 0, 1..xref, im+1-xref..im,
+// im+1, scaled by 1/(yref zref) (outer) or 1/(xref yref zref) (inner; the slopes take the
+// outer factor for their inner cells too).  f = 0 is m, f = 1 + a nt + (t-1) tracer array a.
+__global__ void __launch_bounds__(256) k_put_x(ZR P, ZR C, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n2 = (long long)C.jm * C.lm;
+    if (idx >= n2 * nf) return;
+    const int jc = (int)(idx % C.jm) + 1, lc = (int)((idx / C.jm) % C.lm) + 1, f = (int)(idx / n2);
+    const int jp = C.jbeg + (jc - 1) / C.yr, lp = 1 + (lc - 1) / C.zr;
+    const double yz = 1.0 / (double)(C.yr * C.zr), xyz = 1.0 / (double)(C.xr * C.yr * C.zr);
+    const int ib = C.ibeg, ie = C.iend;
+    if (f == 0) {
+        const double west = P.m(ib - 1, jp, lp) * yz, west_in = P.m(ib, jp, lp) * xyz;
+        const double east_in = P.m(ie, jp, lp) * xyz, east = P.m(ie + 1, jp, lp) * yz;
+        C.m(0, jc, lc) = west;
+        C.m(C.im + 1, jc, lc) = east;
+        for (int i = 1; i <= C.xr; i++) {
+            C.m(i, jc, lc) = west_in;
+            C.m(C.im + 1 - i, jc, lc) = east_in;
+        }
+        return;
+    }
+    const int a = (f - 1) / C.nt, t = (f - 1) % C.nt + 1;
+    const HV &src = P.q[a], &dst = C.q[a];
+    const double inner = a == 0 ? xyz : yz;
+    const double west = src(ib - 1, jp, lp, t) * yz, west_in = src(ib, jp, lp, t) * inner;
+    const double east_in = src(ie, jp, lp, t) * inner, east = src(ie + 1, jp, lp, t) * yz;
+    dst(0, jc, lc, t) = west;
+    dst(C.im + 1, jc, lc, t) = east;
+    for (int i = 1; i <= C.xr; i++) {
+        dst(i, jc, lc, t) = west_in;
+        dst(C.im + 1 - i, jc, lc, t) = east_in;
+    }
+}
+
+// pydyn/zoom/exchange.py:63-105: the same across the y-walls, column ic of the child from the
+// parent column mod(ibeg - 1 + (ic-1)/xref, im) + 1.
+__global__ void __launch_bounds__(256) k_put_y(ZR P, ZR C, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n2 = (long long)C.im * C.lm;
+    if (idx >= n2 * nf) return;
+    const int ic = (int)(idx % C.im) + 1, lc = (int)((idx / C.im) % C.lm) + 1, f = (int)(idx / n2);
+    const int ip = (C.ibeg - 1 + (ic - 1) / C.xr) % P.im + 1, lp = 1 + (lc - 1) / C.zr;
+    const double xz = 1.0 / (double)(C.xr * C.zr), xyz = 1.0 / (double)(C.xr * C.yr * C.zr);
+    const int jb = C.jbeg, je = C.jend;
+    if (f == 0) {
+        const double south = P.m(ip, jb - 1, lp) * xz, south_in = P.m(ip, jb, lp) * xyz;
+        const double north_in = P.m(ip, je, lp) * xyz, north = P.m(ip, je + 1, lp) * xz;
+        C.m(ic, 0, lc) = south;
+        C.m(ic, C.jm + 1, lc) = north;
+        for (int j = 1; j <= C.yr; j++) {
+            C.m(ic, j, lc) = south_in;
+            C.m(ic, C.jm + 1 - j, lc) = north_in;
+        }
+        return;
+    }
+    const int a = (f - 1) / C.nt, t = (f - 1) % C.nt + 1;
+    const HV &src = P.q[a], &dst = C.q[a];
+    const double inner = a == 0 ? xyz : xz;
+    const double south = src(ip, jb - 1, lp, t) * xz, south_in = src(ip, jb, lp, t) * inner;
+    const double north_in = src(ip, je, lp, t) * inner, north = src(ip, je + 1, lp, t) * xz;
+    dst(ic, 0, lc, t) = south;
+    dst(ic, C.jm + 1, lc, t) = north;
+    for (int j = 1; j <= C.yr; j++) {
+        dst(ic, j, lc, t) = south_in;
+        dst(ic, C.jm + 1 - j, lc, t) = north_in;
+    }
+}
+
+// ------------------------------------------------------------------ compress / uncompress
+// pydyn/zoom/exchange.py:109-129 (x) and :147-165 (y) for one line (j, l) or (i, l): the
+// interface cells 1..xref summed in order into cell xref (and im-xref+1..im into im-xref+1),
+// the halo cell moved next to them; the sum for m and rm, the move for every array.  f = 0 is
+// m, f = t (1..nt) the tracer arrays of tracer t.  `axis` 0 = x (lines (j, l)), 1 = y.
+__device__ __forceinline__ double &zat(const HV &q, int axis, int k, int a, int l, int t)
+{
+    return axis == 0 ? q(k, a, l, t) : q(a, k, l, t);
+}
+
+__global__ void __launch_bounds__(256) k_compress(ZR C, int axis, int a0, int a1, int l0, int l1, int nf, int *words,
+                                                  int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int na = a1 - a0 + 1, nl = l1 - l0 + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)na * nl * nf) return;
+    const int a = a0 + (int)(idx % na), l = l0 + (int)((idx / na) % nl), f = (int)(idx / ((long long)na * nl));
+    const int r = axis == 0 ? C.xr : C.yr, n = axis == 0 ? C.im : C.jm;
+    if (f == 0) {
+        const HV &m = C.m;
+        double s = zat(m, axis, 1, a, l, 1);
+        for (int k = 2; k <= r; k++) s = s + zat(m, axis, k, a, l, 1);
+        zat(m, axis, r, a, l, 1) = s;
+        zat(m, axis, r - 1, a, l, 1) = zat(m, axis, 0, a, l, 1);
+        s = zat(m, axis, n - r + 1, a, l, 1);
+        for (int k = n - r + 2; k <= n; k++) s = s + zat(m, axis, k, a, l, 1);
+        zat(m, axis, n - r + 1, a, l, 1) = s;
+        zat(m, axis, n - r + 2, a, l, 1) = zat(m, axis, n + 1, a, l, 1);
+        return;
+    }
+    const int t = f;
+    const HV &rm = C.q[0];
+    double s = zat(rm, axis, 1, a, l, t);
+    for (int k = 2; k <= r; k++) s = s + zat(rm, axis, k, a, l, t);
+    zat(rm, axis, r, a, l, t) = s;
+    s = zat(rm, axis, n - r + 1, a, l, t);
+    for (int k = n - r + 2; k <= n; k++) s = s + zat(rm, axis, k, a, l, t);
+    zat(rm, axis, n - r + 1, a, l, t) = s;
+    for (int q = 0; q < 4; q++) {
+        zat(C.q[q], axis, r - 1, a, l, t) = zat(C.q[q], axis, 0, a, l, t);
+        zat(C.q[q], axis, n - r + 2, a, l, t) = zat(C.q[q], axis, n + 1, a, l, t);
+    }
+}
+
+// pydyn/zoom/exchange.py:132-144 (x), :168-177 (y): the edge cell's value over r shared evenly
+// over the interface cells 1..r (and n-r+1..n) again.
+__global__ void __launch_bounds__(256) k_uncompress(ZR C, int axis, int a0, int a1, int l0, int l1, int nf, int *words,
+                                                    int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int na = a1 - a0 + 1, nl = l1 - l0 + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)na * nl * nf) return;
+    const int a = a0 + (int)(idx % na), l = l0 + (int)((idx / na) % nl), f = (int)(idx / ((long long)na * nl));
+    const int r = axis == 0 ? C.xr : C.yr, n = axis == 0 ? C.im : C.jm;
+    const int t = f == 0 ? 1 : f;
+    const int q0 = f == 0 ? -1 : 0, q1 = f == 0 ? -1 : 3;
+    for (int q = q0; q <= q1; q++) {
+        const HV &v = q < 0 ? C.m : C.q[q];
+        const double lo = zat(v, axis, r, a, l, t) / (double)r;
+        for (int k = 1; k <= r; k++) zat(v, axis, k, a, l, t) = lo;
+        const double hi = zat(v, axis, n - r + 1, a, l, t) / (double)r;
+        for (int k = n - r + 1; k <= n; k++) zat(v, axis, k, a, l, t) = hi;
+    }
+}
+
+// ------------------------------------------------------------------ mix_edges
+// pydyn/zoom/exchange.py:181-231: every parent-cell block (xref x yref child cells) of the
+// child's ring (edge == -1) replaced by the mean of its ring cells, accumulated from zero j-outer,
+// i-inner; blocks at the x walls only when x edges are mixed, at a y wall unless the region
+// touches that pole.  One thread per (block, level, array).
+__global__ void __launch_bounds__(256) k_mix_edges(ZR C, int xedges, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int imp = C.im / C.xr, jmp = C.jm / C.yr;
+    const long long nb = (long long)imp * jmp;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * C.lm * nf) return;
+    const int ib = (int)(idx % imp), jb = (int)((idx / imp) % jmp);
+    const int l = (int)((idx / nb) % C.lm) + 1, f = (int)(idx / (nb * C.lm));
+    const bool spread = (xedges && (ib == 0 || ib == imp - 1)) || (C.touch_sp != 1 && jb == 0) ||
+                        (C.touch_np != 1 && jb == jmp - 1);
+    if (!spread) return;
+    const HV &v = f == 0 ? C.m : C.q[(f - 1) / C.nt];
+    const int t = f == 0 ? 1 : (f - 1) % C.nt + 1;
+    const double xyr = (double)(C.xr * C.yr);
+    double acc = 0.0;
+    for (int b = 0; b < C.yr; b++)
+        for (int a = 0; a < C.xr; a++) {
+            const int i = ib * C.xr + a, j = jb * C.yr + b;  // 0-based within the region
+            const double term = C.edge[(long long)j * C.im + i] == -1 ? v(i + 1, j + 1, l, t) / xyr : 0.0;
+            acc = acc + term;
+        }
+    for (int b = 0; b < C.yr; b++)
+        for (int a = 0; a < C.xr; a++) v(ib * C.xr + a + 1, jb * C.yr + b + 1, l, t) = acc;
+}
+
+// ------------------------------------------------------------------ update_parent
+// pydyn/zoom/exchange.py:235-258: the parent's m and rm over the footprint from the child's
+// block sums (from zero, i outer, j inner); f = 0 m, f = t rm of tracer t.
+__global__ void __launch_bounds__(256) k_update_parent(ZR P, ZR C, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int imp = C.im / C.xr, jmp = C.jm / C.yr;
+    const long long nb = (long long)imp * jmp;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * C.lm * nf) return;
+    const int ip = (int)(idx % imp), jp = (int)((idx / imp) % jmp);
+    const int l = (int)((idx / nb) % C.lm) + 1, f = (int)(idx / (nb * C.lm));
+    const HV &src = f == 0 ? C.m : C.q[0];
+    const HV &dst = f == 0 ? P.m : P.q[0];
+    const int t = f == 0 ? 1 : f;
+    double acc = 0.0;
+    for (int a = 0; a < C.xr; a++)
+        for (int b = 0; b < C.yr; b++) acc = acc + src(ip * C.xr + a + 1, jp * C.yr + b + 1, l, t);
+    dst(C.ibeg + ip, C.jbeg + jp, l, t) = acc;
+}
+
+// ------------------------------------------------------------------ coarsen_block
+// pydyn/zoom/massflow.py:79-147: the parent's am, bm, cm, m over the footprint from the child's,
+// fluxes times tref_ (relative).  am: child faces xref ip summed over each block's rows (j
+// ascending); bm: child faces 1 + (jp-1) yref summed over each block's columns (i ascending);
+// cm: over the block i outer / j inner; m: j outer / i inner.  All from zero.
+__global__ void __launch_bounds__(256) k_coarsen(ZR P, ZR C, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int imp = C.im / C.xr, jmp = C.jm / C.yr, lm = C.lm;
+    const long long na = (long long)(imp + 1) * jmp * lm, nbm = (long long)imp * (jmp + 1) * lm;
+    const long long nc = (long long)imp * jmp * lm;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double tr = (double)C.tr;
+    const int ib = C.ibeg, jb = C.jbeg;
+    auto pcol = [&](int k) { return (ib - 2 + k) % P.im + 1; };  // parent i of child block k = 0..imp
+    if (idx < na) {
+        const int ipf = (int)(idx % (imp + 1)), jp = (int)((idx / (imp + 1)) % jmp), l = (int)(idx / ((long long)(imp + 1) * jmp)) + 1;
+        double acc = 0.0;
+        for (int b = 0; b < C.yr; b++) acc = acc + C.am(C.xr * ipf, jp * C.yr + b + 1, l) * tr;
+        P.am(pcol(ipf), jb + jp, l) = acc;
+        return;
+    }
+    idx -= na;
+    if (idx < nbm) {
+        const int ip = (int)(idx % imp), jpf = (int)((idx / imp) % (jmp + 1)), l = (int)(idx / ((long long)imp * (jmp + 1))) + 1;
+        double acc = 0.0;
+        for (int a = 0; a < C.xr; a++) acc = acc + C.bm(ip * C.xr + a + 1, 1 + C.yr * jpf, l) * tr;
+        P.bm(pcol(ip + 1), jb + jpf, l) = acc;
+        return;
+    }
+    idx -= nbm;
+    if (idx < 2 * nc) {
+        const bool mass = idx >= nc;
+        if (mass) idx -= nc;
+        const int ip = (int)(idx % imp), jp = (int)((idx / imp) % jmp), l = (int)(idx / ((long long)imp * jmp)) + 1;
+        double acc = 0.0;
+        if (!mass) {
+            for (int a = 0; a < C.xr; a++)
+                for (int b = 0; b < C.yr; b++) acc = acc + C.cm(ip * C.xr + a + 1, jp * C.yr + b + 1, l) * tr;
+            P.cm(pcol(ip + 1), jb + jp, l) = acc;
+        } else {
+            for (int b = 0; b < C.yr; b++)
+                for (int a = 0; a < C.xr; a++) acc = acc + C.m(ip * C.xr + a + 1, jp * C.yr + b + 1, l);
+            P.m(pcol(ip + 1), jb + jp, l) = acc;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ coarsen_ambm
+// pydyn/zoom/massflow.py:150-177: the child's outer-interface fluxes from the parent's face
+// flux divided over the child faces it covers.  First (jm lm) threads: the x walls (unless the
+// child spans the globe); then (im lm): the y walls of a child touching no pole.
+__global__ void __launch_bounds__(256) k_coarsen_ambm(ZR P, ZR C, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nx = (long long)C.jm * C.lm, ny = (long long)C.im * C.lm;
+    if (idx < nx) {
+        if (C.whole) return;
+        const int jc = (int)(idx % C.jm) + 1, lc = (int)(idx / C.jm) + 1;
+        const int j = C.jbeg + (jc - 1) / C.yr, lev = 1 + (lc - 1) / C.zr;
+        const double div = (double)(C.yr * C.zr);
+        C.am(C.xr - 1, jc, lc) = P.am(C.ibeg - 1, j, lev) / div;
+        C.am(C.im - C.xr + 1, jc, lc) = P.am(C.iend, j, lev) / div;
+        return;
+    }
+    idx -= nx;
+    if (idx < ny) {
+        const int ic = (int)(idx % C.im) + 1, lc = (int)(idx / C.im) + 1;
+        const int i = C.ibeg + (ic - 1) / C.xr, lev = 1 + (lc - 1) / C.zr;
+        const double div = (double)(C.xr * C.zr);
+        if (C.touch_sp != 1) C.bm(ic, C.yr, lc) = P.bm(i, C.jbeg, lev) / div;
+        if (C.touch_np != 1) C.bm(ic, C.jm - C.yr + 2, lc) = P.bm(i, C.jend + 1, lev) / div;
+    }
+}
+
+// ------------------------------------------------------------------ m2phlb / m2phlb1
+// pydyn/zoom/massflow.py:24-58: sp = seqsum(m over l) g / dxyp(j) per column, a child's ring
+// cells rescaled p dxyp(j) xref yref / dxyp_parent(jpar); phlb = at + bt sp.  The sp halo stays
+// zero (it is never written).  pdxyp null: region 1 (m2phlb1).
+__global__ void __launch_bounds__(256) k_m2phlb_zoom(ZR R, const double *pdxyp, const double *at, const double *bt,
+                                                     int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= R.im * R.jm) return;
+    const int i = col % R.im + 1, j = col / R.im + 1;
+    double p = R.m(i, j, 1);
+    for (int l = 2; l <= R.lm; l++) p = p + R.m(i, j, l);
+    p = p * 9.80665 / R.dxyp[j - 1];
+    if (pdxyp) {
+        const int xr = R.xr, yr = R.yr;
+        const bool inner_i = xr < i && i < R.im - xr + 1, inner_j = yr < j && j < R.jm - yr + 1;
+        const bool keep = (inner_i && inner_j) || (R.xcyc == 1 && inner_j) || (R.touch_np == 1 && j > R.jm - yr) ||
+                          (R.touch_sp == 1 && j < yr + 1);
+        if (!keep) {
+            const int jpar = R.jbeg + (j - 1) / yr;
+            p = p * R.dxyp[j - 1] * (double)xr * (double)yr / pdxyp[jpar - 1];
+        }
+    }
+    R.sp(i, j, 1) = p;
+    for (int l = 1; l <= R.lm + 1; l++) R.phlb(i, j, l) = at[l - 1] + bt[l - 1] * p;
+}
+
+// ------------------------------------------------------------------ region 1's x halos
+// pydyn/zoom/advect.py:283-288 (and check_cfl.py:113-117 for m alone): after dynamu, every row
+// js..je of m and the tracer arrays gets its periodic halo q(0) = q(im), q(im+1) = q(1).
+__global__ void __launch_bounds__(256) k_x_halo(ZR R, int js, int je, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int nj = je - js + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nj * R.lm * nf) return;
+    const int j = js + (int)(idx % nj), l = (int)((idx / nj) % R.lm) + 1, f = (int)(idx / ((long long)nj * R.lm));
+    const HV &v = f == 0 ? R.m : R.q[(f - 1) / R.nt];
+    const int t = f == 0 ? 1 : (f - 1) % R.nt + 1;
+    v(0, j, l, t) = v(R.im, j, l, t);
+    v(R.im + 1, j, l, t) = v(1, j, l, t);
+}
+
+// ------------------------------------------------------------------ outer interface faces
+// pydyn/zoom/advect.py:121-157: a face slab (am at i = k, all j and l of the halo'd array; or bm
+// at j = k, all i and l) saved and zeroed (restore = 0), or put back (restore = 1).
+__global__ void __launch_bounds__(256) k_slab(HV f, int axis, int k, int n0, int n1, double *save, int restore,
+                                              int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n0 * n1) return;
+    const int a = idx % n0, l = idx / n0;  // a over the other horizontal axis (from its lower bound)
+    double &v = axis == 0 ? f(k, f.l1 + a, f.l2 + l) : f(f.l0 + a, k, f.l2 + l);
+    if (restore) {
+        v = save[idx];
+    } else {
+        save[idx] = v;
+        v = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ whole arrays
+__global__ void __launch_bounds__(256) k_copy_flat(const double *src, double *dst, long long n)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) dst[idx] = src[idx];
+}
+
+__global__ void __launch_bounds__(256) k_scale_flat(double *a, long long n, double factor)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) a[idx] = a[idx] * factor;
+}
+
+// the cyclic x halo faces of region 1's fluxes: am(im+1) = am(1) for every j, l
+__global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int n0 = R.jm + 2, n1 = R.lm + 2;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n0 * n1) return;
+    const int j = idx % n0, l = idx / n0;
+    R.am(R.im + 1, j, l) = R.am(1, j, l);
+}
+
+// ------------------------------------------------------------------ a child's x rows
+// pydyn/zoom/advect.py:301-366 (and check_cfl.py:121-145 mass-only): one 64-lane workgroup per
+// (j, l) row of cells xref-1..im-xref+2 (n cells; the two ends are the parent's, held) and its
+// n-1 faces am(xref-1..im-xref+1).  The row's sub-cycle search scales its faces by (n-1)/n in
+// place per retry (kernels.py:169-199 with the ends held), the limiter runs on every cell of the
+// row first, the sub-steps update the interior cells, and the faces are multiplied back.
+template <bool MASS_ONLY>
+__global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
+                                                int fail_bit, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    extern __shared__ double lds[];
+    const int nj = je - js + 1;
+    const int j = js + (int)(blockIdx.x % nj), l = ls + (int)(blockIdx.x / nj);
+    const int lane = threadIdx.x;
+    const int i0 = C.xr - 1;  // Fortran i of the row's cell 0
+    const int n = C.im - 2 * C.xr + 4, nf = n - 1;
+    double *M = lds, *M0 = M + n, *A = M0 + n, *R = A + nf, *F = R + 4 * n;
+
+    for (int c = lane; c < n; c += 64) M0[c] = M[c] = C.m(i0 + c, j, l);
+    for (int k = lane; k < nf; k += 64) A[k] = C.am(i0 + k, j, l);
+    __syncthreads();
+
+    int nl = 1;
+    for (;;) {
+        bool ok = true;
+        for (int k = lane; k < nf; k += 64) {
+            const double ak = A[k];
+            const int u = ak >= 0.0 ? k : k + 1;
+            const bool held = u == 0 || u == n - 1;
+            const double alo = held ? 0.0 : A[u - 1], ahi = held ? 0.0 : A[u];
+            double mx = M[u];
+            for (int s = 0; s < nl; s++) {
+                if (!(fabs(ak / mx) < 1.0)) {
+                    ok = false;
+                    break;
+                }
+                if (!held) mx = (mx + alo) - ahi;
+            }
+        }
+        if (__all(ok)) break;
+        nl += 1;
+        __syncthreads();
+        for (int k = lane; k < nf; k += 64) A[k] = shrink(A[k], nl);
+        __syncthreads();
+        if (nl > nmax) {
+            if (lane == 0) atomicOr(words + W_FLAGS, fail_bit);
+            return;
+        }
+    }
+
+    if (MASS_ONLY) {
+        for (int s = 0; s < nl; s++) {
+            for (int c = lane + 1; c < n - 1; c += 64) M[c] = (M[c] + A[c - 1]) - A[c];
+            __syncthreads();
+        }
+        for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
+    } else {
+        for (int t = 1; t <= C.nt; t++) {
+            if (t > 1) {
+                for (int c = lane; c < n; c += 64) M[c] = M0[c];
+            }
+            for (int c = lane; c < n; c += 64) {
+                const double rm = C.q[0](i0 + c, j, l, t);
+                R[c] = rm;
+                R[n + c] = limits ? limit(C.q[1](i0 + c, j, l, t), rm) : C.q[1](i0 + c, j, l, t);
+                R[2 * n + c] = C.q[2](i0 + c, j, l, t);
+                R[3 * n + c] = C.q[3](i0 + c, j, l, t);
+            }
+            __syncthreads();
+            for (int s = 0; s < nl; s++) {
+                for (int k = lane; k < nf; k += 64) {
+                    const double ak = A[k];
+                    const int up = ak >= 0.0 ? k : k + 1;
+                    const double alpha = ak * (1.0 / M[up]);
+                    const FaceFlux ff = face_flux(ak, alpha, R[up], R[n + up], R[2 * n + up], R[3 * n + up]);
+                    F[k] = ff.f;
+                    F[nf + k] = ff.pf;
+                    F[2 * nf + k] = ff.ft;
+                    F[3 * nf + k] = ff.fw;
+                }
+                __syncthreads();
+                for (int c = lane + 1; c < n - 1; c += 64) {
+                    const double m_old = M[c], a_lo = A[c - 1], a_hi = A[c];
+                    const double m_new = (m_old + a_lo) - a_hi;
+                    const double *f = F, *pf = F + nf, *fy = F + 2 * nf, *fz = F + 3 * nf;
+                    const double rm_new = R[c] - (f[c] - f[c - 1]);
+                    double rx = slope_update(R[n + c], pf[c - 1], pf[c], a_lo, a_hi, rm_new, f[c - 1], f[c], m_old, m_new);
+                    if (limits) rx = limit(rx, rm_new);
+                    R[c] = rm_new;
+                    R[n + c] = rx;
+                    R[2 * n + c] = R[2 * n + c] - (fy[c] - fy[c - 1]);
+                    R[3 * n + c] = R[3 * n + c] - (fz[c] - fz[c - 1]);
+                    M[c] = m_new;
+                }
+                __syncthreads();
+            }
+            for (int c = lane; c < n; c += 64) {
+                C.q[0](i0 + c, j, l, t) = R[c];
+                C.q[1](i0 + c, j, l, t) = R[n + c];
+                C.q[2](i0 + c, j, l, t) = R[2 * n + c];
+                C.q[3](i0 + c, j, l, t) = R[3 * n + c];
+            }
+            __syncthreads();
+        }
+        for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
+    }
+    for (int k = lane; k < nf; k += 64) C.am(i0 + k, j, l) = A[k] * (double)nl;  // "restore old am"
+}
+
+// ------------------------------------------------------------------ a child's y columns
+// pydyn/zoom/advect.py:369-432 (check_cfl.py:171-201 mass-only): per layer l, the columns
+// is..ie with cells yref-1..jm-yref+2 (n cells, the ends held) and faces bm(yref..jm-yref+1).
+// The layer's search: nloop = 1, 2, ... until every face passes its trial; on a retry the whole
+// halo'd layer of bm is scaled (the scope's faces only in the mass-only form).  One workgroup
+// per layer; nloop[l] for the sub-step kernel.
+template <bool MASS_ONLY>
+__global__ void __launch_bounds__(1024) k_y_child_search(ZR C, int is, int ie, int ls, int nmax, int *nloop,
+                                                         int fail_bit, int *words, int skip_mask)
+{
+    const int l = ls + blockIdx.x;
+    if (zskip(words, skip_mask)) return;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int j0 = C.yr - 1, n = C.jm - 2 * C.yr + 4, nf = n - 1, ni = ie - is + 1;
+    int nl = 1;
+    for (;;) {
+        bool ok = true;
+        for (int idx = tid; idx < ni * nf; idx += nth) {
+            const int i = is + idx % ni, k = idx / ni;
+            const double bk = C.bm(i, j0 + 1 + k, l);
+            const int u = bk >= 0.0 ? k : k + 1;
+            const bool held = u == 0 || u == n - 1;
+            const double blo = held ? 0.0 : C.bm(i, j0 + u, l), bhi = held ? 0.0 : C.bm(i, j0 + 1 + u, l);
+            double mx = C.m(i, j0 + u, l);
+            for (int s = 0; s < nl; s++) {
+                if (!(fabs(bk / mx) < 1.0)) {
+                    ok = false;
+                    break;
+                }
+                if (!held) mx = (mx + blo) - bhi;
+            }
+        }
+        if (__syncthreads_and(ok)) break;
+        nl += 1;
+        if (MASS_ONLY) {
+            for (int idx = tid; idx < ni * nf; idx += nth) {
+                double &b = C.bm(is + idx % ni, j0 + 1 + idx / ni, l);
+                b = shrink(b, nl);
+            }
+        } else {
+            const int n0 = C.im + 2, n1 = C.jm + 2;
+            for (int idx = tid; idx < n0 * n1; idx += nth) {
+                double &b = C.bm(C.bm.l0 + idx % n0, C.bm.l1 + idx / n0, l);
+                b = shrink(b, nl);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (nl > nmax) {
+            if (tid == 0) {
+                atomicOr(words + W_FLAGS, fail_bit);
+                nloop[blockIdx.x] = nl;
+            }
+            return;
+        }
+    }
+    if (tid == 0) nloop[blockIdx.x] = nl;
+}
+
+// The sub-steps of the layer's columns: one thread per cell of TI whole columns; each sub-step
+// every thread reads its cell's neighbourhood, computes its two faces' fluxes and its update,
+// then -- after the workgroup's barrier -- writes it.  The end cells are held; the limiter of
+// the first sub-step is taken on the values as read (the reference limits rym on every cell of
+// the scope, ends included, before the loop; the ends are written back limited).
+template <bool MASS_ONLY>
+__global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, int ls, int TI, const int *nloop,
+                                                        int limits, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int l = ls + blockIdx.y;
+    const int j0 = C.yr - 1, n = C.jm - 2 * C.yr + 4;
+    const int ti = threadIdx.x % TI, c = threadIdx.x / TI;
+    const int i = is + blockIdx.x * TI + ti;
+    const bool mine = c < n && i <= ie;
+    const int ii = mine ? i : is, cc = mine ? c : 0;
+    const int jc = j0 + cc;  // Fortran j of the cell
+    const bool inner = cc > 0 && cc < n - 1;
+    const int nl = nloop[blockIdx.y];
+    // faces below (bm(jc)) and above (bm(jc+1)) an interior cell
+    const double b_lo = inner ? C.bm(ii, jc, l) : 0.0, b_hi = inner ? C.bm(ii, jc + 1, l) : 0.0;
+    for (int s = 0; s < nl; s++) {
+        const double m_old = C.m(ii, jc, l);
+        const double m_new = (m_old + b_lo) - b_hi;
+        if (!MASS_ONLY) {
+            const bool lim0 = limits && s == 0;
+            for (int t = 1; t <= C.nt; t++) {
+                const double rm = C.q[0](ii, jc, l, t);
+                double rym = C.q[2](ii, jc, l, t);
+                if (lim0) rym = limit(rym, rm);
+                double out0 = rm, out1 = C.q[1](ii, jc, l, t), out2 = rym, out3 = C.q[3](ii, jc, l, t);
+                if (inner) {
+                    FaceFlux lo, hi;
+                    {
+                        const int u = b_lo >= 0.0 ? jc - 1 : jc;
+                        const double beta = b_lo / C.m(ii, u, l);
+                        const double rmu = C.q[0](ii, u, l, t);
+                        const double ryu = lim0 ? limit(C.q[2](ii, u, l, t), rmu) : C.q[2](ii, u, l, t);
+                        lo = face_flux(b_lo, beta, rmu, ryu, C.q[1](ii, u, l, t), C.q[3](ii, u, l, t));
+                    }
+                    {
+                        const int u = b_hi >= 0.0 ? jc : jc + 1;
+                        const double beta = b_hi / C.m(ii, u, l);
+                        const double rmu = C.q[0](ii, u, l, t);
+                        const double ryu = lim0 ? limit(C.q[2](ii, u, l, t), rmu) : C.q[2](ii, u, l, t);
+                        hi = face_flux(b_hi, beta, rmu, ryu, C.q[1](ii, u, l, t), C.q[3](ii, u, l, t));
+                    }
+                    const double inv_mnew = 1.0 / m_new;
+                    const double flow_conv = b_lo - b_hi, flow_mean = 3.0 * (b_lo + b_hi), mass_x3 = 3.0 * m_old;
+                    const double rm_new = rm + (lo.f - hi.f);
+                    const double f_sum = lo.f + hi.f;
+                    double ry = rym + ((((lo.pf - hi.pf) - flow_conv * rym) + flow_mean * rm_new) - mass_x3 * f_sum) * inv_mnew;
+                    if (limits) ry = limit(ry, rm_new);
+                    out0 = rm_new;
+                    out2 = ry;
+                    out1 = out1 + (lo.ft - hi.ft);
+                    out3 = out3 + (lo.fw - hi.fw);
+                }
+                __syncthreads();
+                if (mine && (inner || lim0)) {
+                    C.q[0](ii, jc, l, t) = out0;
+                    C.q[1](ii, jc, l, t) = out1;
+                    C.q[2](ii, jc, l, t) = out2;
+                    C.q[3](ii, jc, l, t) = out3;
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        if (mine && inner) C.m(ii, jc, l) = m_new;
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// the whole halo'd layer of bm multiplied back by its nloop ("restore old bm's")
+__global__ void __launch_bounds__(256) k_y_child_restore(ZR C, int ls, const int *nloop, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int l = ls + blockIdx.y, nl = nloop[blockIdx.y];
+    if (nl == 1) return;  // times 1.0 is the identity
+    const int n0 = C.im + 2, n1 = C.jm + 2;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n0 * n1) return;
+    double &b = C.bm(C.bm.l0 + idx % n0, C.bm.l1 + idx / n0, l);
+    b = b * (double)nl;
+}
+
+// ------------------------------------------------------------------ launchers
+static int nfields(const ZR &R, int tracers) { return 1 + (tracers ? 4 * R.nt : 0); }
+
+int zoom_put_x(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
+{
+    if (C.whole) return 0;
+    const int nf = nfields(C, tracers);
+    TM5_LAUNCH(ctx, k_put_x, zgrid((long long)C.jm * C.lm * nf), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
+               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    return status(hipGetLastError());
+}
+
+int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
+{
+    const int nf = nfields(C, tracers);
+    TM5_LAUNCH(ctx, k_put_y, zgrid((long long)C.im * C.lm * nf), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
+               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    return status(hipGetLastError());
+}
+
+int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, int l1, int tracers, bool un)
+{
+    const int nf = tracers ? 1 + C.nt : 1;
+    const long long n = (long long)(a1 - a0 + 1) * (l1 - l0 + 1) * nf;
+    if (n <= 0) return 0;
+    const int mask = TM5_ERRORS | TM5_REHEARSAL * !tracers;
+    if (un)
+        TM5_LAUNCH(ctx, k_uncompress, zgrid(n), dim3(256), 0, ctx->stream, C, axis, a0, a1, l0, l1, nf, ctx->words, mask);
+    else
+        TM5_LAUNCH(ctx, k_compress, zgrid(n), dim3(256), 0, ctx->stream, C, axis, a0, a1, l0, l1, nf, ctx->words, mask);
+    return status(hipGetLastError());
+}
+
+int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers)
+{
+    const int nf = nfields(C, tracers);
+    const long long n = (long long)(C.im / C.xr) * (C.jm / C.yr) * C.lm * nf;
+    TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, xedges, nf, ctx->words,
+               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    return status(hipGetLastError());
+}
+
+int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
+{
+    const int nf = tracers ? 1 + C.nt : 1;
+    const long long n = (long long)(C.im / C.xr) * (C.jm / C.yr) * C.lm * nf;
+    TM5_LAUNCH(ctx, k_update_parent, zgrid(n), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
+               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    return status(hipGetLastError());
+}
+
+int zoom_coarsen(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
+{
+    const long long imp = C.im / C.xr, jmp = C.jm / C.yr;
+    const long long n = ((imp + 1) * jmp + imp * (jmp + 1) + 2 * imp * jmp) * C.lm;
+    TM5_LAUNCH(ctx, k_coarsen, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_coarsen_ambm(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
+{
+    const long long n = (long long)(C.jm + C.im) * C.lm;
+    TM5_LAUNCH(ctx, k_coarsen_ambm, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at, const double *bt, int skip_mask)
+{
+    TM5_LAUNCH(ctx, k_m2phlb_zoom, zgrid((long long)R.im * R.jm), dim3(256), 0, ctx->stream, R, pdxyp, at, bt,
+               ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_x_halo(tm5_ctx *ctx, const ZR &R, int js, int je, int tracers, int skip_mask)
+{
+    if (je < js) return 0;
+    const int nf = nfields(R, tracers);
+    TM5_LAUNCH(ctx, k_x_halo, zgrid((long long)(je - js + 1) * R.lm * nf), dim3(256), 0, ctx->stream, R, js, je, nf,
+               ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_slab(tm5_ctx *ctx, const HV &f, int axis, int k, int n0, int n1, double *save, int restore, int skip_mask)
+{
+    TM5_LAUNCH(ctx, k_slab, zgrid((long long)n0 * n1), dim3(256), 0, ctx->stream, f, axis, k, n0, n1, save, restore,
+               ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_copy(tm5_ctx *ctx, const double *src, double *dst, long long n)
+{
+    TM5_LAUNCH(ctx, k_copy_flat, zgrid(n), dim3(256), 0, ctx->stream, src, dst, n);
+    return status(hipGetLastError());
+}
+
+int zoom_scale(tm5_ctx *ctx, double *a, long long n, double factor)
+{
+    TM5_LAUNCH(ctx, k_scale_flat, zgrid(n), dim3(256), 0, ctx->stream, a, n, factor);
+    return status(hipGetLastError());
+}
+
+int zoom_am_halo(tm5_ctx *ctx, const ZR &R, int skip_mask)
+{
+    TM5_LAUNCH(ctx, k_am_halo, zgrid((long long)(R.jm + 2) * (R.lm + 2)), dim3(256), 0, ctx->stream, R, ctx->words,
+               skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
+                 int fail_bit, int skip_mask)
+{
+    const int n = C.im - 2 * C.xr + 4;
+    if (n < 3 || n > 512 || je < js || le < ls) return TM5_EINVAL;  // LDS: 11 n doubles
+    const size_t lds = (size_t)(2 * n + (n - 1) + 4 * n + 4 * (n - 1)) * sizeof(double);
+    const dim3 grid((unsigned)((je - js + 1) * (le - ls + 1)));
+    if (mass_only)
+        TM5_LAUNCH(ctx, k_x_child<true>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, 0, nmax, fail_bit,
+                   ctx->words, skip_mask);
+    else
+        TM5_LAUNCH(ctx, k_x_child<false>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, limits, nmax, fail_bit,
+                   ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int limits, int nmax, int mass_only,
+                 int fail_bit, int skip_mask)
+{
+    const int n = C.jm - 2 * C.yr + 4;
+    const int nlay = le - ls + 1;
+    if (n < 3 || n > 1024 || ie < is || nlay < 1 || nlay > ctx->y_cap) return TM5_EINVAL;
+    int *nloop = ctx->y_nloop;
+    if (mass_only)
+        TM5_LAUNCH(ctx, k_y_child_search<true>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax, nloop,
+                   fail_bit, ctx->words, skip_mask);
+    else
+        TM5_LAUNCH(ctx, k_y_child_search<false>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax, nloop,
+                   fail_bit, ctx->words, skip_mask);
+    const int TI = (1024 / n) < 16 ? (1024 / n) : 16;
+    const int ni = ie - is + 1;
+    const dim3 grid((unsigned)((ni + TI - 1) / TI), (unsigned)nlay), block((unsigned)((TI * n + 63) / 64 * 64));
+    if (mass_only)
+        TM5_LAUNCH(ctx, k_y_child_steps<true>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, 0, ctx->words,
+                   skip_mask);
+    else
+        TM5_LAUNCH(ctx, k_y_child_steps<false>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, limits,
+                   ctx->words, skip_mask);
+    const int n0 = C.im + 2, n1 = C.jm + 2;
+    TM5_LAUNCH(ctx, k_y_child_restore, dim3((unsigned)((n0 * n1 + 255) / 256), (unsigned)nlay), dim3(256), 0,
+               ctx->stream, C, ls, nloop, ctx->words, skip_mask);
+    return status(hipGetLastError());
+}
+
+}  // namespace tm5
+
+// ------------------------------------------------------------------ the C ABI (include/tm5hip.h)
+using namespace tm5;
+
+#define ZTRY(x)                   \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+static bool zbad(const tm5_zregion *r) { return r == nullptr || r->m.data == nullptr || r->im < 1 || r->jm < 1 || r->lm < 1; }
+static bool zglobal(const tm5_zregion *r) { return r->xcyc == 1 && r->touch_sp == 1 && r->touch_np == 1; }
+static int zskipmask(int rehearsal) { return rehearsal ? (TM5_ERRORS | TM5_REHEARSAL) : TM5_ERRORS; }
+
+extern "C" int tm5_zoom_put_xedges(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch, int tracers)
+{
+    if (!c || zbad(p) || zbad(ch)) return TM5_EINVAL;
+    return zoom_put_x(c, zr(*p), zr(*ch), tracers);
+}
+
+extern "C" int tm5_zoom_put_yedges(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch, int tracers)
+{
+    if (!c || zbad(p) || zbad(ch)) return TM5_EINVAL;
+    return zoom_put_y(c, zr(*p), zr(*ch), tracers);
+}
+
+static bool xedges_apply(const tm5_zregion *r) { return !(r->xref == 1 || r->whole); }
+static bool yedges_apply(const tm5_zregion *r) { return !(r->yref == 1 || zglobal(r)); }
+
+extern "C" int tm5_zoom_compress_xedges(tm5_ctx *c, const tm5_zregion *r, int js, int je, int ls, int le, int tracers)
+{
+    if (!c || zbad(r)) return TM5_EINVAL;
+    return xedges_apply(r) ? zoom_compress(c, zr(*r), 0, js, je, ls, le, tracers, false) : 0;
+}
+
+extern "C" int tm5_zoom_uncompress_xedges(tm5_ctx *c, const tm5_zregion *r, int js, int je, int ls, int le, int tracers)
+{
+    if (!c || zbad(r)) return TM5_EINVAL;
+    return xedges_apply(r) ? zoom_compress(c, zr(*r), 0, js, je, ls, le, tracers, true) : 0;
+}
+
+extern "C" int tm5_zoom_compress_yedges(tm5_ctx *c, const tm5_zregion *r, int is, int ie, int ls, int le, int tracers)
+{
+    if (!c || zbad(r)) return TM5_EINVAL;
+    return yedges_apply(r) ? zoom_compress(c, zr(*r), 1, is, ie, ls, le, tracers, false) : 0;
+}
+
+extern "C" int tm5_zoom_uncompress_yedges(tm5_ctx *c, const tm5_zregion *r, int is, int ie, int ls, int le, int tracers)
+{
+    if (!c || zbad(r)) return TM5_EINVAL;
+    return yedges_apply(r) ? zoom_compress(c, zr(*r), 1, is, ie, ls, le, tracers, true) : 0;
+}
+
+extern "C" int tm5_zoom_mix_edges(tm5_ctx *c, const tm5_zregion *r, int tracers)
+{
+    if (!c || zbad(r) || !r->edge) return TM5_EINVAL;
+    if (r->zref != 1) return TM5_EINVAL;  // "mix_edges: zooming in z direction not supported!"
+    const bool xedges = xedges_apply(r);
+    if (!xedges && zglobal(r)) return 0;
+    return zoom_mix_edges(c, zr(*r), xedges ? 1 : 0, tracers);
+}
+
+extern "C" int tm5_zoom_update_parent(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch, int tracers)
+{
+    if (!c || zbad(p) || zbad(ch) || ch->zref != 1) return TM5_EINVAL;
+    return zoom_update_parent(c, zr(*p), zr(*ch), tracers);
+}
+
+extern "C" int tm5_zoom_coarsen_block(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch)
+{
+    if (!c || zbad(p) || zbad(ch) || ch->zref != 1) return TM5_EINVAL;
+    return zoom_coarsen(c, zr(*p), zr(*ch), TM5_ERRORS);
+}
+
+extern "C" int tm5_zoom_coarsen_ambm(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch)
+{
+    if (!c || zbad(p) || zbad(ch)) return TM5_EINVAL;
+    return zoom_coarsen_ambm(c, zr(*p), zr(*ch), TM5_ERRORS);
+}
+
+extern "C" int tm5_zoom_m2phlb(tm5_ctx *c, const tm5_zregion *r, const tm5_zregion *p, const double *at,
+                               const double *bt)
+{
+    if (!c || zbad(r) || !r->dxyp || !at || !bt || (p && !p->dxyp)) return TM5_EINVAL;
+    return zoom_m2phlb(c, zr(*r), p ? p->dxyp : nullptr, at, bt, TM5_ERRORS);
+}
+
+extern "C" int tm5_zoom_dynam0(tm5_ctx *c, const tm5_zregion *r, const double *bt, int ndyn, int rehearsal)
+{
+    if (!c || zbad(r) || !bt || r->tref_abs < 1) return TM5_EINVAL;
+    const ZR R = zr(*r);
+    const double dt = (double)ndyn / (2.0 * (double)r->tref_abs);
+    const int skip = zskipmask(rehearsal);
+    ZTRY(launch_dynam0(c, R.im, R.jm, R.lm, sub3(R.pu, 0, 1, 1), sub3(R.pv, 1, 1, 1), 1.0, bt, dt, sub3(R.am, 0, 1, 1),
+                       sub3(R.bm, 1, 1, 1), sub3(R.cm, 1, 1, 0), skip, R.xcyc));
+    if (R.xcyc == 1) ZTRY(zoom_am_halo(c, R, skip));
+    return 0;
+}
+
+extern "C" int tm5_zoom_outer_faces(tm5_ctx *c, const tm5_zregion *r, int axis, int restore, double *save,
+                                    int rehearsal)
+{
+    if (!c || zbad(r) || !save || axis < 0 || axis > 1) return TM5_EINVAL;
+    const ZR R = zr(*r);
+    const int skip = zskipmask(rehearsal);
+    if (axis == 0) {
+        const int n0 = R.jm + 2, n1 = R.lm + 2;
+        ZTRY(zoom_slab(c, R.am, 0, R.xr - 1, n0, n1, save, restore, skip));
+        return zoom_slab(c, R.am, 0, R.im - R.xr + 1, n0, n1, save + (long long)n0 * n1, restore, skip);
+    }
+    const int n0 = R.im + 2, n1 = R.lm + 2;
+    if (R.touch_sp != 1) ZTRY(zoom_slab(c, R.bm, 1, R.yr, n0, n1, save, restore, skip));
+    if (R.touch_np != 1) ZTRY(zoom_slab(c, R.bm, 1, R.jm - R.yr + 2, n0, n1, save + (long long)n0 * n1, restore, skip));
+    return 0;
+}
+
+extern "C" int tm5_zoom_advect_x(tm5_ctx *c, const tm5_zregion *r, int js, int je, int limits, int max_nloop,
+                                 int mass_only)
+{
+    if (!c || zbad(r) || max_nloop < 1) return TM5_EINVAL;
+    const ZR R = zr(*r);
+    const int skip = zskipmask(mass_only);
+    if (zglobal(r)) {
+        const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_X;
+        ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
+                      sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip));
+        return zoom_x_halo(c, R, js, je, !mass_only, skip);
+    }
+    const int tr = mass_only ? 0 : 1;
+    ZTRY(tm5_zoom_compress_xedges(c, r, js, je, 1, R.lm, tr));
+    ZTRY(zoom_x_child(c, R, js, je, 1, R.lm, limits, max_nloop - 1, mass_only, mass_only ? TM5_REHEARSAL : TM5_FAIL_XC,
+                      skip));
+    if (mass_only) {  // dynamum's own uncompressm and mixm_edges (pydyn/zoom/check_cfl.py:143-144)
+        ZTRY(tm5_zoom_uncompress_xedges(c, r, js, je, 1, R.lm, 0));
+        ZTRY(tm5_zoom_mix_edges(c, r, 0));
+    }
+    return 0;
+}
+
+extern "C" int tm5_zoom_advect_y(tm5_ctx *c, const tm5_zregion *r, int is, int ie, int limits, int max_nloop,
+                                 int mass_only)
+{
+    if (!c || zbad(r) || max_nloop < 1) return TM5_EINVAL;
+    const ZR R = zr(*r);
+    const int skip = zskipmask(mass_only);
+    if (zglobal(r)) {
+        const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_Y;
+        return launch_y(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
+                        sub3(R.bm, 1, 1, 1), mass_only ? 0 : limits, mass_only ? max_nloop : max_nloop - 1, fail, skip);
+    }
+    const int tr = mass_only ? 0 : 1;
+    ZTRY(tm5_zoom_compress_yedges(c, r, is, ie, 1, R.lm, tr));
+    ZTRY(zoom_y_child(c, R, is, ie, 1, R.lm, limits, mass_only ? max_nloop : max_nloop - 1, mass_only,
+                      mass_only ? TM5_REHEARSAL : TM5_FAIL_YC, skip));
+    if (mass_only) {
+        ZTRY(tm5_zoom_uncompress_yedges(c, r, is, ie, 1, R.lm, 0));
+        ZTRY(tm5_zoom_mix_edges(c, r, 0));
+    }
+    return 0;
+}
+
+extern "C" int tm5_zoom_advect_z(tm5_ctx *c, const tm5_zregion *r, int is, int ie, int js, int je, int limits,
+                                 int iter_limit, int mass_only)
+{
+    if (!c || zbad(r) || iter_limit < 1) return TM5_EINVAL;
+    const ZR R = zr(*r);
+    const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_Z;
+    return launch_z_scoped(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
+                           sub3(R.cm, 1, 1, 0), is - 1, ie - 1, js - 1, je - 1, mass_only ? 0 : limits, iter_limit,
+                           fail, zskipmask(mass_only));
+}
+
+extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops)
+{
+    if (!c || zbad(r) || !p || !ops || !column_supported(p->lmax) || r->lm > 64 || p->lmax > r->lm) return TM5_EINVAL;
+    ColumnArgs a{};
+    if (!strcmp(ops, "v")) {
+    } else if (!strcmp(ops, "csv")) {
+        a.pre = 1;
+    } else if (!strcmp(ops, "vsc")) {
+        a.post = 1;
+    } else if (!strcmp(ops, "vsccsv")) {
+        a.twice = 1;
+    } else if (!strcmp(ops, "vscMcsv")) {
+        a.twice = 1;
+        a.mid = 1;
+    } else {
+        return TM5_EINVAL;
+    }
+    const ZR R = zr(*r);
+    a.im = R.im;
+    a.jm = R.jm;
+    a.lm = R.lm;
+    a.m = sub3(R.m, 1, 1, 1);
+    a.q = subq(R, 1, 1, 1);
+    a.entu = view3(&p->entu);
+    a.detu = view3(&p->detu);
+    a.entd = view3(&p->entd);
+    a.detd = view3(&p->detd);
+    a.dkg = view3(&p->dkg);
+    a.top = iview2(&p->cloud_top);
+    a.lfs = iview2(&p->cloud_lfs);
+    a.dt = p->dt_conv;
+    a.owned = r->owned;
+    a.emis = view3(&p->emis);
+    a.factor = p->factor;
+    a.ratio = p->ratio;
+    a.efrac = p->efrac;
+    a.level = p->level - 1;
+    a.tracer = p->tracer - 1;
+    a.loss = view3(&p->loss);
+    a.loss_sr = p->loss_stride;
+    a.dt_chem = p->dt_chem;
+    a.dxyp = r->dxyp;
+    a.at = p->at;
+    a.bt = p->bt;
+    a.sp = sub3(R.sp, 1, 1, 1);
+    a.phlb = sub3(R.phlb, 1, 1, 1);
+    if ((a.pre || a.post || a.twice) && (!p->emis.data || !p->loss.data)) return TM5_EINVAL;
+    if (a.mid && (!p->at || !p->bt || !r->dxyp)) return TM5_EINVAL;
+    return launch_column(c, a, p->lmax, TM5_ERRORS);
+}
+
+extern "C" int tm5_zoom_copy(tm5_ctx *c, const double *src, double *dst, int64_t n)
+{
+    if (!c || !src || !dst || n < 0) return TM5_EINVAL;
+    return n ? zoom_copy(c, src, dst, n) : 0;
+}
+
+extern "C" int tm5_zoom_scale(tm5_ctx *c, double *a, int64_t n, double factor)
+{
+    if (!c || !a || n < 0) return TM5_EINVAL;
+    return n ? zoom_scale(c, a, n, factor) : 0;
+}

@@ -57,6 +57,10 @@ class Context:
             raise CFLError("dynamv: nloop too high")
         if f & _lib.FAIL_Z:
             raise CFLError("dynamw: too many CFL violations!")
+        if f & _lib.FAIL_XC:
+            raise CFLError("dynuamu: nloop too high")  # the reference's spelling (pydyn/zoom/advect.py:330)
+        if f & _lib.FAIL_YC:
+            raise CFLError("nloop too high in dynamv")
         return f
 
     def nloop(self, lm):
@@ -143,6 +147,16 @@ def view(t):
         raise TypeError("view: float64 device arrays only")
     strides = list(t.stride()) + [0] * (4 - t.dim())
     return _lib.View(t.data_ptr(), (ctypes.c_int64 * 4)(*strides))
+
+
+def hview(t, lower):
+    """tm5_hview of a float64 device array with Fortran lower bounds `lower` (3 or 4 axes; a
+    3-axis array gets a unit t axis at lower bound 1)."""
+    if t.dtype.is_floating_point is False or t.element_size() != 8:
+        raise TypeError("hview: float64 device arrays only")
+    strides = list(t.stride()) + [0] * (4 - t.dim())
+    lo = list(lower) + [1] * (4 - len(lower))
+    return _lib.HView(t.data_ptr(), (ctypes.c_int64 * 4)(*strides), (ctypes.c_int32 * 4)(*lo))
 
 
 def iview(t):

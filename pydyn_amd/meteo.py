@@ -297,3 +297,107 @@ class Meteo:
         """The initial prognostic state: m and the four tracer arrays (slopes zero)."""
         z = np.zeros_like(self.rm)
         return {"m": self.m.copy(), "rm": self.rm.copy(), "rxm": z.copy(), "rym": z.copy(), "rzm": z.copy()}
+
+
+# ------------------------------------------------------------------ nested (zoom) regions
+@dataclass
+class Child:
+    """A nested region: im x jm cells over the parent cells ibeg..iend x jbeg..jend (1-based,
+    inclusive), refined xref x yref in space and tref in time relative to the globe; the
+    longitude/latitude bounds are the ones the region is declared with (the tree snaps them to
+    the parent's cells)."""
+
+    name: str
+    im: int
+    jm: int
+    xbeg: float
+    xend: float
+    ybeg: float
+    yend: float
+    xref: int
+    yref: int
+    tref: int
+    ibeg: int
+    jbeg: int
+
+    @property
+    def iend(self):
+        return self.ibeg + self.im // self.xref - 1
+
+    @property
+    def jend(self):
+        return self.jbeg + self.jm // self.yref - 1
+
+
+# parent grid name, seed, child
+ZOOMS = {
+    "tiny2": ("tiny", 3, Child("tiny_child", 8, 8, -90.0, 30.0, -45.0, 45.0, 2, 2, 2, 4, 3)),
+    "small2": ("small", 5, Child("small_child", 24, 18, -120.0, 60.0, -56.25, 45.0, 2, 2, 2, 5, 4)),
+    # glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N: parent cells 4..43 x 51..85
+    "c2": ("glb3x2", 1, Child("nam1x1", 120, 70, -170.0, -50.0, 10.0, 80.0, 3, 2, 2, 4, 51)),
+}
+
+
+class NestedMeteo:
+    """The inputs of a child region, refined from its parent's Meteo so that the two agree: the
+    surface pressure, cloud levels, Kzz, heights and loss rates of the parent cell; extensive
+    fields (en/detrainment, emission) split evenly over the cell's xref x yref children; air
+    mass from the child's own cell areas; the tracer at the parent cell's mixing ratio.  The
+    child's face fluxes interpolate the parent's linearly between the parent faces and share
+    the cross-section (pu / yref, pv / xref), so every child cell's flux convergence is its
+    parent cell's over xref yref and the vertical fluxes agree too."""
+
+    def __init__(self, parent, child):
+        p, c = parent, child
+        g = p.g
+        self.parent, self.c, self.nt = parent, child, parent.nt
+        xr, yr = c.xref, c.yref
+        im, jm, lm = c.im, c.jm, g.lm
+        ip = (c.ibeg - 1) + np.arange(im) // xr  # parent cell of each child cell (0-based)
+        jp = (c.jbeg - 1) + np.arange(jm) // yr
+        cell = np.ix_(ip, jp)
+        rad = np.pi / 180.0
+        lat_f = (-90.0 + g.dy * (c.jbeg - 1) + (g.dy / yr) * np.arange(jm + 1)) * rad
+        s = detmath.sin(lat_f)
+        self.dxyp = ((g.dx / xr) * rad) * (s[1:] - s[:-1]) * (AE * AE)
+        self.ps = p.ps[cell]
+        phlb = g.at[None, None, :] + g.bt[None, None, :] * self.ps[..., None]
+        self.m = (phlb[..., :-1] - phlb[..., 1:]) * self.dxyp[None, :, None] / GRAV
+
+        kx = np.arange(im + 1)
+        f0, w = (c.ibeg - 1) + kx // xr, (kx % xr) / float(xr)
+        pu = p.pu0[:, jp]  # (im_p+1, jm, lm)
+        nxt = np.minimum(f0 + 1, g.im)
+        self.pu0 = ((1.0 - w)[:, None, None] * pu[f0] + w[:, None, None] * pu[nxt]) / float(yr)
+        ky = np.arange(jm + 1)
+        g0, v = (c.jbeg - 1) + ky // yr, (ky % yr) / float(yr)
+        pv = p.pv0[ip]  # (im, jm_p+1, lm)
+        nxt = np.minimum(g0 + 1, g.jm)
+        self.pv0 = ((1.0 - v)[None, :, None] * pv[:, g0] + v[None, :, None] * pv[:, nxt]) / float(xr)
+
+        area = float(xr * yr)
+        for k in ("entu", "detu", "entd", "detd"):
+            setattr(self, k, getattr(p, k)[cell] / area)
+        self.cloud_top = p.cloud_top[cell].copy()
+        self.cloud_lfs = p.cloud_lfs[cell].copy()
+        self.kvh, self.gph, self.loss = p.kvh[cell], p.gph[cell], p.loss[cell]
+        self.emis = p.emis[cell] / area
+        self.rm = (p.rm / p.m[..., None])[cell] * self.m[..., None]
+
+    def amplitude(self, t_sec):
+        return self.parent.amplitude(t_sec)
+
+    def fluxes(self, t_sec):
+        a = self.amplitude(t_sec)
+        return a * self.pu0, a * self.pv0
+
+    def state(self):
+        z = np.zeros_like(self.rm)
+        return {"m": self.m.copy(), "rm": self.rm.copy(), "rxm": z.copy(), "rym": z.copy(), "rzm": z.copy()}
+
+
+def zoom(name, nt=1):
+    """(parent Meteo, NestedMeteo) of a configuration in ZOOMS."""
+    gname, seed, child = ZOOMS[name]
+    parent = Meteo(grid(gname), seed=seed, nt=nt)
+    return parent, NestedMeteo(parent, child)

@@ -1,0 +1,192 @@
+"""The bodies the zoom driver runs on a device state: pydyn.gpu.zoom's names and arguments
+(pydyn/gpu/zoom.py:38-74, the list of pydyn/zoom/host.py:42-48), each a call into libtm5hip.
+
+Every body is asynchronous on the context's stream.  The rehearsal's mass-only bodies do not
+return a verdict: a failure sets TM5_REHEARSAL in the context's flag word, after which every
+later rehearsal kernel does nothing, so the driver reads the verdict once per Check_CFL
+iteration.  The main bodies' errors (CFLError / ValueError) are read by Context.raise_errors.
+"""
+
+import ctypes
+from contextlib import contextmanager
+
+from .. import _lib
+from .. import device as D
+
+__all__ = [
+    "_coarsen_block", "_dynamu_child", "_dynamu_region1", "_dynamum_child", "_dynamum_region1",
+    "_dynamv_child", "_dynamv_region1", "_dynamvm_child", "_dynamvm_region1", "_dynamw_child",
+    "_dynamw_region1", "_dynamwm", "_uncompress_xedges", "_uncompress_yedges", "coarsen_ambm",
+    "column_physics", "dynam0", "m2phlb", "m2phlb1", "mix_edges", "outer_x_faces_zeroed",
+    "outer_y_faces_zeroed", "put_xedges", "put_yedges", "restore", "save", "scale", "update_parent",
+]  # fmt: skip
+
+
+def _call(name, *args):
+    _lib.check(getattr(_lib.lib(), name)(D.context().ptr, *args), name)
+
+
+# ------------------------------------------------------------------ exchange
+def put_xedges(zoom, st, region, tracers=True):
+    for c in zoom.children[region]:
+        _call("tm5_zoom_put_xedges", st.ptr(region), st.ptr(c), int(tracers))
+
+
+def put_yedges(zoom, st, region, tracers=True):
+    for c in zoom.children[region]:
+        _call("tm5_zoom_put_yedges", st.ptr(region), st.ptr(c), int(tracers))
+
+
+def _uncompress_xedges(zoom, st, region, js, je, ls, le, tracers=True):
+    _call("tm5_zoom_uncompress_xedges", st.ptr(region), js, je, ls, le, int(tracers))
+
+
+def _uncompress_yedges(zoom, st, region, is_, ie, ls, le, tracers=True):
+    _call("tm5_zoom_uncompress_yedges", st.ptr(region), is_, ie, ls, le, int(tracers))
+
+
+def mix_edges(zoom, st, region, tracers=True):
+    _call("tm5_zoom_mix_edges", st.ptr(region), int(tracers))
+
+
+def update_parent(zoom, st, region, tracers=True):
+    if region != 1:
+        _call("tm5_zoom_update_parent", st.ptr(zoom.g(region).parent), st.ptr(region), int(tracers))
+
+
+# ------------------------------------------------------------------ mass flow
+def m2phlb1(zoom, st, region=1):
+    _call("tm5_zoom_m2phlb", st.ptr(region), None, st.at.data_ptr(), st.bt.data_ptr())
+
+
+def m2phlb(zoom, st, region):
+    _call("tm5_zoom_m2phlb", st.ptr(region), st.ptr(zoom.g(region).parent), st.at.data_ptr(), st.bt.data_ptr())
+
+
+def _coarsen_block(zoom, st, region):
+    _call("tm5_zoom_coarsen_block", st.ptr(zoom.g(region).parent), st.ptr(region))
+
+
+def coarsen_ambm(zoom, st, region):
+    for c in zoom.children[region]:
+        _call("tm5_zoom_coarsen_ambm", st.ptr(region), st.ptr(c))
+
+
+def dynam0(zoom, st, region, ndyn, rehearsal=False):
+    _call("tm5_zoom_dynam0", st.ptr(region), st.bt.data_ptr(), int(ndyn), int(rehearsal))
+
+
+# ------------------------------------------------------------------ outer interface faces
+@contextmanager
+def outer_x_faces_zeroed(zoom, st, region, second, rehearsal=False):
+    """A child's outer x interface faces zeroed around its x advection in the second half of the
+    parent step (pydyn/zoom/advect.py:121-136)."""
+    g = zoom.g(region)
+    zero = second and g.im // g.xref < zoom.g(1).im
+    buf = st.slabs[region][0].data_ptr()
+    if zero:
+        _call("tm5_zoom_outer_faces", st.ptr(region), 0, 0, buf, int(rehearsal))
+    yield
+    if zero:
+        _call("tm5_zoom_outer_faces", st.ptr(region), 0, 1, buf, int(rehearsal))
+
+
+@contextmanager
+def outer_y_faces_zeroed(zoom, st, region, second, rehearsal=False):
+    """The same for the y interface faces of a region touching no pole (advect.py:139-157)."""
+    g = zoom.g(region)
+    zero = second and (g.touch_sp != 1 or g.touch_np != 1)
+    buf = st.slabs[region][1].data_ptr()
+    if zero:
+        _call("tm5_zoom_outer_faces", st.ptr(region), 1, 0, buf, int(rehearsal))
+    yield
+    if zero:
+        _call("tm5_zoom_outer_faces", st.ptr(region), 1, 1, buf, int(rehearsal))
+
+
+# ------------------------------------------------------------------ advection
+def _dynamu_region1(zoom, st, js, je, limits, stats=None, max_nloop=50):
+    _call("tm5_zoom_advect_x", st.ptr(1), js, je, int(limits), max_nloop, 0)
+
+
+def _dynamu_child(zoom, st, region, js, je, ls, le, limits, stats=None, max_nloop=50):
+    if (ls, le) != (1, zoom.lm):
+        raise NotImplementedError("_dynamu_child: zoom2D levels only")
+    _call("tm5_zoom_advect_x", st.ptr(region), js, je, int(limits), max_nloop, 0)
+
+
+def _dynamv_region1(zoom, st, limits, stats=None, max_nloop=10):
+    _call("tm5_zoom_advect_y", st.ptr(1), 1, zoom.g(1).im, int(limits), max_nloop, 0)
+
+
+def _dynamv_child(zoom, st, region, is_, ie, ls, le, limits, stats=None, max_nloop=10):
+    if (ls, le) != (1, zoom.lm):
+        raise NotImplementedError("_dynamv_child: zoom2D levels only")
+    _call("tm5_zoom_advect_y", st.ptr(region), is_, ie, int(limits), max_nloop, 0)
+
+
+def _dynamw_region1(zoom, st, limits, stats=None, iter_limit=15):
+    g = zoom.g(1)
+    _call("tm5_zoom_advect_z", st.ptr(1), 1, g.im, 1, g.jm, int(limits), iter_limit, 0)
+
+
+def _dynamw_child(zoom, st, region, is_, ie, js, je, limits, stats=None, iter_limit=15):
+    _call("tm5_zoom_advect_z", st.ptr(region), is_, ie, js, je, int(limits), iter_limit, 0)
+
+
+# the rehearsal's mass-only forms (pydyn/zoom/check_cfl.py:101-226)
+def _dynamum_region1(zoom, st, js, je, max_nloop):
+    _call("tm5_zoom_advect_x", st.ptr(1), js, je, 0, max_nloop, 1)
+
+
+def _dynamum_child(zoom, st, region, js, je, ls, le, max_nloop):
+    _call("tm5_zoom_advect_x", st.ptr(region), js, je, 0, max_nloop, 1)
+
+
+def _dynamvm_region1(zoom, st, max_nloop):
+    _call("tm5_zoom_advect_y", st.ptr(1), 1, zoom.g(1).im, 0, max_nloop, 1)
+
+
+def _dynamvm_child(zoom, st, region, is_, ie, ls, le, max_nloop):
+    _call("tm5_zoom_advect_y", st.ptr(region), is_, ie, 0, max_nloop, 1)
+
+
+def _dynamwm(zoom, st, region, is_, ie, js, je, iter_limit):
+    _call("tm5_zoom_advect_z", st.ptr(region), is_, ie, js, je, 0, iter_limit, 1)
+
+
+# ------------------------------------------------------------------ column physics
+def column_physics(zoom, st, region, phys, ops):
+    """convec with the sources and chemistry around it (pydyn/zoom/physics.py:34-78) in one
+    pass over the region's owned columns; `ops` one of v, csv, vsc, vsccsv, vscMcsv."""
+    _call("tm5_zoom_physics", st.ptr(region), ctypes.byref(phys), ops.encode())
+
+
+# ------------------------------------------------------------------ save / restore / scale
+def save(st, key, names):
+    """Copies of the named arrays of every region, for restore(st, key)."""
+    bufs = st.saved.setdefault(key, {})
+    for r, f in st.fields.items():
+        for k in names:
+            if (r, k) not in bufs:
+                bufs[(r, k)] = D.empty((f[k].numel(),))
+            _call("tm5_zoom_copy", f[k].t.data_ptr(), bufs[(r, k)].data_ptr(), f[k].numel())
+
+
+def restore(st, key):
+    for (r, k), buf in st.saved[key].items():
+        f = st.fields[r][k]
+        _call("tm5_zoom_copy", buf.data_ptr(), f.t.data_ptr(), f.numel())
+
+
+def scale(st, names, factor):
+    """a = a * factor for the named arrays of every region."""
+    for f in st.fields.values():
+        for k in names:
+            _call("tm5_zoom_scale", f[k].t.data_ptr(), f[k].numel(), float(factor))
+
+
+def scaled_copy(src, dst, factor):
+    """dst = src * factor (whole arrays): a meteo field at amplitude `factor`."""
+    _call("tm5_zoom_copy", src.data_ptr(), dst.data_ptr(), dst.numel())
+    _call("tm5_zoom_scale", dst.data_ptr(), dst.numel(), float(factor))

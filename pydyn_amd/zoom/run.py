@@ -1,0 +1,117 @@
+"""A zoom tree resident on the device, run interval by interval: a global region with one child
+on nested synthetic meteo (pydyn_amd.meteo.zoom), Check_CFL of the tree, then the steps with
+Setup_MassFlow, every region's convection, sources (its emission at level 1 of tracer 1,
+month2dt of its half step ndyn / (2 tref)) and chemistry (the decay of its loss rates over
+ndyn / (2 tref)) on the cells it owns.
+"""
+
+import numpy as np
+
+from .. import _lib
+from .. import device as D
+from ..gpu import kernels as K
+from ..meteo import SEC_MONTH, XMCH4
+from . import driver as Z
+from . import lib as L
+from .state import ZoomState
+from .tree import Region, Zoom
+
+__all__ = ["Physics", "ZoomRun", "tree_of"]
+
+
+def tree_of(parent_grid, child):
+    """The Zoom of a global grid and one pydyn_amd.meteo.Child."""
+    g, c = parent_grid, child
+    glb = Region("globe", g.im, g.jm, xcyc=1, touch_sp=1, touch_np=1)
+    kid = Region(c.name, c.im, c.jm, c.xbeg, c.xend, c.ybeg, c.yend, c.xref, c.yref, 1, c.tref, parent=1)
+    zoom = Zoom([glb, kid], g.lm, g.at, g.bt)
+    if (zoom.ibeg[2], zoom.jbeg[2], zoom.iend[2], zoom.jend[2]) != (c.ibeg, c.jbeg, c.iend, c.jend):
+        raise ValueError("the child's declared bounds do not snap to its footprint")
+    return zoom
+
+
+class Physics:
+    """Every region's column-physics inputs on the device, and their tm5_zphysics per ndyn."""
+
+    def __init__(self, zoom, st, mets, lmax):
+        torch = __import__("torch")
+        self.zoom, self.st, self.lmax = zoom, st, lmax
+        self.arrays, self._structs = {}, {}
+        for r, met in mets.items():
+            up = D.upload
+            a = {k: up(getattr(met, k)) for k in ("entu", "detu", "entd", "detd", "emis", "loss")}
+            a["dkg"] = K.dkg(up(met.kvh), up(met.m), up(met.gph), lmax)
+            a["cloud_top"] = up(met.cloud_top.astype(np.int32)).to(torch.int32)
+            a["cloud_lfs"] = up(met.cloud_lfs.astype(np.int32)).to(torch.int32)
+            self.arrays[r] = a
+
+    def struct(self, r, ndyn):
+        key = (r, ndyn)
+        if key not in self._structs:
+            a, tref = self.arrays[r], self.zoom.g(r).tref
+            p = _lib.ZPhysics()
+            for k in ("entu", "detu", "entd", "detd", "dkg", "emis", "loss"):
+                setattr(p, k, D.view(a[k]))
+            p.cloud_top, p.cloud_lfs = D.iview(a["cloud_top"]), D.iview(a["cloud_lfs"])
+            p.lmax = self.lmax
+            half = float(ndyn) / (2.0 * tref)
+            p.dt_conv = half
+            p.factor = half / SEC_MONTH  # month2dt (pydyn/emis.py:65-68)
+            p.ratio, p.efrac = XMCH4 / XMCH4, 1.0
+            p.level, p.tracer = 1, 1
+            p.loss_stride = a["loss"].stride(3)
+            p.dt_chem = half
+            p.at, p.bt = self.st.at.data_ptr(), self.st.bt.data_ptr()
+            self._structs[key] = p
+        return self._structs[key]
+
+
+class ZoomRun:
+    """The device state, inputs and clock of a two-region tree built from (parent Meteo,
+    NestedMeteo)."""
+
+    def __init__(self, pmet, cmet, limits=True):
+        g = pmet.g
+        self.mets = {1: pmet, 2: cmet}
+        self.zoom = tree_of(g, cmet.c)
+        self.limits = limits
+        self.st = ZoomState(self.zoom, pmet.nt)
+        for r, met in self.mets.items():
+            self.st.upload_interior(r, met.state(), met.dxyp if r > 1 else g.dxyp)
+        L.update_parent(self.zoom, self.st, 2)  # the footprint holds the child's mass from the start
+        self.phys = Physics(self.zoom, self.st, self.mets, g.lmax)
+        self.flux0 = {r: (D.upload(met.pu0), D.upload(met.pv0)) for r, met in self.mets.items()}
+        self.set_fluxes(0.0)
+        Z.setup_massflow(self.zoom, self.st, 3600)
+        L.m2phlb(self.zoom, self.st, 2)
+        L.m2phlb1(self.zoom, self.st, 1)
+        self.t = 0.0
+        self.log = []
+
+    def set_fluxes(self, t_sec):
+        """Every region's pu, pv at time t_sec: the steady flow times the amplitude."""
+        a = self.mets[1].amplitude(t_sec)
+        for r, (pu0, pv0) in self.flux0.items():
+            f = self.st[r]
+            L.scaled_copy(pu0, f["pu"].t, a)
+            L.scaled_copy(pv0, f["pv"].t, a)
+
+    def interval(self, nread=3600, ndyn_max=3600):
+        """Check_CFL for the tree, then its n steps of ndyn.  Returns (ndyn, n)."""
+        t0, zoom, st = self.t, self.zoom, self.st
+
+        def interpolate(zoom_, st_, i, ndyn):
+            self.set_fluxes(t0 + (i - 0.5) * ndyn)
+
+        ndyn, n, _ = Z.check_cfl_zoom(zoom, st, ndyn_max, nread // ndyn_max, nread, interpolate)
+        for s in range(n):
+            self.set_fluxes(t0 + s * ndyn + ndyn / 2.0)
+            Z.setup_massflow(zoom, st, ndyn)
+            Z.step(zoom, st, self.phys, ndyn, self.limits)
+        D.context().raise_errors()
+        self.t = t0 + n * ndyn
+        self.log.append((t0, ndyn, n))
+        return ndyn, n
+
+    def host(self):
+        return self.st.host()

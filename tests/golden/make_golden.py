@@ -6,6 +6,7 @@ Run in the build container only (the reference is not on the GPU machines):
     python tests/golden/make_golden.py run tiny 48    # whole single-region runs: sha1 per interval
     python tests/golden/make_golden.py run glb6x4 24
     python tests/golden/make_golden.py run glb3x2 720 # long: written interval by interval
+    python tests/golden/make_golden.py zoom tiny2 48  # a global region with one child (zoom tree)
 
 It imports the reference package from /root/reference (read-only) and routes the reference's
 libm through the oracle's fdlibm library with the reference's own `ops.use_libm`, so that its
@@ -238,12 +239,106 @@ def run_golden(name, n_intervals):
         np.savez_compressed(GOLDEN / f"run_{name}_final.npz", **final)
 
 
+# ------------------------------------------------------------------ zoom trees
+ZFIELDS = ("m", "rm", "rxm", "rym", "rzm", "am", "bm", "cm", "pu", "pv", "sp", "phlb", "dxyp")
+
+
+def reference_zoom_run(pmet, cmet, n_intervals, every=None):
+    """A global region with one child on pydyn_amd.meteo's nested inputs through the reference's
+    zoom path: the parent's footprint set to the child's sums (its update_parent), Check_CFL of
+    the tree, then the steps with Setup_MassFlow; every region's `s` adds its emission (month2dt
+    of its half step ndyn / (2 tref)) at level 1 of tracer 1 and its `c` applies the decay of its
+    scope's loss rates over ndyn / (2 tref).  every(it, t0, ndyn, n, st)."""
+    pg, c, nt = pmet.g, cmet.c, pmet.nt
+    lm = pg.lm
+    glb = Z.Region("globe", pg.im, pg.jm, xcyc=1, touch_sp=1, touch_np=1)
+    kid = Z.Region(c.name, c.im, c.jm, xbeg=c.xbeg, xend=c.xend, ybeg=c.ybeg, yend=c.yend, xref=c.xref,
+                   yref=c.yref, tref=c.tref, parent=1)  # fmt: skip
+    zm = Z.Zoom([glb, kid], lm, pg.at, pg.bt)
+    assert (zm.ibeg[2], zm.jbeg[2], zm.iend[2], zm.jend[2]) == (c.ibeg, c.jbeg, c.iend, c.jend)
+    st = Z.ZoomState.zeros(zm, nt)
+    mets = {1: pmet, 2: cmet}
+    conv, emis, loss = {}, {}, {}
+    for r, met in mets.items():
+        g, f = zm.g(r), st[r]
+        init = met.state()
+        f["m"][1 : g.im, 1 : g.jm, 1:lm] = init["m"]
+        for k in ("rm", "rxm", "rym", "rzm"):
+            f[k][1 : g.im, 1 : g.jm, 1:lm] = init[k]
+        f["dxyp"][1 : g.jm] = met.dxyp if r > 1 else pg.dxyp
+        conv[r] = {
+            "entu": met.entu, "detu": met.detu, "entd": met.entd, "detd": met.detd,
+            "dkg": diffusion.dkg(met.kvh, met.m, met.gph, pg.lmax),
+            "cloud_top": met.cloud_top, "cloud_lfs": met.cloud_lfs, "lmax": pg.lmax,
+        }  # fmt: skip
+        emis[r] = met.emis
+        si, sj = Z.scope(zm, r)
+        loss[r] = met.loss[si, sj]
+    Z.update_parent(zm, st, 2)
+
+    def put_fluxes(s_, t_sec):
+        for r, met in mets.items():
+            g = zm.g(r)
+            pu, pv = met.fluxes(t_sec)
+            s_[r]["pu"][0 : g.im, 1 : g.jm, 1:lm] = pu
+            s_[r]["pv"][1 : g.im, 1 : g.jm + 1, 1:lm] = pv
+
+    put_fluxes(st, 0.0)
+    Z.setup_massflow(zm, st, 3600)
+    Z.m2phlb(zm, st, 2)
+    Z.m2phlb1(zm, st, 1)
+    t = 0.0
+    for it in range(n_intervals):
+        t0 = t
+
+        def interpolate(zoom, s_, i, ndyn, t0=t0):
+            put_fluxes(s_, t0 + (i - 0.5) * ndyn)
+
+        ndyn, n, _ = Z.check_cfl_zoom(zm, st, 3600, 1, 3600, interpolate)
+        for s in range(n):
+            put_fluxes(st, t0 + s * ndyn + ndyn / 2.0)
+            Z.setup_massflow(zm, st, ndyn)
+
+            def sources(zoom, s_, region, ndyn=ndyn):
+                Z.sources_region(zoom, s_, region, [(_inject(emis[region], ndyn, zoom.g(region).tref), 1, 1)])
+
+            def chemistry(zoom, s_, region, ndyn=ndyn):
+                decay = chem.decay(loss[region], ndyn / (2.0 * zoom.g(region).tref))
+                Z.chemistry_region(zoom, s_, region, decay, range(1, nt + 1))
+
+            Z.step(zm, st, conv, ndyn, sources=sources, chemistry=chemistry)
+        t = t0 + n * ndyn
+        if every is not None:
+            every(it, t0, ndyn, n, st)
+    return zm, st
+
+
+def zoom_golden(name, n_intervals):
+    pmet, cmet = meteo.zoom(name)
+    path = GOLDEN / f"zoom_{name}_sha1.json"
+    record = {"config": name, "intervals": []}
+
+    def every(it, t0, ndyn, n, st):
+        arrays = {f"r{r}.{k}": sha(st[r][k].data) for r in (1, 2) for k in ZFIELDS}
+        record["intervals"].append({"t0": t0, "ndyn": ndyn, "n": n, **arrays})
+        if it % 6 == 5 or it == n_intervals - 1:
+            path.write_text(json.dumps(record, indent=0))
+            print(f"{name}: interval {it + 1}/{n_intervals} t={t0 + n * ndyn:.0f} ndyn={ndyn}", flush=True)
+
+    _, st = reference_zoom_run(pmet, cmet, n_intervals, every)
+    if name in ("tiny2", "small2"):
+        np.savez_compressed(GOLDEN / f"zoom_{name}_final.npz",
+                            **{f"r{r}.{k}": st[r][k].data for r in (1, 2) for k in ZFIELDS})  # fmt: skip
+
+
 def main(argv):
     _use_twin()
     if argv[0] == "ops":
         np.savez_compressed(GOLDEN / "ops_tiny.npz", **ops_cases())
     elif argv[0] == "run":
         run_golden(argv[1], int(argv[2]))
+    elif argv[0] == "zoom":
+        zoom_golden(argv[1], int(argv[2]))
     else:
         raise SystemExit(__doc__)
 

@@ -213,3 +213,43 @@ def test_fdlibm_exp_within_one_ulp_of_the_c_library():
     special = oracle_libm.exp(np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 710.0, -746.0]))
     assert special[0] == 1.0 and special[1] == 1.0 and special[2] == np.inf and special[3] == 0.0
     assert np.isnan(special[4]) and special[5] == np.inf and special[6] == 0.0
+
+
+# ------------------------------------------------------------------ the zoom tree
+def _zoom_against_golden(name, limit=None):
+    import hashlib
+    import json
+
+    from oracle import zoom as OZ
+    from pydyn_amd import meteo
+
+    ref = json.loads((GOLDEN / f"zoom_{name}_sha1.json").read_text())["intervals"][:limit]
+    run = OZ.ZoomRun(*meteo.zoom(name))
+    for it, want in enumerate(ref):
+        assert run.interval() == (want["ndyn"], want["n"]), f"{name} interval {it}"
+        for key, digest in want.items():
+            if "." in key:
+                r, k = int(key[1]), key.split(".")[1]
+                got = hashlib.sha1(np.ascontiguousarray(run.st[r][k].data, dtype="<f8").tobytes()).hexdigest()[:20]
+                assert got == digest, f"{name} interval {it}: region {r} {k}"
+    return run
+
+
+def test_zoom_tiny_tree_bitwise():
+    """A global region with a nested child, 4 simulated days: every array of both regions (halos
+    included) bitwise equal to the reference's zoom path, interval by interval."""
+    run = _zoom_against_golden("tiny2")
+    final = np.load(GOLDEN / "zoom_tiny2_final.npz")
+    for key in final.files:
+        r, k = int(key[1]), key.split(".")[1]
+        assert np.array_equal(run.st[r][k].data.view(np.int64), final[key].view(np.int64)), key
+
+
+def test_zoom_small_tree_with_cfl_cuts_bitwise():
+    run = _zoom_against_golden("small2", limit=36)
+    assert any(nd == 1800 for _, nd, _ in run.log), "the sample must include a Check_CFL cut"
+
+
+def test_zoom_production_tree_first_intervals_bitwise():
+    """glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N (120 x 70 x 34)."""
+    _zoom_against_golden("c2", limit=2)
