@@ -194,6 +194,12 @@ typedef struct tm5_zregion {
     const double *dxyp;              /* device, jm */
     const int32_t *owned;            /* device (im, jm), i fastest: 1 where convec, sources, chemistry act */
     const int32_t *edge;             /* device (im, jm), i fastest: blank_zoom_region's edge map */
+    /* region 1's reduced grid (pydyn/redgrid.py): nred rows jred[] (device, 1-based), each row's
+       cell count imred[] (device, jm; im on a uniform row); nred = 0: uniform */
+    int nred;
+    const int32_t *jred;
+    const int32_t *imred;
+    double *scratch;                 /* device, >= (elements of m) + (elements of am) doubles when nred > 0 */
 } tm5_zregion;
 
 /* put_xedges / put_yedges: the parent's cells at each wall of the child into the child's halo

@@ -101,6 +101,76 @@ class Region:
     xcyc: int = 0
     touch_sp: int = 0
     touch_np: int = 0
+    redgrid: object = None  # RedGrid of region 1's x advection, or None
+
+
+# ------------------------------------------------------------------ the reduced grid (redgrid.py)
+class RedGrid:
+    """Rows near the poles advected in x with clusters of `clustsize` cells merged into one
+    (pydyn/redgrid.py:26-54): bands [(j 0-based, clustsize, imred)], imred per row."""
+
+    def __init__(self, im, jm, jred, clustsize):
+        self.im, self.jm = im, jm
+        self.bands = []
+        self.imred = np.full(jm, im, dtype=int)
+        for j, c in zip(jred, clustsize):
+            if im % c:
+                raise ValueError(f"redgrid: im={im} not a multiple of clustsize {c}")
+            self.bands.append((j - 1, c, im // c))
+            self.imred[j - 1] = im // c
+
+    @property
+    def rows(self):
+        return [b[0] for b in self.bands]
+
+
+def _cluster_sums(row, n, c):
+    """(n, ...) sums of consecutive clusters of c cells, accumulated in index order."""
+    parts = row[: n * c].reshape(n, c, *row.shape[1:])
+    acc = parts[:, 0].copy()
+    for k in range(1, c):
+        acc = acc + parts[:, k]
+    return acc
+
+
+def uni2red(grid, m, qs):
+    """Clusters summed into cells 0..imred-1 of their row; the tracers beyond zeroed, m not
+    (pydyn/redgrid.py:57-68)."""
+    for j, c, n in grid.bands:
+        m[:n, j] = _cluster_sums(m[:, j], n, c)
+        for q in qs:
+            q[:n, j] = _cluster_sums(q[:, j], n, c)
+            q[n:, j] = 0.0
+
+
+def uni2red_mass(grid, m):
+    for j, c, n in grid.bands:
+        m[:n, j] = _cluster_sums(m[:, j], n, c)
+
+
+def uni2red_mf(grid, am):
+    """am(i) = am(i clustsize) for i = 1..imred, am(0) = am(imred), the rest -1 (redgrid.py:71-77)."""
+    for j, c, n in grid.bands:
+        row = am[:, j]
+        row[1 : n + 1] = row[c * np.arange(1, n + 1)]
+        row[n + 1 :] = -1.0
+        row[0] = row[n]
+
+
+def red2uni(grid, m, qs, m_uni):
+    """The clusters split again: tracers spread by m_uni / m of the cluster, m = m_uni on the
+    reduced rows (redgrid.py:80-92)."""
+    for j, c, n in grid.bands:
+        scale = m_uni[: n * c, j].reshape(n, c, *m_uni.shape[2:]) / m[:n, j, None]
+        for q in qs:
+            q[: n * c, j] = (scale[..., None] * q[:n, j, None]).reshape(n * c, *q.shape[2:])
+    for j, _, _ in grid.bands:
+        m[:, j] = m_uni[:, j]
+
+
+def red2uni_mass(grid, m, m_uni):
+    for j, _, _ in grid.bands:
+        m[:, j] = m_uni[:, j]
 
 
 class Tree:
@@ -714,16 +784,37 @@ def advect_z_zoom(tree, st, r, limits=True):
 
 
 def x_region1(tree, st, js, je, limits):
-    """dynamu(1) on the interior rows, then the periodic x halos its closing loop writes for
-    rows js..je (advect.py:253-297)."""
+    """dynamu(1) on the interior rows (on the reduced grid when region 1 has one), then the
+    periodic x halos its closing loop writes for rows js..je -- in the reduced representation,
+    before the clusters are split -- and, for every reduced row, q(0) = q(im) after the split
+    (pydyn/zoom/advect.py:253-297)."""
     g, f, lm = tree.g(1), st[1], tree.lm
     im, jm = g.im, g.jm
-    advect_x(*interior(f, im, jm, lm), f["am"][0:im, 1:jm, 1:lm], limits)
+    fields = interior(f, im, jm, lm)
+    am = f["am"][0:im, 1:jm, 1:lm]
+    grid = g.redgrid
+    if grid is None:
+        advect_x(*fields, am, limits)
+    else:
+        m_uni, am_uni = fields[0].copy(), am.copy()
+        uni2red(grid, fields[0], fields[1:])
+        uni2red_mf(grid, am)
+        for j in range(jm):
+            n = grid.imred[j]
+            advect_x(*(q[:n, j : j + 1] for q in fields), am[: n + 1, j : j + 1], limits)
     for j in range(js, je + 1):
+        n = grid.imred[j - 1] if grid is not None else im
         for name in ("rm", "rxm", "rym", "rzm", "m"):
             q = f[name]
-            q[0, j] = q[im, j].copy()
-            q[im + 1, j] = q[1, j].copy()
+            q[0, j] = q[n, j].copy()
+            q[n + 1, j] = q[1, j].copy()
+    if grid is not None:
+        red2uni(grid, fields[0], fields[1:], (m_uni + am_uni[:-1]) - am_uni[1:])
+        am[...] = am_uni
+        for j0 in grid.rows:
+            for name in ("rm", "rxm", "rym", "rzm", "m"):
+                q = f[name]
+                q[0, j0 + 1] = q[im, j0 + 1].copy()
 
 
 def y_region1(tree, st, limits):
@@ -1066,15 +1157,31 @@ def mass_x_zoom(tree, st, run, r, cap):
 
 
 def mass_x_region1(tree, st, js, je, cap):
-    """dynamum(1) and the halos m(0) = m(im), m(im+1) = m(1) of rows js..je (check_cfl.py:101-118)."""
+    """dynamum(1) (on the reduced grid when there is one, pydyn/cfl.py:30-57) and the halos
+    m(0) = m(im) of rows js..je, m(im+1) = m(1) on the uniform rows (check_cfl.py:101-118)."""
     g, f, lm = tree.g(1), st[1], tree.lm
     im, jm = g.im, g.jm
-    if not mass_x(f["m"][1:im, 1:jm, 1:lm], f["am"][0:im, 1:jm, 1:lm], cap):
-        return False
+    m_int, am_int = f["m"][1:im, 1:jm, 1:lm], f["am"][0:im, 1:jm, 1:lm]
+    grid = g.redgrid
+    if grid is None:
+        if not mass_x(m_int, am_int, cap):
+            return False
+    else:
+        m_uni, am_uni = m_int.copy(), am_int.copy()
+        uni2red_mass(grid, m_int)
+        uni2red_mf(grid, am_int)
+        for j in range(jm):
+            n = grid.imred[j]
+            if not mass_x(m_int[:n, j : j + 1], am_int[: n + 1, j : j + 1], cap):
+                return False
+        red2uni_mass(grid, m_int, (m_uni + am_uni[:-1]) - am_uni[1:])
+        am_int[...] = am_uni
+    reduced = set(grid.rows) if grid is not None else set()
     m = f["m"]
     for j in range(js, je + 1):
         m[0, j] = m[im, j].copy()
-        m[im + 1, j] = m[1, j].copy()
+        if j - 1 not in reduced:
+            m[im + 1, j] = m[1, j].copy()
     return True
 
 
@@ -1234,10 +1341,12 @@ SEC_MONTH = 30.0 * 86400.0
 XMCH4 = 12.01115 + 1.0079 * 4.0
 
 
-def tree_of(parent, child, lm, at, bt):
-    """The Tree of a global parent grid and one pydyn_amd.meteo.Child."""
+def tree_of(parent, child, lm, at, bt, redgrid=None):
+    """The Tree of a global parent grid (its x advection on the reduced grid (jred, clustsize)
+    when given) and one pydyn_amd.meteo.Child."""
     c = child
-    glb = Region("globe", parent.im, parent.jm, xcyc=1, touch_sp=1, touch_np=1)
+    grid = None if redgrid is None else RedGrid(parent.im, parent.jm, *redgrid)
+    glb = Region("globe", parent.im, parent.jm, xcyc=1, touch_sp=1, touch_np=1, redgrid=grid)
     kid = Region(c.name, c.im, c.jm, c.xbeg, c.xend, c.ybeg, c.yend, c.xref, c.yref, 1, c.tref, parent=1)
     tree = Tree([glb, kid], lm, at, bt)
     if (tree.ibeg[2], tree.jbeg[2], tree.iend[2], tree.jend[2]) != (c.ibeg, c.jbeg, c.iend, c.jend):
@@ -1251,10 +1360,10 @@ class ZoomRun:
     step) and chemistry (the region's decay over its scope, dt = ndyn / (2 tref)) as callbacks
     on every region's s and c (pydyn/forward.py:411-451 generalised to the tree)."""
 
-    def __init__(self, pmet, cmet, limits=True):
+    def __init__(self, pmet, cmet, limits=True, redgrid=None):
         g = pmet.g
         self.mets = {1: pmet, 2: cmet}
-        self.tree = tree_of(g, cmet.c, g.lm, g.at, g.bt)
+        self.tree = tree_of(g, cmet.c, g.lm, g.at, g.bt, redgrid)
         self.limits = limits
         self.nt = pmet.nt
         tree, lm = self.tree, g.lm

@@ -46,6 +46,10 @@ class ZRegion(ctypes.Structure):
         ("dxyp", ctypes.c_void_p),
         ("owned", ctypes.c_void_p),
         ("edge", ctypes.c_void_p),
+        ("nred", ctypes.c_int),
+        ("jred", ctypes.c_void_p),
+        ("imred", ctypes.c_void_p),
+        ("scratch", ctypes.c_void_p),
     ]  # fmt: skip
 
 

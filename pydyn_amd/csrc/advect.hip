@@ -26,13 +26,14 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // every face is scaled by (nloop-1)/nloop in place.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
-                                          int nmax, int *words, int *rows_out, int fail_bit, int skip_mask)
+                                          int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
+                                          const int *imred)
 {
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
-    const int n = im;
     const int row = blockIdx.x;
     const int j = row % jm, l = row / jm;
+    const int n = imred ? imred[j] : im;  // a reduced row has imred(j) cells
     const int lane = threadIdx.x;
     double *M = lds;       // n   current mass
     double *M0 = M + n;    // n   the row's mass at the start
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
 }
 
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask)
+             int skip_mask, const int *imred)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
     const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
@@ -146,10 +147,10 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
     ctx->x_nrows = jm * lm;
     if (q.nt > 0)
         TM5_LAUNCH(ctx, k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                           ctx->x_rows, fail_bit, skip_mask);
+                   ctx->x_rows, fail_bit, skip_mask, imred);
     else
         TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                           ctx->x_rows, fail_bit, skip_mask);
+                   ctx->x_rows, fail_bit, skip_mask, imred);
     return status(hipGetLastError());
 }
 

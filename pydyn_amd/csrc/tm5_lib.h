@@ -116,6 +116,8 @@ struct ZR {
     HV m, q[4], am, bm, cm, pu, pv, sp, phlb;
     const double *dxyp;
     const int *owned, *edge;
+    int nred;
+    const int *jred, *imred;
 };
 
 inline ZR zr(const tm5_zregion &d)
@@ -130,6 +132,7 @@ inline ZR zr(const tm5_zregion &d)
     r.am = hv(d.am), r.bm = hv(d.bm), r.cm = hv(d.cm), r.pu = hv(d.pu), r.pv = hv(d.pv);
     r.sp = hv(d.sp), r.phlb = hv(d.phlb);
     r.dxyp = d.dxyp, r.owned = d.owned, r.edge = d.edge;
+    r.nred = d.nred, r.jred = d.jred, r.imred = d.imred;
     return r;
 }
 
@@ -182,7 +185,7 @@ inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 
 // launchers (advect.hip)
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask);
+             int skip_mask, const int *imred = nullptr);
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
              int skip_mask);
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,

@@ -642,6 +642,101 @@ __global__ void __launch_bounds__(256) k_y_child_restore(ZR C, int ls, const int
     b = b * (double)nl;
 }
 
+
+// ------------------------------------------------------------------ region 1's reduced grid
+// pydyn/redgrid.py:57-92 on the rows jred[b] (1-based) with c = im / imred cells per cluster.
+// One 64-lane workgroup per (reduced row, level): the row staged in LDS, since the clusters are
+// summed (and later split) in place.  f = 0 is m; then the tracer arrays, rm..rzm of each tracer.
+__global__ void __launch_bounds__(64) k_uni2red(ZR R, int tracers, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    extern __shared__ double row[];
+    const int j = R.jred[blockIdx.x], l = blockIdx.y + 1, lane = threadIdx.x;
+    const int n = R.imred[j - 1], c = R.im / n;
+    const int nf = tracers ? 1 + 4 * R.nt : 1;
+    for (int f = 0; f < nf; f++) {
+        const HV &v = f == 0 ? R.m : R.q[(f - 1) % 4];
+        const int t = f == 0 ? 1 : (f - 1) / 4 + 1;
+        for (int i = lane; i < R.im; i += 64) row[i] = v(i + 1, j, l, t);
+        __syncthreads();
+        for (int k = lane; k < n; k += 64) {
+            double acc = row[k * c];
+            for (int a = 1; a < c; a++) acc = acc + row[k * c + a];
+            v(k + 1, j, l, t) = acc;
+        }
+        if (f > 0)  // the tracers beyond the reduced row are zeroed (m is not)
+            for (int i = n + lane; i < R.im; i += 64) v(i + 1, j, l, t) = 0.0;
+        __syncthreads();
+    }
+}
+
+// am(i) = am(i c) for i = 1..imred, am(imred+1..im) = -1, am(0) = am(imred), in index order
+__global__ void __launch_bounds__(64) k_uni2red_mf(ZR R, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= R.nred * R.lm) return;
+    const int j = R.jred[idx % R.nred], l = idx / R.nred + 1;
+    const int n = R.imred[j - 1], c = R.im / n;
+    for (int i = 1; i <= n; i++) R.am(i, j, l) = R.am(i * c, j, l);
+    for (int i = n + 1; i <= R.im; i++) R.am(i, j, l) = -1.0;
+    R.am(0, j, l) = R.am(n, j, l);
+}
+
+// The clusters split again with the uniform-grid masses m_uni(i) + am_uni(i-1) - am_uni(i) (the
+// row's mass advanced on the uniform grid): q(i) = m_uni(i) / m(cluster) q(cluster) for every
+// tracer array, then m = m_uni on the row.  mu, au: the uniform m and am before the x step
+// (copies laid out as m and am).
+__global__ void __launch_bounds__(64) k_red2uni(ZR R, HV mu, HV au, int tracers, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    extern __shared__ double sh[];
+    double *red = sh, *mred = sh + R.im;
+    const int j = R.jred[blockIdx.x], l = blockIdx.y + 1, lane = threadIdx.x;
+    const int n = R.imred[j - 1], c = R.im / n;
+    for (int k = lane; k < n; k += 64) mred[k] = R.m(k + 1, j, l);
+    __syncthreads();
+    if (tracers) {
+        for (int f = 0; f < 4 * R.nt; f++) {
+            const HV &v = R.q[f % 4];
+            const int t = f / 4 + 1;
+            for (int k = lane; k < n; k += 64) red[k] = v(k + 1, j, l, t);
+            __syncthreads();
+            for (int i = lane; i < n * c; i += 64) {
+                const int k = i / c;
+                const double m_new = (mu(i + 1, j, l) + au(i, j, l)) - au(i + 1, j, l);
+                v(i + 1, j, l, t) = (m_new / mred[k]) * red[k];
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < R.im; i += 64) R.m(i + 1, j, l) = (mu(i + 1, j, l) + au(i, j, l)) - au(i + 1, j, l);
+}
+
+// region 1's x halos of rows js..je on the reduced grid: main form (before the split) q(0) =
+// q(imred), q(imred+1) = q(1); after the split the reduced rows get q(0) = q(im) (quirk = 1).
+// The mass-only form after its split: m(0) = m(im), and m(im+1) = m(1) on uniform rows only.
+__global__ void __launch_bounds__(256) k_x_halo_red(ZR R, int js, int je, int nf, int stage, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const int nj = je - js + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nj * R.lm * nf) return;
+    const int j = js + (int)(idx % nj), l = (int)((idx / nj) % R.lm) + 1, f = (int)(idx / ((long long)nj * R.lm));
+    const HV &v = f == 0 ? R.m : R.q[(f - 1) / R.nt];
+    const int t = f == 0 ? 1 : (f - 1) % R.nt + 1;
+    const int n = R.imred[j - 1];
+    if (stage == 0) {
+        v(0, j, l, t) = v(n, j, l, t);
+        v(n + 1, j, l, t) = v(1, j, l, t);
+    } else if (stage == 1) {
+        if (n != R.im) v(0, j, l, t) = v(R.im, j, l, t);
+    } else {
+        v(0, j, l, t) = v(R.im, j, l, t);
+        if (n == R.im) v(R.im + 1, j, l, t) = v(1, j, l, t);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 static int nfields(const ZR &R, int tracers) { return 1 + (tracers ? 4 * R.nt : 0); }
 
@@ -805,6 +900,46 @@ using namespace tm5;
         if (rc_ != 0) return rc_; \
     } while (0)
 
+// Region 1's dynamu on its reduced grid (pydyn/zoom/advect.py:253-297, pydyn/cfl.py:30-57): the
+// uniform m and am kept, the clusters summed and the faces subsampled, every row advected with
+// its own cell count, the halos, the clusters split with the uniformly advanced mass, am put back.
+static int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int limits, int max_nloop, int mass_only,
+                          int fail, int skip)
+{
+    const ZR R = zr(d);
+    if (!d.jred || !d.imred || !d.scratch || R.im > 4096) return TM5_EINVAL;
+    const long long nm = (long long)(R.im + 4) * (R.jm + 4) * R.lm, na = (long long)(R.im + 2) * (R.jm + 2) * (R.lm + 2);
+    HV mu = R.m, au = R.am;
+    mu.p = d.scratch;
+    au.p = d.scratch + nm;
+    ZTRY(zoom_copy(c, R.m.p, mu.p, nm));
+    ZTRY(zoom_copy(c, R.am.p, au.p, na));
+    const dim3 rows((unsigned)R.nred, (unsigned)R.lm);
+    const int tracers = mass_only ? 0 : 1;
+    TM5_LAUNCH(c, k_uni2red, rows, dim3(64), (size_t)R.im * sizeof(double), c->stream, R, tracers, c->words, skip);
+    TM5_LAUNCH(c, k_uni2red_mf, zgrid((long long)R.nred * R.lm, 64), dim3(64), 0, c->stream, R, c->words, skip);
+    ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
+                  sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip, R.imred));
+    const int nf = mass_only ? 1 : 1 + 4 * R.nt;
+    const long long nh = (long long)(je - js + 1) * R.lm * nf;
+    if (!mass_only && nh > 0)
+        TM5_LAUNCH(c, k_x_halo_red, zgrid(nh), dim3(256), 0, c->stream, R, js, je, nf, 0, c->words, skip);
+    TM5_LAUNCH(c, k_red2uni, rows, dim3(64), (size_t)2 * R.im * sizeof(double), c->stream, R, mu, au, tracers,
+               c->words, skip);
+    ZTRY(zoom_copy(c, au.p, R.am.p, na));
+    if (nh > 0) {
+        if (mass_only) {
+            TM5_LAUNCH(c, k_x_halo_red, zgrid(nh), dim3(256), 0, c->stream, R, js, je, nf, 2, c->words, skip);
+        }
+    }
+    if (!mass_only) {  // every reduced row: q(0) = q(im) after the split
+        const long long nq = (long long)R.jm * R.lm * nf;
+        TM5_LAUNCH(c, k_x_halo_red, zgrid(nq), dim3(256), 0, c->stream, R, 1, R.jm, nf, 1, c->words, skip);
+    }
+    return status(hipGetLastError());
+}
+
+
 static bool zbad(const tm5_zregion *r) { return r == nullptr || r->m.data == nullptr || r->im < 1 || r->jm < 1 || r->lm < 1; }
 static bool zglobal(const tm5_zregion *r) { return r->xcyc == 1 && r->touch_sp == 1 && r->touch_np == 1; }
 static int zskipmask(int rehearsal) { return rehearsal ? (TM5_ERRORS | TM5_REHEARSAL) : TM5_ERRORS; }
@@ -919,9 +1054,12 @@ extern "C" int tm5_zoom_advect_x(tm5_ctx *c, const tm5_zregion *r, int js, int j
     const int skip = zskipmask(mass_only);
     if (zglobal(r)) {
         const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_X;
-        ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
-                      sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip));
-        return zoom_x_halo(c, R, js, je, !mass_only, skip);
+        if (r->nred == 0) {
+            ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
+                          sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip));
+            return zoom_x_halo(c, R, js, je, !mass_only, skip);
+        }
+        return zoom_x_reduced(c, *r, js, je, limits, max_nloop, mass_only, fail, skip);
     }
     const int tr = mass_only ? 0 : 1;
     ZTRY(tm5_zoom_compress_xedges(c, r, js, je, 1, R.lm, tr));

@@ -329,12 +329,21 @@ class Child:
         return self.jbeg + self.jm // self.yref - 1
 
 
-# parent grid name, seed, child
+_TINY_CHILD = Child("tiny_child", 8, 8, -90.0, 30.0, -45.0, 45.0, 2, 2, 2, 4, 3)
+_SMALL_CHILD = Child("small_child", 24, 18, -120.0, 60.0, -56.25, 45.0, 2, 2, 2, 5, 4)
+# glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N: parent cells 4..43 x 51..85
+_NAM_CHILD = Child("nam1x1", 120, 70, -170.0, -50.0, 10.0, 80.0, 3, 2, 2, 4, 51)
+# glb3x2's reduced grid as production runs it: rows (1-based) and their cluster sizes
+GLB3X2_REDGRID = ((1, 2, 3, 4, 5, 6, 7, 84, 85, 86, 87, 88, 89, 90), (40, 8, 8, 4, 4, 4, 2, 2, 4, 4, 4, 8, 8, 40))
+
+# parent grid name, seed, child, region 1's reduced grid (jred, clustsize) or None
 ZOOMS = {
-    "tiny2": ("tiny", 3, Child("tiny_child", 8, 8, -90.0, 30.0, -45.0, 45.0, 2, 2, 2, 4, 3)),
-    "small2": ("small", 5, Child("small_child", 24, 18, -120.0, 60.0, -56.25, 45.0, 2, 2, 2, 5, 4)),
-    # glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N: parent cells 4..43 x 51..85
-    "c2": ("glb3x2", 1, Child("nam1x1", 120, 70, -170.0, -50.0, 10.0, 80.0, 3, 2, 2, 4, 51)),
+    "tiny2": ("tiny", 3, _TINY_CHILD, None),
+    "small2": ("small", 5, _SMALL_CHILD, None),
+    "c2": ("glb3x2", 1, _NAM_CHILD, None),
+    "tiny2r": ("tiny", 3, _TINY_CHILD, ((1, 2, 8), (12, 4, 3))),
+    "small2r": ("small", 5, _SMALL_CHILD, ((1, 2, 3, 14, 15, 16), (24, 8, 4, 4, 8, 24))),
+    "c2r": ("glb3x2", 1, _NAM_CHILD, GLB3X2_REDGRID),
 }
 
 
@@ -398,6 +407,11 @@ class NestedMeteo:
 
 def zoom(name, nt=1):
     """(parent Meteo, NestedMeteo) of a configuration in ZOOMS."""
-    gname, seed, child = ZOOMS[name]
+    gname, seed, child, _ = ZOOMS[name]
     parent = Meteo(grid(gname), seed=seed, nt=nt)
     return parent, NestedMeteo(parent, child)
+
+
+def redgrid(name):
+    """Region 1's reduced grid (jred, clustsize) of a configuration in ZOOMS, or None."""
+    return ZOOMS[name][3]

@@ -14,15 +14,17 @@ from ..meteo import SEC_MONTH, XMCH4
 from . import driver as Z
 from . import lib as L
 from .state import ZoomState
-from .tree import Region, Zoom
+from .tree import RedGrid, Region, Zoom
 
 __all__ = ["Physics", "ZoomRun", "tree_of"]
 
 
-def tree_of(parent_grid, child):
-    """The Zoom of a global grid and one pydyn_amd.meteo.Child."""
+def tree_of(parent_grid, child, redgrid=None):
+    """The Zoom of a global grid (its x advection on the reduced grid (jred, clustsize) when
+    given) and one pydyn_amd.meteo.Child."""
     g, c = parent_grid, child
-    glb = Region("globe", g.im, g.jm, xcyc=1, touch_sp=1, touch_np=1)
+    grid = None if redgrid is None else RedGrid(g.im, g.jm, *redgrid)
+    glb = Region("globe", g.im, g.jm, xcyc=1, touch_sp=1, touch_np=1, redgrid=grid)
     kid = Region(c.name, c.im, c.jm, c.xbeg, c.xend, c.ybeg, c.yend, c.xref, c.yref, 1, c.tref, parent=1)
     zoom = Zoom([glb, kid], g.lm, g.at, g.bt)
     if (zoom.ibeg[2], zoom.jbeg[2], zoom.iend[2], zoom.jend[2]) != (c.ibeg, c.jbeg, c.iend, c.jend):
@@ -70,10 +72,10 @@ class ZoomRun:
     """The device state, inputs and clock of a two-region tree built from (parent Meteo,
     NestedMeteo)."""
 
-    def __init__(self, pmet, cmet, limits=True):
+    def __init__(self, pmet, cmet, limits=True, redgrid=None):
         g = pmet.g
         self.mets = {1: pmet, 2: cmet}
-        self.zoom = tree_of(g, cmet.c)
+        self.zoom = tree_of(g, cmet.c, redgrid)
         self.limits = limits
         self.st = ZoomState(self.zoom, pmet.nt)
         for r, met in self.mets.items():

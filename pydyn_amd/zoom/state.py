@@ -106,6 +106,14 @@ class ZoomState:
             d.dxyp = f["dxyp"].t.data_ptr()
             d.owned = self._owned[r].data_ptr()
             d.edge = self._edge[r].data_ptr()
+            grid = g.redgrid
+            if grid is not None:
+                torch = __import__("torch")
+                jred = torch.tensor(grid.jred, dtype=torch.int32, device="cuda")
+                imred = torch.tensor(grid.imred, dtype=torch.int32, device="cuda")
+                scratch = D.empty((f["m"].numel() + f["am"].numel(),))
+                self._keep = getattr(self, "_keep", []) + [jred, imred, scratch]
+                d.nred, d.jred, d.imred, d.scratch = len(grid.jred), jred.data_ptr(), imred.data_ptr(), scratch.data_ptr()
             self._desc[r] = d
         return self._desc[r]
 

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["N_OPERATORS", "SPLITORDER", "Region", "Zoom", "split_orders"]
+__all__ = ["N_OPERATORS", "SPLITORDER", "RedGrid", "Region", "Zoom", "split_orders"]
 
 SPLITORDER = "xyzvsccsvzyx"  # the model's operator order of a step (pydyn/zoom/tree.py:18)
 N_OPERATORS = len(SPLITORDER) // 2
@@ -20,6 +20,25 @@ def _nint(x):
     """Fortran nint: half away from zero."""
     r = int(np.floor(abs(x) + 0.5))
     return r if x >= 0 else -r
+
+
+class RedGrid:
+    """A reduced grid for a region's x advection (pydyn/redgrid.py:26-54): on row jred[k]
+    (1-based) clusters of clustsize[k] cells are merged, leaving imred = im / clustsize cells."""
+
+    def __init__(self, im, jm, jred, clustsize):
+        self.im, self.jm = im, jm
+        self.jred = [int(j) for j in jred]
+        self.imred = np.full(jm, im, dtype=np.int32)
+        for j, c in zip(self.jred, clustsize):
+            if im % c:
+                raise ValueError(f"redgrid: im={im} not a multiple of clustsize {c}")
+            self.imred[j - 1] = im // c
+
+    @property
+    def rows(self):
+        """The reduced rows, 0-based."""
+        return [j - 1 for j in self.jred]
 
 
 @dataclass
@@ -42,6 +61,7 @@ class Region:
     xcyc: int = 0
     touch_sp: int = 0
     touch_np: int = 0
+    redgrid: object = None  # RedGrid of the region's x advection (region 1 only), or None
 
 
 def split_orders(zoom, base, nops):
@@ -178,6 +198,8 @@ class Zoom:
                 raise NotImplementedError(f"region {r}: put_xedges would read the parent's halo")
             if self.jbeg[r] <= 1 or self.jend[r] >= p.jm:
                 raise NotImplementedError(f"region {r}: put_yedges would read the parent's halo")
+            if g.redgrid is not None:
+                raise NotImplementedError(f"region {r}: a reduced grid in a child is not supported")
 
     def scope(self, r):
         """The interior scope as 0-based slices of an (im, jm) array."""

@@ -37,6 +37,7 @@ sys.path.insert(1, str(REFERENCE))
 
 from pydyn import cfl, chem, diffusion, kernels, ops, synth  # noqa: E402
 from pydyn import zoom as Z  # noqa: E402
+from pydyn.redgrid import RedGrid  # noqa: E402
 
 from oracle import libm as oracle_libm  # noqa: E402
 from pydyn_amd import meteo  # noqa: E402
@@ -243,7 +244,7 @@ def run_golden(name, n_intervals):
 ZFIELDS = ("m", "rm", "rxm", "rym", "rzm", "am", "bm", "cm", "pu", "pv", "sp", "phlb", "dxyp")
 
 
-def reference_zoom_run(pmet, cmet, n_intervals, every=None):
+def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None):
     """A global region with one child on pydyn_amd.meteo's nested inputs through the reference's
     zoom path: the parent's footprint set to the child's sums (its update_parent), Check_CFL of
     the tree, then the steps with Setup_MassFlow; every region's `s` adds its emission (month2dt
@@ -251,7 +252,8 @@ def reference_zoom_run(pmet, cmet, n_intervals, every=None):
     scope's loss rates over ndyn / (2 tref).  every(it, t0, ndyn, n, st)."""
     pg, c, nt = pmet.g, cmet.c, pmet.nt
     lm = pg.lm
-    glb = Z.Region("globe", pg.im, pg.jm, xcyc=1, touch_sp=1, touch_np=1)
+    grid = None if redgrid is None else RedGrid(pg.im, pg.jm, *redgrid)
+    glb = Z.Region("globe", pg.im, pg.jm, xcyc=1, touch_sp=1, touch_np=1, redgrid=grid)
     kid = Z.Region(c.name, c.im, c.jm, xbeg=c.xbeg, xend=c.xend, ybeg=c.ybeg, yend=c.yend, xref=c.xref,
                    yref=c.yref, tref=c.tref, parent=1)  # fmt: skip
     zm = Z.Zoom([glb, kid], lm, pg.at, pg.bt)
@@ -325,8 +327,8 @@ def zoom_golden(name, n_intervals):
             path.write_text(json.dumps(record, indent=0))
             print(f"{name}: interval {it + 1}/{n_intervals} t={t0 + n * ndyn:.0f} ndyn={ndyn}", flush=True)
 
-    _, st = reference_zoom_run(pmet, cmet, n_intervals, every)
-    if name in ("tiny2", "small2"):
+    _, st = reference_zoom_run(pmet, cmet, n_intervals, every, meteo.redgrid(name))
+    if name in ("tiny2", "small2", "tiny2r"):
         np.savez_compressed(GOLDEN / f"zoom_{name}_final.npz",
                             **{f"r{r}.{k}": st[r][k].data for r in (1, 2) for k in ZFIELDS})  # fmt: skip
 

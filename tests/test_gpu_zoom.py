@@ -52,8 +52,8 @@ def _run(name, limit=None, oracle_every=None):
 
     ref = json.loads((GOLDEN / f"zoom_{name}_sha1.json").read_text())["intervals"][:limit]
     pmet, cmet = meteo.zoom(name)
-    run = ZoomRun(pmet, cmet)
-    orun = OZ.ZoomRun(pmet, cmet) if oracle_every else None
+    run = ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name))
+    orun = OZ.ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name)) if oracle_every else None
     for it, want in enumerate(ref):
         got_nd = run.interval()
         if orun is not None:
@@ -76,6 +76,15 @@ def test_zoom_tiny_tree_matches_reference(dev):
 
 def test_zoom_small_tree_matches_reference(dev):
     _run("small2", limit=24, oracle_every=6)
+
+
+def test_zoom_reduced_grid_tiny_tree_matches_reference(dev):
+    """Region 1's x advection on a reduced grid (rows of 1, 3 and 4 merged cells)."""
+    _run("tiny2r", oracle_every=1)
+
+
+def test_zoom_reduced_grid_small_tree_matches_reference(dev):
+    _run("small2r", limit=24, oracle_every=6)
 
 
 def test_zoom_production_tree_matches_oracle_and_reference(dev):

@@ -224,7 +224,7 @@ def _zoom_against_golden(name, limit=None):
     from pydyn_amd import meteo
 
     ref = json.loads((GOLDEN / f"zoom_{name}_sha1.json").read_text())["intervals"][:limit]
-    run = OZ.ZoomRun(*meteo.zoom(name))
+    run = OZ.ZoomRun(*meteo.zoom(name), redgrid=meteo.redgrid(name))
     for it, want in enumerate(ref):
         assert run.interval() == (want["ndyn"], want["n"]), f"{name} interval {it}"
         for key, digest in want.items():
@@ -253,3 +253,18 @@ def test_zoom_small_tree_with_cfl_cuts_bitwise():
 def test_zoom_production_tree_first_intervals_bitwise():
     """glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N (120 x 70 x 34)."""
     _zoom_against_golden("c2", limit=2)
+
+
+def test_zoom_reduced_grid_tiny_tree_bitwise():
+    """Region 1's x advection on a reduced grid (pydyn/redgrid.py), 4 simulated days."""
+    _zoom_against_golden("tiny2r")
+
+
+def test_redgrid_bands_known_answer():
+    """The reference test's production rc (tests/test_redgrid.py:25-37): glb6x4 rows 1, 2, 3, 45
+    merged 60, 20, 10, 60."""
+    from oracle.zoom import RedGrid
+
+    g = RedGrid(60, 45, (1, 2, 3, 45), (60, 20, 10, 60))
+    assert g.bands == [(0, 60, 1), (1, 20, 3), (2, 10, 6), (44, 60, 1)]
+    assert g.imred[3] == 60 and g.imred[44] == 1
