@@ -255,6 +255,8 @@ typedef struct tm5_zphysics {
     tm5_view emis;                        /* (im, jm) */
     double factor, ratio, efrac;
     int level, tracer;                    /* 1-based */
+    const double *escale;                 /* device, nt: tracer t gets ((emis escale[t]) factor) ratio) efrac where
+                                             escale[t] != 0 (an ensemble); NULL: tracer `tracer` gets it unscaled */
     tm5_view loss;                        /* (im, jm, lm) with the 4 rates at stride loss_stride */
     int64_t loss_stride;
     double dt_chem;

@@ -1360,17 +1360,18 @@ class ZoomRun:
     step) and chemistry (the region's decay over its scope, dt = ndyn / (2 tref)) as callbacks
     on every region's s and c (pydyn/forward.py:411-451 generalised to the tree)."""
 
-    def __init__(self, pmet, cmet, limits=True, redgrid=None):
+    def __init__(self, pmet, cmet, limits=True, redgrid=None, ensemble=None):
         g = pmet.g
         self.mets = {1: pmet, 2: cmet}
         self.tree = tree_of(g, cmet.c, g.lm, g.at, g.bt, redgrid)
         self.limits = limits
-        self.nt = pmet.nt
+        self.ensemble = ensemble
+        self.nt = pmet.nt if ensemble is None else ensemble.nt
         tree, lm = self.tree, g.lm
         self.st = new_state(tree, self.nt)
         for r, met in self.mets.items():
             f, gr = self.st[r], tree.g(r)
-            init = met.state()
+            init = met.state() if ensemble is None else ensemble.state(met)
             f["m"][1 : gr.im, 1 : gr.jm, 1:lm] = init["m"]
             for k in ("rm", "rxm", "rym", "rzm"):
                 f[k][1 : gr.im, 1 : gr.jm, 1:lm] = init[k]
@@ -1404,10 +1405,17 @@ class ZoomRun:
             f["pv"][1 : gr.im, 1 : gr.jm + 1, 1:lm] = pv
 
     def sources(self, ndyn):
+        """Each region's emission at level 1: of tracer 1, or with an ensemble of every member's
+        first tracer, the field scaled by the member's factor first."""
+
         def add(tree, st, r):
             factor = physics.month2dt(ndyn, SEC_MONTH, tree.g(r).tref)
-            x = physics.inject(self.emis[r], factor, XMCH4, XMCH4, 1.0)
-            sources_region(tree, st, r, [(x, 1, 1)])
+            if self.ensemble is None:
+                injections = [(physics.inject(self.emis[r], factor, XMCH4, XMCH4, 1.0), 1, 1)]
+            else:
+                injections = [(physics.inject(self.emis[r] * s, factor, XMCH4, XMCH4, 1.0), 1, t + 1)
+                              for t, s in enumerate(self.ensemble.escale()) if s != 0.0]  # fmt: skip
+            sources_region(tree, st, r, injections)
 
         return add
 

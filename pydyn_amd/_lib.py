@@ -66,6 +66,7 @@ class ZPhysics(ctypes.Structure):
         ("efrac", ctypes.c_double),
         ("level", ctypes.c_int),
         ("tracer", ctypes.c_int),
+        ("escale", ctypes.c_void_p),
         ("loss", View),
         ("loss_stride", ctypes.c_int64),
         ("dt_chem", ctypes.c_double),

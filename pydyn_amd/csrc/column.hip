@@ -247,14 +247,15 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 
     // ---- apply: the flagged sequence around v for each tracer of the column
     const bool sc = a.pre || a.twice || a.post;
-    double decay = 1.0, x = 0.0;
+    double decay = 1.0, x = 0.0, emis_ij = 0.0;
     if (sc) {
         if (lane < lm) {
             const double *L = &a.loss(i, j, lane);
             const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
             decay = fdlibm_exp(-(total * a.dt_chem));
         }
-        x = ((a.emis(i, j, 0) * a.factor) * a.ratio) * a.efrac;
+        emis_ij = a.emis(i, j, 0);
+        x = ((emis_ij * a.factor) * a.ratio) * a.efrac;
     }
     for (int t = 0; t < a.q.nt; t++) {
         const long long o = a.q.off(i, j, lane < lm ? lane : 0, t);
@@ -265,11 +266,19 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             ry = a.q.q[2][o];
             rz = a.q.q[3][o];
         }
-        const bool here = lane == a.level && t == a.tracer;
-        auto s_op = [&]() {  // do_add_2d: rm + x, rzm - x at the level
+        // do_add_2d: rm + x, rzm - x at the level (pydyn/zoom/physics.py:55-64); an ensemble member's
+        // emission field is its scale times the region's, scaled first
+        bool here = lane == a.level && t == a.tracer;
+        double xt = x;
+        if (a.escale) {
+            const double es = a.escale[t];
+            here = lane == a.level && es != 0.0;
+            xt = (((emis_ij * es) * a.factor) * a.ratio) * a.efrac;
+        }
+        auto s_op = [&]() {
             if (here) {
-                rm = rm + x;
-                rz = rz - x;
+                rm = rm + xt;
+                rz = rz - xt;
             }
         };
         auto c_op = [&]() {  // the decay of every moment

@@ -204,7 +204,8 @@ struct ColumnArgs {
     const int *owned;  // null: every column; else (im, jm) i fastest, 1 where the region acts
     View3 emis;
     double factor, ratio, efrac;
-    int level, tracer;  // 0-based
+    int level, tracer;      // 0-based
+    const double *escale;   // null, or per tracer: the emission field's scale (0: no emission)
     View3 loss;
     long long loss_sr;
     double dt_chem;

@@ -1143,6 +1143,7 @@ extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphy
     a.efrac = p->efrac;
     a.level = p->level - 1;
     a.tracer = p->tracer - 1;
+    a.escale = p->escale;
     a.loss = view3(&p->loss);
     a.loss_sr = p->loss_stride;
     a.dt_chem = p->dt_chem;

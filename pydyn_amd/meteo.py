@@ -365,6 +365,7 @@ class NestedMeteo:
         ip = (c.ibeg - 1) + np.arange(im) // xr  # parent cell of each child cell (0-based)
         jp = (c.jbeg - 1) + np.arange(jm) // yr
         cell = np.ix_(ip, jp)
+        self.cell = cell
         rad = np.pi / 180.0
         lat_f = (-90.0 + g.dy * (c.jbeg - 1) + (g.dy / yr) * np.arange(jm + 1)) * rad
         s = detmath.sin(lat_f)
@@ -415,3 +416,50 @@ def zoom(name, nt=1):
 def redgrid(name):
     """Region 1's reduced grid (jred, clustsize) of a configuration in ZOOMS, or None."""
     return ZOOMS[name][3]
+
+
+# ------------------------------------------------------------------ ensembles
+class Ensemble:
+    """`members` members of `tracers` tracers each (tracer t = member * tracers + k): member m's
+    emission scaled by scale[m], drawn uniformly in [0.5, 1.5), into its first tracer only,
+    and its initial tracers those of the meteo times (1 + 0.02 p_m), p_m a smooth field of
+    max |p| = 1 (a child takes its parent cell's value)."""
+
+    def __init__(self, parent, members, seed=11):
+        rng = np.random.default_rng(seed)
+        self.members, self.tracers = members, parent.nt
+        self.scale = rng.uniform(0.5, 1.5, members)
+        self.pert = [_smooth(rng, parent.g) for _ in range(members)]
+
+    @property
+    def nt(self):
+        return self.members * self.tracers
+
+    def subset(self, members):
+        """The ensemble of the listed members only (their plain runs)."""
+        out = object.__new__(Ensemble)
+        out.members, out.tracers = len(members), self.tracers
+        out.scale = self.scale[list(members)]
+        out.pert = [self.pert[m] for m in members]
+        return out
+
+    def escale(self):
+        """The emission scale of every tracer (zero: no emission)."""
+        out = np.zeros(self.nt)
+        out[:: self.tracers] = self.scale
+        return out
+
+    def state(self, met):
+        """met.state() with the ensemble's tracers (im, jm, lm, members * tracers)."""
+        st = met.state()
+        rm = met.rm
+        out = np.empty((*rm.shape[:3], self.nt))
+        for m, p in enumerate(self.pert):
+            field = p if not isinstance(met, NestedMeteo) else p[met.cell]
+            factor = (1.0 + 0.02 * field)[:, :, None]
+            for k in range(self.tracers):
+                out[..., m * self.tracers + k] = rm[..., k] * factor
+        st["rm"] = out
+        for k in ("rxm", "rym", "rzm"):
+            st[k] = np.zeros_like(out)
+        return st

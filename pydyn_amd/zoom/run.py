@@ -35,10 +35,11 @@ def tree_of(parent_grid, child, redgrid=None):
 class Physics:
     """Every region's column-physics inputs on the device, and their tm5_zphysics per ndyn."""
 
-    def __init__(self, zoom, st, mets, lmax):
+    def __init__(self, zoom, st, mets, lmax, escale=None):
         torch = __import__("torch")
         self.zoom, self.st, self.lmax = zoom, st, lmax
         self.arrays, self._structs = {}, {}
+        self.escale = None if escale is None else D.upload(np.asarray(escale, dtype=np.float64))
         for r, met in mets.items():
             up = D.upload
             a = {k: up(getattr(met, k)) for k in ("entu", "detu", "entd", "detd", "emis", "loss")}
@@ -61,6 +62,7 @@ class Physics:
             p.factor = half / SEC_MONTH  # month2dt (pydyn/emis.py:65-68)
             p.ratio, p.efrac = XMCH4 / XMCH4, 1.0
             p.level, p.tracer = 1, 1
+            p.escale = None if self.escale is None else self.escale.data_ptr()
             p.loss_stride = a["loss"].stride(3)
             p.dt_chem = half
             p.at, p.bt = self.st.at.data_ptr(), self.st.bt.data_ptr()
@@ -72,16 +74,21 @@ class ZoomRun:
     """The device state, inputs and clock of a two-region tree built from (parent Meteo,
     NestedMeteo)."""
 
-    def __init__(self, pmet, cmet, limits=True, redgrid=None):
+    def __init__(self, pmet, cmet, limits=True, redgrid=None, ensemble=None):
+        """ensemble: a pydyn_amd.meteo.Ensemble (its members' tracers side by side, each member's
+        emission scaled), or None (the meteo's tracers, the emission into tracer 1)."""
         g = pmet.g
         self.mets = {1: pmet, 2: cmet}
         self.zoom = tree_of(g, cmet.c, redgrid)
         self.limits = limits
-        self.st = ZoomState(self.zoom, pmet.nt)
+        self.ensemble = ensemble
+        nt = pmet.nt if ensemble is None else ensemble.nt
+        self.st = ZoomState(self.zoom, nt)
         for r, met in self.mets.items():
-            self.st.upload_interior(r, met.state(), met.dxyp if r > 1 else g.dxyp)
+            init = met.state() if ensemble is None else ensemble.state(met)
+            self.st.upload_interior(r, init, met.dxyp if r > 1 else g.dxyp)
         L.update_parent(self.zoom, self.st, 2)  # the footprint holds the child's mass from the start
-        self.phys = Physics(self.zoom, self.st, self.mets, g.lmax)
+        self.phys = Physics(self.zoom, self.st, self.mets, g.lmax, None if ensemble is None else ensemble.escale())
         self.flux0 = {r: (D.upload(met.pu0), D.upload(met.pv0)) for r, met in self.mets.items()}
         self.set_fluxes(0.0)
         Z.setup_massflow(self.zoom, self.st, 3600)

@@ -244,13 +244,14 @@ def run_golden(name, n_intervals):
 ZFIELDS = ("m", "rm", "rxm", "rym", "rzm", "am", "bm", "cm", "pu", "pv", "sp", "phlb", "dxyp")
 
 
-def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None):
+def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None, ensemble=None):
     """A global region with one child on pydyn_amd.meteo's nested inputs through the reference's
     zoom path: the parent's footprint set to the child's sums (its update_parent), Check_CFL of
     the tree, then the steps with Setup_MassFlow; every region's `s` adds its emission (month2dt
     of its half step ndyn / (2 tref)) at level 1 of tracer 1 and its `c` applies the decay of its
     scope's loss rates over ndyn / (2 tref).  every(it, t0, ndyn, n, st)."""
-    pg, c, nt = pmet.g, cmet.c, pmet.nt
+    pg, c = pmet.g, cmet.c
+    nt = pmet.nt if ensemble is None else ensemble.nt
     lm = pg.lm
     grid = None if redgrid is None else RedGrid(pg.im, pg.jm, *redgrid)
     glb = Z.Region("globe", pg.im, pg.jm, xcyc=1, touch_sp=1, touch_np=1, redgrid=grid)
@@ -263,7 +264,7 @@ def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None):
     conv, emis, loss = {}, {}, {}
     for r, met in mets.items():
         g, f = zm.g(r), st[r]
-        init = met.state()
+        init = met.state() if ensemble is None else ensemble.state(met)
         f["m"][1 : g.im, 1 : g.jm, 1:lm] = init["m"]
         for k in ("rm", "rxm", "rym", "rzm"):
             f[k][1 : g.im, 1 : g.jm, 1:lm] = init[k]
@@ -302,7 +303,13 @@ def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None):
             Z.setup_massflow(zm, st, ndyn)
 
             def sources(zoom, s_, region, ndyn=ndyn):
-                Z.sources_region(zoom, s_, region, [(_inject(emis[region], ndyn, zoom.g(region).tref), 1, 1)])
+                tref = zoom.g(region).tref
+                if ensemble is None:
+                    injections = [(_inject(emis[region], ndyn, tref), 1, 1)]
+                else:  # every member's first tracer, its emission field scaled by its factor
+                    injections = [(_inject(emis[region] * s, ndyn, tref), 1, t + 1)
+                                  for t, s in enumerate(ensemble.escale()) if s != 0.0]  # fmt: skip
+                Z.sources_region(zoom, s_, region, injections)
 
             def chemistry(zoom, s_, region, ndyn=ndyn):
                 decay = chem.decay(loss[region], ndyn / (2.0 * zoom.g(region).tref))
@@ -315,8 +322,18 @@ def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None):
     return zm, st
 
 
+# ensembles: (zoom configuration, tracers per member, members)
+ENSEMBLES = {"tiny2e": ("tiny2", 4, 4), "small2e": ("small2", 2, 3)}
+
+
 def zoom_golden(name, n_intervals):
-    pmet, cmet = meteo.zoom(name)
+    ensemble = None
+    if name in ENSEMBLES:
+        base, ntr, members = ENSEMBLES[name]
+        pmet, cmet = meteo.zoom(base, nt=ntr)
+        ensemble = meteo.Ensemble(pmet, members)
+    else:
+        pmet, cmet = meteo.zoom(name)
     path = GOLDEN / f"zoom_{name}_sha1.json"
     record = {"config": name, "intervals": []}
 
@@ -327,8 +344,9 @@ def zoom_golden(name, n_intervals):
             path.write_text(json.dumps(record, indent=0))
             print(f"{name}: interval {it + 1}/{n_intervals} t={t0 + n * ndyn:.0f} ndyn={ndyn}", flush=True)
 
-    _, st = reference_zoom_run(pmet, cmet, n_intervals, every, meteo.redgrid(name))
-    if name in ("tiny2", "small2", "tiny2r"):
+    redgrid = meteo.redgrid(ENSEMBLES[name][0] if name in ENSEMBLES else name)
+    _, st = reference_zoom_run(pmet, cmet, n_intervals, every, redgrid, ensemble)
+    if name in ("tiny2", "small2", "tiny2r", "tiny2e"):
         np.savez_compressed(GOLDEN / f"zoom_{name}_final.npz",
                             **{f"r{r}.{k}": st[r][k].data for r in (1, 2) for k in ZFIELDS})  # fmt: skip
 

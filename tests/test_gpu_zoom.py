@@ -93,3 +93,63 @@ def test_zoom_production_tree_matches_oracle_and_reference(dev):
     holds."""
     _run("c2", limit=1, oracle_every=1)
     _run("c2")
+
+
+# ------------------------------------------------------------------ ensembles (members x tracers)
+def _ensemble(base, ntr, members):
+    from pydyn_amd import meteo
+
+    pmet, cmet = meteo.zoom(base, nt=ntr)
+    return pmet, cmet, meteo.Ensemble(pmet, members)
+
+
+def test_zoom_ensemble_tiny_matches_reference(dev):
+    """4 members x 4 tracers on the tiny tree: every array against the reference's run of the
+    same ensemble (each member's emission scaled into its first tracer)."""
+    from oracle import zoom as OZ
+    from pydyn_amd.zoom.run import ZoomRun
+
+    pmet, cmet, ens = _ensemble("tiny2", 4, 4)
+    ref = json.loads((GOLDEN / "zoom_tiny2e_sha1.json").read_text())["intervals"]
+    run = ZoomRun(pmet, cmet, ensemble=ens)
+    orun = OZ.ZoomRun(pmet, cmet, ensemble=ens)
+    for it, want in enumerate(ref):
+        assert run.interval() == orun.interval() == (want["ndyn"], want["n"]), f"interval {it}"
+        host = run.host()
+        bad = [k for k in want if "." in k and sha(host[int(k[1])][k.split(".")[1]]) != want[k]]
+        if bad:
+            bad.append(_first_difference(host, orun.st))
+        assert not bad, f"tiny2e interval {it}: {bad}"
+
+
+def _member_slices_equal(batched, plain, m, ntr):
+    import torch
+
+    for r, fields in batched.st.fields.items():
+        for k in ("rm", "rxm", "rym", "rzm"):
+            a = fields[k].t[..., m * ntr : (m + 1) * ntr]
+            b = plain.st.fields[r][k].t
+            assert torch.equal(a.contiguous().view(torch.int64), b.contiguous().view(torch.int64)), (m, r, k)
+        for k in ("m", "am", "bm", "cm", "sp", "phlb"):
+            a, b = fields[k].t, plain.st.fields[r][k].t
+            assert torch.equal(a.contiguous().view(torch.int64), b.contiguous().view(torch.int64)), (m, r, k)
+
+
+@pytest.mark.parametrize(("base", "ntr", "members", "intervals", "check"), [
+    ("tiny2", 4, 4, 48, (0, 1, 2, 3)),
+    ("c2", 4, 64, 2, (0, 63)),
+])  # fmt: skip
+def test_zoom_ensemble_members_equal_their_plain_runs(dev, base, ntr, members, intervals, check):
+    """The batched form (all members' tracers side by side in one run) gives every member the bits
+    of its own plain run; at c2 the 64 members x 4 tracers of the production tree."""
+    from pydyn_amd.zoom.run import ZoomRun
+
+    pmet, cmet, ens = _ensemble(base, ntr, members)
+    batched = ZoomRun(pmet, cmet, ensemble=ens)
+    plains = {m: ZoomRun(pmet, cmet, ensemble=ens.subset([m])) for m in check}
+    for it in range(intervals):
+        nd = batched.interval()
+        for m, run in plains.items():
+            assert run.interval() == nd, (it, m)
+    for m, run in plains.items():
+        _member_slices_equal(batched, run, m, ntr)
