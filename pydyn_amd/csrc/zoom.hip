@@ -173,19 +173,49 @@ __global__ void __launch_bounds__(256) k_uncompress(ZR C, int axis, int a0, int 
 // pydyn/zoom/exchange.py:181-231: every parent-cell block (xref x yref child cells) of the
 // child's ring (edge == -1) replaced by the mean of its ring cells, accumulated from zero j-outer,
 // i-inner; blocks at the x walls only when x edges are mixed, at a y wall unless the region
-// touches that pole.  One thread per (block, level, array).
-__global__ void __launch_bounds__(256) k_mix_edges(ZR C, int xedges, int nf, int *words, int skip_mask)
+// touches that pole.  One thread per (ring block, level, array).
+// The blocks that are spread, enumerated: the x-wall columns of blocks (ib = 0 and imp-1, every
+// jb) when x edges are mixed, then the y-wall rows (jb = 0 unless the region touches the south
+// pole, jmp-1 unless it touches the north pole) over the columns the x walls did not take.
+struct RingBlocks {
+    int imp, jmp, nx, ib0, ib1, ny_s, ny_n;
+    __host__ __device__ int count() const { return nx + (ny_s + ny_n) * (ib1 - ib0 + 1); }
+    __device__ void at(int k, int &ib, int &jb) const
+    {
+        if (k < nx) {
+            jb = k % jmp;
+            ib = k < jmp ? 0 : imp - 1;
+            return;
+        }
+        k -= nx;
+        const int w = ib1 - ib0 + 1;
+        ib = ib0 + k % w;
+        jb = (k / w == 0 && ny_s) ? 0 : jmp - 1;
+    }
+};
+
+static RingBlocks ring_blocks(const ZR &C, int xedges)
+{
+    RingBlocks r;
+    r.imp = C.im / C.xr, r.jmp = C.jm / C.yr;
+    r.nx = xedges ? (r.imp > 1 ? 2 : 1) * r.jmp : 0;
+    r.ib0 = xedges ? 1 : 0;
+    r.ib1 = xedges ? r.imp - 2 : r.imp - 1;
+    r.ny_s = C.touch_sp != 1 ? 1 : 0;
+    r.ny_n = (C.touch_np != 1 && !(r.ny_s && r.jmp == 1)) ? 1 : 0;
+    if (r.ib1 < r.ib0) r.ny_s = r.ny_n = 0;
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_mix_edges(ZR C, RingBlocks ring, int nf, int *words, int skip_mask)
 {
     if (zskip(words, skip_mask)) return;
-    const int imp = C.im / C.xr, jmp = C.jm / C.yr;
-    const long long nb = (long long)imp * jmp;
+    const long long nb = ring.count();
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb * C.lm * nf) return;
-    const int ib = (int)(idx % imp), jb = (int)((idx / imp) % jmp);
+    int ib, jb;
+    ring.at((int)(idx % nb), ib, jb);
     const int l = (int)((idx / nb) % C.lm) + 1, f = (int)(idx / (nb * C.lm));
-    const bool spread = (xedges && (ib == 0 || ib == imp - 1)) || (C.touch_sp != 1 && jb == 0) ||
-                        (C.touch_np != 1 && jb == jmp - 1);
-    if (!spread) return;
     const HV &v = f == 0 ? C.m : C.q[(f - 1) / C.nt];
     const int t = f == 0 ? 1 : (f - 1) % C.nt + 1;
     const double xyr = (double)(C.xr * C.yr);
@@ -773,8 +803,10 @@ int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, i
 int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers)
 {
     const int nf = nfields(C, tracers);
-    const long long n = (long long)(C.im / C.xr) * (C.jm / C.yr) * C.lm * nf;
-    TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, xedges, nf, ctx->words,
+    const RingBlocks ring = ring_blocks(C, xedges);
+    const long long n = (long long)ring.count() * C.lm * nf;
+    if (n == 0) return 0;
+    TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, ring, nf, ctx->words,
                TM5_ERRORS | TM5_REHEARSAL * !tracers);
     return status(hipGetLastError());
 }
