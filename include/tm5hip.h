@@ -170,6 +170,10 @@ int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *desc, tm5_region **ou
 int tm5_region_destroy(tm5_region *region);
 /* One interval of nread seconds from t0: returns the (ndyn, n) Check_CFL chose. */
 int tm5_region_interval(tm5_region *region, double t0, int nread, int ndyn_max, int *ndyn, int *n);
+/* The operators of a step one by one, as tm5_region_interval runs them: 'D' Setup_MassFlow at
+   time t, 'x' 'y' 'z' advection, 'V' the middle "vsc m2phlb1 csv", 'M' m2phlb1; a step of ndyn
+   from t0 is "D" at t0 + ndyn/2, then "xyzVzyxM". */
+int tm5_region_ops(tm5_region *region, double t, int ndyn, const char *ops);
 
 
 /* ------------------------------------------------------------------ the zoom tree */
@@ -264,6 +268,20 @@ typedef struct tm5_zphysics {
 } tm5_zphysics;
 
 int tm5_zoom_physics(tm5_ctx *ctx, const tm5_zregion *r, const tm5_zphysics *p, const char *ops);
+
+/* ------------------------------------------------------------------ adjoints */
+/* The adjoint of an advection operator's tracer map (axis 0 dynamu, 1 dynamv, 2 dynamw) at the
+   forward trajectory: m, q, flux the operator's forward inputs (unchanged), lam the lambdas after
+   it, overwritten with those before it.  The limiter's branches come from the recomputed forward
+   sub-steps.  work: device scratch of (im+1)(jm+1)(lm+1) + im jm lm doubles.  No counterpart in
+   the reference (it has no adjoint); the forward operators are pydyn/kernels.py:129-346. */
+int tm5_advect_adjoint(tm5_ctx *ctx, int axis, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                       const tm5_view *flux, const tm5_tracers *lam, int limits, int max_nloop, double *work);
+/* The adjoint of a column-physics sequence ("v", "csv", "vsc", "vsccsv"/"vscMcsv") of a single
+   region on lambdas lam, with m the air mass the forward saw; the emission's lambda is added
+   into lam_emis (im, jm).  p as for tm5_zoom_physics (at, bt unused). */
+int tm5_column_adjoint(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *lam,
+                       const tm5_zphysics *p, const char *ops, const tm5_view *lam_emis);
 
 /* save / restore / scale of whole arrays (pydyn/zoom/host.py:52-71) */
 int tm5_zoom_copy(tm5_ctx *ctx, const double *src, double *dst, int64_t n);

@@ -124,6 +124,8 @@ _SIGNATURES = {
     "tm5_region_create": [_VP, _P(RegionDesc), _P(_VP)],
     "tm5_region_destroy": [_VP],
     "tm5_region_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
+    "tm5_region_ops": [_VP, _D, _I, ctypes.c_char_p],
+    "tm5_advect_adjoint": [_VP, _I, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(Tracers), _I, _I, _VP],
     "tm5_zoom_put_xedges": [_VP, _P(ZRegion), _P(ZRegion), _I],
     "tm5_zoom_put_yedges": [_VP, _P(ZRegion), _P(ZRegion), _I],
     "tm5_zoom_compress_xedges": [_VP, _P(ZRegion), _I, _I, _I, _I, _I],
@@ -142,6 +144,7 @@ _SIGNATURES = {
     "tm5_zoom_advect_z": [_VP, _P(ZRegion), _I, _I, _I, _I, _I, _I, _I],
     "tm5_zoom_physics": [_VP, _P(ZRegion), _P(ZPhysics), ctypes.c_char_p],
     "tm5_zoom_copy": [_VP, _VP, _VP, ctypes.c_int64],
+    "tm5_column_adjoint": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(ZPhysics), ctypes.c_char_p, _P(View)],
     "tm5_zoom_scale": [_VP, _VP, ctypes.c_int64, _D],
 }
 

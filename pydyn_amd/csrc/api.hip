@@ -553,3 +553,21 @@ extern "C" int tm5_region_interval(tm5_region *r, double t0, int nread, int ndyn
     if (n_out) *n_out = n;
     return 0;
 }
+
+extern "C" int tm5_region_ops(tm5_region *r, double t, int ndyn, const char *ops)
+{
+    if (!r || !ops || ndyn < 1) return TM5_EINVAL;
+    const tm5_region_desc &d = r->d;
+    for (const char *op = ops; *op; op++) {
+        switch (*op) {
+        case 'D': TRY(dynam0_at(r, t, ndyn, view3(&d.am), view3(&d.bm), view3(&d.cm))); break;
+        case 'x': case 'y': case 'z': TRY(operator_(r, *op, ndyn)); break;
+        case 'V': TRY(middle(r, ndyn)); break;
+        case 'M':
+            TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
+            break;
+        default: return TM5_EINVAL;
+        }
+    }
+    return 0;
+}

@@ -57,7 +57,9 @@ __device__ __forceinline__ void column_m2phlb1(const ColumnArgs &a, int i, int j
     for (int l = lane; l <= a.lm; l += 64) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
 }
 
-template <int N>
+// ADJ: the transposed sequence on lambdas (a.q), C^T from the columns of C each lane holds, the
+// emission's lambda accumulated into a.lam_emis.
+template <int N, bool ADJ>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
@@ -237,8 +239,8 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         for (int r = 0; r < N; r++) U[r * LD + lane] = y[r];
     }
     __syncthreads();
-    double crow[N];
-    {
+    double crow[ADJ ? 1 : N];
+    if (!ADJ) {
         const int r = lane < N ? lane : N - 1;
 #pragma unroll
         for (int k = 0; k < N; k++) crow[k] = U[r * LD + k];
@@ -257,6 +259,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         emis_ij = a.emis(i, j, 0);
         x = ((emis_ij * a.factor) * a.ratio) * a.efrac;
     }
+    double lam_e = 0.0;  // ADJ: this lane's share of the emission's lambda
     for (int t = 0; t < a.q.nt; t++) {
         const long long o = a.q.off(i, j, lane < lm ? lane : 0, t);
         double rm = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
@@ -287,10 +290,15 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             ry = ry * decay;
             rz = rz * decay;
         };
-        auto v_op = [&]() {  // rm <- C rm over levels < N, slopes times diag(C)
+        auto v_op = [&]() {  // rm <- C rm over levels < N (lane k: C^T, its column), slopes times diag(C)
             double acc = 0.0;
+            if (ADJ) {
 #pragma unroll
-            for (int k = 0; k < N; k++) acc = acc + crow[k] * bcast(rm, k);
+                for (int k = 0; k < N; k++) acc = acc + y[k] * bcast(rm, k);
+            } else {
+#pragma unroll
+                for (int k = 0; k < N; k++) acc = acc + crow[ADJ ? 0 : k] * bcast(rm, k);
+            }
             if (lane < N) {
                 rm = acc;
                 rx = diag * rx;
@@ -298,6 +306,29 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 rz = diag * rz;
             }
         };
+        if (ADJ) {  // the transposes in reverse: v^T; s^T adds (rm - rzm) times dx/demis at the level
+            const double es = a.escale ? a.escale[t] : 1.0;
+            const double kt = ((es * a.factor) * a.ratio) * a.efrac;
+            auto s_ad = [&]() {
+                if (here) lam_e = lam_e + kt * (rm - rz);
+            };
+            if (a.post) {
+                c_op();
+                s_ad();
+            }
+            if (a.twice) {
+                v_op();
+                s_ad();
+                c_op();
+                c_op();
+                s_ad();
+            }
+            v_op();
+            if (a.pre) {
+                s_ad();
+                c_op();
+            }
+        } else {
         if (a.pre) {
             c_op();
             s_op();
@@ -314,12 +345,17 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             s_op();
             c_op();
         }
+        }
         if (active && own && lane < lm) {
             a.q.q[0][o] = rm;
             a.q.q[1][o] = rx;
             a.q.q[2][o] = ry;
             a.q.q[3][o] = rz;
         }
+    }
+    if (ADJ) {
+        if (active && own && lane == a.level) a.lam_emis(i, j, 0) = a.lam_emis(i, j, 0) + lam_e;
+        return;
     }
     // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290)
     if (a.mid && active) column_m2phlb1(a, i, j, m_l, mc);
@@ -338,7 +374,12 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
                                  "k_column<20>", "k_column<21>", "k_column<22>", "k_column<23>", "k_column<24>",
                                  "k_column<25>", "k_column<26>", "k_column<27>", "k_column<28>", "k_column<29>",
                                  "k_column<30>", "k_column<31>"};
-    TM5_LAUNCH_NAMED(ctx, name[N], k_column<N>, dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
+    if (args.adjoint) {
+        TM5_LAUNCH_NAMED(ctx, "k_column_adjoint", (k_column<N, true>), dim3((ncol + 3) / 4), dim3(256), 4 * per_wave,
+                         ctx->stream, args, ctx->words, skip_mask);
+        return status(hipGetLastError());
+    }
+    TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
                        skip_mask);
     return status(hipGetLastError());
 }

@@ -211,6 +211,8 @@ struct ColumnArgs {
     double dt_chem;
     const double *dxyp, *at, *bt;
     View3 sp, phlb;
+    int adjoint;      // the transposed sequence on lambdas in q; the emission's lambda into lam_emis
+    View3 lam_emis;   // (im, jm)
 };
 bool column_supported(int n);
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,

@@ -59,7 +59,7 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
             assert int(out[f"{cname}.{fname}"]) == getattr(cls, fname).offset, f"{cname}.{fname}"
 
 
-@pytest.mark.parametrize("source", ["advect.hip", "convec.hip", "column.hip", "physics.hip", "zoom.hip", "api.hip"])
+@pytest.mark.parametrize("source", ["advect.hip", "convec.hip", "column.hip", "physics.hip", "zoom.hip", "adjoint.hip", "api.hip"])
 def test_no_contracted_fp64_fma_in_device_code(source, tmp_path):
     """Every v_fma_f64 / v_fmac_f64 in the gfx950 code belongs to a correctly rounded division
     (v_div_scale .. v_div_fmas .. v_div_fixup, five FMAs each): the kernels' own adds and
