@@ -11,7 +11,9 @@ pydyn_amd.meteo).  Every interval is bit-identical to the reference (tests/test_
 value = simulated seconds advanced / wall seconds x 3600 / (365 x 86400), the reference's
 years/hour (pydyn/gpu/bench.py:26, 262) with the ndyn Check_CFL actually chose.
 
-Also reported: the dominant kernel's roofline (its algorithmic bytes or FP64 operations per
+Also reported, in the same line: the zoom tree's rate ("zoom"), the 64-member x 4-tracer batched
+form on it against one member's plain run ("batched"), the cost of one gradient over 24 hourly
+intervals through the adjoint ("adjoint"), the dominant kernel's roofline (its algorithmic bytes or FP64 operations per
 launch over its average launch time, from HIP events the library records around each launch on
 its stream, over intervals run after the timed ones) and the oracle (NumPy, one core) timed on a
 bounded sample of the same workload on this host.
@@ -39,6 +41,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-intervals", type=int, default=24)
     p.add_argument("--profile-intervals", type=int, default=12, help="intervals timed per kernel after the run")
+    p.add_argument("--no-zoom", action="store_true", help="skip the zoom tree's rate")
+    p.add_argument("--no-batched", action="store_true", help="skip the 64-member batched rate")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window's cost")
+    p.add_argument("--members", type=int, default=64)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="end with three plain device copies of MIB MiB (a known size for rocprofv3 --pmc)")
     return p.parse_args()
@@ -135,6 +141,108 @@ def roofline(reg, intervals):
     return out
 
 
+def _timed(run, intervals, warmup=2):
+    """Wall seconds of `intervals` intervals after `warmup`, and the parent steps and simulated
+    seconds they advanced."""
+    import torch
+
+    for _ in range(warmup):
+        run.interval()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    steps = sim = 0
+    for _ in range(intervals):
+        ndyn, n = run.interval()
+        steps += n
+        sim += ndyn * n
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, steps, sim
+
+
+def zoom_section(intervals=12):
+    """The zoom tree (glb3x2x34 parent + the 1x1 degree child over 170W-50W, 10N-80N,
+    120 x 70 x 34, tref 2), one CH4 tracer: simulated years/hour."""
+    from pydyn_amd import meteo
+    from pydyn_amd.zoom.run import ZoomRun
+
+    p, c = meteo.zoom("c2")
+    run = ZoomRun(p, c)
+    wall, steps, sim = _timed(run, intervals)
+    return {
+        "workload": "glb3x2x34 + child 120x70x34 (xref 3, yref 2, tref 2), two-way exchange, 1 CH4 tracer",
+        "value": round(sim / wall * 3600.0 / SECONDS_A_YEAR, 3),
+        "unit": "simulated years/hour",
+        "ms_per_interval": round(wall / intervals * 1e3, 3),
+        "intervals": intervals,
+    }
+
+
+def batched_section(members, intervals=6):
+    """members x 4 tracers on the zoom tree in one run (each member's emission scaled, each
+    bit-identical to its own plain run) against one member's plain run: member-steps/s, a step
+    being one parent step of the tree."""
+    import torch
+
+    from pydyn_amd import meteo
+    from pydyn_amd.zoom.run import ZoomRun
+
+    p, c = meteo.zoom("c2", nt=4)
+    ens = meteo.Ensemble(p, members)
+    plain = ZoomRun(p, c, ensemble=ens.subset([0]))
+    wall_p, steps_p, _ = _timed(plain, 2 * intervals)
+    del plain
+    torch.cuda.empty_cache()
+    batched = ZoomRun(p, c, ensemble=ens)
+    wall_b, steps_b, _ = _timed(batched, intervals)
+    sequential = steps_p / wall_p  # one member's steps/s = members run one after another
+    value = steps_b * members / wall_b
+    return {
+        "workload": f"{members} members x 4 tracers on the zoom tree (nt={members * 4}), FP64",
+        "value": round(value, 2),
+        "unit": "member-steps/s",
+        "sequential_member_steps_per_s": round(sequential, 2),
+        "speedup_vs_sequential": round(value / sequential, 3),
+        "ms_per_interval": round(wall_b / intervals * 1e3, 3),
+        "intervals": intervals,
+    }
+
+
+def adjoint_section(met):
+    """The gradient of the 500-sample column-average misfit over 24 hourly intervals at
+    glb3x2x34 with respect to the emission field and the initial tracers, through the
+    torch.autograd.Function (checkpointed forward, replayed intervals, device adjoint)."""
+    import torch
+
+    from pydyn_amd import adjoint as AD
+    from pydyn_amd import device as D
+
+    window = AD.Window(met, 24)
+    i, j, obs = AD.samples(met.g)
+    st = met.state()
+    init = [D.upload(st[k]) for k in ("rm", "rxm", "rym", "rzm")]
+    emis = D.upload(met.emis)
+    times = []
+    for _ in range(2):  # the first pass warms up
+        x = [t.clone().requires_grad_(True) for t in init]
+        e = emis.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = AD.transport(window, *x, e)
+        L = AD.loss(out[0], window.m_final, i, j, obs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        times.append((t1 - t0, time.perf_counter() - t1))
+    fwd, bwd = times[-1]
+    return {
+        "workload": "24 x 3600 s of glb3x2x34, loss over 500 seeded columns, d/d(emission 120x90, initial tracers)",
+        "forward_s": round(fwd, 4),
+        "backward_s": round(bwd, 4),
+        "gradients_per_hour": round(3600.0 / (fwd + bwd), 1),
+    }
+
+
 def cpu_baseline(met, intervals):
     from oracle.driver import Run
 
@@ -204,6 +312,12 @@ def main():
         },
         "roofline": roofline(reg, a.profile_intervals),
     }
+    if not a.no_zoom:
+        line["zoom"] = zoom_section()
+    if not a.no_batched:
+        line["batched"] = batched_section(a.members)
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_section(met)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(met, a.cpu_intervals)
     print(json.dumps(line), flush=True)

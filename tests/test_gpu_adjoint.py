@@ -80,27 +80,6 @@ def test_operator_adjoint_matches_oracle(dev, axis, scaled, limits):
             assert np.array_equal(D.download(x), y)
 
 
-def test_scaled_fluxes_sub_cycle():
-    """The scaled operator cases do exercise sub-cycling on every axis."""
-    from oracle.advection import closed_search, periodic_search
-
-    met, W = oracle_window("small", 1)
-    found = {"x": 1, "y": 1, "z": 1}
-    for op, ndyn, m, flux, Q in W.ops:
-        if op not in "xyz":
-            continue
-        f = flux * SCALE[op]
-        if op == "x":
-            im = m.shape[0]
-            found["x"] = max(found["x"], int(periodic_search(m.reshape(im, -1).copy(), f.reshape(im + 1, -1), 49).max()))
-        elif op == "y":
-            for k in range(m.shape[2]):
-                found["y"] = max(found["y"], closed_search(m[:, :, k].T.copy(), f[:, :, k].T.copy(), 9, fresh=False)[0])
-        elif op == "z":
-            found["z"] = max(found["z"], closed_search(np.moveaxis(m, 2, 0).copy(), np.moveaxis(f, 2, 0), 15, fresh=True)[0])
-    assert min(found.values()) >= 2, found
-
-
 def test_column_adjoint_matches_oracle(dev):
     D, AD = dev
     print("column adjoint small", flush=True)

@@ -268,3 +268,113 @@ def test_redgrid_bands_known_answer():
     g = RedGrid(60, 45, (1, 2, 3, 45), (60, 20, 10, 60))
     assert g.bands == [(0, 60, 1), (1, 20, 3), (2, 10, 6), (44, 60, 1)]
     assert g.imred[3] == 60 and g.imred[44] == 1
+
+
+# ------------------------------------------------------------------ the tangent-linear and adjoint models
+def _window(name, intervals, limits, clipped=True, seed=3, **kw):
+    from oracle import adjoint as A
+
+    met = meteo.Meteo(meteo.grid(name), seed=seed)
+    rm = met.state()["rm"]
+    rng = np.random.default_rng(5)
+    init = [rm] + [(2.0 * rm * rng.uniform(-1.0, 1.0, rm.shape) if clipped else np.zeros_like(rm)) for _ in range(3)]
+    return met, init, A.Window(met, intervals, limits=limits, init=init, **kw)
+
+
+def _directions(met, shape, seed):
+    rng = np.random.default_rng(seed)
+    rm = met.state()["rm"]
+    return [rm * rng.standard_normal(shape) for _ in range(4)], met.emis * rng.standard_normal(met.emis.shape)
+
+
+def test_tangent_is_the_forward_when_linear():
+    """Limiter off, the tracer map is linear in (initial tracers, emission): the forward run from
+    the perturbation itself (the forward the golden vectors pin) equals the tangent."""
+    from oracle import adjoint as A
+
+    met, init, W = _window("small", 3, limits=False)
+    dx, de = _directions(met, W.final[0].shape, 21)
+    tl = W.tlm(dx, de)
+    F = A.Window(met, 3, limits=False, init=dx, emis=de).final
+    for k in range(4):
+        assert np.abs(tl[k] - F[k]).max() <= 1e-13 * np.abs(F[k]).max(), A.FIELDS[k]
+
+
+@pytest.mark.parametrize("limits", [True, False])
+@pytest.mark.parametrize("name", ["tiny", "small"])
+def test_adjoint_is_the_transpose_of_the_tangent(name, limits):
+    met, init, W = _window(name, 3, limits)
+    dx, de = _directions(met, W.final[0].shape, 22)
+    lam = [np.random.default_rng(23).standard_normal(W.final[0].shape) for _ in range(4)]
+    tl = W.tlm(dx, de)
+    ad, ae = W.adm(lam)
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(tl, lam))
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, ad)) + float(np.sum(de * ae))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_tangent_matches_finite_differences_with_the_limiter_clipping():
+    """Central differences of the forward with slopes the limiter clips: the tangent freezes the
+    limiter's branches, which small perturbations keep."""
+    from oracle import adjoint as A
+
+    met, init, W = _window("small", 3, limits=True)
+    dx, de = _directions(met, W.final[0].shape, 24)
+    clipped = sum(int((A.lim_branch(Q[1], Q[0]) != 0).sum()) for op, _, _, _, Q in W.ops if op in "xyz")
+    assert clipped > 1000
+    tl = W.tlm(dx, de)
+    eps = 1e-7
+    plus = A.Window(met, 3, init=[x + eps * d for x, d in zip(init, dx)], emis=met.emis + eps * de).final
+    minus = A.Window(met, 3, init=[x - eps * d for x, d in zip(init, dx)], emis=met.emis - eps * de).final
+    for k in range(4):
+        fd = (plus[k] - minus[k]) / (2.0 * eps)
+        assert np.abs(fd - tl[k]).max() <= 1e-6 * np.abs(tl[k]).max(), A.FIELDS[k]
+
+
+@pytest.mark.parametrize("axis", "xyz")
+def test_operator_adjoints_with_sub_cycling(axis):
+    """Each advection operator's adjoint is the transpose of its tangent when its fluxes are
+    scaled up so that it sub-cycles (x 3, y 3, z 8 times the window's fluxes)."""
+    from oracle import adjoint as A
+    from oracle.advection import closed_search, periodic_search
+
+    met, init, W = _window("small", 1, limits=True)
+    op_fn = {"x": A.x_op, "y": A.y_op, "z": A.z_op}[axis]
+    scale = {"x": 3.0, "y": 3.0, "z": 8.0}[axis]
+    rng = np.random.default_rng(25)
+    first_seen = False
+    for op, _, m, flux, Q in W.ops:
+        if op != axis:
+            continue
+        f = flux * scale
+        if axis == "x":
+            n = int(periodic_search(m.reshape(m.shape[0], -1).copy(), f.reshape(m.shape[0] + 1, -1).copy(), 49).max())
+        elif axis == "y":
+            n = max(closed_search(m[:, :, k].T.copy(), f[:, :, k].T.copy(), 9, fresh=False)[0] for k in range(m.shape[2]))
+        else:
+            n = closed_search(np.moveaxis(m, 2, 0).copy(), np.moveaxis(f, 2, 0).copy(), 15, fresh=True)[0]
+        assert n >= 2
+        d = [rng.standard_normal(Q[0].shape) for _ in range(4)]
+        lam = [rng.standard_normal(Q[0].shape) for _ in range(4)]
+        tl = op_fn(m, f, Q, d, True, "tl")
+        ad = op_fn(m, f, Q, lam, True, "ad")
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(tl, lam))
+        rhs = sum(float(np.sum(a * b)) for a, b in zip(d, ad))
+        assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+        # and the tangent against central differences of the forward operator, on the step's
+        # first operator of the axis: later ones can be handed slopes exactly at the limiter's
+        # bound (a clip carried through physics that scales s and rm alike), a kink where the
+        # one-sided derivatives differ and the tangent takes the forward's branch
+        if first_seen:
+            continue
+        first_seen = True
+        forward = {"x": advection.advect_x, "y": advection.advect_y, "z": advection.advect_z}[axis]
+        eps = 1e-6 * float(np.abs(Q[0]).max())
+        out = []
+        for sign in (1.0, -1.0):
+            st = [m.copy(), *(q + sign * eps * x for q, x in zip(Q, d)), f.copy()]
+            forward(*st)
+            out.append(st[1:5])
+        for k in range(4):
+            fd = (out[0][k] - out[1][k]) / (2.0 * eps)
+            assert np.abs(fd - tl[k]).max() <= 1e-6 * np.abs(tl[k]).max(), (axis, k)
