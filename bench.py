@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true", help="skip the 64-member batched rate")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window's cost")
     p.add_argument("--members", type=int, default=64)
+    p.add_argument("--only", choices=("zoom", "batched", "adjoint"),
+                   help="run that section alone and print its object (for profiling one workload)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="end with three plain device copies of MIB MiB (a known size for rocprofv3 --pmc)")
     return p.parse_args()
@@ -276,6 +278,11 @@ def main():
     from pydyn_amd.region import Region
 
     met = meteo.Meteo(meteo.grid(a.grid), seed=1)
+    if a.only:
+        out = {"zoom": zoom_section, "batched": lambda: batched_section(a.members),
+               "adjoint": lambda: adjoint_section(met)}[a.only]()
+        print(json.dumps({a.only: out}), flush=True)
+        return 0
     reg = Region(met)
     for _ in range(a.warmup):
         reg.interval()

@@ -7,17 +7,22 @@
 // "vscMcsv" (M = m2phlb1) to every tracer of the column in one pass; in a zoom tree only on the
 // columns the region owns.
 //
-// One column per 64-lane wavefront, N = lmax known at compile time so that the matrix lives in
-// registers:
+// Two columns per 64-lane wavefront: lanes 0-31 hold one column, lanes 32-63 the next (half
+// h = lane / 32, lane li = lane % 32 of the half), N = lmax known at compile time so that the
+// matrix lives in registers:
 //   build      lane c owns matrix column c (the up/downdraft recursions, pydyn/kernels.py:350-396),
-//              through LDS;
-//   LU         lane r holds row r; right-looking Crout without pivoting, each element updated in
-//              increasing k as in the reference's left-looking loops (pydyn/kernels.py:404-407);
+//              in registers, then written to the half's LDS matrix;
+//   LU         lane r holds row r (read back from LDS); right-looking Crout without pivoting,
+//              each element updated in increasing k as in the reference's left-looking loops
+//              (pydyn/kernels.py:404-407);
 //   inverse    lane c solves for the right-hand side e_c (forward, then back substitution in the
 //              reference's order, pydyn/kernels.py:409-419);
 //   apply      lane l holds row l of C: rm(l) = sum_k C(l,k) rm(k) from zero in increasing k,
 //              slopes times C(l,l) (pydyn/kernels.py:448-457).
-// Uniform values (a pivot, an L or U entry) come from the owning lane by readlane.
+// A value uniform over a column (a pivot row, an L or U entry, the column's rm) is written to the
+// half's LDS once and read back by every lane of the half: one LDS read serves both halves.
+// Levels 32 and up (lm <= 64) sit in a second register set of the lane, where only the sources
+// and the chemistry act (N < 32).
 //
 // The matrix of a column with cloud top T and level of free sinking F is dense in its leading
 // DB x DB block (DB = max(T, F) + 1) and tridiagonal outside it; the entries outside are exact
@@ -25,7 +30,9 @@
 // so the products outside it all have a +-0 factor and are skipped: subtracting +-0 leaves
 // every non-zero value as it is and can only change the sign of a zero, and no zero's sign
 // reaches a result (zeros only ever enter products, sums with non-zero terms, or numerators),
-// so the outputs are the reference's bits for any non-singular matrix of finite entries.
+// so the outputs are the reference's bits for any non-singular matrix of finite entries.  A
+// wave works to the larger envelope of its two columns: the other column's extra products have
+// a +-0 factor too.
 #include "tm5_lib.h"
 
 namespace tm5 {
@@ -35,247 +42,293 @@ __device__ __forceinline__ bool skipped_k(const int *words, int mask)
     return (words[W_FLAGS] & mask) != 0;  // written by earlier kernels of the stream
 }
 
-__device__ __forceinline__ double bcast(double v, int lane)
+// Lanes of one wavefront exchange values through LDS: the wavefront's LDS operations are
+// performed in program order, the wait lets its writes land before the reads that follow, and
+// the fences keep the compiler from moving LDS accesses across this point.
+__device__ __forceinline__ void wsync()
 {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-
-constexpr int VEC = 64 + 13 * 32;  // per-level vectors of a wave's LDS
-
-// m2phlb1 of one column (pydyn/zoom/massflow.py:24-32): every lane holds m of its level in m_l
-__device__ __forceinline__ void column_m2phlb1(const ColumnArgs &a, int i, int j, double m_l, const double *mc)
+// the larger of a value over the two halves of the wavefront (uniform)
+__device__ __forceinline__ int wave_max2(int v)
 {
-    double p = bcast(m_l, 0);
-    for (int l = 1; l < a.lm; l++) p = p + mc[l];
-    p = p * 9.80665 / a.dxyp[j];
-    const int lane = threadIdx.x & 63;
-    if (lane == 0) a.sp(i, j, 0) = p;
-    for (int l = lane; l <= a.lm; l += 64) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 32);
+    return a > b ? a : b;
 }
 
-// ADJ: the transposed sequence on lambdas (a.q), C^T from the columns of C each lane holds, the
-// emission's lambda accumulated into a.lam_emis.
+template <int N> struct ColumnLds {
+    static constexpr int LD = (N % 2 == 0) ? N + 1 : N;  // odd: a row read by every lane spreads over the banks
+    static constexpr int BUILD = 11 * 32;                  // the build's per-level vectors (share the matrix's space)
+    static constexpr int R0 = N * LD > BUILD ? N * LD : BUILD;
+    // per half: matrix / build vectors, the column's m at every level, the broadcast slot of rm;
+    // a multiple of 32 plus 8 doubles, so the two halves' broadcast reads fall in different banks
+    static constexpr int S = ((R0 + 96 + 31) / 32) * 32 + 8;
+};
+
 template <int N, bool ADJ>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
-    constexpr int LD = (N % 2 == 0) ? N + 1 : N;
+    constexpr int LD = ColumnLds<N>::LD, S = ColumnLds<N>::S;
+    static_assert(N < 32, "a half-wave holds the convective levels");
     extern __shared__ double lds[];
-    // the column is uniform over the wavefront: said so, the branches on it are scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int h = lane >> 5, li = lane & 31, l1 = li + 32;
     const int ncol = a.im * a.jm;
-    int col = xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
+    int col = (xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * 2 + h;
     const bool active = col < ncol;
     if (!active) col = ncol - 1;
     const int i = col % a.im, j = col / a.im;
     const int lm = a.lm;
-    // per wave: fu / conv1 / C^T (N+1)*LD, fd (N+1)*LD, vectors
-    static_assert(N < 32, "the per-level vectors hold 32 levels");
-    double *U = lds + (size_t)wave * (2 * (N + 1) * LD + VEC);
-    double *D = U + (N + 1) * LD;
-    double *vec = D + (N + 1) * LD;
-    double *mc = vec;  // all lm levels (m2phlb1)
-    double *eu = vec + 64, *du = eu + 32, *ed = du + 32, *dd = ed + 32, *dk = dd + 32;
-    double *amu = dk + 32, *amd = amu + 32, *eum = amd + 32, *edm = eum + 32;
-    double *cu = edm + 32, *cd = cu + 32, *cdl = cd + 32, *cdk = cdl + 32;
+    double *H = lds + (size_t)(wave * 2 + h) * S;
+    double *Mt = H;  // N x LD: conv1, then L\U, then C (column c written by lane c)
+    double *eu = H, *du = H + 32, *ed = H + 64, *dd = H + 96, *dk = H + 128, *eum = H + 160, *edm = H + 192;
+    double *amu = H + 224, *amd = H + 256, *cd = H + 288, *cdl = H + 320;
+    double *mc = H + ColumnLds<N>::R0;  // m at all lm levels (m2phlb1)
+    double *rv = mc + 64;               // the column's rm by level, for v
 
-    const double m_l = lane < lm ? a.m(i, j, lane) : 0.0;
-    mc[lane] = m_l;
-    // a column the region does not own is computed like the others (every wave reaches every
-    // barrier) but keeps its tracers; the middle m2phlb1 is every column's
+    const double m_l = li < lm ? a.m(i, j, li) : 0.0;
+    mc[li] = m_l;
+    mc[l1] = l1 < lm ? a.m(i, j, l1) : 0.0;
+    // a column the region does not own is computed like the others but keeps its tracers; the
+    // middle m2phlb1 is every column's
     const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
-    for (int x = lane; x < (N + 1) * LD; x += 64) U[x] = D[x] = 0.0;
-    if (lane < 32) amu[lane] = amd[lane] = 0.0;
-    __syncthreads();
-    if (lane < N) {
-        const double e_u = a.entu(i, j, lane), e_d = a.entd(i, j, lane);
-        eu[lane] = e_u;
-        du[lane] = a.detu(i, j, lane);
-        ed[lane] = e_d;
-        dd[lane] = a.detd(i, j, lane);
-        dk[lane] = a.dkg(i, j, lane);
-        eum[lane] = e_u / m_l;   // the fu(k, k-1) and fd(k-1, k-1) quotients, every level at once
-        edm[lane] = -e_d / m_l;
+    amu[li] = amd[li] = 0.0;
+    if (li < N) {
+        const double e_u = a.entu(i, j, li), e_d = a.entd(i, j, li);
+        eu[li] = e_u;
+        du[li] = a.detu(i, j, li);
+        ed[li] = e_d;
+        dd[li] = a.detd(i, j, li);
+        dk[li] = a.dkg(i, j, li);
+        eum[li] = e_u / m_l;  // the fu(k, k-1) and fd(k-1, k-1) quotients, every level at once
+        edm[li] = -e_d / m_l;
     }
-    __syncthreads();
+    wsync();
 
     // ---- build conv1: lane c owns column c (pydyn/kernels.py:366-395)
-    const int ctop = __builtin_amdgcn_readfirstlane(a.top(i, j)), clfs = __builtin_amdgcn_readfirstlane(a.lfs(i, j));
+    const int ctop = a.top(i, j), clfs = a.lfs(i, j);
     // levels above the cloud top (updraft) or the level of free sinking (downdraft) contribute
-    // exact zeros (the reference's where(act, .., 0.0)): their iterations are skipped
+    // exact zeros (the reference's where(act, .., 0.0)); a wave runs to the larger of its two
+    // columns' levels, the other column's extra levels giving the same zeros
     const int kup = ctop < N ? ctop : N, kdn = clfs < N ? clfs : N;
-    // the dense leading block of conv1; outside it the matrix is tridiagonal
-    const int DB = __builtin_amdgcn_readfirstlane((kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N);
-    {
+    const int db = (kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N;
+    const int KUP = wave_max2(kup), KDN = wave_max2(kdn), DB = wave_max2(db);
+    double colv[N + 1];  // rows 0..N of this lane's column of the flux matrix; rows 0 and N stay zero
+#pragma unroll
+    for (int k = 0; k <= N; k++) colv[k] = 0.0;
+    {  // downdraft rows k-1 for k = kdn..2
         double run = 0.0, prev = 0.0;
-        for (int k = 1; k <= kup; k++) {
-            const bool on = ctop >= k;
-            const double inflow = prev + eu[k - 1];
-            const double aa = inflow - du[k - 1];
-            const bool rising = aa > 0.0;
-            const double keep = rising ? max0(1.0 - du[k - 1] / inflow) : 0.0;
-            const double amu_k = (on && rising) ? aa : 0.0;
-            if (lane < N) {
+#pragma unroll
+        for (int k = N; k >= 2; k--) {
+            if (k <= KDN) {
+                const bool on = clfs >= k;
+                const double edk = ed[k - 1], ddk = dd[k - 1];
+                const double inflow = prev - edk;
+                const double aa = inflow + ddk;
+                const bool sinking = aa < 0.0;
+                const double keep = sinking ? max0(1.0 + ddk / inflow) : 0.0;
+                const double amd_k = (on && sinking) ? aa : 0.0;
                 double v = 0.0;
-                if (lane < k - 1)
+                if (li >= k)
                     v = on ? run * keep : 0.0;
-                else if (lane == k - 1)
-                    v = on ? eum[k - 1] * keep : 0.0;
-                U[k * LD + lane] = v;
-                run = v;
-            }
-            if (lane == 0) amu[k] = amu_k;
-            prev = amu_k;
-        }
-    }
-    {
-        double run = 0.0, prev = 0.0;
-        for (int k = kdn; k >= 2; k--) {
-            const bool on = clfs >= k;
-            const double inflow = prev - ed[k - 1];
-            const double aa = inflow + dd[k - 1];
-            const bool sinking = aa < 0.0;
-            const double keep = sinking ? max0(1.0 + dd[k - 1] / inflow) : 0.0;
-            const double amd_k = (on && sinking) ? aa : 0.0;
-            if (lane < N) {
-                double v = 0.0;
-                if (lane >= k)
-                    v = on ? run * keep : 0.0;
-                else if (lane == k - 1)
+                else if (li == k - 1)
                     v = on ? edm[k - 1] * keep : 0.0;
-                D[(k - 1) * LD + lane] = v;
+                colv[k - 1] = v;
                 run = v;
+                if (li == 0) amd[k - 1] = amd_k;
+                prev = amd_k;
             }
-            if (lane == 0) amd[k - 1] = amd_k;
-            prev = amd_k;
         }
     }
-    __syncthreads();
-    // the four corrections of interface k (pydyn/kernels.py:388-391), every k at once in lane k
+    {  // updraft rows k = 1..kup, added in front: row k is fu + fd
+        double run = 0.0, prev = 0.0;
+#pragma unroll
+        for (int k = 1; k <= N; k++) {
+            double v = 0.0;
+            if (k <= KUP) {
+                const bool on = ctop >= k;
+                const double euk = eu[k - 1], duk = du[k - 1];
+                const double inflow = prev + euk;
+                const double aa = inflow - duk;
+                const bool rising = aa > 0.0;
+                const double keep = rising ? max0(1.0 - duk / inflow) : 0.0;
+                const double amu_k = (on && rising) ? aa : 0.0;
+                if (li < k - 1)
+                    v = on ? run * keep : 0.0;
+                else if (li == k - 1)
+                    v = on ? eum[k - 1] * keep : 0.0;
+                run = v;
+                if (li == 0) amu[k] = amu_k;
+                prev = amu_k;
+            }
+            if (k < N) colv[k] = v + colv[k];
+        }
+    }
+    wsync();
+    // the four corrections of interface k (pydyn/kernels.py:388-391), every k at once in lane k:
+    // lane k takes -cu(k) then -cdk(k) on row k; lane k-1 takes -cd(k) then +cdl(k)
+    double q0 = 0.0, q3 = 0.0;
+    if (li >= 1 && li < N) {
+        q0 = amu[li] / mc[li];
+        cd[li] = amd[li] / mc[li - 1];
+        cdl[li] = dk[li - 1] / mc[li - 1];
+        q3 = dk[li - 1] / mc[li];
+    }
+    wsync();
     {
-        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
-        if (lane >= 1 && lane < N) {
-            q0 = amu[lane] / mc[lane];
-            q1 = amd[lane] / mc[lane - 1];
-            q2 = dk[lane - 1] / mc[lane - 1];
-            q3 = dk[lane - 1] / mc[lane];
-        }
-        if (lane >= 1 && lane < N) {
-            cu[lane] = q0;
-            cd[lane] = q1;
-            cdl[lane] = q2;
-            cdk[lane] = q3;
-        }
-        __syncthreads();
-    }
-    if (lane < N) {
+        const double cdn = li + 1 < N ? cd[li + 1] : 0.0, cln = li + 1 < N ? cdl[li + 1] : 0.0;
+#pragma unroll
         for (int k = 1; k < N; k++) {
-            double v = U[k * LD + lane] + D[k * LD + lane];
-            if (lane == k) v = v - cu[k];
-            if (lane == k - 1) v = v - cd[k];
-            if (lane == k - 1) v = v + cdl[k];
-            if (lane == k) v = v - cdk[k];
-            D[k * LD + lane] = v;
+            if (li == k) colv[k] = (colv[k] - q0) - q3;
+            if (li + 1 == k) colv[k] = (colv[k] - cdn) + cln;
         }
     }
-    __syncthreads();
+    wsync();  // the build vectors are read: the matrix takes their space
+    if (li < N) {
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            double v = (-a.dt) * (colv[r] - colv[r + 1]);
+            if (r == li) v = v + 1.0;
+            Mt[r * LD + li] = v;
+        }
+    }
+    wsync();
     // conv1 row r into lane r's registers
     double A[N];
+    {
+        const int r = li < N ? li : N - 1;
 #pragma unroll
-    for (int c = 0; c < N; c++) {
-        const int r = lane < N ? lane : N - 1;
-        double v = (-a.dt) * (D[r * LD + c] - D[(r + 1) * LD + c]);
-        if (r == c) v = v + 1.0;
-        A[c] = v;
+        for (int c = 0; c < N; c++) A[c] = Mt[r * LD + c];
     }
 
     // ---- LU, lanes over rows.  Entry (r, c) is non-zero only inside the dense block (r, c < DB)
     // or on the three diagonals, and LU fill-in stays inside that envelope; the products skipped
-    // outside it have a factor that is exactly +-0.
-    // Rows r <= k take the update with a zero multiplier: r - (+-0) is r up to a zero's sign.
+    // outside it have a factor that is exactly +-0.  Rows r <= k take the update with a zero
+    // multiplier: r - (+-0) is r up to a zero's sign.  Row k, final after step k-1, goes to LDS
+    // for the step (it becomes the U part of the factors there).
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        const double piv = bcast(A[k], k);
-        const double q = A[k] / piv;
-        A[k] = lane > k ? q : A[k];
-        const double lrk = lane > k ? q : 0.0;
-        if (k + 1 < N) A[k + 1] = A[k + 1] - lrk * bcast(A[k + 1], k);
+        if (li == k) {
 #pragma unroll
-        for (int c = k + 2; c < N; c++) {
-            if (c < DB) A[c] = A[c] - lrk * bcast(A[c], k);
+            for (int c = k; c < N; c++) Mt[k * LD + c] = A[c];
+        }
+        wsync();
+        double prow[N];  // row k, entries k.. (the loads issue together, the updates wait on them)
+#pragma unroll
+        for (int c = k; c < N; c++) prow[c] = Mt[k * LD + c];
+        const double q = A[k] / prow[k];
+        A[k] = li > k ? q : A[k];
+        const double lrk = li > k ? q : 0.0;
+        if (k + 1 < N) A[k + 1] = A[k + 1] - lrk * prow[k + 1];
+        // the dense block in chunks of four columns (a chunk's columns past DB have a +-0 factor)
+#pragma unroll
+        for (int c0 = k + 2; c0 < N; c0 += 4) {
+            if (c0 < DB) {
+#pragma unroll
+                for (int c = c0; c < c0 + 4 && c < N; c++) A[c] = A[c] - lrk * prow[c];
+            }
         }
     }
+    // the L part of row r
+    if (li < N) {
+#pragma unroll
+        for (int c = 0; c + 1 < N; c++)
+            if (c < li) Mt[li * LD + c] = A[c];
+    }
+    wsync();
 
     // ---- inverse: lane c solves L U x = e_c
     double y[N];
 #pragma unroll
-    for (int r = 0; r < N; r++) y[r] = (lane == r) ? 1.0 : 0.0;
+    for (int r = 0; r < N; r++) y[r] = (li == r) ? 1.0 : 0.0;
 #pragma unroll
     for (int r = 1; r < N; r++) {
+        double lrow[N];  // row r of L (loaded whole: the loads do not wait on the branch)
+#pragma unroll
+        for (int s = 0; s < r; s++) lrow[s] = Mt[r * LD + s];
         if (r < DB) {
 #pragma unroll
-            for (int s = 0; s < r - 1; s++) y[r] = y[r] - bcast(A[s], r) * y[s];
+            for (int s = 0; s < r - 1; s++) y[r] = y[r] - lrow[s] * y[s];
         }
-        y[r] = y[r] - bcast(A[r - 1], r) * y[r - 1];
+        y[r] = y[r] - lrow[r - 1] * y[r - 1];
     }
 #pragma unroll
     for (int r = N - 1; r >= 0; r--) {
+        double urow[N];  // row r of U, entries r..
+#pragma unroll
+        for (int s = r; s < N; s++) urow[s] = Mt[r * LD + s];
         double acc = y[r];
-        if (r + 1 < N) acc = acc - bcast(A[r + 1], r) * y[r + 1];
+        if (r + 1 < N) acc = acc - urow[r + 1] * y[r + 1];
 #pragma unroll
-        for (int s = r + 2; s < N; s++) {
-            if (s < DB) acc = acc - bcast(A[s], r) * y[s];
+        for (int s0 = r + 2; s0 < N; s0 += 4) {
+            if (s0 < DB) {
+#pragma unroll
+                for (int s = s0; s < s0 + 4 && s < N; s++) acc = acc - urow[s] * y[s];
+            }
         }
-        y[r] = acc / bcast(A[r], r);
+        y[r] = acc / urow[r];
     }
-    // C^T through LDS: lane c wrote column c; lane l reads row l
-    __syncthreads();
-    if (lane < N) {
+    // C through LDS: lane c writes column c; lane l reads row l
+    wsync();
+    if (li < N) {
 #pragma unroll
-        for (int r = 0; r < N; r++) U[r * LD + lane] = y[r];
+        for (int r = 0; r < N; r++) Mt[r * LD + li] = y[r];
     }
-    __syncthreads();
+    wsync();
     double crow[ADJ ? 1 : N];
     if (!ADJ) {
-        const int r = lane < N ? lane : N - 1;
+        const int r = li < N ? li : N - 1;
 #pragma unroll
-        for (int k = 0; k < N; k++) crow[k] = U[r * LD + k];
+        for (int k = 0; k < N; k++) crow[k] = Mt[r * LD + k];
     }
-    const double diag = lane < N ? U[lane * LD + lane] : 0.0;
+    const double diag = li < N ? Mt[li * LD + li] : 0.0;
 
-    // ---- apply: the flagged sequence around v for each tracer of the column
+    // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
     const bool sc = a.pre || a.twice || a.post;
-    double decay = 1.0, x = 0.0, emis_ij = 0.0;
+    double decay = 1.0, decay1 = 1.0, x = 0.0, emis_ij = 0.0;
     if (sc) {
-        if (lane < lm) {
-            const double *L = &a.loss(i, j, lane);
+        if (li < lm) {
+            const double *L = &a.loss(i, j, li);
             const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
             decay = fdlibm_exp(-(total * a.dt_chem));
+        }
+        if (l1 < lm) {
+            const double *L = &a.loss(i, j, l1);
+            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
+            decay1 = fdlibm_exp(-(total * a.dt_chem));
         }
         emis_ij = a.emis(i, j, 0);
         x = ((emis_ij * a.factor) * a.ratio) * a.efrac;
     }
-    double lam_e = 0.0;  // ADJ: this lane's share of the emission's lambda
+    double lam_e = 0.0;  // ADJ: the emission's lambda from this lane's level
     for (int t = 0; t < a.q.nt; t++) {
-        const long long o = a.q.off(i, j, lane < lm ? lane : 0, t);
-        double rm = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-        if (lane < lm) {
+        const long long o = a.q.off(i, j, li < lm ? li : 0, t), o1 = a.q.off(i, j, l1 < lm ? l1 : 0, t);
+        double rm = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rm1 = 0.0, rx1 = 0.0, ry1 = 0.0, rz1 = 0.0;
+        if (li < lm) {
             rm = a.q.q[0][o];
             rx = a.q.q[1][o];
             ry = a.q.q[2][o];
             rz = a.q.q[3][o];
         }
+        if (l1 < lm) {
+            rm1 = a.q.q[0][o1];
+            rx1 = a.q.q[1][o1];
+            ry1 = a.q.q[2][o1];
+            rz1 = a.q.q[3][o1];
+        }
         // do_add_2d: rm + x, rzm - x at the level (pydyn/zoom/physics.py:55-64); an ensemble member's
         // emission field is its scale times the region's, scaled first
-        bool here = lane == a.level && t == a.tracer;
-        double xt = x;
+        bool here = li == a.level && t == a.tracer, here1 = l1 == a.level && t == a.tracer;
+        double xt = x, es = 1.0;
         if (a.escale) {
-            const double es = a.escale[t];
-            here = lane == a.level && es != 0.0;
+            es = a.escale[t];
+            here = li == a.level && es != 0.0;
+            here1 = l1 == a.level && es != 0.0;
             xt = (((emis_ij * es) * a.factor) * a.ratio) * a.efrac;
         }
         auto s_op = [&]() {
@@ -283,23 +336,29 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 rm = rm + xt;
                 rz = rz - xt;
             }
+            if (here1) {
+                rm1 = rm1 + xt;
+                rz1 = rz1 - xt;
+            }
         };
         auto c_op = [&]() {  // the decay of every moment
             rm = rm * decay;
             rx = rx * decay;
             ry = ry * decay;
             rz = rz * decay;
+            rm1 = rm1 * decay1;
+            rx1 = rx1 * decay1;
+            ry1 = ry1 * decay1;
+            rz1 = rz1 * decay1;
         };
-        auto v_op = [&]() {  // rm <- C rm over levels < N (lane k: C^T, its column), slopes times diag(C)
+        auto v_op = [&]() {  // rm <- C rm over levels < N (ADJ: C^T, lane k holds column k), slopes times diag(C)
+            wsync();
+            rv[li] = rm;
+            wsync();
             double acc = 0.0;
-            if (ADJ) {
 #pragma unroll
-                for (int k = 0; k < N; k++) acc = acc + y[k] * bcast(rm, k);
-            } else {
-#pragma unroll
-                for (int k = 0; k < N; k++) acc = acc + crow[ADJ ? 0 : k] * bcast(rm, k);
-            }
-            if (lane < N) {
+            for (int k = 0; k < N; k++) acc = acc + (ADJ ? y[k] : crow[k]) * rv[k];
+            if (li < N) {
                 rm = acc;
                 rx = diag * rx;
                 ry = diag * ry;
@@ -307,10 +366,10 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             }
         };
         if (ADJ) {  // the transposes in reverse: v^T; s^T adds (rm - rzm) times dx/demis at the level
-            const double es = a.escale ? a.escale[t] : 1.0;
             const double kt = ((es * a.factor) * a.ratio) * a.efrac;
             auto s_ad = [&]() {
                 if (here) lam_e = lam_e + kt * (rm - rz);
+                if (here1) lam_e = lam_e + kt * (rm1 - rz1);
             };
             if (a.post) {
                 c_op();
@@ -329,44 +388,59 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 c_op();
             }
         } else {
-        if (a.pre) {
-            c_op();
-            s_op();
-        }
-        v_op();
-        if (a.twice) {
-            s_op();
-            c_op();
-            c_op();
-            s_op();
+            if (a.pre) {
+                c_op();
+                s_op();
+            }
             v_op();
+            if (a.twice) {
+                s_op();
+                c_op();
+                c_op();
+                s_op();
+                v_op();
+            }
+            if (a.post) {
+                s_op();
+                c_op();
+            }
         }
-        if (a.post) {
-            s_op();
-            c_op();
-        }
-        }
-        if (active && own && lane < lm) {
-            a.q.q[0][o] = rm;
-            a.q.q[1][o] = rx;
-            a.q.q[2][o] = ry;
-            a.q.q[3][o] = rz;
+        if (active && own) {
+            if (li < lm) {
+                a.q.q[0][o] = rm;
+                a.q.q[1][o] = rx;
+                a.q.q[2][o] = ry;
+                a.q.q[3][o] = rz;
+            }
+            if (l1 < lm) {
+                a.q.q[0][o1] = rm1;
+                a.q.q[1][o1] = rx1;
+                a.q.q[2][o1] = ry1;
+                a.q.q[3][o1] = rz1;
+            }
         }
     }
     if (ADJ) {
-        if (active && own && lane == a.level) a.lam_emis(i, j, 0) = a.lam_emis(i, j, 0) + lam_e;
+        if (active && own && (li == a.level || l1 == a.level)) a.lam_emis(i, j, 0) = a.lam_emis(i, j, 0) + lam_e;
         return;
     }
-    // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290)
-    if (a.mid && active) column_m2phlb1(a, i, j, m_l, mc);
+    // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290, 24-32): sp from
+    // the column's mass summed upward, phlb = at + bt sp
+    if (a.mid && active) {
+        double p = mc[0];
+        for (int l = 1; l < lm; l++) p = p + mc[l];
+        p = p * 9.80665 / a.dxyp[j];
+        if (li == 0) a.sp(i, j, 0) = p;
+        for (int l = li; l <= lm; l += 32) a.phlb(i, j, l) = a.at[l] + a.bt[l] * p;
+    }
 }
 
 template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
 {
-    constexpr int LD = (N % 2 == 0) ? N + 1 : N;
-    const size_t per_wave = (size_t)(2 * (N + 1) * LD + VEC) * sizeof(double);
+    const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
     const int ncol = args.im * args.jm;
+    const dim3 grid((ncol + 7) / 8), block(256);
     static const char *name[] = {"k_column<0>", "k_column<1>", "k_column<2>", "k_column<3>", "k_column<4>",
                                  "k_column<5>", "k_column<6>", "k_column<7>", "k_column<8>", "k_column<9>",
                                  "k_column<10>", "k_column<11>", "k_column<12>", "k_column<13>", "k_column<14>",
@@ -374,18 +448,17 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
                                  "k_column<20>", "k_column<21>", "k_column<22>", "k_column<23>", "k_column<24>",
                                  "k_column<25>", "k_column<26>", "k_column<27>", "k_column<28>", "k_column<29>",
                                  "k_column<30>", "k_column<31>"};
-    if (args.adjoint) {
-        TM5_LAUNCH_NAMED(ctx, "k_column_adjoint", (k_column<N, true>), dim3((ncol + 3) / 4), dim3(256), 4 * per_wave,
-                         ctx->stream, args, ctx->words, skip_mask);
-        return status(hipGetLastError());
-    }
-    TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), dim3((ncol + 3) / 4), dim3(256), 4 * per_wave, ctx->stream, args, ctx->words,
-                       skip_mask);
+    if (args.adjoint)
+        TM5_LAUNCH_NAMED(ctx, "k_column_adjoint", (k_column<N, true>), grid, block, lds, ctx->stream, args, ctx->words,
+                         skip_mask);
+    else
+        TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), grid, block, lds, ctx->stream, args, ctx->words,
+                         skip_mask);
     return status(hipGetLastError());
 }
 
-// the lmax values compiled in register form (a wave's LDS stays under 16 KB, four waves a
-// workgroup); others go through the LDS kernel (convec.hip)
+// the lmax values compiled in register form (two columns a wave, four waves a workgroup); others
+// go through the LDS kernel (convec.hip)
 bool column_supported(int n)
 {
     switch (n) {
