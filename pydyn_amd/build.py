@@ -29,6 +29,9 @@ FLAGS = [
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the column kernel's fully unrolled LU and substitutions (register-indexed arrays) exceed
+    # the default size limit for an unroll pragma; unrolled they need no scratch memory
+    "-mllvm", "-pragma-unroll-threshold=200000",
     "-Wall",
     "-Wno-unused-function",
 ]  # fmt: skip

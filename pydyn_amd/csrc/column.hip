@@ -53,6 +53,25 @@ __device__ __forceinline__ void wsync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A uniform value the compiler must take as redefined here: comparisons with it are then made
+// where they are used, as scalar compares, instead of being hoisted and kept as lane masks.
+__device__ __forceinline__ int fresh(int v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+// lane k's value of each 32-lane half, to every lane of that half (ds_swizzle in bit-mask mode:
+// and-mask 0, or-mask k; no LDS memory is touched)
+template <int K> __device__ __forceinline__ double half_bcast(double v)
+{
+    static_assert(K >= 0 && K < 32, "a lane of the half");
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), K << 5);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), K << 5);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // the larger of a value over the two halves of the wavefront (uniform)
 __device__ __forceinline__ int wave_max2(int v)
 {
@@ -68,6 +87,40 @@ template <int N> struct ColumnLds {
     // a multiple of 32 plus 8 doubles, so the two halves' broadcast reads fall in different banks
     static constexpr int S = ((R0 + 96 + 31) / 32) * 32 + 8;
 };
+
+// Step K of the LU and the steps after it (a template recursion: the pivot lane must be a
+// compile-time constant of the swizzle).  Lane r holds row r in A.  Row K, final after step
+// K-1, comes from lane K of the half: its diagonal, the next entry and the dense block in chunks
+// of four columns (a chunk's columns past DB have a +-0 factor).  Rows r <= K take the update
+// with a zero multiplier: r - (+-0) is r up to a zero's sign.
+template <int N, int K> __device__ __forceinline__ void lu_steps(double (&A)[N], int li, int DB)
+{
+    if constexpr (K < N) {
+        const int DBk = fresh(DB);
+        double prow[N];
+        prow[K] = half_bcast<K>(A[K]);
+        if (K + 1 < N) prow[K + 1] = half_bcast<K>(A[K + 1]);
+#pragma unroll
+        for (int c0 = K + 2; c0 < N; c0 += 4) {
+            if (c0 < DBk) {
+#pragma unroll
+                for (int c = c0; c < c0 + 4 && c < N; c++) prow[c] = half_bcast<K>(A[c]);
+            }
+        }
+        const double q = A[K] / prow[K];
+        A[K] = li > K ? q : A[K];
+        const double lrk = li > K ? q : 0.0;
+        if (K + 1 < N) A[K + 1] = A[K + 1] - lrk * prow[K + 1];
+#pragma unroll
+        for (int c0 = K + 2; c0 < N; c0 += 4) {
+            if (c0 < DBk) {
+#pragma unroll
+                for (int c = c0; c < c0 + 4 && c < N; c++) A[c] = A[c] - lrk * prow[c];
+            }
+        }
+        lu_steps<N, K + 1>(A, li, DB);
+    }
+}
 
 template <int N, bool ADJ>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
@@ -208,37 +261,14 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 
     // ---- LU, lanes over rows.  Entry (r, c) is non-zero only inside the dense block (r, c < DB)
     // or on the three diagonals, and LU fill-in stays inside that envelope; the products skipped
-    // outside it have a factor that is exactly +-0.  Rows r <= k take the update with a zero
-    // multiplier: r - (+-0) is r up to a zero's sign.  Row k, final after step k-1, goes to LDS
-    // for the step (it becomes the U part of the factors there).
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-        if (li == k) {
-#pragma unroll
-            for (int c = k; c < N; c++) Mt[k * LD + c] = A[c];
-        }
-        wsync();
-        double prow[N];  // row k, entries k.. (the loads issue together, the updates wait on them)
-#pragma unroll
-        for (int c = k; c < N; c++) prow[c] = Mt[k * LD + c];
-        const double q = A[k] / prow[k];
-        A[k] = li > k ? q : A[k];
-        const double lrk = li > k ? q : 0.0;
-        if (k + 1 < N) A[k + 1] = A[k + 1] - lrk * prow[k + 1];
-        // the dense block in chunks of four columns (a chunk's columns past DB have a +-0 factor)
-#pragma unroll
-        for (int c0 = k + 2; c0 < N; c0 += 4) {
-            if (c0 < DB) {
-#pragma unroll
-                for (int c = c0; c < c0 + 4 && c < N; c++) A[c] = A[c] - lrk * prow[c];
-            }
-        }
-    }
-    // the L part of row r
+    // outside it have a factor that is exactly +-0.
+    lu_steps<N, 0>(A, li, DB);
+    // the factors to LDS, row r by lane r: what the substitutions read (the three diagonals, and
+    // for rows inside the dense block its columns with the last chunk's padding)
     if (li < N) {
 #pragma unroll
-        for (int c = 0; c + 1 < N; c++)
-            if (c < li) Mt[li * LD + c] = A[c];
+        for (int c = 0; c < N; c++)
+            if ((c + 1 >= li && c <= li + 1) || (li < DB && c < DB + 3)) Mt[li * LD + c] = A[c];
     }
     wsync();
 
@@ -248,25 +278,33 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     for (int r = 0; r < N; r++) y[r] = (li == r) ? 1.0 : 0.0;
 #pragma unroll
     for (int r = 1; r < N; r++) {
-        double lrow[N];  // row r of L (loaded whole: the loads do not wait on the branch)
+        if (r < fresh(DB)) {
+            double lrow[N];
 #pragma unroll
-        for (int s = 0; s < r; s++) lrow[s] = Mt[r * LD + s];
-        if (r < DB) {
+            for (int s = 0; s < r - 1; s++) lrow[s] = Mt[r * LD + s];
 #pragma unroll
             for (int s = 0; s < r - 1; s++) y[r] = y[r] - lrow[s] * y[s];
         }
-        y[r] = y[r] - lrow[r - 1] * y[r - 1];
+        y[r] = y[r] - Mt[r * LD + r - 1] * y[r - 1];
     }
 #pragma unroll
     for (int r = N - 1; r >= 0; r--) {
-        double urow[N];  // row r of U, entries r..
+        const int DBr = fresh(DB);
+        double urow[N];  // row r of U inside the envelope
+        urow[r] = Mt[r * LD + r];
+        if (r + 1 < N) urow[r + 1] = Mt[r * LD + r + 1];
 #pragma unroll
-        for (int s = r; s < N; s++) urow[s] = Mt[r * LD + s];
+        for (int s0 = r + 2; s0 < N; s0 += 4) {
+            if (s0 < DBr) {
+#pragma unroll
+                for (int s = s0; s < s0 + 4 && s < N; s++) urow[s] = Mt[r * LD + s];
+            }
+        }
         double acc = y[r];
         if (r + 1 < N) acc = acc - urow[r + 1] * y[r + 1];
 #pragma unroll
         for (int s0 = r + 2; s0 < N; s0 += 4) {
-            if (s0 < DB) {
+            if (s0 < DBr) {
 #pragma unroll
                 for (int s = s0; s < s0 + 4 && s < N; s++) acc = acc - urow[s] * y[s];
             }

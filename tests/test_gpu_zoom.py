@@ -95,6 +95,12 @@ def test_zoom_production_tree_matches_oracle_and_reference(dev):
     _run("c2")
 
 
+def test_zoom_reduced_grid_production_tree_matches_reference(dev):
+    """The production tree with region 1's x advection on the glb3x2 reduced grid (rows 1-7 and
+    84-90 merged in clusters of 40, 8, 8, 4, 4, 4, 2 cells)."""
+    _run("c2r")
+
+
 # ------------------------------------------------------------------ ensembles (members x tracers)
 def _ensemble(base, ntr, members):
     from pydyn_amd import meteo
