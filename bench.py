@@ -288,10 +288,7 @@ def main():
         reg.interval()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sim = 0
-    for _ in range(a.steps):
-        ndyn, n = reg.interval()
-        sim += ndyn * n
+    sim = sum(ndyn * n for ndyn, n in reg.run(a.steps))  # one read of the device for the K intervals
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     reg.check()

@@ -63,7 +63,9 @@ typedef struct tm5_iview {
 #define TM5_REHEARSAL 32 /* a Check_CFL mass-only operator exceeded its cap (the verdict) */
 #define TM5_FAIL_XC 64   /* a child's dynamu: nloop too high (CFLError "dynuamu: nloop too high") */
 #define TM5_FAIL_YC 128  /* a child's dynamv: nloop too high (CFLError "nloop too high in dynamv") */
-#define TM5_ERRORS 223   /* any error bit (all but TM5_REHEARSAL) */
+#define TM5_ERRORS 223   /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
+#define TM5_SPEC 256     /* tm5_region_run: a queued interval's rehearsal failed; the intervals queued after it
+                            did nothing (cleared before tm5_region_run returns) */
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
@@ -174,6 +176,14 @@ int tm5_region_interval(tm5_region *region, double t0, int nread, int ndyn_max, 
    time t, 'x' 'y' 'z' advection, 'V' the middle "vsc m2phlb1 csv", 'M' m2phlb1; a step of ndyn
    from t0 is "D" at t0 + ndyn/2, then "xyzVzyxM". */
 int tm5_region_ops(tm5_region *region, double t, int ndyn, const char *ops);
+/* count intervals of nread seconds from t0, as count calls of tm5_region_interval would run
+   them (interval k starts at t0 + k nread), with the host reading the device once per batch
+   instead of once per interval: every interval is queued with its rehearsal at (ndyn_max,
+   nread / ndyn_max), its steps skipping themselves on the device if the rehearsal fails; when
+   one fails, the run resumes from it through tm5_region_interval (its Check_CFL cut), then
+   queues the rest again.  ndyn[k], n[k]: what Check_CFL chose for interval k.  The reference's
+   queued submission is Pipeline.submit (pydyn/gpu/zoom.py:944-952). */
+int tm5_region_run(tm5_region *region, double t0, int nread, int ndyn_max, int count, int *ndyn, int *n);
 
 
 /* ------------------------------------------------------------------ the zoom tree */

@@ -9,7 +9,7 @@ import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libtm5hip.so"
 
-FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC = 1, 2, 4, 8, 16, 32, 64, 128
+FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC, SPEC = 1, 2, 4, 8, 16, 32, 64, 128, 256
 ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC
 EINVAL, EFLAGS = -1, -2
 
@@ -125,6 +125,7 @@ _SIGNATURES = {
     "tm5_region_destroy": [_VP],
     "tm5_region_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
     "tm5_region_ops": [_VP, _D, _I, ctypes.c_char_p],
+    "tm5_region_run": [_VP, _D, _I, _I, _I, _P(_I), _P(_I)],
     "tm5_advect_adjoint": [_VP, _I, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(Tracers), _I, _I, _VP],
     "tm5_zoom_put_xedges": [_VP, _P(ZRegion), _P(ZRegion), _I],
     "tm5_zoom_put_yedges": [_VP, _P(ZRegion), _P(ZRegion), _I],

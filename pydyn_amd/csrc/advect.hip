@@ -160,6 +160,7 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
 // by (nloop-1)/nloop in place (the reference scales bm(:,:,l) whole).  The polar outer faces
 // must be zero (pydyn/kernels.py:233-234).
 constexpr int Y_FPT = 2;  // interior faces per thread in the y check at nloop = 1
+constexpr int Y_NTH = 1024;  // threads of a y check workgroup
 
 // grid (chunks, lm): every workgroup checks its chunk of layer l's faces at nloop = 1 (chunk 0
 // also the polar faces) and or-s its verdict into the layer's word; the last workgroup of the
@@ -351,8 +352,8 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
     const bool mass_only = q.nt == 0;
     const int TI = lines_per_block(jm, 16);
     if (TI < 1) return TM5_EINVAL;
-    const int chunks = (im * (jm - 1) + 1024 * Y_FPT - 1) / (1024 * Y_FPT);
-    TM5_LAUNCH(ctx, k_y_search, dim3(chunks, lm), dim3(1024), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
+    const int chunks = (im * (jm - 1) + Y_NTH * Y_FPT - 1) / (Y_NTH * Y_FPT);
+    TM5_LAUNCH(ctx, k_y_search, dim3(chunks, lm), dim3(Y_NTH), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
                ctx->y_nloop + ctx->y_cap, ctx->y_nloop + 2 * ctx->y_cap, ctx->words, fail_bit, skip_mask);
     const dim3 grid((im + TI - 1) / TI, lm), block = line_block(TI, jm);
     if (mass_only)
@@ -544,6 +545,92 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
 
 // The search over the whole region (the trial advancing only the scope's columns), then the
 // sub-steps on the scope: region 1's scope is the region, a child's its block's.
+// ------------------------------------------------------------------ Check_CFL's y and z at cap 1
+// The rehearsal allows dynamvm and dynamwm no sub-cycling (max_nloop 1, pydyn/zoom/check_cfl.py:
+// 258-290): a trial passes only if every interior face has |Courant| < 1 at nloop = 1, and then
+// its one mass-only sub-step is the operator.  So the check needs no search: each face is tested
+// where its column is updated, a failure raises the rehearsal's bit (the sub-steps already done
+// are then discarded with the trial), and the closed ends are checked as the searches do.
+// TI whole columns of a layer (y) or of a row (z) per workgroup: every read of the workgroup's
+// cells precedes its writes.
+__global__ void __launch_bounds__(1024) k_y_mass1(int im, int jm, int lm, int TI, View3 m, View3 bm, int *words,
+                                                  int fail_bit, int skip_mask)
+{
+    if (skipped(words, skip_mask)) return;
+    const int l = blockIdx.y, i0 = blockIdx.x * TI;
+    const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
+    const bool mine = j < jm && i0 + ti < im;
+    const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
+    const double m_old = m(i, jj, l), b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
+    bool ok = true, bad = false;
+    if (mine) {
+        if (jj >= 1) {
+            const double mlo = m(i, jj - 1, l);
+            ok = fabs(b_lo / (b_lo >= 0.0 ? mlo : m_old)) < 1.0;
+        } else {
+            bad = b_lo != 0.0;
+        }
+        if (jj == jm - 1) bad = bad || b_hi != 0.0;
+    }
+    const double m_new = (m_old + b_lo) - b_hi;
+    __syncthreads();
+    if (mine) m(i, jj, l) = m_new;
+    const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+    if ((threadIdx.x & 63) == 0) {
+        if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Y);
+        if (any_fail) atomicOr(words + W_FLAGS, fail_bit);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_z_mass1(int im, int jm, int lm, int TI, View3 m, View3 cm, int *words,
+                                                  int fail_bit, int skip_mask)
+{
+    if (skipped(words, skip_mask)) return;
+    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
+    const bool mine = l < lm && i0 + ti < im;
+    const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;
+    const double m_old = m(i, j, ll);
+    const double c_lo = cm(i, j, ll) / 1.0, c_hi = cm(i, j, ll + 1) / 1.0;  // cm / n_advz at n = 1
+    bool ok = true, bad = false;
+    if (mine) {
+        if (ll >= 1) {
+            const double ck = cm(i, j, ll), mlo = m(i, j, ll - 1);
+            ok = fabs(ck / (ck >= 0.0 ? mlo : m_old)) < 1.0;
+        } else {
+            bad = cm(i, j, 0) != 0.0 || cm(i, j, lm) != 0.0;
+        }
+    }
+    const double m_new = (m_old + c_lo) - c_hi;
+    __syncthreads();
+    if (mine) m(i, j, ll) = m_new;
+    const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+    if ((threadIdx.x & 63) == 0) {
+        if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+        if (any_fail) atomicOr(words + W_FLAGS, fail_bit);
+    }
+}
+
+int launch_y_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 bm, int fail_bit, int skip_mask)
+{
+    if (im < 1 || jm < 2 || lm < 1) return TM5_EINVAL;
+    const int TI = lines_per_block(jm, 16);
+    if (TI < 1) return TM5_EINVAL;
+    TM5_LAUNCH(ctx, k_y_mass1, dim3((im + TI - 1) / TI, lm), line_block(TI, jm), 0, ctx->stream, im, jm, lm, TI, m,
+               bm, ctx->words, fail_bit, skip_mask);
+    return status(hipGetLastError());
+}
+
+int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int fail_bit, int skip_mask)
+{
+    if (im < 1 || jm < 1 || lm < 2) return TM5_EINVAL;
+    const int TI = lines_per_block(lm, 32);
+    if (TI < 1) return TM5_EINVAL;
+    TM5_LAUNCH(ctx, k_z_mass1, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI, m,
+               cm, ctx->words, fail_bit, skip_mask);
+    return status(hipGetLastError());
+}
+
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask)
 {

@@ -34,8 +34,6 @@ void prof_mark(tm5_ctx *ctx, const char *name, bool stop)
 
 int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 entu, View3 detu, View3 entd,
                   View3 detd, IView2 top, IView2 lfs, View3 dkg, double dt, int lmax);
-int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
-                   const double *bt, View3 sp, View3 phlb);
 int launch_dkg(tm5_ctx *ctx, int im, int jm, View3 kvh, View3 m, View3 gph, int lmax, View3 dkg);
 int launch_add_2d(tm5_ctx *ctx, int im, int jm, View3 emis, double factor, double ratio, double efrac, Tracers q,
                   int level, int tracer);
@@ -49,6 +47,21 @@ int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor);
 __global__ void k_set_word(int *words, int index, int value, int mask)
 {
     if (threadIdx.x == 0) words[index] = (words[index] & ~mask) | (value & mask);
+}
+
+// A queued trial's verdict against the plan (expect_fail: the plan cuts the step after this
+// trial): a different verdict makes every kernel queued after it skip.  The trial's bit is cleared.
+__global__ void k_spec_check(int *words, int k, int expect_fail)
+{
+    if (threadIdx.x != 0) return;
+    const int f = words[W_FLAGS];
+    const bool failed = (f & TM5_REHEARSAL) != 0;
+    int out = f & ~TM5_REHEARSAL;
+    if (!(f & TM5_SPEC) && failed != (expect_fail != 0)) {
+        out |= TM5_SPEC;
+        words[W_SPEC_AT] = k;
+    }
+    words[W_FLAGS] = out;
 }
 
 __global__ void __launch_bounds__(256) k_copy3(int n0, int n1, int n2, View3 src, View3 dst)
@@ -352,6 +365,11 @@ extern "C" int tm5_scale(tm5_ctx *c, int n0, int n1, int n2, const tm5_view *a, 
 struct tm5_region {
     tm5_ctx *ctx;
     tm5_region_desc d;
+    int skip;         // the step kernels' skip mask: TM5_ERRORS, with TM5_SPEC while queuing ahead
+    bool copy_ms;     // while queuing: the step's last m2phlb1 also copies m into the rehearsal's ms
+    bool ms_fresh;    // ms holds m (the next rehearsal need not copy it)
+    int plan;         // tm5_region_run: the step the last interval settled on (0: none yet)
+    int batch;        // tm5_region_run: intervals it queues per read of the device
     double *scratch;  // the rehearsal's m, am, bm, cm (i-fastest, contiguous)
     View3 ms, ams, bms, cms;
 };
@@ -365,6 +383,7 @@ extern "C" int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *d, tm5_reg
     if (!r) return TM5_EINVAL;
     r->ctx = ctx;
     r->d = *d;
+    r->skip = TM5_ERRORS;
     const int im = d->im, jm = d->jm, lm = d->lm;
     const size_t nm = (size_t)im * jm * lm, na = (size_t)(im + 1) * jm * lm, nb = (size_t)im * (jm + 1) * lm,
                  nc = (size_t)im * jm * (lm + 1);
@@ -409,12 +428,41 @@ static int new_valid_timestep(int dtime, int nread, int cfl_outputstep, int tref
 // Fortran nint (pydyn/ops.py:78-80)
 static int nint(double x) { return (int)(floor(fabs(x) + 0.5) * (x >= 0 ? 1.0 : -1.0)); }
 
-static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View3 cm)
+static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View3 cm, const View3 *dual = nullptr)
 {
     const tm5_region_desc &d = r->d;
     const double a = amplitude(d, t);
     return launch_dynam0(r->ctx, d.im, d.jm, d.lm, view3(&d.pu0), view3(&d.pv0), a, d.bt, (double)ndyn / 2.0, am, bm,
-                         cm, TM5_ERRORS);
+                         cm, r->skip, 1, dual);
+}
+
+// The kernels of Check_CFL's rehearsal of the n steps of ndyn on the scratch copy.  With n = 1
+// the trial's flux set is the step's own (Setup_MassFlow at the same time, t0 + ndyn/2): dual
+// then has its dynam0 write the region's am, bm, cm too.
+static int rehearse_queue(tm5_region *r, double t0, int ndyn, int n, bool dual = false, bool clear = true)
+{
+    tm5_ctx *c = r->ctx;
+    const tm5_region_desc &d = r->d;
+    const int im = d.im, jm = d.jm, lm = d.lm;
+    const int skip = REHEARSAL_SKIP | (r->skip & TM5_SPEC);
+    const View3 step_fluxes[3] = {view3(&d.am), view3(&d.bm), view3(&d.cm)};
+    if (clear) TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL));
+    if (!r->ms_fresh) TRY(copy3(c, im, jm, lm, view3(&d.m), r->ms));
+    r->ms_fresh = false;  // the trial moves ms
+    for (int i = 1; i <= n; i++) {
+        TRY(dynam0_at(r, t0 + ((double)i - 0.5) * (double)ndyn, ndyn, r->ams, r->bms, r->cms,
+                      dual && n == 1 ? step_fluxes : nullptr));
+        const char *order = "xyzzyx";
+        for (const char *op = order; *op; op++) {
+            if (*op == 'x')
+                TRY(launch_x(c, im, jm, lm, r->ms, no_tracers(), r->ams, 0, 10 - 1, TM5_REHEARSAL, skip));
+            else if (*op == 'y')  // caps 1: the check at nloop 1 and its step
+                TRY(launch_y_mass1(c, im, jm, lm, r->ms, r->bms, TM5_REHEARSAL, skip));
+            else
+                TRY(launch_z_mass1(c, im, jm, lm, r->ms, r->cms, TM5_REHEARSAL, skip));
+        }
+    }
+    return 0;
 }
 
 // Check_CFL's rehearsal of the n steps of ndyn on the scratch copy (pydyn/zoom/check_cfl.py:
@@ -422,22 +470,7 @@ static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View
 static int rehearse(tm5_region *r, double t0, int ndyn, int n, bool *ok)
 {
     tm5_ctx *c = r->ctx;
-    const tm5_region_desc &d = r->d;
-    const int im = d.im, jm = d.jm, lm = d.lm;
-    TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL));
-    TRY(copy3(c, im, jm, lm, view3(&d.m), r->ms));
-    for (int i = 1; i <= n; i++) {
-        TRY(dynam0_at(r, t0 + ((double)i - 0.5) * (double)ndyn, ndyn, r->ams, r->bms, r->cms));
-        const char *order = "xyzzyx";
-        for (const char *op = order; *op; op++) {
-            if (*op == 'x')
-                TRY(launch_x(c, im, jm, lm, r->ms, no_tracers(), r->ams, 0, 10 - 1, TM5_REHEARSAL, REHEARSAL_SKIP));
-            else if (*op == 'y')
-                TRY(launch_y(c, im, jm, lm, r->ms, no_tracers(), r->bms, 0, 1, TM5_REHEARSAL, REHEARSAL_SKIP));
-            else
-                TRY(launch_z(c, im, jm, lm, r->ms, no_tracers(), r->cms, 0, 1, TM5_REHEARSAL, REHEARSAL_SKIP));
-        }
-    }
+    TRY(rehearse_queue(r, t0, ndyn, n));
     TRY(read_words(c));
     if (c->hwords[W_FLAGS] & TM5_ERRORS) return TM5_EFLAGS;
     *ok = (c->hwords[W_FLAGS] & TM5_REHEARSAL) == 0;
@@ -452,11 +485,11 @@ static int operator_(tm5_region *r, char op, int ndyn)
     const Tracers q = tracers(&d.q);
     switch (op) {
     case 'x':
-        return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, TM5_ERRORS);
+        return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, r->skip);
     case 'y':
-        return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, TM5_ERRORS);
+        return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, r->skip);
     case 'z':
-        return launch_z(c, im, jm, lm, view3(&d.m), q, view3(&d.cm), d.limits, 15, TM5_FAIL_Z, TM5_ERRORS);
+        return launch_z(c, im, jm, lm, view3(&d.m), q, view3(&d.cm), d.limits, 15, TM5_FAIL_Z, r->skip);
     case 'v':
         return launch_convec(c, im, jm, lm, view3(&d.m), q, view3(&d.entu), view3(&d.detu), view3(&d.entd),
                              view3(&d.detd), iview2(&d.cloud_top), iview2(&d.cloud_lfs), view3(&d.dkg),
@@ -512,7 +545,7 @@ static int middle(tm5_region *r, int ndyn)
     a.bt = d.bt;
     a.sp = view3(&d.sp);
     a.phlb = view3(&d.phlb);
-    return launch_column(r->ctx, a, d.lmax, TM5_ERRORS);
+    return launch_column(r->ctx, a, d.lmax, r->skip);
 }
 
 static int step(tm5_region *r, int ndyn)
@@ -521,13 +554,16 @@ static int step(tm5_region *r, int ndyn)
     for (const char *op = "xyz"; *op; op++) TRY(operator_(r, *op, ndyn));
     TRY(middle(r, ndyn));
     for (const char *op = "zyx"; *op; op++) TRY(operator_(r, *op, ndyn));
-    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
+    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
+                       r->copy_ms ? &r->ms : nullptr));
+    r->ms_fresh = r->copy_ms;
     return 0;
 }
 
 extern "C" int tm5_region_interval(tm5_region *r, double t0, int nread, int ndyn_max, int *ndyn_out, int *n_out)
 {
     if (!r || nread < 1 || ndyn_max < 1 || nread % ndyn_max != 0) return TM5_EINVAL;
+    r->ms_fresh = false;  // the caller may have changed m since the last call
     const tm5_region_desc &d = r->d;
     int ndyn = ndyn_max, n = nread / ndyn_max;
     bool ok = false;
@@ -557,6 +593,7 @@ extern "C" int tm5_region_interval(tm5_region *r, double t0, int nread, int ndyn
 extern "C" int tm5_region_ops(tm5_region *r, double t, int ndyn, const char *ops)
 {
     if (!r || !ops || ndyn < 1) return TM5_EINVAL;
+    r->ms_fresh = false;
     const tm5_region_desc &d = r->d;
     for (const char *op = ops; *op; op++) {
         switch (*op) {
@@ -568,6 +605,103 @@ extern "C" int tm5_region_ops(tm5_region *r, double t, int ndyn, const char *ops
             break;
         default: return TM5_EINVAL;
         }
+    }
+    return 0;
+}
+
+// new_valid_timestep's chain from ndyn_max down to ndyn: the trials Check_CFL makes when it
+// settles on ndyn, each with its n; false if ndyn is not on the chain
+static bool trial_chain(int nread, int ndyn_max, int ndyn, int *nd, int *nn, int *count)
+{
+    int a = ndyn_max, n = nread / ndyn_max, k = 0;
+    for (;;) {
+        if (k >= 100) return false;
+        nd[k] = a;
+        nn[k] = n;
+        k++;
+        if (a == ndyn) break;
+        const int b = new_valid_timestep(a, nread, nread, 1);
+        if (b < 1) return false;
+        n = nint((double)(n * a) / (double)b);
+        a = b;
+    }
+    *count = k;
+    return true;
+}
+
+extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max, int count, int *ndyn_out, int *n_out)
+{
+    if (!r || !ndyn_out || !n_out || count < 0 || nread < 1 || ndyn_max < 1 || nread % ndyn_max != 0)
+        return TM5_EINVAL;
+    tm5_ctx *c = r->ctx;
+    const tm5_region_desc &d = r->d;
+    r->ms_fresh = false;  // the caller may have changed m since the last call
+    // queuing ahead needs every step kernel to honour the skip mask (the register column form)
+    if (!column_supported(d.lmax) || d.lm > 64) {
+        for (int k = 0; k < count; k++)
+            TRY(tm5_region_interval(r, t0 + (double)k * (double)nread, nread, ndyn_max, ndyn_out + k, n_out + k));
+        return 0;
+    }
+    TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL | TM5_SPEC));
+    // intervals queued per read of the device: doubled after a batch that held to its plan, back
+    // to 2 after one that did not (the kernels queued after a miss only skip, but each costs a
+    // dispatch)
+    int &batch = r->batch;
+    if (batch < 2) batch = 2;
+    int &plan = r->plan;  // the step Check_CFL is predicted to settle on: the previous interval's
+    if (plan < 1 || plan > ndyn_max) plan = ndyn_max;
+    int i = 0;
+    while (i < count) {
+        int nd[100], nn[100], trials = 0;
+        if (!trial_chain(nread, ndyn_max, plan, nd, nn, &trials)) {
+            plan = ndyn_max;
+            trial_chain(nread, ndyn_max, plan, nd, nn, &trials);
+        }
+        const int end = i + batch < count ? i + batch : count;
+        r->skip = TM5_ERRORS | TM5_SPEC;
+        int rc = 0;
+        for (int k = i; k < end && rc == 0; k++) {
+            const double t = t0 + (double)k * (double)nread;
+            for (int q = 0; q < trials && rc == 0; q++) {
+                const bool last = q + 1 == trials;
+                rc = rehearse_queue(r, t, nd[q], nn[q], last, false);  // k_spec_check clears the bit
+                if (rc == 0) {
+                    TM5_LAUNCH(c, k_spec_check, dim3(1), dim3(64), 0, c->stream, c->words, k, last ? 0 : 1);
+                    rc = status(hipGetLastError());
+                }
+            }
+            const int ndyn = nd[trials - 1], n = nn[trials - 1];
+            r->copy_ms = true;
+            for (int s = 0; s < n && rc == 0; s++) {
+                if (n > 1)  // with n = 1 the last trial's Setup_MassFlow wrote the step's fluxes
+                    rc = dynam0_at(r, (t + (double)(s * ndyn)) + (double)ndyn / 2.0, ndyn, view3(&d.am), view3(&d.bm),
+                                   view3(&d.cm));
+                if (rc == 0) rc = step(r, ndyn);
+            }
+            r->copy_ms = false;
+        }
+        r->skip = TM5_ERRORS;
+        TRY(rc);
+        TRY(read_words(c));
+        if (c->hwords[W_FLAGS] & TM5_ERRORS) return TM5_EFLAGS;
+        const int f = (c->hwords[W_FLAGS] & TM5_SPEC) ? c->hwords[W_SPEC_AT] : end;
+        for (int k = i; k < f; k++) {
+            ndyn_out[k] = nd[trials - 1];
+            n_out[k] = nn[trials - 1];
+        }
+        if (f >= end) {
+            i = end;
+            batch = batch < 16 ? 2 * batch : 16;
+            continue;
+        }
+        batch = 2;
+        // interval f's trials disagreed with the plan: nothing after them ran; redo it the
+        // reference's way, and plan the next intervals on its step
+        TRY(set_word(c, W_FLAGS, 0, TM5_SPEC | TM5_REHEARSAL));
+        r->ms_fresh = false;
+        TRY(tm5_region_interval(r, t0 + (double)f * (double)nread, nread, ndyn_max, ndyn_out + f, n_out + f));
+        plan = ndyn_out[f];
+        i = f + 1;
     }
     return 0;
 }

@@ -19,8 +19,9 @@ constexpr int D0_TI = 32;
 
 __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
-                                                int *words, int skip_mask)
+                                                int *words, int skip_mask, View3 am2, View3 bm2, View3 cm2)
 {
+    // am2, bm2, cm2 (when am2.p is set): a second copy of every output
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double conv[];  // [lm][TI]
     const int TI = D0_TI;
@@ -35,10 +36,18 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
         const double p0 = a * pu(i, j, l), p1 = a * pu(i + 1, j, l);
         const double v0 = a * pv(i, j, l), v1 = a * pv(i, j + 1, l);
         conv[idx] = ((p0 - p1) + v0) - v1;
-        am(i, j, l) = dt * (i == 0 && cyclic ? a * pu(im, j, l) : p0);  // a cyclic region: am(0) = am(im)
-        if (i == im - 1) am(im, j, l) = dt * p1;
-        bm(i, j, l) = dt * v0;
-        if (j == jm - 1) bm(i, jm, l) = dt * v1;
+        const double a0 = dt * (i == 0 && cyclic ? a * pu(im, j, l) : p0);  // a cyclic region: am(0) = am(im)
+        const double a1 = dt * p1, b0 = dt * v0, b1 = dt * v1;
+        am(i, j, l) = a0;
+        if (i == im - 1) am(im, j, l) = a1;
+        bm(i, j, l) = b0;
+        if (j == jm - 1) bm(i, jm, l) = b1;
+        if (am2.p) {
+            am2(i, j, l) = a0;
+            if (i == im - 1) am2(im, j, l) = a1;
+            bm2(i, j, l) = b0;
+            if (j == jm - 1) bm2(i, jm, l) = b1;
+        }
     }
     __syncthreads();
     for (int ti = tid; ti < ni; ti += nth) {
@@ -47,12 +56,18 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
         for (int l = 1; l < lm; l++) pit = pit + conv[l * TI + ti];
         cm(i, j, 0) = 0.0;
         cm(i, j, lm) = 0.0;
+        if (am2.p) {
+            cm2(i, j, 0) = 0.0;
+            cm2(i, j, lm) = 0.0;
+        }
         if (lm > 1) {
             double sd = conv[(lm - 1) * TI + ti] - (bt[lm - 1] - bt[lm]) * pit;
             cm(i, j, lm - 1) = (-dt) * sd;
+            if (am2.p) cm2(i, j, lm - 1) = (-dt) * sd;
             for (int l = lm - 2; l >= 1; l--) {
                 sd = (sd + conv[l * TI + ti]) - (bt[l] - bt[l + 1]) * pit;
                 cm(i, j, l) = (-dt) * sd;
+                if (am2.p) cm2(i, j, l) = (-dt) * sd;
             }
         }
     }
@@ -65,8 +80,9 @@ constexpr int MP_TI = 32;
 
 __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m, const double *dxyp,
                                                  const double *at, const double *bt, View3 sp, View3 phlb, int *words,
-                                                 int skip_mask)
+                                                 int skip_mask, View3 mcopy)
 {
+    // mcopy (when mcopy.p is set): a copy of m on the way
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double sh[];  // M [lm][TI], P [TI]
     const int TI = MP_TI;
@@ -77,7 +93,11 @@ __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m
 #pragma unroll 8
     for (int idx = tid; idx < lm * TI; idx += nth) {
         const int ti = idx % TI, l = idx / TI;
-        if (ti < ni) M[idx] = m(i0 + ti, j, l);
+        if (ti < ni) {
+            const double v = m(i0 + ti, j, l);
+            M[idx] = v;
+            if (mcopy.p) mcopy(i0 + ti, j, l) = v;
+        }
     }
     __syncthreads();
     for (int ti = tid; ti < ni; ti += nth) {
@@ -154,22 +174,25 @@ __global__ void __launch_bounds__(256) k_scale(int n0, int n1, int n2, View3 a, 
 static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic, const View3 *dual)
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
     const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
+    const View3 none{};
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
-                       pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
+                       pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask, dual ? dual[0] : none,
+                       dual ? dual[1] : none, dual ? dual[2] : none);
     return status(hipGetLastError());
 }
 
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
-                   const double *bt, View3 sp, View3 phlb)
+                   const double *bt, View3 sp, View3 phlb, const View3 *mcopy)
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !dxyp || !at || !bt) return TM5_EINVAL;
     const dim3 grid((im + MP_TI - 1) / MP_TI, jm);
+    const View3 none{};
     TM5_LAUNCH(ctx, k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * MP_TI * sizeof(double), ctx->stream, im, jm,
-                       lm, m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS);
+                       lm, m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS, mcopy ? *mcopy : none);
     return status(hipGetLastError());
 }
 

@@ -14,6 +14,7 @@ enum {
     W_Z_NADV = 2,   // n_advz of the last z operator
     W_Z_TICKET = 3, // blocks of the z search that have finished (reset by the last one)
     W_Z_CAND = 4,   // the z search's running candidate
+    W_SPEC_AT = 5,  // tm5_region_run: the first queued interval whose rehearsal failed
     W_WORDS = 8,
 };
 
@@ -215,13 +216,18 @@ struct ColumnArgs {
     View3 lam_emis;   // (im, jm)
 };
 bool column_supported(int n);
+// dual: a second (am, bm, cm) that receives the same fluxes
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1);
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const View3 *dual = nullptr);
+// mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
-                   const double *bt, View3 sp, View3 phlb);
+                   const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
 int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask);
+// Check_CFL's mass-only y and z with no sub-cycling allowed: the check at nloop 1 and its step
+int launch_y_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 bm, int fail_bit, int skip_mask);
+int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int fail_bit, int skip_mask);
 
 }  // namespace tm5

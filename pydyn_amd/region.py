@@ -66,15 +66,21 @@ class Region:
 
     def interval(self, nread=3600, ndyn_max=3600):
         """One interval of nread seconds (Check_CFL + its steps); returns (ndyn, n)."""
-        ndyn, n = ctypes.c_int(), ctypes.c_int()
-        rc = _lib.lib().tm5_region_interval(self._ptr, float(self.t), int(nread), int(ndyn_max), ctypes.byref(ndyn),
-                                            ctypes.byref(n))  # fmt: skip
+        return self.run(1, nread, ndyn_max)[0]
+
+    def run(self, count, nread=3600, ndyn_max=3600):
+        """count intervals of nread seconds, queued with one read of the device per batch
+        (tm5_region_run); returns [(ndyn, n)] per interval."""
+        ndyn, n = (ctypes.c_int * max(count, 1))(), (ctypes.c_int * max(count, 1))()
+        rc = _lib.lib().tm5_region_run(self._ptr, float(self.t), int(nread), int(ndyn_max), int(count), ndyn, n)
         if rc == _lib.EFLAGS:
             D.context().raise_errors()
-        _lib.check(rc, "tm5_region_interval")
-        self.t = self.t + n.value * ndyn.value
-        self.log.append((ndyn.value, n.value))
-        return ndyn.value, n.value
+        _lib.check(rc, "tm5_region_run")
+        out = [(ndyn[k], n[k]) for k in range(count)]
+        for nd, nn in out:
+            self.t = self.t + nn * nd
+            self.log.append((nd, nn))
+        return out
 
     def check(self):
         """Raise the reference's exception for any error the device recorded."""
