@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
 {
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
-    const int row = blockIdx.x;
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
     const int j = row % jm, l = row / jm;
     const int n = imred ? imred[j] : im;  // a reduced row has imred(j) cells
     const int lane = threadIdx.x;
@@ -286,7 +286,9 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
                                                   const int *nloop, int limits, int *words, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    const int l = blockIdx.y, i0 = blockIdx.x * TI;
+    int bx, l;
+    xcd_grid2(bx, l);
+    const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
     const bool mine = j < jm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
@@ -501,7 +503,9 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
                                                   int limits, int *words, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    int bx, j;
+    xcd_grid2(bx, j);
+    const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
     const bool mine = l < lm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;
@@ -557,7 +561,9 @@ __global__ void __launch_bounds__(1024) k_y_mass1(int im, int jm, int lm, int TI
                                                   int fail_bit, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    const int l = blockIdx.y, i0 = blockIdx.x * TI;
+    int bx, l;
+    xcd_grid2(bx, l);
+    const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
     const bool mine = j < jm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
@@ -586,7 +592,9 @@ __global__ void __launch_bounds__(1024) k_z_mass1(int im, int jm, int lm, int TI
                                                   int fail_bit, int skip_mask)
 {
     if (skipped(words, skip_mask)) return;
-    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    int bx, j;
+    xcd_grid2(bx, j);
+    const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
     const bool mine = l < lm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;

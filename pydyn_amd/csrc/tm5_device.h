@@ -61,6 +61,16 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
+// The workgroup's (x, y) in a 2-d grid renumbered like xcd_tile: neighbouring tiles, which share
+// cache lines at their edges, run on the same XCD and so share its L2.
+__device__ __forceinline__ void xcd_grid2(int &bx, int &by)
+{
+    const int nbx = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    bx = t % nbx;
+    by = t / nbx;
+}
+
 __device__ __forceinline__ double shrink(double f, int n) { return f * (double)(n - 1) / (double)n; }
 
 // ------------------------------------------------------------------ the fdlibm exp
