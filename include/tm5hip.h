@@ -132,6 +132,29 @@ int tm5_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const d
 int tm5_convec(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
                const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
                const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax);
+/* convec on the columns mask (im, jm, i fastest, nonzero = act) marks: pydyn/gpu/kernels.py:351
+   convec's `cols`, the flat i * jm + j list, as a mask; register form only (lmax in 4, 5, 8, 10,
+   15, 20, 25, 26), else TM5_EINVAL. */
+int tm5_convec_masked(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                      const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
+                      const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax,
+                      const int *mask);
+/* advect_x_rows (pydyn/gpu/kernels.py:64): every (j, l) row of nrow[j] cells (NULL: im; a reduced
+   row's cells first), per row nloop and the largest |Courant| of the accepted trial into the
+   device arrays nloop (int, jm lm) and amax (double, jm lm; NULL: not kept), row j + jm l. */
+int tm5_advect_x_rows(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                      const tm5_view *am, const int *nrow, int limits, int max_nloop, int *nloop, double *amax);
+/* mass_x_rows (pydyn/gpu/kernels.py:420) with reduced rows: TM5_REHEARSAL when a row exceeds its cap. */
+int tm5_mass_x_rows(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_view *am, const int *nrow,
+                    int max_nloop);
+/* advect_z_substeps (pydyn/gpu/kernels.py:262): n_advz sub-steps with cm / n_advz (cm not
+   written); prelimit applies dynamw's limiter to rzm first. */
+int tm5_advect_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                          const tm5_view *cm, int n_advz, int limits, int prelimit);
+/* cluster_sum (pydyn/gpu/kernels.py:327): out (n, r1, r2) = seqsum over clusters of c cells of
+   q (n c, r1, r2) along its first axis, from each cluster's first term. */
+int tm5_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, const tm5_view *q, const tm5_view *out);
+
 /* dkg (im, jm, lmax) from kvh (im, jm, lm-1), m and gph (im, jm, lm+1).
    pydyn/diffusion.py:256-263 (device part of pd_calc_kzz, pydyn/gpu/csrc/kzz.cuh:91-242). */
 int tm5_dkg(tm5_ctx *ctx, int im, int jm, const tm5_view *kvh, const tm5_view *m, const tm5_view *gph, int lmax,

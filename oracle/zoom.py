@@ -823,7 +823,13 @@ def y_region1(tree, st, limits):
     g, f, lm = tree.g(1), st[1], tree.lm
     im, jm = g.im, g.jm
     layers = advect_y(*interior(f, im, jm, lm), f["bm"][1:im, 1 : jm + 1, 1:lm], limits)
-    bm = f["bm"]
+    layer_halo(f["bm"].data, im, jm, layers)
+
+
+def layer_halo(bm, im, jm, layers):
+    """Region 1's bm faces outside bm(1:im, 1:jm+1, l) (data index = Fortran index, lower bound
+    0) through dynamv's whole-layer scaling: per layer l+1 that sub-cycled, (n-1)/n for n = 2..
+    nloop, then times nloop (advect.py:212-226)."""
     for lev, nl in enumerate(layers, start=1):
         if nl == 1:
             continue

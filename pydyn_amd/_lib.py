@@ -124,6 +124,12 @@ _SIGNATURES = {
     "tm5_region_create": [_VP, _P(RegionDesc), _P(_VP)],
     "tm5_region_destroy": [_VP],
     "tm5_region_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
+    "tm5_advect_x_rows": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _VP, _I, _I, _VP, _VP],
+    "tm5_mass_x_rows": [_VP, _I, _I, _I, _P(View), _P(View), _VP, _I],
+    "tm5_advect_z_substeps": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I, _I],
+    "tm5_cluster_sum": [_VP, _I, _I, _I, _I, _P(View), _P(View)],
+    "tm5_convec_masked": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(View), _P(View), _P(View), _P(IView),
+                          _P(IView), _P(View), _D, _I, _VP],
     "tm5_region_ops": [_VP, _D, _I, ctypes.c_char_p],
     "tm5_region_run": [_VP, _D, _I, _I, _I, _P(_I), _P(_I)],
     "tm5_advect_adjoint": [_VP, _I, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(Tracers), _I, _I, _VP],

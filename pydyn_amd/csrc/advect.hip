@@ -27,8 +27,9 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
-                                          const int *imred)
+                                          const int *imred, double *amax_out)
 {
+    // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
     const int row = xcd_tile(blockIdx.x, gridDim.x);
@@ -46,18 +47,22 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     __syncthreads();
 
     int nl = 1;
+    double cmax = 0.0;
     for (;;) {
         bool ok = true;
+        cmax = 0.0;
         for (int k = lane; k <= n; k += 64) {
             const double ak = A[k];
             const int u = ak >= 0.0 ? (k == 0 ? n - 1 : k - 1) : (k == n ? 0 : k);
             const double alo = A[u], ahi = A[u + 1];
             double mx = M[u];
             for (int s = 0; s < nl; s++) {
-                if (!(fabs(ak / mx) < 1.0)) {
+                const double c = fabs(ak / mx);
+                if (!(c < 1.0)) {
                     ok = false;
                     break;
                 }
+                if (c > cmax) cmax = c;
                 mx = (mx + alo) - ahi;
             }
         }
@@ -75,6 +80,13 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         }
     }
     if (lane == 0) rows_out[row] = nl;  // one plain store per row: no contended atomic
+    if (amax_out) {  // the row's largest Courant number (a maximum: any order gives the same value)
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(cmax, off);
+            if (o > cmax) cmax = o;
+        }
+        if (lane == 0) amax_out[row] = cmax;
+    }
 
     if (MASS_ONLY) {  // each cell reads only itself and its two faces: in place
         for (int s = 0; s < nl; s++)
@@ -139,18 +151,19 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
 }
 
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask, const int *imred)
+             int skip_mask, const int *imred, int *rows_out, double *amax_out)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
     const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
+    int *rows = rows_out ? rows_out : ctx->x_rows;
     if (q.nt > 0)
         TM5_LAUNCH(ctx, k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   ctx->x_rows, fail_bit, skip_mask, imred);
+                   rows, fail_bit, skip_mask, imred, amax_out);
     else
         TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   ctx->x_rows, fail_bit, skip_mask, imred);
+                   rows, fail_bit, skip_mask, imred, amax_out);
     return status(hipGetLastError());
 }
 
@@ -500,8 +513,9 @@ __device__ __forceinline__ FaceFlux z_face(const View3 &m, const Tracers &q, con
 
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
-                                                  int limits, int *words, int skip_mask)
+                                                  int limits, int prelimit, int *words, int skip_mask)
 {
+    // prelimit: dynamw's limiter on rzm before the first sub-step (advect_z's, not the bare sub-steps')
     if (skipped(words, skip_mask)) return;
     int bx, j;
     xcd_grid2(bx, j);
@@ -516,7 +530,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
         const double m_old = m(i, j, ll);
         const double m_new = (m_old + c_lo) - c_hi;
         if (!MASS_ONLY) {
-            const bool lim0 = limits && s == 0;
+            const bool lim0 = prelimit && s == 0;
             for (int t = 0; t < q.nt; t++) {
                 const FaceFlux lo = z_face(m, q, cm, i, j, ll, t, lm, dn, lim0);
                 const FaceFlux hi = z_face(m, q, cm, i, j, ll + 1, t, lm, dn, lim0);
@@ -549,6 +563,11 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
 
 // The search over the whole region (the trial advancing only the scope's columns), then the
 // sub-steps on the scope: region 1's scope is the region, a child's its block's.
+__global__ void k_set_nadv(int *words, int n, int skip_mask)
+{
+    if (threadIdx.x == 0 && !skipped(words, skip_mask)) words[W_Z_NADV] = n;
+}
+
 // ------------------------------------------------------------------ Check_CFL's y and z at cap 1
 // The rehearsal allows dynamvm and dynamwm no sub-cycling (max_nloop 1, pydyn/zoom/check_cfl.py:
 // 258-290): a trial passes only if every interior face has |Courant| < 1 at nloop = 1, and then
@@ -662,10 +681,12 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (i0 * q.si + j0 * q.sj);
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
     if (mass_only)
-        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, ctx->words,
+        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
+                   ctx->words,
                    skip_mask);
     else
-        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, ctx->words,
+        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
+                   ctx->words,
                    skip_mask);
     return status(hipGetLastError());
 }
@@ -674,6 +695,19 @@ int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm,
              int skip_mask)
 {
     return launch_z_scoped(ctx, im, jm, lm, m, q, cm, 0, im - 1, 0, jm - 1, limits, nmax, fail_bit, skip_mask);
+}
+
+// dynamw's sub-steps alone with a given n_advz (left in the context's word for the kernel)
+int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int n_advz, int limits,
+                      int prelimit)
+{
+    if (im < 1 || jm < 1 || lm < 2 || n_advz < 1 || q.nt < 1) return TM5_EINVAL;
+    const int TI = lines_per_block(lm, 32);
+    if (TI < 1) return TM5_EINVAL;
+    TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
+    TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI,
+               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS);
+    return status(hipGetLastError());
 }
 
 }  // namespace tm5

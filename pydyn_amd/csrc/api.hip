@@ -334,6 +334,64 @@ extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m,
                          iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
 }
 
+// convec on the columns an (im, jm) mask (i fastest) marks; the others keep their tracers
+extern "C" int tm5_convec_masked(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                                 const tm5_view *entu, const tm5_view *detu, const tm5_view *entd,
+                                 const tm5_view *detd, const tm5_iview *cloud_top, const tm5_iview *cloud_lfs,
+                                 const tm5_view *dkg, double dt, int lmax, const int *mask)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(entu) || bad_view(detu) || bad_view(entd) || bad_view(detd) ||
+        bad_view(dkg) || !cloud_top || !cloud_lfs || !cloud_top->data || !cloud_lfs->data || !mask)
+        return TM5_EINVAL;
+    if (!column_supported(lmax) || lm > 64) return TM5_EINVAL;
+    ColumnArgs a{};
+    a.im = im;
+    a.jm = jm;
+    a.lm = lm;
+    a.m = view3(m);
+    a.q = tracers(q);
+    a.entu = view3(entu);
+    a.detu = view3(detu);
+    a.entd = view3(entd);
+    a.detd = view3(detd);
+    a.dkg = view3(dkg);
+    a.top = iview2(cloud_top);
+    a.lfs = iview2(cloud_lfs);
+    a.dt = dt;
+    a.owned = mask;
+    return launch_column(c, a, lmax, TM5_ERRORS);
+}
+
+extern "C" int tm5_advect_x_rows(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                                 const tm5_view *am, const int *nrow, int limits, int max_nloop, int *nloop,
+                                 double *amax)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(am)) return TM5_EINVAL;
+    return launch_x(c, im, jm, lm, view3(m), tracers(q), view3(am), limits, max_nloop - 1, TM5_FAIL_X, TM5_ERRORS, nrow,
+                    nloop, amax);
+}
+
+extern "C" int tm5_mass_x_rows(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_view *am,
+                               const int *nrow, int max_nloop)
+{
+    if (!c || bad_view(m) || bad_view(am)) return TM5_EINVAL;
+    return launch_x(c, im, jm, lm, view3(m), no_tracers(), view3(am), 0, max_nloop - 1, TM5_REHEARSAL, REHEARSAL_SKIP,
+                    nrow);
+}
+
+extern "C" int tm5_advect_z_substeps(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                                     const tm5_view *cm, int n_advz, int limits, int prelimit)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(cm)) return TM5_EINVAL;
+    return launch_z_substeps(c, im, jm, lm, view3(m), tracers(q), view3(cm), n_advz, limits, prelimit);
+}
+
+extern "C" int tm5_cluster_sum(tm5_ctx *c, int n, int cs, int r1, int r2, const tm5_view *q, const tm5_view *out)
+{
+    if (!c || bad_view(q) || bad_view(out)) return TM5_EINVAL;
+    return launch_cluster_sum(c, n, cs, r1, r2, view3(q), view3(out));
+}
+
 extern "C" int tm5_dkg(tm5_ctx *c, int im, int jm, const tm5_view *kvh, const tm5_view *m, const tm5_view *gph,
                        int lmax, const tm5_view *dkg)
 {

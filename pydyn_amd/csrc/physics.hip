@@ -114,6 +114,28 @@ __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m
     }
 }
 
+// ------------------------------------------------------------------ cluster sums
+// The reduced grid's seqsum over clusters (pydyn/redgrid.py:56-62 through ops.seqsum): out(k, a,
+// b) = q(k c, a, b) + q(k c + 1, a, b) + ... + q(k c + c - 1, a, b), summed from the first term.
+__global__ void __launch_bounds__(256) k_cluster_sum(int n, int c, int r1, int r2, View3 q, View3 out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * r1 * r2) return;
+    const int k = (int)(t % n), a = (int)((t / n) % r1), b = (int)(t / ((long long)n * r1));
+    double acc = q(k * c, a, b);
+    for (int s = 1; s < c; s++) acc = acc + q(k * c + s, a, b);
+    out(k, a, b) = acc;
+}
+
+int launch_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, View3 q, View3 out)
+{
+    if (n < 1 || c < 1 || r1 < 1 || r2 < 1) return TM5_EINVAL;
+    const long long total = (long long)n * r1 * r2;
+    TM5_LAUNCH(ctx, k_cluster_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, n, c, r1, r2, q,
+               out);
+    return status(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ dkg
 // pydyn/diffusion.py:256-263: dkg(l) = max(0, kvh(l)) 2 (m(l+1) + m(l)) / dz^2, dz = gph(l+2) -
 // gph(l), for l < lmax - 1; zero at lmax - 1.

@@ -185,8 +185,9 @@ inline IView2 iview2(const tm5_iview *v) { return IView2{v->data, v->stride[0], 
 inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 
 // launchers (advect.hip)
+// rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask, const int *imred = nullptr);
+             int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr);
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
              int skip_mask);
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
@@ -226,6 +227,9 @@ int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask)
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask);
+int launch_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, View3 q, View3 out);
+int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int n_advz, int limits,
+                      int prelimit);
 // Check_CFL's mass-only y and z with no sub-cycling allowed: the check at nloop 1 and its step
 int launch_y_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 bm, int fail_bit, int skip_mask);
 int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int fail_bit, int skip_mask);

@@ -18,6 +18,8 @@ __all__ = [
     "advect_x_rows",
     "advect_y",
     "advect_z",
+    "advect_z_substeps",
+    "cluster_sum",
     "convec",
     "decay",
     "dkg",
@@ -59,15 +61,56 @@ def advect_x(m, rm, rxm, rym, rzm, am, limits=True, max_nloop=50, deferred=False
 
 
 def advect_x_rows(m, rm, rxm, rym, rzm, a, nrow=None, limits=True, max_nloop=None, deferred=False):
-    """pydyn/gpu/kernels.py:64: advect_x on uniform rows (a reduced grid's nrow is not supported)."""
-    if nrow is not None and any(int(n) != m.shape[0] for n in nrow):
-        raise NotImplementedError("advect_x_rows: reduced rows are not supported")
-    return advect_x(m, rm, rxm, rym, rzm, a, limits, 50 if max_nloop is None else max_nloop, deferred)
+    """dynamu on every (j, l) row of a region at once (pydyn/gpu/kernels.py:64): m (n, jm, lm),
+    tracers (n, jm, lm, nt), a (n+1, jm, lm) with a[0] == a[n] per row, nrow (jm,) int32 device
+    array of the cells of each latitude (a reduced row's cells first; None: every row whole).
+    Returns the device arrays (nloop, alpha_max), one value per row j + jm l."""
+    torch = D._torch()
+    ctx = D.context()
+    im, jm, lm = _dims(m)
+    nloop = torch.empty(jm * lm, dtype=torch.int32, device="cuda")
+    amax = torch.empty(jm * lm, dtype=torch.float64, device="cuda")
+    rows = None if nrow is None else nrow.to(torch.int32).contiguous()
+    _lib.check(_lib.lib().tm5_advect_x_rows(ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm), D.view(a),
+                                            None if rows is None else rows.data_ptr(), int(limits),
+                                            50 if max_nloop is None else int(max_nloop), nloop.data_ptr(),
+                                            amax.data_ptr()), "tm5_advect_x_rows")  # fmt: skip
+    _finish(ctx, deferred)
+    return nloop, amax
 
 
-def advect_y(m, rm, rxm, rym, rzm, bm, limits=True, max_nloop=10, stats=None, deferred=False):
+def _layer_halo(full, bm, layers, level_offset):
+    """dynamv's whole-layer scaling of `full` outside its view bm (the reference's layer scale,
+    pydyn/gpu/csrc/advect.cuh:192-205 and pydyn/zoom/advect.py:212-226): per layer that
+    sub-cycled, the faces outside bm through the retry scalings (n-1)/n, n = 2..nloop, and back
+    times nloop.  bm is a slice of full with its strides."""
+    torch = D._torch()
+    off = bm.storage_offset() - full.storage_offset()
+    origin = []
+    for ax in sorted(range(3), key=lambda k: -full.stride(k)):
+        origin.append((ax, off // full.stride(ax)))
+        off %= full.stride(ax)
+    start = dict(origin)  # the view's first element in full, per axis
+    inside = torch.zeros(full.shape[:2], dtype=torch.bool, device=full.device)
+    inside[start[0] : start[0] + bm.shape[0], start[1] : start[1] + bm.shape[1]] = True
+    for lev, n in enumerate(layers):
+        n = int(n)
+        if n == 1:
+            continue
+        layer = full[:, :, lev + level_offset]
+        halo = layer[~inside]
+        for r in range(2, n + 1):
+            halo = halo * float(r - 1) / float(r)
+        layer[~inside] = halo * float(n)
+
+
+def advect_y(m, rm, rxm, rym, rzm, bm, limits=True, max_nloop=10, stats=None, full_bm=None, level_offset=0,
+             deferred=False):
     """dynamv, one nloop per layer (pydyn/gpu/kernels.py:103, pydyn/kernels.py:223-291).
-    Returns the per-layer nloop."""
+    full_bm: the array whose whole layers dynamv scales and restores (region 1's halo'd bm, bm a
+    slice of it), its levels offset by level_offset from bm's.  Returns the device array of the
+    layers' nloop; stats gets nloop_max (its 'xi' is not computed here: None)."""
+    torch = D._torch()
     ctx = D.context()
     im, jm, lm = _dims(m)
     _lib.check(_lib.lib().tm5_advect_y(ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm), D.view(bm),
@@ -76,14 +119,16 @@ def advect_y(m, rm, rxm, rym, rzm, bm, limits=True, max_nloop=10, stats=None, de
     if deferred:
         return None
     layers = ctx.nloop(lm)[1]
+    if full_bm is not None and full_bm is not bm:
+        _layer_halo(full_bm, bm, layers, level_offset)
     if stats is not None:
-        stats["nloop_max"] = int(layers.max())
-    return layers
+        stats["nloop_max"], stats["xi"] = int(layers.max()), None
+    return torch.as_tensor(layers, dtype=torch.int32).to("cuda")
 
 
 def advect_z(m, rm, rxm, rym, rzm, cm, limits=True, iter_limit=15, stats=None, deferred=False):
     """dynamw, one n_advz for the region (pydyn/gpu/kernels.py:227, pydyn/kernels.py:295-346).
-    Returns n_advz."""
+    Returns n_advz; stats gets nloop_max (its 'xi': None, as the reference's)."""
     ctx = D.context()
     im, jm, lm = _dims(m)
     _lib.check(_lib.lib().tm5_advect_z(ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm), D.view(cm),
@@ -93,8 +138,32 @@ def advect_z(m, rm, rxm, rym, rzm, cm, limits=True, iter_limit=15, stats=None, d
         return None
     n = ctx.nloop(0)[2]
     if stats is not None:
-        stats["nloop_max"] = n
+        stats["nloop_max"], stats["xi"] = n, None
     return n
+
+
+def advect_z_substeps(m, rm, rxm, rym, rzm, cm, n_advz, limits=True, prelimit=False):
+    """dynamw's n_advz sub-steps with cm / n_advz, cm not written (pydyn/gpu/kernels.py:262,
+    pydyn/kernels.py:324-346); prelimit applies dynamw's limiter to rzm first.  n_advz: an int
+    or a one-element device array."""
+    ctx = D.context()
+    im, jm, lm = _dims(m)
+    n = int(n_advz.reshape(-1)[0]) if hasattr(n_advz, "reshape") else int(n_advz)
+    _lib.check(_lib.lib().tm5_advect_z_substeps(ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm), D.view(cm),
+                                                n, int(limits), int(prelimit)), "tm5_advect_z_substeps")  # fmt: skip
+
+
+def cluster_sum(q, n, c):
+    """seqsum(clusters(q, n, c), axis=1) of a (cells, a[, b]) device array: a new (n, a[, b])
+    array, each cluster summed from its first cell (pydyn/gpu/kernels.py:327, pydyn/redgrid.py)."""
+    ctx = D.context()
+    v = q if q.dim() == 3 else q.unsqueeze(2)
+    if v.shape[0] < n * c:
+        raise ValueError("cluster_sum: fewer cells than n * c")
+    out = D.empty((n, v.shape[1], v.shape[2]))
+    _lib.check(_lib.lib().tm5_cluster_sum(ctx.ptr, int(n), int(c), v.shape[1], v.shape[2], D.view(v), D.view(out)),
+               "tm5_cluster_sum")  # fmt: skip
+    return out if q.dim() == 3 else out[..., 0]
 
 
 # ------------------------------------------------------------------ Check_CFL's mass-only kernels
@@ -110,11 +179,20 @@ def _mass(fn, m, flux, cap):
 
 
 def mass_x_rows(m, a, nrow=None, max_nloop=10):
-    """dynamum (pydyn/gpu/kernels.py:420, pydyn/cfl.py:140-167): True when every row stayed
-    under its cap (m, a updated), False otherwise."""
-    if nrow is not None and any(int(n) != m.shape[0] for n in nrow):
-        raise NotImplementedError("mass_x_rows: reduced rows are not supported")
-    return _mass("tm5_mass_x", m, a, max_nloop)
+    """dynamum (pydyn/gpu/kernels.py:420, pydyn/cfl.py:140-167) on rows of nrow[j] cells (None:
+    every row whole): True when every row stayed under its cap (m, a updated), False otherwise."""
+    torch = D._torch()
+    ctx = D.context()
+    ctx.flags(clear=True)
+    im, jm, lm = _dims(m)
+    rows = None if nrow is None else nrow.to(torch.int32).contiguous()
+    _lib.check(_lib.lib().tm5_mass_x_rows(ctx.ptr, im, jm, lm, D.view(m), D.view(a),
+                                          None if rows is None else rows.data_ptr(), int(max_nloop)),
+               "tm5_mass_x_rows")  # fmt: skip
+    f = ctx.flags(clear=True)
+    if f & _lib.ERRORS:
+        raise ValueError(f"tm5_mass_x_rows: device error flags {f}")
+    return not (f & _lib.REHEARSAL)
 
 
 def mass_y(m, bm, max_nloop=1):
@@ -146,16 +224,26 @@ def m2phlb1(m, dxyp, at, bt, sp, phlb):
 
 
 # ------------------------------------------------------------------ physics
-def convec(m, rm, rxm, rym, rzm, conv, dt, lmax):
-    """convec on every column (pydyn/gpu/kernels.py:351, pydyn/kernels.py:423-457): conv holds
-    device arrays entu, detu, entd, detd, dkg (float64) and cloud_top, cloud_lfs (int32)."""
+def convec(m, rm, rxm, rym, rzm, conv, dt, lmax, cols=None, inverse="crout"):
+    """convec (pydyn/gpu/kernels.py:351, pydyn/kernels.py:423-457): conv holds device arrays
+    entu, detu, entd, detd, dkg (float64) and cloud_top, cloud_lfs (int32).  cols: the columns
+    to act on (device int32, flat i * jm + j; None: every column).  inverse: 'crout' (fastminv,
+    the reference's default); the LAPACK kind is not provided."""
+    if inverse != "crout":
+        raise ValueError(f"convec: inverse {inverse!r}: only 'crout' (fastminv) is provided")
+    torch = D._torch()
     ctx = D.context()
     im, jm, lm = _dims(m)
-    _lib.check(_lib.lib().tm5_convec(
-        ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm),
-        *(D.view(conv[k]) for k in ("entu", "detu", "entd", "detd")),
-        D.iview(conv["cloud_top"]), D.iview(conv["cloud_lfs"]), D.view(conv["dkg"]), float(dt), int(lmax),
-    ), "tm5_convec")  # fmt: skip
+    args = (ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm),
+            *(D.view(conv[k]) for k in ("entu", "detu", "entd", "detd")),
+            D.iview(conv["cloud_top"]), D.iview(conv["cloud_lfs"]), D.view(conv["dkg"]), float(dt), int(lmax))  # fmt: skip
+    if cols is None:
+        _lib.check(_lib.lib().tm5_convec(*args), "tm5_convec")
+        return
+    flat = cols.to(torch.int64).reshape(-1)
+    mask = torch.zeros(jm, im, dtype=torch.int32, device="cuda")  # (j, i): i fastest
+    mask[flat % jm, flat // jm] = 1
+    _lib.check(_lib.lib().tm5_convec_masked(*args, mask.data_ptr()), "tm5_convec_masked")
 
 
 def dkg(kvh, m, gph, lmax):

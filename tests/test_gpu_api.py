@@ -1,0 +1,203 @@
+"""The operator layer's names beyond what the step itself calls (pydyn/gpu/kernels.py): reduced
+rows and the per-row Courant maximum of advect_x_rows / mass_x_rows, bare z sub-steps, cluster
+sums, convection on a column list, and dynamv on a slice of region 1's halo'd bm.  Each against
+the oracle on the same inputs, bit for bit.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("m", "rm", "rxm", "rym", "rzm")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pydyn_amd import device as D
+    from pydyn_amd.gpu import kernels as K
+
+    D.context().flags(clear=True)
+    return D, K
+
+
+def assert_bitwise(got, want, what=""):
+    got, want = np.asarray(got, dtype=float), np.asarray(want, dtype=float)
+    assert got.shape == want.shape, what
+    differ = got.view(np.int64) != want.view(np.int64)
+    assert not differ.any(), f"{what}: {differ.sum()} of {differ.size} differ"
+
+
+def records(op, seed=3):
+    """The inputs (m, flux, tracers) of every `op` operator of one hourly interval of the small
+    grid, as the oracle's run recorded them, with slopes the limiter clips."""
+    from oracle import adjoint as A
+    from pydyn_amd import meteo
+
+    met = meteo.Meteo(meteo.grid("small"), seed=seed)
+    rm = met.state()["rm"]
+    rng = np.random.default_rng(5)
+    init = [rm] + [2.0 * rm * rng.uniform(-1.0, 1.0, rm.shape) for _ in range(3)]
+    W = A.Window(met, 1, init=init)
+    return [(m.copy(), flux.copy(), [x.copy() for x in Q]) for o, _, m, flux, Q in W.ops if o == op]
+
+
+def test_advect_x_rows_whole_rows(dev):
+    from oracle import advection
+
+    D, K = dev
+    m, a, q = records("x")[0]
+    a = 3.0 * a  # rows sub-cycle
+    im, jm, lm = m.shape
+    m0, a0 = m.copy(), a.copy()
+    dm, dq, da = D.upload(m), [D.upload(x) for x in q], D.upload(a)
+    nloop, amax = K.advect_x_rows(dm, *dq, da, None)
+    want_n = advection.advect_x(m, *q, a, True)  # in place; (jm, lm)
+    for k, (g, w) in enumerate(zip([dm, *dq, da], [m, *q, a])):
+        assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "am")
+    got_n = nloop.cpu().numpy().reshape(lm, jm).T  # row j + jm l
+    assert np.array_equal(got_n, want_n)
+    assert want_n.max() > 1 and want_n.min() == 1  # some rows sub-cycle, some not
+    # rows that did not sub-cycle: the largest |Courant| of their one trial sub-step
+    up_lo = np.concatenate([m0[-1:], m0], axis=0)  # upwind cell of face k: k-1 (periodic) or k
+    up_hi = np.concatenate([m0, m0[:1]], axis=0)
+    c = np.abs(a0 / np.where(a0 >= 0.0, up_lo, up_hi))
+    got_a = amax.cpu().numpy().reshape(lm, jm).T
+    one = want_n == 1
+    assert np.array_equal(got_a[one], c.max(axis=0)[one])
+    assert np.all((got_a > 0.0) & (got_a < 1.0))
+
+
+@pytest.mark.parametrize("mass_only", [False, True])
+def test_x_rows_reduced(dev, mass_only):
+    """Rows of nrow[j] cells (a reduced latitude's clusters first): each the periodic row of its
+    first nrow[j] cells and faces 0..nrow[j]; the cells beyond untouched."""
+    from oracle import advection, cfl
+
+    D, K = dev
+    torch = D._torch()
+    m, a, q = records("x")[0]
+    a = 2.5 * a  # every row sub-cycles
+    im, jm, lm = m.shape
+    nrow = np.array([im if j in (0, jm - 1) else (im // 2, im // 3, im // 4)[j % 3] for j in range(jm)],
+                    dtype=np.int32)  # fmt: skip
+    for j, n in enumerate(nrow):
+        a[n, j] = a[0, j]  # a reduced row closes at its own last face
+    dm, dq, da = D.upload(m), [D.upload(x) for x in q], D.upload(a)
+    rows = torch.as_tensor(nrow, device="cuda")
+    if mass_only:
+        ok = K.mass_x_rows(dm, da, rows, max_nloop=10)
+        assert ok
+    else:
+        nloop, _ = K.advect_x_rows(dm, *dq, da, rows)
+    for j, n in enumerate(nrow):
+        sm, sa = m[:n, j : j + 1].copy(), a[: n + 1, j : j + 1].copy()
+        sq = [x[:n, j : j + 1].copy() for x in q]
+        if mass_only:
+            assert cfl.mass_x(sm, sa, 10)
+        else:
+            want = advection.advect_x(sm, *sq, sa, True)
+            assert np.array_equal(nloop.cpu().numpy().reshape(lm, jm)[:, j], want[0])
+            assert want.max() > 1
+        assert_bitwise(D.download(dm)[:n, j], sm[:, 0], f"m row {j}")
+        assert_bitwise(D.download(dm)[n:, j], m[n:, j], f"m beyond row {j}")
+        assert_bitwise(D.download(da)[: n + 1, j], sa[:, 0], f"am row {j}")
+        if not mass_only:
+            for k in range(4):
+                assert_bitwise(D.download(dq[k])[:n, j], sq[k][:, 0], f"{FIELDS[k + 1]} row {j}")
+                assert_bitwise(D.download(dq[k])[n:, j], q[k][n:, j], f"{FIELDS[k + 1]} beyond row {j}")
+
+
+@pytest.mark.parametrize("prelimit", [False, True])
+@pytest.mark.parametrize("n_advz", [1, 3])
+def test_advect_z_substeps(dev, n_advz, prelimit):
+    from oracle import advection
+    from oracle.lines import limit
+
+    D, K = dev
+    m, cm, q = records("z")[0]
+    dm, dq, dc = D.upload(m), [D.upload(x) for x in q], D.upload(cm)
+    K.advect_z_substeps(dm, *dq, dc, n_advz, True, prelimit)
+    if prelimit:
+        q[3][...] = limit(q[3], q[0])
+    advection.z_substeps(m, *q, cm, n_advz, True)
+    for k, (g, w) in enumerate(zip([dm, *dq, dc], [m, *q, cm])):
+        assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "cm")
+
+
+def test_cluster_sum(dev):
+    D, K = dev
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal((30, 4, 3))
+    got = D.download(K.cluster_sum(D.upload(q), 5, 6))
+    want = q[0:30:6].copy()
+    for s in range(1, 6):  # each cluster from its first cell, in index order
+        want = want + q[s:30:6]
+    assert_bitwise(got, want, "3-d")
+    got2 = D.download(K.cluster_sum(D.upload(q[:, :, 0]), 10, 3))
+    want2 = (q[0:30:3, :, 0] + q[1:30:3, :, 0]) + q[2:30:3, :, 0]
+    assert_bitwise(got2, want2, "2-d")
+
+
+def test_convec_on_a_column_list(dev):
+    """convec's `cols` (flat i * jm + j): the listed columns as the whole-region operator leaves
+    them, the others unchanged."""
+    import torch
+
+    from oracle import convection
+    from pydyn_amd import meteo
+
+    D, K = dev
+    met = meteo.Meteo(meteo.grid("small"), seed=5)
+    st = met.state()
+    g = met.g
+    im, jm = g.im, g.jm
+    rng = np.random.default_rng(5)
+    cols = np.sort(rng.choice(im * jm, size=im * jm // 3, replace=False)).astype(np.int32)
+    from oracle import physics
+
+    dkg = physics.dkg(met.kvh, met.m, met.gph, g.lmax)
+    conv = {k: D.upload(getattr(met, k)) for k in ("entu", "detu", "entd", "detd")}
+    conv["dkg"] = D.upload(dkg)
+    for k in ("cloud_top", "cloud_lfs"):
+        conv[k] = D.upload(getattr(met, k).astype(np.int32)).to(torch.int32)
+    dst = {k: D.upload(st[k]) for k in FIELDS}
+    K.convec(*(dst[k] for k in FIELDS), conv, 1800.0, g.lmax, cols=torch.as_tensor(cols, device="cuda"))
+    full = {k: st[k].copy() for k in FIELDS}
+    convection.convec(*(full[k] for k in FIELDS), met.entu, met.detu, met.entd, met.detd, met.cloud_top,
+                      met.cloud_lfs, dkg, 1800.0, g.lmax)  # fmt: skip
+    listed = np.zeros((im, jm), dtype=bool)
+    listed[cols // jm, cols % jm] = True
+    for k in FIELDS[1:]:
+        got = D.download(dst[k])
+        assert_bitwise(got[listed], full[k][listed], f"{k} listed")
+        assert_bitwise(got[~listed], st[k][~listed], f"{k} not listed")
+
+
+def test_advect_y_on_region1_halo(dev):
+    """dynamv on the interior slice of region 1's halo'd bm (0:im+1, 0:jm+1, 0:lm+1): the
+    faces outside it through the whole-layer scaling when a layer sub-cycles."""
+    from oracle import advection
+    from oracle.zoom import layer_halo
+
+    D, K = dev
+    m, b, q = records("y")[0]
+    im, jm, lm = m.shape
+    rng = np.random.default_rng(6)
+    full = rng.uniform(-1.0, 1.0, (im + 2, jm + 2, lm + 2)) * np.abs(b).max()
+    bm = full[1 : im + 1, 1 : jm + 2, 1 : lm + 1]  # data index = Fortran index (lower bound 0)
+    bm[...] = 3.0 * b  # layers sub-cycle
+    dm, dq, dfull = D.upload(m), [D.upload(x) for x in q], D.upload(full)
+    dbm = dfull[1 : im + 1, 1 : jm + 2, 1 : lm + 1]
+    nloop = K.advect_y(dm, *dq, dbm, full_bm=dfull, level_offset=1)
+    layers = advection.advect_y(m, *q, bm, True)
+    assert int(np.max(layers)) > 1  # some layers sub-cycle
+    layer_halo(full, im, jm, layers)
+    assert np.array_equal(nloop.cpu().numpy(), layers)
+    for k, (g, w) in enumerate(zip([dm, *dq], [m, *q])):
+        assert_bitwise(D.download(g), w, FIELDS[k])
+    assert_bitwise(D.download(dfull), full, "full bm")
