@@ -15,7 +15,18 @@ import csv
 import glob
 import json
 import os
+import re
 import sys
+
+
+def _short(kernel):
+    """The library's own name of a kernel (tm5_profile_*): the column kernel's template flag
+    dropped, its adjoint named as such."""
+    short = kernel.split("(")[0].replace("void ", "").replace("tm5::", "")
+    m = re.fullmatch(r"k_column<(\d+), (true|false)>", short)
+    if m:
+        return f"k_column<{m.group(1)}>" if m.group(2) == "false" else "k_column_adjoint"
+    return short
 
 
 def _csv(d, suffix):
@@ -56,7 +67,7 @@ def traffic(fetch_dir, write_dir, copy_bytes):
     for k in fetch:
         if k in write:
             b = fetch[k][1] * 1024.0 * kf + write[k][1] * 1024.0 * kw
-            short = k.split("(")[0].replace("void ", "").replace("tm5::", "")
+            short = _short(k)
             out[short] = {"bytes_per_launch": round(b), "fetch_kib": round(fetch[k][1], 1),
                           "write_kib": round(write[k][1], 1), "dispatches": fetch[k][0]}  # fmt: skip
     return out
