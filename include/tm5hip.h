@@ -278,6 +278,13 @@ int tm5_zoom_advect_y(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int li
 int tm5_zoom_advect_z(tm5_ctx *ctx, const tm5_zregion *r, int is, int ie, int js, int je, int limits,
                       int iter_limit, int mass_only);
 
+/* sources_region's do_add_2d (pydyn/zoom/physics.py:55-64): rm + x, rzm - x at 1-based (level,
+   tracer) on the cells the region owns; x (im, jm) in kg. */
+int tm5_zoom_inject(tm5_ctx *ctx, const tm5_zregion *r, const tm5_view *x, int level, int tracer);
+/* chemistry_region (pydyn/zoom/physics.py:67-78): rm and the slopes of 1-based tracers t0 ..
+   t0+count-1 times decay (the scope's factors, (ier-isr+1, jer-jsr+1, lm)) on the owned cells. */
+int tm5_zoom_decay(tm5_ctx *ctx, const tm5_zregion *r, const tm5_view *decay, int t0, int count);
+
 /* The column physics of a region on the cells it owns: convec (convection and Kzz diffusion,
    dt_conv), do_add_2d (x = ((emis factor) ratio) efrac at level/tracer) and the CH4 decay
    (exp(-(sum of the four loss rates) dt_chem)).  `ops` is the sequence run in one pass over

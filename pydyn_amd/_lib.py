@@ -151,6 +151,8 @@ _SIGNATURES = {
     "tm5_zoom_advect_z": [_VP, _P(ZRegion), _I, _I, _I, _I, _I, _I, _I],
     "tm5_zoom_physics": [_VP, _P(ZRegion), _P(ZPhysics), ctypes.c_char_p],
     "tm5_zoom_copy": [_VP, _VP, _VP, ctypes.c_int64],
+    "tm5_zoom_inject": [_VP, _VP, _P(View), _I, _I],
+    "tm5_zoom_decay": [_VP, _VP, _P(View), _I, _I],
     "tm5_column_adjoint": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(ZPhysics), ctypes.c_char_p, _P(View)],
     "tm5_zoom_scale": [_VP, _VP, ctypes.c_int64, _D],
 }

@@ -1137,6 +1137,64 @@ extern "C" int tm5_zoom_advect_z(tm5_ctx *c, const tm5_zregion *r, int is, int i
                            fail, zskipmask(mass_only));
 }
 
+// ------------------------------------------------------------------ sources and chemistry alone
+// sources_region's do_add_2d (pydyn/zoom/physics.py:55-64): rm + x, rzm - x at (level, tracer)
+// (1-based) on the cells the region owns; x (im, jm).
+__global__ void __launch_bounds__(256) k_zoom_inject(ZR R, View3 x, int level, int tracer, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)R.im * R.jm) return;
+    const int i = (int)(idx % R.im), j = (int)(idx / R.im);
+    if (!R.owned[(long long)j * R.im + i]) return;
+    const double v = x(i, j, 0);
+    double &rm = R.q[0](i + 1, j + 1, level, tracer);
+    double &rz = R.q[3](i + 1, j + 1, level, tracer);
+    rm = rm + v;
+    rz = rz - v;
+}
+
+// chemistry_region (pydyn/zoom/physics.py:67-78): rm and the slopes of tracers t0..t0+count-1
+// (1-based) times the factor, decay (the scope's, (ier-isr+1, jer-jsr+1, lm)) on the owned cells
+// and 1.0 elsewhere.
+__global__ void __launch_bounds__(256) k_zoom_decay(ZR R, View3 decay, int t0, int count, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)R.im * R.jm * R.lm) return;
+    const int i = (int)(idx % R.im), j = (int)((idx / R.im) % R.jm), l = (int)(idx / ((long long)R.im * R.jm));
+    const int fi = i + 1, fj = j + 1;
+    double f = 1.0;
+    if (fi >= R.isr && fi <= R.ier && fj >= R.jsr && fj <= R.jer) f = decay(fi - R.isr, fj - R.jsr, l);
+    if (!R.owned[(long long)j * R.im + i]) f = 1.0;
+    for (int t = t0; t < t0 + count; t++)
+        for (int a = 0; a < 4; a++) {
+            double &q = R.q[a](fi, fj, l + 1, t);
+            q = q * f;
+        }
+}
+
+extern "C" int tm5_zoom_inject(tm5_ctx *c, const tm5_zregion *r, const tm5_view *x, int level, int tracer)
+{
+    if (!c || zbad(r) || !x || !x->data || !r->owned || level < 1 || level > r->lm || tracer < 1 || tracer > r->nt)
+        return TM5_EINVAL;
+    const ZR R = zr(*r);
+    TM5_LAUNCH(c, k_zoom_inject, zgrid((long long)R.im * R.jm), dim3(256), 0, c->stream, R, view3(x), level, tracer,
+               c->words, TM5_ERRORS);
+    return status(hipGetLastError());
+}
+
+extern "C" int tm5_zoom_decay(tm5_ctx *c, const tm5_zregion *r, const tm5_view *decay, int t0, int count)
+{
+    if (!c || zbad(r) || !decay || !decay->data || !r->owned || t0 < 1 || count < 0 || t0 + count - 1 > r->nt)
+        return TM5_EINVAL;
+    if (count == 0) return 0;
+    const ZR R = zr(*r);
+    TM5_LAUNCH(c, k_zoom_decay, zgrid((long long)R.im * R.jm * R.lm), dim3(256), 0, c->stream, R, view3(decay), t0,
+               count, c->words, TM5_ERRORS);
+    return status(hipGetLastError());
+}
+
 extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops)
 {
     if (!c || zbad(r) || !p || !ops || !column_supported(p->lmax) || r->lm > 64 || p->lmax > r->lm) return TM5_EINVAL;

@@ -16,9 +16,10 @@ from .. import device as D
 __all__ = [
     "_coarsen_block", "_dynamu_child", "_dynamu_region1", "_dynamum_child", "_dynamum_region1",
     "_dynamv_child", "_dynamv_region1", "_dynamvm_child", "_dynamvm_region1", "_dynamw_child",
-    "_dynamw_region1", "_dynamwm", "_uncompress_xedges", "_uncompress_yedges", "coarsen_ambm",
-    "column_physics", "dynam0", "m2phlb", "m2phlb1", "mix_edges", "outer_x_faces_zeroed",
-    "outer_y_faces_zeroed", "put_xedges", "put_yedges", "restore", "save", "scale", "update_parent",
+    "_dynamw_region1", "_dynamwm", "_uncompress_xedges", "_uncompress_yedges", "assign", "chemistry_region",
+    "coarsen_ambm", "column_physics", "convec_region", "dynam0", "m2phlb", "m2phlb1", "mix_edges",
+    "outer_x_faces_zeroed", "outer_y_faces_zeroed", "put_xedges", "put_yedges", "restore", "save", "scale",
+    "sources_region", "update_parent",
 ]  # fmt: skip
 
 
@@ -160,6 +161,43 @@ def column_physics(zoom, st, region, phys, ops):
     """convec with the sources and chemistry around it (pydyn/zoom/physics.py:34-78) in one
     pass over the region's owned columns; `ops` one of v, csv, vsc, vsccsv, vscMcsv."""
     _call("tm5_zoom_physics", st.ptr(region), ctypes.byref(phys), ops.encode())
+
+
+def convec_region(zoom, st, region, conv, ndyn, inverse="crout"):
+    """convec on the columns the region owns (pydyn/zoom/physics.py:33-52), dt = ndyn / (2 tref):
+    conv holds device arrays entu, detu, entd, detd, dkg (im, jm, >= lmax) and cloud_top,
+    cloud_lfs (im, jm) int32, plus 'lmax'.  inverse: 'crout' only."""
+    if inverse != "crout":
+        raise ValueError(f"convec_region: inverse {inverse!r}: only 'crout' (fastminv) is provided")
+    p = _lib.ZPhysics()
+    for k in ("entu", "detu", "entd", "detd", "dkg"):
+        setattr(p, k, D.view(conv[k]))
+    p.cloud_top, p.cloud_lfs = D.iview(conv["cloud_top"]), D.iview(conv["cloud_lfs"])
+    p.lmax = int(conv["lmax"])
+    p.dt_conv = float(ndyn) / (2.0 * zoom.g(region).tref)
+    _call("tm5_zoom_physics", st.ptr(region), ctypes.byref(p), b"v")
+
+
+def sources_region(zoom, st, region, injections):
+    """do_add_2d on the cells the region owns (pydyn/zoom/physics.py:55-64): each (x (im, jm)
+    device array in kg, level, tracer), 1-based, added to rm and subtracted from rzm."""
+    for x, level, tracer in injections:
+        _call("tm5_zoom_inject", st.ptr(region), ctypes.byref(D.view(x)), int(level), int(tracer))
+
+
+def chemistry_region(zoom, st, region, decay, tracers):
+    """rm and the slopes of the 1-based `tracers` times `decay` (the scope's factors, a device
+    array (ier-isr+1, jer-jsr+1, lm)) on the owned cells (pydyn/zoom/physics.py:67-78)."""
+    v = D.view(decay)
+    for t in tracers:
+        _call("tm5_zoom_decay", st.ptr(region), ctypes.byref(v), int(t), 1)
+
+
+def assign(dst, src, region, names):
+    """dst's named arrays of the region set to src's (pydyn/zoom/host.py:51-54)."""
+    for k in names:
+        a, b = src[region][k], dst[region][k]
+        _call("tm5_zoom_copy", a.t.data_ptr(), b.t.data_ptr(), b.numel())
 
 
 # ------------------------------------------------------------------ save / restore / scale

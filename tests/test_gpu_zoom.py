@@ -159,3 +159,49 @@ def test_zoom_ensemble_members_equal_their_plain_runs(dev, base, ntr, members, i
             assert run.interval() == nd, (it, m)
     for m, run in plains.items():
         _member_slices_equal(batched, run, m, ntr)
+
+
+# ------------------------------------------------------------------ the physics bodies alone
+def test_zoom_physics_bodies_match_oracle(dev):
+    """convec_region, sources_region, chemistry_region and assign of pydyn_amd.zoom.lib (the
+    body names of pydyn/zoom/host.py:42-48) on both regions of the tiny tree, against the
+    oracle's bodies on the same state."""
+    import torch
+
+    from oracle import zoom as OZ
+    from pydyn_amd import meteo
+    from pydyn_amd.zoom import lib as L
+    from pydyn_amd.zoom.run import ZoomRun
+
+    D = dev
+    pmet, cmet = meteo.zoom("tiny2")
+    run, orun = ZoomRun(pmet, cmet), OZ.ZoomRun(pmet, cmet)
+    run.interval(), orun.interval()  # a state with slopes
+    assert _first_difference(run.host(), orun.st) is None
+    rng = np.random.default_rng(9)
+    tree, otree = run.zoom, orun.tree
+    for r in (1, 2):
+        g = tree.g(r)
+        conv = {k: D.upload(np.asarray(v)) for k, v in orun.conv[r].items() if k not in ("cloud_top", "cloud_lfs", "lmax")}
+        for k in ("cloud_top", "cloud_lfs"):
+            conv[k] = D.upload(np.asarray(orun.conv[r][k]).astype(np.int32)).to(torch.int32)
+        conv["lmax"] = orun.conv[r]["lmax"]
+        L.convec_region(tree, run.st, r, conv, 1800)
+        OZ.convec_region(otree, orun.st, r, orun.conv[r], 1800)
+        assert _first_difference(run.host(), orun.st) is None, f"convec_region {r}"
+        x = rng.uniform(0.0, 1e-3, (g.im, g.jm)) * np.abs(orun.st[r]["rm"].data).max()
+        L.sources_region(tree, run.st, r, [(D.upload(x), 1, 1)])
+        OZ.sources_region(otree, orun.st, r, [(x, 1, 1)])
+        assert _first_difference(run.host(), orun.st) is None, f"sources_region {r}"
+        sc = OZ.scope(otree, r)
+        shape = (sc[0].stop - sc[0].start, sc[1].stop - sc[1].start, tree.lm)
+        decay = rng.uniform(0.5, 1.0, shape)
+        L.chemistry_region(tree, run.st, r, D.upload(decay), [1])
+        OZ.chemistry_region(otree, orun.st, r, decay, [1])
+        assert _first_difference(run.host(), orun.st) is None, f"chemistry_region {r}"
+    # assign: region 2's tracers from a second run's
+    run2 = ZoomRun(pmet, cmet)
+    L.assign(run.st, run2.st, 2, ("rm", "rzm"))
+    got, want = run.host(), run2.host()
+    for k in ("rm", "rzm"):
+        assert np.array_equal(got[2][k].view(np.int64), want[2][k].view(np.int64)), k
