@@ -64,8 +64,8 @@ typedef struct tm5_iview {
 #define TM5_FAIL_XC 64   /* a child's dynamu: nloop too high (CFLError "dynuamu: nloop too high") */
 #define TM5_FAIL_YC 128  /* a child's dynamv: nloop too high (CFLError "nloop too high in dynamv") */
 #define TM5_ERRORS 223   /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
-#define TM5_SPEC 256     /* tm5_region_run: a queued interval's rehearsal failed; the intervals queued after it
-                            did nothing (cleared before tm5_region_run returns) */
+#define TM5_SPEC 256     /* tm5_region_run: a queued interval's trial or checked step disagreed with its plan;
+                            the kernels queued after it did nothing (cleared before tm5_region_run returns) */
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
@@ -201,11 +201,16 @@ int tm5_region_interval(tm5_region *region, double t0, int nread, int ndyn_max, 
 int tm5_region_ops(tm5_region *region, double t, int ndyn, const char *ops);
 /* count intervals of nread seconds from t0, as count calls of tm5_region_interval would run
    them (interval k starts at t0 + k nread), with the host reading the device once per batch
-   instead of once per interval: every interval is queued with its rehearsal at (ndyn_max,
-   nread / ndyn_max), its steps skipping themselves on the device if the rehearsal fails; when
-   one fails, the run resumes from it through tm5_region_interval (its Check_CFL cut), then
-   queues the rest again.  ndyn[k], n[k]: what Check_CFL chose for interval k.  The reference's
-   queued submission is Pipeline.submit (pydyn/gpu/zoom.py:944-952). */
+   instead of once per interval.  Every interval is queued on a planned step (the previous
+   interval's): the Check_CFL trials the plan expects to fail are rehearsed and their verdicts
+   checked on the device, and the planned steps run as checked steps, which make the passing
+   rehearsal's tests themselves on the same bits of m (x rows at nloop <= 9, y and z faces at
+   nloop 1) and keep the interval's starting m and moments in a backup.  When a verdict or a
+   check disagrees with the plan, the kernels queued after it skip; the interval's state comes
+   back from the backup, the run resumes from it through tm5_region_interval, then queues the
+   rest again.  ndyn[k], n[k]: what Check_CFL chose for interval k.  The reference's queued
+   submission is Pipeline.submit (pydyn/gpu/zoom.py:944-952).  The backup (m and four moments
+   per tracer, im jm lm doubles each) is allocated on the first call. */
 int tm5_region_run(tm5_region *region, double t0, int nread, int ndyn_max, int count, int *ndyn, int *n);
 
 

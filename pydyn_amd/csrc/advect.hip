@@ -24,12 +24,25 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // k-1 and k, face 0 the same as face n).  The search (kernels.py:169-199): nloop = 1, 2, ...,
 // each trial carrying every face's upwind cell through nloop mass-only sub-steps; on a failure
 // every face is scaled by (nloop-1)/nloop in place.
+// A checked step's cap failure (spec_k: its interval in the queue): the kernels queued after it
+// skip, and the host restores the interval's state from the backup before redoing it.
+__device__ __forceinline__ void spec_fail(int *words, int spec_k)
+{
+    words[W_SPEC_AT] = spec_k;  // every raiser of one launch writes the same interval
+    words[W_SPEC_KIND] = 1;
+    __threadfence();
+    atomicOr(words + W_FLAGS, TM5_SPEC);
+}
+
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
-                                          const int *imred, double *amax_out)
+                                          const int *imred, double *amax_out, int spec_k, double *bk)
 {
     // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
+    // spec_k >= 0: a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
+    // bk (whole rows only): the row's m and moments as read, i fastest, m first and then every
+    // tracer's rm, rxm, rym, rzm (each im * jm * lm)
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
     const int row = xcd_tile(blockIdx.x, gridDim.x);
@@ -42,7 +55,12 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     double *R = A + n + 1; // 4n  rm, rxm, rym, rzm of one tracer
     double *F = R + 4 * n; // 4(n+1) f, pf, ft (=fy), fw (=fz)
 
-    for (int i = lane; i < n; i += 64) M0[i] = M[i] = m(i, j, l);
+    const long long ncell = (long long)im * jm * lm, brow = (long long)im * (j + (long long)jm * l);
+    for (int i = lane; i < n; i += 64) {
+        const double v = m(i, j, l);
+        M0[i] = M[i] = v;
+        if (bk) bk[brow + i] = v;
+    }
     for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
     __syncthreads();
 
@@ -73,9 +91,18 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         __syncthreads();
         if (nl > nmax) {
             if (lane == 0) {
-                atomicOr(words + W_FLAGS, fail_bit);
+                if (spec_k >= 0)
+                    spec_fail(words, spec_k);
+                else
+                    atomicOr(words + W_FLAGS, fail_bit);
                 rows_out[row] = nl;
             }
+            if (!MASS_ONLY && bk)  // the row's moments are not read below
+                for (int t = 0; t < q.nt; t++)
+                    for (int i = lane; i < n; i += 64) {
+                        const long long o = q.off(i, j, l, t);
+                        for (int a = 0; a < 4; a++) bk[(1 + 4 * t + a) * ncell + brow + i] = q.q[a][o];
+                    }
             return;
         }
     }
@@ -99,11 +126,18 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
             }
             for (int i = lane; i < n; i += 64) {
                 const long long o = q.off(i, j, l, t);
-                const double rm = q.q[0][o];
+                const double rm = q.q[0][o], rx = q.q[1][o], ry = q.q[2][o], rz = q.q[3][o];
                 R[i] = rm;
-                R[n + i] = limits ? limit(q.q[1][o], rm) : q.q[1][o];
-                R[2 * n + i] = q.q[2][o];
-                R[3 * n + i] = q.q[3][o];
+                R[n + i] = limits ? limit(rx, rm) : rx;
+                R[2 * n + i] = ry;
+                R[3 * n + i] = rz;
+                if (bk) {
+                    double *b = bk + (1 + 4 * t) * ncell + brow + i;
+                    b[0] = rm;
+                    b[ncell] = rx;
+                    b[2 * ncell] = ry;
+                    b[3 * ncell] = rz;
+                }
             }
             __syncthreads();
             for (int s = 0; s < nl; s++) {
@@ -151,19 +185,20 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
 }
 
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask, const int *imred, int *rows_out, double *amax_out)
+             int skip_mask, const int *imred, int *rows_out, double *amax_out, int spec_k, double *backup)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
+    if (backup && imred) return TM5_EINVAL;  // the backup holds whole rows
     const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
     int *rows = rows_out ? rows_out : ctx->x_rows;
     if (q.nt > 0)
         TM5_LAUNCH(ctx, k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out);
+                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup);
     else
         TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out);
+                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup);
     return status(hipGetLastError());
 }
 
@@ -294,9 +329,12 @@ __device__ __forceinline__ FaceFlux y_face(const View3 &m, const Tracers &q, con
     return ff;
 }
 
+// spec_k >= 0 (no table, every layer at nloop 1): a checked step's dynamv, which first makes
+// Check_CFL's test of dynamvm at cap 1 on its own faces (as k_y_mass1) and, where a face fails,
+// raises spec_fail -- the kernels after it skip
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 bm,
-                                                  const int *nloop, int limits, int *words, int skip_mask)
+                                                  const int *nloop, int limits, int *words, int skip_mask, int spec_k)
 {
     if (skipped(words, skip_mask)) return;
     int bx, l;
@@ -305,7 +343,23 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
     const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
     const bool mine = j < jm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
-    const int nl = nloop[l];
+    const int nl = nloop ? nloop[l] : 1;  // no table: every layer at nloop 1
+    if (spec_k >= 0) {
+        bool ok = true, bad = false;
+        if (mine) {
+            const double b_lo = bm(i, jj, l);
+            if (jj >= 1)
+                ok = fabs(b_lo / (b_lo >= 0.0 ? m(i, jj - 1, l) : m(i, jj, l))) < 1.0;
+            else
+                bad = b_lo != 0.0;
+            if (jj == jm - 1) bad = bad || bm(i, jm, l) != 0.0;
+        }
+        const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+        if ((threadIdx.x & 63) == 0) {
+            if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Y);
+            if (any_fail) spec_fail(words, spec_k);
+        }
+    }
     for (int s = 0; s < nl; s++) {
         const double m_old = m(i, jj, l), b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
         const double m_new = (m_old + b_lo) - b_hi;
@@ -360,23 +414,29 @@ static int lines_per_block(int len, int cap) { return len > 1024 ? 0 : (1024 / l
 
 static dim3 line_block(int TI, int len) { return dim3((unsigned)(((TI * len) + 63) / 64 * 64)); }
 
+// known_one: the caller knows every layer passes at nloop 1 (a Check_CFL rehearsal at cap 1 on
+// the same bits of m and bm passed), so the search is not launched
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
-             int skip_mask)
+             int skip_mask, bool known_one, int spec_k)
 {
+    if (spec_k >= 0) known_one = true;
     if (im < 1 || jm < 2 || lm < 1 || lm > ctx->y_cap) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
     const int TI = lines_per_block(jm, 16);
     if (TI < 1) return TM5_EINVAL;
     const int chunks = (im * (jm - 1) + Y_NTH * Y_FPT - 1) / (Y_NTH * Y_FPT);
-    TM5_LAUNCH(ctx, k_y_search, dim3(chunks, lm), dim3(Y_NTH), 0, ctx->stream, im, jm, lm, m, bm, nmax, ctx->y_nloop,
-               ctx->y_nloop + ctx->y_cap, ctx->y_nloop + 2 * ctx->y_cap, ctx->words, fail_bit, skip_mask);
+    if (!known_one)
+        TM5_LAUNCH(ctx, k_y_search, dim3(chunks, lm), dim3(Y_NTH), 0, ctx->stream, im, jm, lm, m, bm, nmax,
+                   ctx->y_nloop, ctx->y_nloop + ctx->y_cap, ctx->y_nloop + 2 * ctx->y_cap, ctx->words, fail_bit,
+                   skip_mask);
+    const int *nloop = known_one ? nullptr : ctx->y_nloop;
     const dim3 grid((im + TI - 1) / TI, lm), block = line_block(TI, jm);
     if (mass_only)
-        TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop, limits,
-                           ctx->words, skip_mask);
+        TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
+                   ctx->words, skip_mask, spec_k);
     else
-        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, ctx->y_nloop,
-                           limits, ctx->words, skip_mask);
+        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
+                   ctx->words, skip_mask, spec_k);
     return status(hipGetLastError());
 }
 
@@ -513,9 +573,12 @@ __device__ __forceinline__ FaceFlux z_face(const View3 &m, const Tracers &q, con
 
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
-                                                  int limits, int prelimit, int *words, int skip_mask)
+                                                  int limits, int prelimit, int *words, int skip_mask, int fixed_n,
+                                                  int spec_k)
 {
     // prelimit: dynamw's limiter on rzm before the first sub-step (advect_z's, not the bare sub-steps')
+    // fixed_n > 0: n_advz given by the caller, else the search's (the context's word)
+    // spec_k >= 0 (fixed_n 1): a checked step's dynamw, first testing its faces as k_z_mass1 does
     if (skipped(words, skip_mask)) return;
     int bx, j;
     xcd_grid2(bx, j);
@@ -523,7 +586,23 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
     const bool mine = l < lm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;
-    const int nz = words[W_Z_NADV];
+    const int nz = fixed_n > 0 ? fixed_n : words[W_Z_NADV];
+    if (spec_k >= 0) {
+        bool ok = true, bad = false;
+        if (mine) {
+            if (ll >= 1) {
+                const double ck = cm(i, j, ll);
+                ok = fabs(ck / (ck >= 0.0 ? m(i, j, ll - 1) : m(i, j, ll))) < 1.0;
+            } else {
+                bad = cm(i, j, 0) != 0.0 || cm(i, j, lm) != 0.0;
+            }
+        }
+        const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+        if ((threadIdx.x & 63) == 0) {
+            if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+            if (any_fail) spec_fail(words, spec_k);
+        }
+    }
     const double dn = (double)nz;
     const double c_lo = cm(i, j, ll) / dn, c_hi = cm(i, j, ll + 1) / dn;
     for (int s = 0; s < nz; s++) {
@@ -659,8 +738,9 @@ int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int 
 }
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
-                    int limits, int nmax, int fail_bit, int skip_mask)
+                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one, int spec_k)
 {
+    if (spec_k >= 0) known_one = true;
     if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
     if (i0 < 0 || j0 < 0 || i1 >= im || j1 >= jm || i1 < i0 || j1 < j0) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
@@ -672,8 +752,10 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     if (nb < nb_cols) nb = nb_cols;
     if (nb > ctx->z_cap) return TM5_EINVAL;
     const ZScope scope{i0, i1, j0, j1};
-    TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, scope, nmax,
-               ctx->z_ok, ctx->words, fail_bit, skip_mask);
+    if (!known_one)  // as launch_y's known_one: n_advz 1 without a search
+        TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, scope, nmax,
+                   ctx->z_ok, ctx->words, fail_bit, skip_mask);
+    const int fixed_n = known_one ? 1 : 0;
     // the scope's views
     const int si = i1 - i0 + 1, sj = j1 - j0 + 1;
     View3 ms{m.p + i0 * m.si + j0 * m.sj, m.si, m.sj, m.sl}, cs{cm.p + i0 * cm.si + j0 * cm.sj, cm.si, cm.sj, cm.sl};
@@ -682,19 +764,18 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
     if (mass_only)
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words,
-                   skip_mask);
+                   ctx->words, skip_mask, fixed_n, spec_k);
     else
         TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words,
-                   skip_mask);
+                   ctx->words, skip_mask, fixed_n, spec_k);
     return status(hipGetLastError());
 }
 
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
-             int skip_mask)
+             int skip_mask, bool known_one, int spec_k)
 {
-    return launch_z_scoped(ctx, im, jm, lm, m, q, cm, 0, im - 1, 0, jm - 1, limits, nmax, fail_bit, skip_mask);
+    return launch_z_scoped(ctx, im, jm, lm, m, q, cm, 0, im - 1, 0, jm - 1, limits, nmax, fail_bit, skip_mask,
+                           known_one, spec_k);
 }
 
 // dynamw's sub-steps alone with a given n_advz (left in the context's word for the kernel)
@@ -706,7 +787,7 @@ int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, 
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI,
-               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS);
+               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, -1);
     return status(hipGetLastError());
 }
 

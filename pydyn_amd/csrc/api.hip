@@ -72,6 +72,21 @@ __global__ void __launch_bounds__(256) k_copy3(int n0, int n1, int n2, View3 src
     dst(i, j, l) = src(i, j, l);
 }
 
+// the region's m and moments back from a checked step's backup (k_x's layout: i fastest, m first,
+// then every tracer's rm, rxm, rym, rzm)
+__global__ void __launch_bounds__(256) k_restore(int im, int jm, int lm, const double *bk, View3 m, Tracers q)
+{
+    const long long ncell = (long long)im * jm * lm;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncell) return;
+    const int i = (int)(idx % im), j = (int)((idx / im) % jm), l = (int)(idx / ((long long)im * jm));
+    m(i, j, l) = bk[idx];
+    for (int t = 0; t < q.nt; t++) {
+        const long long o = q.off(i, j, l, t);
+        for (int a = 0; a < 4; a++) q.q[a][o] = bk[(1 + 4 * t + a) * ncell + idx];
+    }
+}
+
 static int copy3(tm5_ctx *ctx, int n0, int n1, int n2, View3 src, View3 dst)
 {
     const long long n = (long long)n0 * n1 * n2;
@@ -430,6 +445,8 @@ struct tm5_region {
     int batch;        // tm5_region_run: intervals it queues per read of the device
     double *scratch;  // the rehearsal's m, am, bm, cm (i-fastest, contiguous)
     View3 ms, ams, bms, cms;
+    double *backup;   // tm5_region_run: the queued interval's m and moments at its start (k_x's layout)
+    size_t backup_n;  // its length in doubles
 };
 
 static View3 packed(double *p, int n0, int n1) { return View3{p, 1, n0, (long long)n0 * n1}; }
@@ -463,6 +480,7 @@ extern "C" int tm5_region_destroy(tm5_region *r)
     if (!r) return 0;
     hipStreamSynchronize(r->ctx->stream);
     hipFree(r->scratch);
+    if (r->backup) hipFree(r->backup);
     free(r);
     return 0;
 }
@@ -486,30 +504,26 @@ static int new_valid_timestep(int dtime, int nread, int cfl_outputstep, int tref
 // Fortran nint (pydyn/ops.py:78-80)
 static int nint(double x) { return (int)(floor(fabs(x) + 0.5) * (x >= 0 ? 1.0 : -1.0)); }
 
-static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View3 cm, const View3 *dual = nullptr)
+static int dynam0_at(tm5_region *r, double t, int ndyn, View3 am, View3 bm, View3 cm)
 {
     const tm5_region_desc &d = r->d;
     const double a = amplitude(d, t);
     return launch_dynam0(r->ctx, d.im, d.jm, d.lm, view3(&d.pu0), view3(&d.pv0), a, d.bt, (double)ndyn / 2.0, am, bm,
-                         cm, r->skip, 1, dual);
+                         cm, r->skip, 1);
 }
 
-// The kernels of Check_CFL's rehearsal of the n steps of ndyn on the scratch copy.  With n = 1
-// the trial's flux set is the step's own (Setup_MassFlow at the same time, t0 + ndyn/2): dual
-// then has its dynam0 write the region's am, bm, cm too.
-static int rehearse_queue(tm5_region *r, double t0, int ndyn, int n, bool dual = false, bool clear = true)
+// The kernels of Check_CFL's rehearsal of the n steps of ndyn on the scratch copy.
+static int rehearse_queue(tm5_region *r, double t0, int ndyn, int n, bool clear = true)
 {
     tm5_ctx *c = r->ctx;
     const tm5_region_desc &d = r->d;
     const int im = d.im, jm = d.jm, lm = d.lm;
     const int skip = REHEARSAL_SKIP | (r->skip & TM5_SPEC);
-    const View3 step_fluxes[3] = {view3(&d.am), view3(&d.bm), view3(&d.cm)};
     if (clear) TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL));
     if (!r->ms_fresh) TRY(copy3(c, im, jm, lm, view3(&d.m), r->ms));
     r->ms_fresh = false;  // the trial moves ms
     for (int i = 1; i <= n; i++) {
-        TRY(dynam0_at(r, t0 + ((double)i - 0.5) * (double)ndyn, ndyn, r->ams, r->bms, r->cms,
-                      dual && n == 1 ? step_fluxes : nullptr));
+        TRY(dynam0_at(r, t0 + ((double)i - 0.5) * (double)ndyn, ndyn, r->ams, r->bms, r->cms));
         const char *order = "xyzzyx";
         for (const char *op = order; *op; op++) {
             if (*op == 'x')
@@ -535,7 +549,17 @@ static int rehearse(tm5_region *r, double t0, int ndyn, int n, bool *ok)
     return 0;
 }
 
-static int operator_(tm5_region *r, char op, int ndyn)
+// A step after a passing Check_CFL rehearsal of its ndyn (known_one), or a checked step that makes
+// the rehearsal's tests itself (spec_k >= 0: queued interval spec_k; a failed test raises
+// TM5_SPEC).  Both rest on one fact: the rehearsal's mass-only operators -- dynamum at cap 10,
+// dynamvm and dynamwm at cap 1, in the order x y z z y x -- move m through the same bits as the
+// step's operators, given the same fluxes (the mass-only x sub-steps add the same faces as
+// dynamu's; convection, sources, chemistry and m2phlb1 leave m alone).  So after a passing
+// rehearsal every y layer and the z region pass at 1, and a step whose x rows pass at nloop <= 9
+// and whose y and z faces pass at 1 is exactly a step whose rehearsal passes.
+// backup (checked x only): where the x kernel leaves the m and moments it reads.
+static int operator_(tm5_region *r, char op, int ndyn, bool known_one = false, int spec_k = -1,
+                     double *backup = nullptr)
 {
     tm5_ctx *c = r->ctx;
     const tm5_region_desc &d = r->d;
@@ -543,11 +567,16 @@ static int operator_(tm5_region *r, char op, int ndyn)
     const Tracers q = tracers(&d.q);
     switch (op) {
     case 'x':
+        if (spec_k >= 0)  // the rehearsal's cap (its search stops past 10 - 1)
+            return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 10 - 1, TM5_FAIL_X, r->skip,
+                            nullptr, nullptr, nullptr, spec_k, backup);
         return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, r->skip);
     case 'y':
-        return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, r->skip);
+        return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, r->skip,
+                        known_one, spec_k);
     case 'z':
-        return launch_z(c, im, jm, lm, view3(&d.m), q, view3(&d.cm), d.limits, 15, TM5_FAIL_Z, r->skip);
+        return launch_z(c, im, jm, lm, view3(&d.m), q, view3(&d.cm), d.limits, 15, TM5_FAIL_Z, r->skip, known_one,
+                        spec_k);
     case 'v':
         return launch_convec(c, im, jm, lm, view3(&d.m), q, view3(&d.entu), view3(&d.detu), view3(&d.entd),
                              view3(&d.detd), iview2(&d.cloud_top), iview2(&d.cloud_lfs), view3(&d.dkg),
@@ -606,12 +635,14 @@ static int middle(tm5_region *r, int ndyn)
     return launch_column(r->ctx, a, d.lmax, r->skip);
 }
 
-static int step(tm5_region *r, int ndyn)
+// one step of ndyn after Check_CFL's rehearsal of it passed, or (spec_k >= 0) a checked step of
+// queued interval spec_k, the first of its interval with a backup
+static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullptr)
 {
     const tm5_region_desc &d = r->d;
-    for (const char *op = "xyz"; *op; op++) TRY(operator_(r, *op, ndyn));
+    for (const char *op = "xyz"; *op; op++) TRY(operator_(r, *op, ndyn, true, spec_k, *op == 'x' ? backup : nullptr));
     TRY(middle(r, ndyn));
-    for (const char *op = "zyx"; *op; op++) TRY(operator_(r, *op, ndyn));
+    for (const char *op = "zyx"; *op; op++) TRY(operator_(r, *op, ndyn, true, spec_k));
     TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
                        r->copy_ms ? &r->ms : nullptr));
     r->ms_fresh = r->copy_ms;
@@ -700,7 +731,16 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
             TRY(tm5_region_interval(r, t0 + (double)k * (double)nread, nread, ndyn_max, ndyn_out + k, n_out + k));
         return 0;
     }
+    const size_t ncell = (size_t)d.im * d.jm * d.lm, nb = ncell * (1 + 4 * (size_t)d.q.nt);
+    if (r->backup_n < nb) {
+        if (r->backup) TRY(status(hipFree(r->backup)));
+        r->backup = nullptr;
+        r->backup_n = 0;
+        TRY(status(hipMalloc((void **)&r->backup, nb * sizeof(double))));
+        r->backup_n = nb;
+    }
     TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL | TM5_SPEC));
+    TRY(set_word(c, W_SPEC_KIND, 0, -1));
     // intervals queued per read of the device: doubled after a batch that held to its plan, back
     // to 2 after one that did not (the kernels queued after a miss only skip, but each costs a
     // dispatch)
@@ -720,21 +760,21 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         int rc = 0;
         for (int k = i; k < end && rc == 0; k++) {
             const double t = t0 + (double)k * (double)nread;
-            for (int q = 0; q < trials && rc == 0; q++) {
-                const bool last = q + 1 == trials;
-                rc = rehearse_queue(r, t, nd[q], nn[q], last, false);  // k_spec_check clears the bit
+            // the trials the plan expects to fail are rehearsed; the last one is the steps
+            // themselves, checked
+            for (int q = 0; q + 1 < trials && rc == 0; q++) {
+                rc = rehearse_queue(r, t, nd[q], nn[q], false);  // k_spec_check clears the bit
                 if (rc == 0) {
-                    TM5_LAUNCH(c, k_spec_check, dim3(1), dim3(64), 0, c->stream, c->words, k, last ? 0 : 1);
+                    TM5_LAUNCH(c, k_spec_check, dim3(1), dim3(64), 0, c->stream, c->words, k, 1);
                     rc = status(hipGetLastError());
                 }
             }
             const int ndyn = nd[trials - 1], n = nn[trials - 1];
             r->copy_ms = true;
             for (int s = 0; s < n && rc == 0; s++) {
-                if (n > 1)  // with n = 1 the last trial's Setup_MassFlow wrote the step's fluxes
-                    rc = dynam0_at(r, (t + (double)(s * ndyn)) + (double)ndyn / 2.0, ndyn, view3(&d.am), view3(&d.bm),
-                                   view3(&d.cm));
-                if (rc == 0) rc = step(r, ndyn);
+                rc = dynam0_at(r, (t + (double)(s * ndyn)) + (double)ndyn / 2.0, ndyn, view3(&d.am), view3(&d.bm),
+                               view3(&d.cm));
+                if (rc == 0) rc = step(r, ndyn, k, s == 0 ? r->backup : nullptr);
             }
             r->copy_ms = false;
         }
@@ -753,9 +793,17 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
             continue;
         }
         batch = 2;
-        // interval f's trials disagreed with the plan: nothing after them ran; redo it the
-        // reference's way, and plan the next intervals on its step
+        // interval f's trials or checks disagreed with the plan: nothing after them ran.  A failed
+        // check left the interval's state part-stepped: it comes back from the backup.  Then the
+        // interval is redone the reference's way, and the next ones planned on its step.
+        if (c->hwords[W_SPEC_KIND]) {
+            const long long nc = (long long)ncell;
+            TM5_LAUNCH(c, k_restore, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, d.im, d.jm, d.lm,
+                       r->backup, view3(&d.m), tracers(&d.q));
+            TRY(status(hipGetLastError()));
+        }
         TRY(set_word(c, W_FLAGS, 0, TM5_SPEC | TM5_REHEARSAL));
+        TRY(set_word(c, W_SPEC_KIND, 0, -1));
         r->ms_fresh = false;
         TRY(tm5_region_interval(r, t0 + (double)f * (double)nread, nread, ndyn_max, ndyn_out + f, n_out + f));
         plan = ndyn_out[f];

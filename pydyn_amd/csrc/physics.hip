@@ -19,9 +19,8 @@ constexpr int D0_TI = 32;
 
 __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
-                                                int *words, int skip_mask, View3 am2, View3 bm2, View3 cm2)
+                                                int *words, int skip_mask)
 {
-    // am2, bm2, cm2 (when am2.p is set): a second copy of every output
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double conv[];  // [lm][TI]
     const int TI = D0_TI;
@@ -42,12 +41,6 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
         if (i == im - 1) am(im, j, l) = a1;
         bm(i, j, l) = b0;
         if (j == jm - 1) bm(i, jm, l) = b1;
-        if (am2.p) {
-            am2(i, j, l) = a0;
-            if (i == im - 1) am2(im, j, l) = a1;
-            bm2(i, j, l) = b0;
-            if (j == jm - 1) bm2(i, jm, l) = b1;
-        }
     }
     __syncthreads();
     for (int ti = tid; ti < ni; ti += nth) {
@@ -56,18 +49,12 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
         for (int l = 1; l < lm; l++) pit = pit + conv[l * TI + ti];
         cm(i, j, 0) = 0.0;
         cm(i, j, lm) = 0.0;
-        if (am2.p) {
-            cm2(i, j, 0) = 0.0;
-            cm2(i, j, lm) = 0.0;
-        }
         if (lm > 1) {
             double sd = conv[(lm - 1) * TI + ti] - (bt[lm - 1] - bt[lm]) * pit;
             cm(i, j, lm - 1) = (-dt) * sd;
-            if (am2.p) cm2(i, j, lm - 1) = (-dt) * sd;
             for (int l = lm - 2; l >= 1; l--) {
                 sd = (sd + conv[l * TI + ti]) - (bt[l] - bt[l + 1]) * pit;
                 cm(i, j, l) = (-dt) * sd;
-                if (am2.p) cm2(i, j, l) = (-dt) * sd;
             }
         }
     }
@@ -196,14 +183,12 @@ __global__ void __launch_bounds__(256) k_scale(int n0, int n1, int n2, View3 a, 
 static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic, const View3 *dual)
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
 {
     if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
     const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
-    const View3 none{};
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
-                       pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask, dual ? dual[0] : none,
-                       dual ? dual[1] : none, dual ? dual[2] : none);
+               pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 

@@ -15,6 +15,7 @@ enum {
     W_Z_TICKET = 3, // blocks of the z search that have finished (reset by the last one)
     W_Z_CAND = 4,   // the z search's running candidate
     W_SPEC_AT = 5,  // tm5_region_run: the first queued interval whose rehearsal failed
+    W_SPEC_KIND = 6, // 1: that interval's checked step failed a check (its state moved), 0: a trial
     W_WORDS = 8,
 };
 
@@ -187,11 +188,14 @@ inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 // launchers (advect.hip)
 // rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr);
+             int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr,
+             int spec_k = -1, double *backup = nullptr);
+// known_one: the caller knows the sub-cycle count is 1 everywhere (no search launched); spec_k >= 0:
+// a checked step of queued interval spec_k, which tests Check_CFL's cap 1 itself
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
-             int skip_mask);
+             int skip_mask, bool known_one = false, int spec_k = -1);
 int launch_z(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int limits, int nmax, int fail_bit,
-             int skip_mask);
+             int skip_mask, bool known_one = false, int spec_k = -1);
 // The column physics of one launch (column.hip): convec with, around it, the sources (s), the
 // chemistry (c) and m2phlb1 (M) in the sequence the flags give:
 //   plain "v";  pre "csv";  post "vsc";  twice "vsccsv";  twice + mid "vscMcsv".
@@ -217,16 +221,15 @@ struct ColumnArgs {
     View3 lam_emis;   // (im, jm)
 };
 bool column_supported(int n);
-// dual: a second (am, bm, cm) that receives the same fluxes
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const View3 *dual = nullptr);
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1);
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
 int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
-                    int limits, int nmax, int fail_bit, int skip_mask);
+                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);
 int launch_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, View3 q, View3 out);
 int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int n_advz, int limits,
                       int prelimit);

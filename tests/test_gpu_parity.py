@@ -193,11 +193,14 @@ def test_run_glb6x4_one_day_matches_reference(dev):
     _run_against_golden("glb6x4")
 
 
-@pytest.mark.parametrize("name, batch", [("small", 40), ("small", 7), ("glb6x4", 24)])
+@pytest.mark.parametrize("name, batch", [("small", 40), ("small", 7), ("glb6x4", 24), ("glb3x2", 24)])
 def test_queued_run_matches_reference(dev, name, batch):
-    """Intervals queued in batches (tm5_region_run: the device read once per batch, the steps of
-    an interval skipping themselves when its rehearsal fails and the run resuming from it with
-    the Check_CFL cut): the same (ndyn, n) and, at each batch's end, the same bits."""
+    """Intervals queued in batches (tm5_region_run: the device read once per batch; each interval
+    on the previous one's step, as checked steps that make Check_CFL's tests themselves; where a
+    trial or a check disagrees, the interval's state restored from the backup and the run resumed
+    from it with the reference's Check_CFL): the same (ndyn, n) and, at each batch's end, the
+    same bits.  small turns from 3600 s to 1800 s (a checked step fails its test) and back (a
+    trial expected to fail passes); glb3x2 x 34 does both 7 times or more in its 30 days."""
     from pydyn_amd import meteo
     from pydyn_amd.region import Region
 
@@ -257,12 +260,12 @@ def test_profile_times_every_launch(dev):
     ctx.profile_begin()
     reg.interval()
     prof = ctx.profile_end()
-    # two of each advection kernel a step; one Setup_MassFlow (the rehearsal's, written for the step too)
-    for k, least in (("k_x<false>", 2), ("k_y_search", 2), ("k_y_steps<false>", 2), ("k_z_search", 2),
-                     ("k_z_steps<false>", 2), ("k_dynam0", 1)):  # fmt: skip
+    # two of each advection kernel and one Setup_MassFlow a step; a checked step searches nothing
+    for k, least in (("k_x<false>", 2), ("k_y_steps<false>", 2), ("k_z_steps<false>", 2), ("k_dynam0", 1)):
         assert k in prof, (k, sorted(prof))
         count, secs = prof[k]
         assert count >= least and 0.0 < secs < 1.0, (k, prof[k])
+    assert "k_y_search" not in prof and "k_z_search" not in prof, sorted(prof)
     assert any(k.startswith("k_column<") for k in prof), sorted(prof)
     reg.interval()  # not recorded
     assert ctx.profile_entries() == prof
