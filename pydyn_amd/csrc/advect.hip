@@ -24,16 +24,6 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // k-1 and k, face 0 the same as face n).  The search (kernels.py:169-199): nloop = 1, 2, ...,
 // each trial carrying every face's upwind cell through nloop mass-only sub-steps; on a failure
 // every face is scaled by (nloop-1)/nloop in place.
-// A checked step's cap failure (spec_k: its interval in the queue): the kernels queued after it
-// skip, and the host restores the interval's state from the backup before redoing it.
-__device__ __forceinline__ void spec_fail(int *words, int spec_k)
-{
-    words[W_SPEC_AT] = spec_k;  // every raiser of one launch writes the same interval
-    words[W_SPEC_KIND] = 1;
-    __threadfence();
-    atomicOr(words + W_FLAGS, TM5_SPEC);
-}
-
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,

@@ -592,14 +592,17 @@ static int operator_(tm5_region *r, char op, int ndyn, bool known_one = false, i
 }
 
 // "v s c m2phlb1 c s v" in one kernel when lmax has a register form (column.hip), else the
-// operators one by one
-static int middle(tm5_region *r, int ndyn)
+// operators one by one.  with_z: a step's "z ... z" around it, at n_advz 1 (operator_'s
+// known_one and spec_k), fused into the column kernel where it has one.
+static int middle(tm5_region *r, int ndyn, bool with_z = false, int spec_k = -1)
 {
     const tm5_region_desc &d = r->d;
     if (!column_supported(d.lmax) || d.lm > 64) {
+        if (with_z) TRY(operator_(r, 'z', ndyn, true, spec_k));
         for (const char *op = "vsc"; *op; op++) TRY(operator_(r, *op, ndyn));
         TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
         for (const char *op = "csv"; *op; op++) TRY(operator_(r, *op, ndyn));
+        if (with_z) TRY(operator_(r, 'z', ndyn, true, spec_k));
         return 0;
     }
     ColumnArgs a{};
@@ -632,6 +635,10 @@ static int middle(tm5_region *r, int ndyn)
     a.bt = d.bt;
     a.sp = view3(&d.sp);
     a.phlb = view3(&d.phlb);
+    a.zfused = with_z ? 1 : 0;
+    a.zlimits = d.limits;
+    a.spec_k = spec_k;
+    a.cm = view3(&d.cm);
     return launch_column(r->ctx, a, d.lmax, r->skip);
 }
 
@@ -640,9 +647,11 @@ static int middle(tm5_region *r, int ndyn)
 static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullptr)
 {
     const tm5_region_desc &d = r->d;
-    for (const char *op = "xyz"; *op; op++) TRY(operator_(r, *op, ndyn, true, spec_k, *op == 'x' ? backup : nullptr));
-    TRY(middle(r, ndyn));
-    for (const char *op = "zyx"; *op; op++) TRY(operator_(r, *op, ndyn, true, spec_k));
+    TRY(operator_(r, 'x', ndyn, true, spec_k, backup));
+    TRY(operator_(r, 'y', ndyn, true, spec_k));
+    TRY(middle(r, ndyn, true, spec_k));  // "z v s c m2phlb1 c s v z"
+    TRY(operator_(r, 'y', ndyn, true, spec_k));
+    TRY(operator_(r, 'x', ndyn, true, spec_k));
     TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
                        r->copy_ms ? &r->ms : nullptr));
     r->ms_fresh = r->copy_ms;

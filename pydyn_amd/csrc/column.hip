@@ -144,9 +144,42 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     double *mc = H + ColumnLds<N>::R0;  // m at all lm levels (m2phlb1)
     double *rv = mc + 64;               // the column's rm by level, for v
 
-    const double m_l = li < lm ? a.m(i, j, li) : 0.0;
+    // zfused: the first dynamw (n_advz 1, c = cm / 1) moves m before convection and m2phlb1 see it
+    const bool zf = !ADJ && a.zfused;
+    double m_l = li < lm ? a.m(i, j, li) : 0.0, m_h = l1 < lm ? a.m(i, j, l1) : 0.0;
+    if (zf) {
+        const double m0_l = m_l, m0_h = m_h;
+        if (li < lm) m_l = (m0_l + a.cm(i, j, li) / 1.0) - a.cm(i, j, li + 1) / 1.0;
+        if (l1 < lm) m_h = (m0_h + a.cm(i, j, l1) / 1.0) - a.cm(i, j, l1 + 1) / 1.0;
+        if (a.spec_k >= 0) {
+            // a checked step: Check_CFL's dynamwm at cap 1 on both dynamw's masses (m before and
+            // after the first), every interior face k tested by lane k and k + 32 of its column
+            H[li] = m0_l;
+            H[l1] = m0_h;
+            mc[li] = m_l;
+            mc[l1] = m_h;
+            wsync();
+            bool ok = true, bad = false;
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+                const int k = h2 ? l1 : li;
+                if (k >= 1 && k < lm) {
+                    const double ck = a.cm(i, j, k);
+                    const int u = ck >= 0.0 ? k - 1 : k;
+                    ok = ok && fabs(ck / H[u]) < 1.0 && fabs(ck / mc[u]) < 1.0;
+                }
+            }
+            if (li == 0) bad = a.cm(i, j, 0) != 0.0 || a.cm(i, j, lm) != 0.0;
+            const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+            if (lane == 0) {
+                if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+                if (any_fail) spec_fail(words, a.spec_k);
+            }
+            wsync();  // H is read: the build vectors take its space
+        }
+    }
     mc[li] = m_l;
-    mc[l1] = l1 < lm ? a.m(i, j, l1) : 0.0;
+    mc[l1] = m_h;
     // a column the region does not own is computed like the others but keeps its tracers; the
     // middle m2phlb1 is every column's
     const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
@@ -326,6 +359,47 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     }
     const double diag = li < N ? Mt[li * LD + li] : 0.0;
 
+    // zfused: dynamw's sub-step through LDS staging Z (the matrix's space, free once C is read):
+    // Z[0, 64) the masses at the first sub-step's start, then rm, rxm, rym, rzm by level; the
+    // second sub-step starts from mc.  Each lane takes the two faces of each of its cells.
+    static_assert(ColumnLds<N>::R0 >= 5 * 64, "the staging fits the matrix's space");
+    double *Z = H;
+    double zm0 = 0.0, zm0h = 0.0, zm2 = 0.0, zm2h = 0.0;
+    if (zf) {
+        if (li < lm) {
+            zm0 = a.m(i, j, li);
+            zm2 = (mc[li] + a.cm(i, j, li) / 1.0) - a.cm(i, j, li + 1) / 1.0;
+        }
+        if (l1 < lm) {
+            zm0h = a.m(i, j, l1);
+            zm2h = (mc[l1] + a.cm(i, j, l1) / 1.0) - a.cm(i, j, l1 + 1) / 1.0;
+        }
+    }
+    const bool zlim = a.zlimits != 0;
+    // face k of the staged sub-step (pydyn/kernels.py:324-346, as k_z_steps' z_face at n_advz 1)
+    auto z_face = [&](int k, double cf, const double *Mz) {
+        FaceFlux ff{0.0, 0.0, 0.0, 0.0};
+        if (k <= 0 || k >= lm) return ff;  // the closed ends
+        const int u = cf >= 0.0 ? k - 1 : k;
+        const double gamma = cf / Mz[u];
+        const double rmu = Z[64 + u];
+        const double rzu = zlim ? limit(Z[256 + u], rmu) : Z[256 + u];
+        return face_flux(cf, gamma, rmu, rzu, Z[128 + u], Z[192 + u]);
+    };
+    auto z_cell = [&](int l, const double *Mz, double m_old, double m_new, double &rm, double &rx, double &ry,
+                      double &rz) {
+        const double c_lo = a.cm(i, j, l) / 1.0, c_hi = a.cm(i, j, l + 1) / 1.0;
+        const FaceFlux lo = z_face(l, c_lo, Mz), hi = z_face(l + 1, c_hi, Mz);
+        const double rzm = zlim ? limit(rz, rm) : rz;
+        const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
+        double rzn = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
+        if (zlim) rzn = limit(rzn, rm_new);
+        rx = (rx + lo.ft) - hi.ft;
+        ry = (ry + lo.fw) - hi.fw;
+        rm = rm_new;
+        rz = rzn;
+    };
+
     // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
     const bool sc = a.pre || a.twice || a.post;
     double decay = 1.0, decay1 = 1.0, x = 0.0, emis_ij = 0.0;
@@ -389,6 +463,29 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             ry1 = ry1 * decay1;
             rz1 = rz1 * decay1;
         };
+        auto z_op = [&](bool second) {  // dynamw at n_advz 1: m0 -> m1 (first), m1 -> m2 (second)
+            wsync();  // earlier reads of Z and of the matrix precede these writes
+            if (!second) {
+                if (li < lm) Z[li] = zm0;
+                if (l1 < lm) Z[l1] = zm0h;
+            }
+            if (li < lm) {
+                Z[64 + li] = rm;
+                Z[128 + li] = rx;
+                Z[192 + li] = ry;
+                Z[256 + li] = rz;
+            }
+            if (l1 < lm) {
+                Z[64 + l1] = rm1;
+                Z[128 + l1] = rx1;
+                Z[192 + l1] = ry1;
+                Z[256 + l1] = rz1;
+            }
+            wsync();
+            const double *Mz = second ? mc : Z;
+            if (li < lm) z_cell(li, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], rm, rx, ry, rz);
+            if (l1 < lm) z_cell(l1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], rm1, rx1, ry1, rz1);
+        };
         auto v_op = [&]() {  // rm <- C rm over levels < N (ADJ: C^T, lane k holds column k), slopes times diag(C)
             wsync();
             rv[li] = rm;
@@ -426,6 +523,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 c_op();
             }
         } else {
+            if (zf) z_op(false);
             if (a.pre) {
                 c_op();
                 s_op();
@@ -442,6 +540,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
                 s_op();
                 c_op();
             }
+            if (zf) z_op(true);
         }
         if (active && own) {
             if (li < lm) {
@@ -462,6 +561,10 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         if (active && own && (li == a.level || l1 == a.level)) a.lam_emis(i, j, 0) = a.lam_emis(i, j, 0) + lam_e;
         return;
     }
+    if (zf && active) {  // the second dynamw's mass
+        if (li < lm) a.m(i, j, li) = zm2;
+        if (l1 < lm) a.m(i, j, l1) = zm2h;
+    }
     // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290, 24-32): sp from
     // the column's mass summed upward, phlb = at + bt sp
     if (a.mid && active) {
@@ -476,6 +579,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
 {
+    if (args.zfused && (args.adjoint || args.owned || !args.cm.p)) return TM5_EINVAL;  // a region's step only
     const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
     const int ncol = args.im * args.jm;
     const dim3 grid((ncol + 7) / 8), block(256);

@@ -185,6 +185,16 @@ inline IView2 iview2(const tm5_iview *v) { return IView2{v->data, v->stride[0], 
 
 inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 
+// A checked step's failed test (spec_k: its interval in the queue): the kernels queued after it
+// skip, and the host restores the interval's state from the backup before redoing it.
+__device__ __forceinline__ void spec_fail(int *words, int spec_k)
+{
+    words[W_SPEC_AT] = spec_k;  // every raiser of one launch writes the same interval
+    words[W_SPEC_KIND] = 1;
+    __threadfence();
+    atomicOr(words + W_FLAGS, TM5_SPEC);
+}
+
 // launchers (advect.hip)
 // rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
@@ -219,6 +229,10 @@ struct ColumnArgs {
     View3 sp, phlb;
     int adjoint;      // the transposed sequence on lambdas in q; the emission's lambda into lam_emis
     View3 lam_emis;   // (im, jm)
+    // zfused: dynamw at n_advz 1 before and after the sequence ("z v s c M c s v z"), on cm with
+    // the limiter flag zlimits; spec_k >= 0: a checked step's, testing both at cap 1 first
+    int zfused, zlimits, spec_k;
+    View3 cm;
 };
 bool column_supported(int n);
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
