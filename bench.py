@@ -106,6 +106,8 @@ def roofline(reg, intervals):
     every launch on its stream; the roofline of the kernel with the largest total."""
     from pydyn_amd import device as D
 
+    if intervals < 1:
+        return None
     ctx = D.context()
     ctx.profile_begin()
     for _ in range(intervals):
