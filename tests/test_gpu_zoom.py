@@ -143,6 +143,7 @@ def _member_slices_equal(batched, plain, m, ntr):
 
 @pytest.mark.parametrize(("base", "ntr", "members", "intervals", "check"), [
     ("tiny2", 4, 4, 48, (0, 1, 2, 3)),
+    ("tiny2", 4, 8, 48, (0, 5, 7)),
     ("c2", 4, 64, 2, (0, 63)),
 ])  # fmt: skip
 def test_zoom_ensemble_members_equal_their_plain_runs(dev, base, ntr, members, intervals, check):
