@@ -84,6 +84,16 @@ int tm5_ctx_flags(tm5_ctx *ctx, int clear, int *flags);
 int tm5_ctx_nloop(tm5_ctx *ctx, int *x_max, int *y_max, int *nloop_layers, int lm, int *z_nadv);
 /* Clear the bits of `mask` in the flag word from the stream (a kernel: no wait, capturable). */
 int tm5_ctx_clear(tm5_ctx *ctx, int mask);
+/* Tile shapes of the line and column kernels, set per context (pydyn_amd.tune measures them;
+   they change speed only, never results).  No counterpart in the reference, whose launch shapes
+   are fixed in its kernel source. */
+#define TM5_TILE_Y_LINES 0 /* columns i per y-line workgroup (dynamv, dynamvm), 1..16, at most 1024 / jm; 16 */
+#define TM5_TILE_Z_LINES 1 /* columns i per z-line workgroup (dynamw, dynamwm), 1..32, at most 1024 / lm; 32 */
+#define TM5_TILE_DYNAM0 2  /* columns i per dynam0 workgroup, 1..64, lm * tile <= 8192; 32 */
+#define TM5_TILE_M2PHLB 3  /* columns i per m2phlb1 workgroup, 1..64, (lm + 1) * tile <= 8192; 32 */
+#define TM5_TILES 4
+int tm5_ctx_set_tile(tm5_ctx *ctx, int which, int value);
+int tm5_ctx_get_tile(tm5_ctx *ctx, int which, int *value);
 /* Per-kernel GPU time: from tm5_profile_begin every launch on the context is bracketed by
    events on its stream; tm5_profile_end waits for the stream and totals them per kernel, and
    tm5_profile_entry reads the idx-th total (TM5_EINVAL past the last).  The reference's

@@ -105,6 +105,8 @@ _SIGNATURES = {
     "tm5_ctx_flags": [_VP, _I, _P(_I)],
     "tm5_ctx_nloop": [_VP, _P(_I), _P(_I), _P(_I), _I, _P(_I)],
     "tm5_ctx_clear": [_VP, _I],
+    "tm5_ctx_set_tile": [_VP, _I, _I],
+    "tm5_ctx_get_tile": [_VP, _I, _P(_I)],
     "tm5_profile_begin": [_VP],
     "tm5_profile_end": [_VP],
     "tm5_profile_entry": [_VP, _I, _P(ctypes.c_char_p), _P(ctypes.c_longlong), _P(_D)],

@@ -412,7 +412,7 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
     if (spec_k >= 0) known_one = true;
     if (im < 1 || jm < 2 || lm < 1 || lm > ctx->y_cap) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = lines_per_block(jm, 16);
+    const int TI = lines_per_block(jm, ctx->tile[TM5_TILE_Y_LINES]);
     if (TI < 1) return TM5_EINVAL;
     const int chunks = (im * (jm - 1) + Y_NTH * Y_FPT - 1) / (Y_NTH * Y_FPT);
     if (!known_one)
@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(1024) k_z_mass1(int im, int jm, int lm, int TI
 int launch_y_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 bm, int fail_bit, int skip_mask)
 {
     if (im < 1 || jm < 2 || lm < 1) return TM5_EINVAL;
-    const int TI = lines_per_block(jm, 16);
+    const int TI = lines_per_block(jm, ctx->tile[TM5_TILE_Y_LINES]);
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_y_mass1, dim3((im + TI - 1) / TI, lm), line_block(TI, jm), 0, ctx->stream, im, jm, lm, TI, m,
                bm, ctx->words, fail_bit, skip_mask);
@@ -720,7 +720,7 @@ int launch_y_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 bm, int 
 int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int fail_bit, int skip_mask)
 {
     if (im < 1 || jm < 1 || lm < 2) return TM5_EINVAL;
-    const int TI = lines_per_block(lm, 32);
+    const int TI = lines_per_block(lm, ctx->tile[TM5_TILE_Z_LINES]);
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_z_mass1, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI, m,
                cm, ctx->words, fail_bit, skip_mask);
@@ -734,7 +734,7 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
     if (i0 < 0 || j0 < 0 || i1 >= im || j1 >= jm || i1 < i0 || j1 < j0) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = lines_per_block(lm, 32);
+    const int TI = lines_per_block(lm, ctx->tile[TM5_TILE_Z_LINES]);
     if (TI < 1) return TM5_EINVAL;
     const long long nface = (long long)im * jm * (lm - 1);
     long long nb = (nface + 256LL * Z_FPT - 1) / (256LL * Z_FPT);
@@ -773,7 +773,7 @@ int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, 
                       int prelimit)
 {
     if (im < 1 || jm < 1 || lm < 2 || n_advz < 1 || q.nt < 1) return TM5_EINVAL;
-    const int TI = lines_per_block(lm, 32);
+    const int TI = lines_per_block(lm, ctx->tile[TM5_TILE_Z_LINES]);
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI,

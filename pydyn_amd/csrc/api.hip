@@ -134,6 +134,10 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     c->y_cap = 256;
     c->x_cap = 1 << 17;
     c->z_cap = 1 << 13;
+    c->tile[TM5_TILE_Y_LINES] = 16;
+    c->tile[TM5_TILE_Z_LINES] = 32;
+    c->tile[TM5_TILE_DYNAM0] = 32;
+    c->tile[TM5_TILE_M2PHLB] = 32;
     if (e == hipSuccess) e = hipMalloc((void **)&c->words, W_WORDS * sizeof(int));
     // per layer: nloop, the y check's tickets and verdicts (zero between launches)
     if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, 3 * c->y_cap * sizeof(int));
@@ -232,6 +236,21 @@ extern "C" int tm5_profile_entry(tm5_ctx *c, int idx, const char **kernel, long 
     if (kernel) *kernel = p->total_names[idx];
     if (count) *count = p->counts[idx];
     if (seconds) *seconds = p->seconds[idx];
+    return 0;
+}
+
+extern "C" int tm5_ctx_set_tile(tm5_ctx *c, int which, int value)
+{
+    static const int hi[TM5_TILES] = {16, 32, 64, 64};
+    if (!c || which < 0 || which >= TM5_TILES || value < 1 || value > hi[which]) return TM5_EINVAL;
+    c->tile[which] = value;
+    return 0;
+}
+
+extern "C" int tm5_ctx_get_tile(tm5_ctx *c, int which, int *value)
+{
+    if (!c || !value || which < 0 || which >= TM5_TILES) return TM5_EINVAL;
+    *value = c->tile[which];
     return 0;
 }
 

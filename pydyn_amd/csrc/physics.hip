@@ -15,15 +15,13 @@ __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 // interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
 // convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
 // pit summed from the surface and sd accumulated from the top, in the reference's order.
-constexpr int D0_TI = 32;
-
-__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu, View3 pv, double a,
+__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, double a,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
                                                 int *words, int skip_mask)
 {
+    // TI columns i per workgroup (the context's TM5_TILE_DYNAM0)
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double conv[];  // [lm][TI]
-    const int TI = D0_TI;
     const int j = blockIdx.y, i0 = blockIdx.x * TI;
     const int ni = min(TI, im - i0);
     const int tid = threadIdx.x, nth = blockDim.x;
@@ -63,16 +61,13 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, View3 pu
 // ------------------------------------------------------------------ m2phlb1
 // pydyn/zoom/massflow.py:282-290 on a tile of TI columns: the column masses staged in LDS,
 // sp = seqsum(m over l) g / dxyp(j) per column, then phlb = at + bt sp in parallel.
-constexpr int MP_TI = 32;
-
-__global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, View3 m, const double *dxyp,
+__global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, int TI, View3 m, const double *dxyp,
                                                  const double *at, const double *bt, View3 sp, View3 phlb, int *words,
                                                  int skip_mask, View3 mcopy)
 {
-    // mcopy (when mcopy.p is set): a copy of m on the way
+    // TI columns i per workgroup (TM5_TILE_M2PHLB); mcopy (when mcopy.p is set): a copy of m on the way
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double sh[];  // M [lm][TI], P [TI]
-    const int TI = MP_TI;
     double *M = sh, *P = sh + lm * TI;
     const int j = blockIdx.y, i0 = blockIdx.x * TI;
     const int ni = min(TI, im - i0);
@@ -185,21 +180,23 @@ static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
 {
-    if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !bt) return TM5_EINVAL;
-    const dim3 grid((im + D0_TI - 1) / D0_TI, jm);
-    TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * D0_TI * sizeof(double), ctx->stream, im, jm, lm, pu,
-               pv, a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
+    const int TI = ctx->tile[TM5_TILE_DYNAM0];
+    if (im < 1 || jm < 1 || lm < 1 || lm * TI > 8192 || !bt) return TM5_EINVAL;
+    const dim3 grid((im + TI - 1) / TI, jm);
+    TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI, pu, pv,
+               a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy)
 {
-    if (im < 1 || jm < 1 || lm < 1 || lm > 200 || !dxyp || !at || !bt) return TM5_EINVAL;
-    const dim3 grid((im + MP_TI - 1) / MP_TI, jm);
+    const int TI = ctx->tile[TM5_TILE_M2PHLB];
+    if (im < 1 || jm < 1 || lm < 1 || (lm + 1) * TI > 8192 || !dxyp || !at || !bt) return TM5_EINVAL;
+    const dim3 grid((im + TI - 1) / TI, jm);
     const View3 none{};
-    TM5_LAUNCH(ctx, k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * MP_TI * sizeof(double), ctx->stream, im, jm,
-                       lm, m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS, mcopy ? *mcopy : none);
+    TM5_LAUNCH(ctx, k_m2phlb1, grid, dim3(256), (size_t)(lm + 1) * TI * sizeof(double), ctx->stream, im, jm, lm, TI,
+               m, dxyp, at, bt, sp, phlb, ctx->words, TM5_ERRORS, mcopy ? *mcopy : none);
     return status(hipGetLastError());
 }
 

@@ -33,6 +33,7 @@ struct tm5_ctx {
     int x_nrows;    // rows the last x operator wrote
     int *z_ok;      // device, per-block verdicts of the z check at n = 1
     int z_cap;      // its length
+    int tile[TM5_TILES];  // tile shapes (tm5_ctx_set_tile)
     tm5_prof *prof; // non-null while profiling (then prof == prof_store)
     tm5_prof *prof_store;  // owned: the events and the last totals
 };

@@ -9,10 +9,18 @@ was made), so torch copies and library kernels are ordered on it.
 """
 
 import ctypes
+import json
+import pathlib
 
 import numpy as np
 
 from . import _lib
+
+# tm5_ctx_set_tile's knobs (include/tm5hip.h), by name
+TILES = {"y_lines": 0, "z_lines": 1, "dynam0": 2, "m2phlb": 3}
+# the shapes pydyn_amd.tune measured best, applied to every new context (absent: the defaults)
+TILES_FILE = pathlib.Path(__file__).resolve().parent / "tiles.json"
+TILES_FILE_ENABLED = True
 
 
 class CFLError(RuntimeError):
@@ -37,6 +45,18 @@ class Context:
         ptr = ctypes.c_void_p()
         _lib.check(lib.tm5_ctx_create(self.stream.cuda_stream, ctypes.byref(ptr)), "tm5_ctx_create")
         self.ptr = ptr
+        if TILES_FILE_ENABLED and TILES_FILE.exists():
+            for name, value in json.loads(TILES_FILE.read_text())["tiles"].items():
+                self.set_tile(name, value)
+
+    def set_tile(self, name, value):
+        """A tile shape of the kernels (TILES; speed only, never results)."""
+        _lib.check(_lib.lib().tm5_ctx_set_tile(self.ptr, TILES[name], int(value)), f"tm5_ctx_set_tile {name}")
+
+    def tile(self, name):
+        out = ctypes.c_int()
+        _lib.check(_lib.lib().tm5_ctx_get_tile(self.ptr, TILES[name], ctypes.byref(out)), "tm5_ctx_get_tile")
+        return out.value
 
     def flags(self, clear=True):
         """Wait for the stream and return the flag word (cleared when `clear`)."""

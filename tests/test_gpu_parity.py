@@ -249,6 +249,26 @@ def test_run_glb3x2_matches_oracle_and_reference(dev):
             print(f"glb3x2: interval {it + 1}/{len(ref)} bitwise equal to the reference", flush=True)
 
 
+def test_tile_shapes_change_no_bits(dev):
+    """Every tile shape of tm5_ctx_set_tile splits the same operations differently into
+    workgroups: the small grid's run (Check_CFL cuts included) against the reference with shapes
+    unlike the defaults and the tuned ones."""
+    D, _ = dev
+    ctx = D.context()
+    saved = {k: ctx.tile(k) for k in D.TILES}
+    try:
+        for shape in ({"y_lines": 3, "z_lines": 5, "dynam0": 7, "m2phlb": 9},
+                      {"y_lines": 16, "z_lines": 32, "dynam0": 64, "m2phlb": 64}):  # fmt: skip
+            for k, v in shape.items():
+                ctx.set_tile(k, v)
+            _run_against_golden("small")
+    finally:
+        for k, v in saved.items():
+            ctx.set_tile(k, v)
+    with pytest.raises(RuntimeError):
+        ctx.set_tile("dynam0", 65)
+
+
 def test_profile_times_every_launch(dev):
     """profile_begin/profile_end total the GPU time per kernel from events around each launch."""
     D, _ = dev
