@@ -298,17 +298,15 @@ __global__ void __launch_bounds__(1024) k_y_search(int im, int jm, int lm, View3
 // writes it.  Along-flow slope rym; carried cross slopes rxm (ft) and rzm (fw).  Flow out of a
 // pole cell carries no slope; the pole cells keep their rxm, rym.  The limiter of the first
 // sub-step is taken on the values as they are read (it is the reference's initial limiter).
-__device__ __forceinline__ FaceFlux y_face(const View3 &m, const Tracers &q, const View3 &bm, int i, int k, int l, int t,
+__device__ __forceinline__ FaceFlux y_face(double bf, double m_u, double rm, double rx, double ry, double rz, int k,
                                            int jm, bool lim0)
 {
+    // face k with flux bf and its upwind cell's mass and moments (ry as stored: the first
+    // sub-step's limiter is taken here)
     FaceFlux ff{0.0, 0.0, 0.0, 0.0};
     if (k <= 0 || k >= jm) return ff;  // the closed polar faces
-    const double bf = bm(i, k, l);
-    const int u = bf >= 0.0 ? k - 1 : k;
-    const double beta = bf / m(i, u, l);
-    const long long o = q.off(i, u, l, t);
-    const double rm = q.q[0][o], rx = q.q[1][o], rz = q.q[3][o];
-    const double ry = lim0 ? limit(q.q[2][o], rm) : q.q[2][o];
+    const double beta = bf / m_u;
+    if (lim0) ry = limit(ry, rm);
     ff = face_flux(bf, beta, rm, ry, rx, rz);
     if ((k == 1 && bf >= 0.0) || (k == jm - 1 && bf < 0.0)) {  // leaving a pole cell
         ff.f = beta * rm;
@@ -319,9 +317,6 @@ __device__ __forceinline__ FaceFlux y_face(const View3 &m, const Tracers &q, con
     return ff;
 }
 
-// spec_k >= 0 (no table, every layer at nloop 1): a checked step's dynamv, which first makes
-// Check_CFL's test of dynamvm at cap 1 on its own faces (as k_y_mass1) and, where a face fails,
-// raises spec_fail -- the kernels after it skip
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 bm,
                                                   const int *nloop, int limits, int *words, int skip_mask, int spec_k)
@@ -350,6 +345,8 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
             if (any_fail) spec_fail(words, spec_k);
         }
     }
+    // the cells below and above (clamped at the poles, where their faces carry nothing)
+    const int jlo = jj > 0 ? jj - 1 : 0, jhi = jj + 1 < jm ? jj + 1 : jm - 1;
     for (int s = 0; s < nl; s++) {
         const double m_old = m(i, jj, l), b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
         const double m_new = (m_old + b_lo) - b_hi;
@@ -360,19 +357,26 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
             const double mass_x3 = 3.0 * m_old;
             const bool lim0 = limits && s == 0;
             const bool pole = jj == 0 || jj == jm - 1;
+            // both candidates of each face's upwind cell are read, independently of the flux signs
+            const double m_dn = m(i, jlo, l), m_up = m(i, jhi, l);
             for (int t = 0; t < q.nt; t++) {
-                const FaceFlux lo = y_face(m, q, bm, i, jj, l, t, jm, lim0);
-                const FaceFlux hi = y_face(m, q, bm, i, jj + 1, l, t, jm, lim0);
-                const long long o = q.off(i, jj, l, t);
-                const double rm = q.q[0][o];
-                double rym = q.q[2][o];
+                const long long o = q.off(i, jj, l, t), od = q.off(i, jlo, l, t), ou = q.off(i, jhi, l, t);
+                const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
+                const double rm_d = q.q[0][od], rx_d = q.q[1][od], ry_d = q.q[2][od], rz_d = q.q[3][od];
+                const double rm_u = q.q[0][ou], rx_u = q.q[1][ou], ry_u = q.q[2][ou], rz_u = q.q[3][ou];
+                const bool lo_dn = b_lo >= 0.0, hi_own = b_hi >= 0.0;  // upwind: below / this cell
+                const FaceFlux lo = y_face(b_lo, lo_dn ? m_dn : m_old, lo_dn ? rm_d : rm, lo_dn ? rx_d : rx0,
+                                           lo_dn ? ry_d : ry0, lo_dn ? rz_d : rz0, jj, jm, lim0);
+                const FaceFlux hi = y_face(b_hi, hi_own ? m_old : m_up, hi_own ? rm : rm_u, hi_own ? rx0 : rx_u,
+                                           hi_own ? ry0 : ry_u, hi_own ? rz0 : rz_u, jj + 1, jm, lim0);
+                double rym = ry0;
                 if (lim0) rym = limit(rym, rm);
                 const double rm_new = rm - (hi.f - lo.f);
                 double ry = rym + ((((lo.pf - hi.pf) - flow_conv * rym) + flow_mean * rm_new) - mass_x3 * (lo.f + hi.f)) *
                                       inv_mnew;
                 if (limits) ry = limit(ry, rm_new);
-                const double rx = q.q[1][o] - (hi.ft - lo.ft);
-                const double rz = q.q[3][o] - (hi.fw - lo.fw);
+                const double rx = rx0 - (hi.ft - lo.ft);
+                const double rz = rz0 - (hi.fw - lo.fw);
                 __syncthreads();
                 if (mine) {
                     q.q[0][o] = rm_new;
