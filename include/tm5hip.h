@@ -180,6 +180,14 @@ int tm5_decay(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *loss, double
               int count);
 /* a = a * factor on a 3-D view of n0 x n1 x n2.  pydyn/zoom/host.py:226-230 scale. */
 int tm5_scale(tm5_ctx *ctx, int n0, int n1, int n2, const tm5_view *a, double factor);
+/* The device's libm over contiguous device arrays of n doubles: out = exp(x), log(x) or
+   pow(x, y) (fn TM5_LIBM_*; y only for pow), by the fdlibm methods the kernels use.  The
+   reference's libm route ops.libm.exp/log/pow (pydyn/ops.py:165-232) with its device twin
+   (pydyn/gpu/csrc/pmath.cuh). */
+#define TM5_LIBM_EXP 0
+#define TM5_LIBM_LOG 1
+#define TM5_LIBM_POW 2
+int tm5_libm(tm5_ctx *ctx, int fn, int64_t n, const double *x, const double *y, double *out);
 
 /* ------------------------------------------------------------------ the region driver */
 /* One global region resident on the device, run interval by interval by a native driver:

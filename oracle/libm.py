@@ -1,5 +1,6 @@
-"""The oracle's exp: the fdlibm method in C (oracle/libm.c), built into oracle/_build/ by
-oracle/Makefile, applied element by element through a C loop."""
+"""The oracle's exp, log and pow: the fdlibm methods in C (oracle/libm.c), built into
+oracle/_build/ by oracle/Makefile, applied element by element through a C loop.  Test
+infrastructure: the checker of the device's libm, never a product path."""
 
 import ctypes
 import pathlib
@@ -21,8 +22,11 @@ def _lib():
         if not LIB_PATH.exists():
             build()
         lib = ctypes.CDLL(str(LIB_PATH))
-        lib.oracle_exp_map.restype = None
-        lib.oracle_exp_map.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        for name in ("oracle_exp_map", "oracle_log_map"):
+            getattr(lib, name).restype = None
+            getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        lib.oracle_pow_map.restype = None
+        lib.oracle_pow_map.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         _LIB.append(lib)
     return _LIB[0]
 
@@ -33,3 +37,20 @@ def exp(x):
     y = np.empty_like(x)
     _lib().oracle_exp_map(x.ctypes.data, y.ctypes.data, x.size)
     return y
+
+
+def log(x):
+    """fdlibm log of every element of x (float64)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    y = np.empty_like(x)
+    _lib().oracle_log_map(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def pow(x, y):  # noqa: A001 -- the libm name
+    """fdlibm pow of every pair of elements (float64, broadcast)."""
+    x, y = np.broadcast_arrays(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
+    x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+    out = np.empty_like(x)
+    _lib().oracle_pow_map(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out

@@ -106,6 +106,7 @@ _SIGNATURES = {
     "tm5_ctx_nloop": [_VP, _P(_I), _P(_I), _P(_I), _I, _P(_I)],
     "tm5_ctx_clear": [_VP, _I],
     "tm5_ctx_set_tile": [_VP, _I, _I],
+    "tm5_libm": [_VP, _I, ctypes.c_int64, _VP, _VP, _VP],
     "tm5_ctx_get_tile": [_VP, _I, _P(_I)],
     "tm5_profile_begin": [_VP],
     "tm5_profile_end": [_VP],

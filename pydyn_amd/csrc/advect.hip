@@ -551,20 +551,8 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
 // pydyn/kernels.py:324-346 with c = cm / n_advz (cm itself not written): one thread per cell of
 // TI whole columns (i0.., j = blockIdx.y), the neighbourhood read into registers, the update
 // written after the workgroup's barrier.  Along-flow slope rzm; cross slopes rxm (ft), rym (fw).
-__device__ __forceinline__ FaceFlux z_face(const View3 &m, const Tracers &q, const View3 &cm, int i, int j, int k,
-                                           int t, int lm, double dn, bool lim0)
-{
-    FaceFlux ff{0.0, 0.0, 0.0, 0.0};
-    if (k <= 0 || k >= lm) return ff;  // the closed ends
-    const double cf = cm(i, j, k) / dn;
-    const int u = cf >= 0.0 ? k - 1 : k;
-    const double gamma = cf / m(i, j, u);
-    const long long o = q.off(i, j, u, t);
-    const double rm = q.q[0][o];
-    const double rz = lim0 ? limit(q.q[3][o], rm) : q.q[3][o];
-    return face_flux(cf, gamma, rm, rz, q.q[1][o], q.q[2][o]);
-}
-
+// A face's flux, its Courant number c / m(upwind) and upwind cell are the same for every tracer
+// of a sub-step and are found once; tracers go two a round, so that their loads overlap.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
                                                   int limits, int prelimit, int *words, int skip_mask, int fixed_n,
@@ -599,29 +587,62 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     }
     const double dn = (double)nz;
     const double c_lo = cm(i, j, ll) / dn, c_hi = cm(i, j, ll + 1) / dn;
+    // each face's upwind cell (the closed ends carry nothing)
+    const bool has_lo = ll >= 1, has_hi = ll + 1 <= lm - 1;
+    const int u_lo = c_lo >= 0.0 ? (has_lo ? ll - 1 : ll) : ll, u_hi = c_hi >= 0.0 ? ll : (has_hi ? ll + 1 : ll);
     for (int s = 0; s < nz; s++) {
         const double m_old = m(i, j, ll);
         const double m_new = (m_old + c_lo) - c_hi;
         if (!MASS_ONLY) {
             const bool lim0 = prelimit && s == 0;
-            for (int t = 0; t < q.nt; t++) {
-                const FaceFlux lo = z_face(m, q, cm, i, j, ll, t, lm, dn, lim0);
-                const FaceFlux hi = z_face(m, q, cm, i, j, ll + 1, t, lm, dn, lim0);
-                const long long o = q.off(i, j, ll, t);
-                const double rm = q.q[0][o];
-                double rzm = q.q[3][o];
-                if (lim0) rzm = limit(rzm, rm);
-                const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
-                double rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
+            // the Courant numbers of the sub-step, the same for every tracer
+            const double g_lo = c_lo / m(i, j, u_lo), g_hi = c_hi / m(i, j, u_hi);
+            // one tracer's update (z_face's fluxes with the Courant numbers above)
+            auto update = [&](int t, long long o, double &rm_new, double &rx, double &ry, double &rz) {
+                FaceFlux lo{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};
+                const long long ol = q.off(i, j, u_lo, t), oh = q.off(i, j, u_hi, t);
+                const double rml = q.q[0][ol], rxl = q.q[1][ol], ryl = q.q[2][ol], rzl = q.q[3][ol];
+                const double rmh = q.q[0][oh], rxh = q.q[1][oh], ryh = q.q[2][oh], rzh = q.q[3][oh];
+                const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
+                if (has_lo) lo = face_flux(c_lo, g_lo, rml, lim0 ? limit(rzl, rml) : rzl, rxl, ryl);
+                if (has_hi) hi = face_flux(c_hi, g_hi, rmh, lim0 ? limit(rzh, rmh) : rzh, rxh, ryh);
+                const double rzm = lim0 ? limit(rz0, rm) : rz0;
+                rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
+                rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
                 if (limits) rz = limit(rz, rm_new);
-                const double rx = (q.q[1][o] + lo.ft) - hi.ft;
-                const double ry = (q.q[2][o] + lo.fw) - hi.fw;
+                rx = (rx0 + lo.ft) - hi.ft;
+                ry = (ry0 + lo.fw) - hi.fw;
+            };
+            int t = 0;
+            for (; t + 1 < q.nt; t += 2) {  // two tracers a round: their loads in flight together
+                const long long o0 = q.off(i, j, ll, t), o1 = q.off(i, j, ll, t + 1);
+                double a0, a1, a2, a3, b0, b1, b2, b3;
+                update(t, o0, a0, a1, a2, a3);
+                update(t + 1, o1, b0, b1, b2, b3);
                 __syncthreads();
                 if (mine) {
-                    q.q[0][o] = rm_new;
-                    q.q[3][o] = rz;
-                    q.q[1][o] = rx;
-                    q.q[2][o] = ry;
+                    q.q[0][o0] = a0;
+                    q.q[1][o0] = a1;
+                    q.q[2][o0] = a2;
+                    q.q[3][o0] = a3;
+                    q.q[0][o1] = b0;
+                    q.q[1][o1] = b1;
+                    q.q[2][o1] = b2;
+                    q.q[3][o1] = b3;
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+            if (t < q.nt) {
+                const long long o0 = q.off(i, j, ll, t);
+                double a0, a1, a2, a3;
+                update(t, o0, a0, a1, a2, a3);
+                __syncthreads();
+                if (mine) {
+                    q.q[0][o0] = a0;
+                    q.q[1][o0] = a1;
+                    q.q[2][o0] = a2;
+                    q.q[3][o0] = a3;
                 }
                 __threadfence_block();
                 __syncthreads();

@@ -447,6 +447,12 @@ extern "C" int tm5_decay(tm5_ctx *c, int im, int jm, int lm, const tm5_view *los
     return launch_decay(c, im, jm, lm, view3(loss), loss->stride[3], dt, tracers(q), t0, count);
 }
 
+extern "C" int tm5_libm(tm5_ctx *c, int fn, int64_t n, const double *x, const double *y, double *out)
+{
+    if (!c) return TM5_EINVAL;
+    return launch_libm(c, fn, (long long)n, x, y, out);
+}
+
 extern "C" int tm5_scale(tm5_ctx *c, int n0, int n1, int n2, const tm5_view *a, double factor)
 {
     if (!c || bad_view(a)) return TM5_EINVAL;

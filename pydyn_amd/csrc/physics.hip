@@ -177,6 +177,24 @@ __global__ void __launch_bounds__(256) k_scale(int n0, int n1, int n2, View3 a, 
 
 static dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// ------------------------------------------------------------------ the libm route
+__global__ void __launch_bounds__(256) k_libm(int fn, long long n, const double *x, const double *y, double *out)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double v = x[k];
+    out[k] = fn == TM5_LIBM_EXP ? fdlibm_exp(v) : fn == TM5_LIBM_LOG ? fdlibm_log(v) : fdlibm_pow(v, y[k]);
+}
+
+int launch_libm(tm5_ctx *ctx, int fn, long long n, const double *x, const double *y, double *out)
+{
+    if (n < 0 || n > (1LL << 40) || !x || !out || fn < TM5_LIBM_EXP || fn > TM5_LIBM_POW || (fn == TM5_LIBM_POW && !y))
+        return TM5_EINVAL;
+    if (n == 0) return 0;
+    TM5_LAUNCH(ctx, k_libm, blocks(n), dim3(256), 0, ctx->stream, fn, n, x, y, out);
+    return status(hipGetLastError());
+}
+
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
 {

@@ -132,4 +132,255 @@ __device__ inline double fdlibm_exp(double x)
     return __longlong_as_double((long long)(yb + ((uint64_t)(int64_t)(k + 1000) << 52))) * twom1000;
 }
 
+// ------------------------------------------------------------------ the fdlibm log and pow
+// By the methods of Sun's fdlibm e_log.c and e_pow.c (log: x = 2^k (1 + f), s = f / (2 + f) and
+// a degree-14 Remez series in s; pow: log2|x| to extra precision in two parts, y split into a
+// head and a tail, 2^(y log2|x|) by the exp method; the special cases of C99 Annex F).  The same
+// operations in the same order as the oracle's C copy, constants by their bit patterns, so host
+// and device agree bit for bit.
+__device__ __forceinline__ double fd_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ int32_t fd_hi(double x) { return (int32_t)((uint64_t)__double_as_longlong(x) >> 32); }
+__device__ __forceinline__ uint32_t fd_lo(double x) { return (uint32_t)(uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ double fd_with_hi(double x, int32_t hi)
+{
+    return fd_bits(((uint64_t)(uint32_t)hi << 32) | ((uint64_t)__double_as_longlong(x) & 0xffffffffull));
+}
+__device__ __forceinline__ double fd_lo_cleared(double x)
+{
+    return fd_bits((uint64_t)__double_as_longlong(x) & 0xffffffff00000000ull);
+}
+
+__device__ inline double fdlibm_log(double x)
+{
+    const double ln2_hi = fd_bits(0x3fe62e42fee00000ull), ln2_lo = fd_bits(0x3dea39ef35793c76ull);
+    const double two54 = fd_bits(0x4350000000000000ull);
+    const double Lg1 = fd_bits(0x3fe5555555555593ull), Lg2 = fd_bits(0x3fd999999997fa04ull);
+    const double Lg3 = fd_bits(0x3fd2492494229359ull), Lg4 = fd_bits(0x3fcc71c51d8e78afull);
+    const double Lg5 = fd_bits(0x3fc7466496cb03deull), Lg6 = fd_bits(0x3fc39a09d078c69full);
+    const double Lg7 = fd_bits(0x3fc2f112df3e5244ull);
+    int32_t hx = fd_hi(x);
+    const uint32_t lx = fd_lo(x);
+    int k = 0;
+    if (hx < 0x00100000) {                                         /* x < 2^-1022 */
+        if (((hx & 0x7fffffff) | (int32_t)lx) == 0) return -__longlong_as_double(0x7ff0000000000000ll); /* log(+-0) */
+        if (hx < 0) return __longlong_as_double(0x7ff8000000000000ll);                                      /* log(-x) */
+        k -= 54;
+        x *= two54; /* subnormal: scale up */
+        hx = fd_hi(x);
+    }
+    if (hx >= 0x7ff00000) return x + x;
+    k += (hx >> 20) - 1023;
+    hx &= 0x000fffff;
+    int i = (hx + 0x95f64) & 0x100000;
+    x = fd_with_hi(x, hx | (i ^ 0x3ff00000)); /* normalise x or x/2 */
+    k += i >> 20;
+    const double f = x - 1.0;
+    if ((0x000fffff & (2 + hx)) < 3) { /* |f| < 2^-20 */
+        if (f == 0.0) {
+            if (k == 0) return 0.0;
+            const double dk = (double)k;
+            return dk * ln2_hi + dk * ln2_lo;
+        }
+        const double R = f * f * (0.5 - 0.33333333333333333 * f);
+        if (k == 0) return f - R;
+        const double dk = (double)k;
+        return dk * ln2_hi - ((R - dk * ln2_lo) - f);
+    }
+    const double s = f / (2.0 + f);
+    const double dk = (double)k;
+    const double z = s * s;
+    i = hx - 0x6147a;
+    const double w = z * z;
+    const int j = 0x6b851 - hx;
+    const double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+    const double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+    i |= j;
+    const double R = t2 + t1;
+    if (i > 0) {
+        const double hfsq = 0.5 * f * f;
+        if (k == 0) return f - (hfsq - s * (hfsq + R));
+        return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+    }
+    if (k == 0) return f - s * (f - R);
+    return dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
+__device__ inline double fdlibm_pow(double x, double y)
+{
+    const double bp[2] = {1.0, 1.5};
+    const double dp_h[2] = {0.0, fd_bits(0x3fe2b80340000000ull)};
+    const double dp_l[2] = {0.0, fd_bits(0x3e4cfdeb43cfd006ull)};
+    const double two53 = fd_bits(0x4340000000000000ull), huge = 1.0e300, tiny = 1.0e-300;
+    const double L1 = fd_bits(0x3fe3333333333303ull), L2 = fd_bits(0x3fdb6db6db6fabffull);
+    const double L3 = fd_bits(0x3fd55555518f264dull), L4 = fd_bits(0x3fd17460a91d4101ull);
+    const double L5 = fd_bits(0x3fcd864a93c9db65ull), L6 = fd_bits(0x3fca7e284a454eefull);
+    const double P1 = fd_bits(0x3fc555555555553eull), P2 = fd_bits(0xbf66c16c16bebd93ull);
+    const double P3 = fd_bits(0x3f11566aaf25de2cull), P4 = fd_bits(0xbebbbd41c5d26bf1ull);
+    const double P5 = fd_bits(0x3e66376972bea4d0ull);
+    const double lg2 = fd_bits(0x3fe62e42fefa39efull), lg2_h = fd_bits(0x3fe62e4300000000ull);
+    const double lg2_l = fd_bits(0xbe205c610ca86c39ull);
+    const double ovt = 8.0085662595372944372e-17; /* -(1024 - log2(ovfl + .5 ulp)) */
+    const double cp = fd_bits(0x3feec709dc3a03fdull), cp_h = fd_bits(0x3feec709e0000000ull);
+    const double cp_l = fd_bits(0xbe3e2fe0145b01f5ull);
+    const double ivln2 = fd_bits(0x3ff71547652b82feull), ivln2_h = fd_bits(0x3ff7154760000000ull);
+    const double ivln2_l = fd_bits(0x3e54ae0bf85ddf44ull);
+
+    const int32_t hx = fd_hi(x), hy = fd_hi(y);
+    const uint32_t lx = fd_lo(x), ly = fd_lo(y);
+    int32_t ix = hx & 0x7fffffff;
+    const int32_t iy = hy & 0x7fffffff;
+
+    if ((iy | (int32_t)ly) == 0) return 1.0; /* x^0 = 1, also for NaN x */
+    if (x == 1.0) return 1.0;                /* 1^y = 1, also for NaN y (C99) */
+    if (ix > 0x7ff00000 || (ix == 0x7ff00000 && lx != 0) || iy > 0x7ff00000 || (iy == 0x7ff00000 && ly != 0))
+        return x + y; /* NaN */
+    /* yisint: 0 y not an integer, 1 an odd integer, 2 an even integer (x < 0 only) */
+    int yisint = 0;
+    if (hx < 0) {
+        if (iy >= 0x43400000)
+            yisint = 2;
+        else if (iy >= 0x3ff00000) {
+            const int k = (iy >> 20) - 0x3ff;
+            if (k > 20) {
+                const uint32_t j = ly >> (52 - k);
+                if ((j << (52 - k)) == ly) yisint = 2 - (int)(j & 1);
+            } else if (ly == 0) {
+                const int32_t j = iy >> (20 - k);
+                if ((j << (20 - k)) == iy) yisint = 2 - (j & 1);
+            }
+        }
+    }
+    if (ly == 0) {
+        if (iy == 0x7ff00000) { /* y = +-inf */
+            if (((ix - 0x3ff00000) | (int32_t)lx) == 0) return 1.0; /* (-1)^+-inf = 1 (C99) */
+            if (ix >= 0x3ff00000) return hy >= 0 ? y : 0.0;        /* |x| > 1 */
+            return hy < 0 ? -y : 0.0;                                /* |x| < 1 */
+        }
+        if (iy == 0x3ff00000) return hy < 0 ? 1.0 / x : x; /* y = +-1 */
+        if (hy == 0x40000000) return x * x;                /* y = 2 */
+        if (hy == 0x3fe00000 && hx >= 0) return sqrt(x);   /* y = 0.5, x >= +0 */
+    }
+    double ax = fabs(x);
+    if (lx == 0 && (ix == 0x7ff00000 || ix == 0 || ix == 0x3ff00000)) { /* x = +-0, +-inf, +-1 */
+        double z = ax;
+        if (hy < 0) z = 1.0 / z;
+        if (hx < 0) {
+            if (((ix - 0x3ff00000) | yisint) == 0)
+                z = __longlong_as_double(0x7ff8000000000000ll); /* (-1)^non-integer */
+            else if (yisint == 1)
+                z = -z;
+        }
+        return z;
+    }
+    int n = (int)((uint32_t)hx >> 31) ^ 1; /* 1 for x >= 0, 0 for x < 0 */
+    if ((n | yisint) == 0) return __longlong_as_double(0x7ff8000000000000ll);     /* (x < 0)^non-integer */
+    double s = 1.0;
+    if ((n | (yisint - 1)) == 0) s = -1.0; /* (x < 0)^odd */
+
+    double t1, t2;
+    if (iy > 0x41e00000) { /* |y| > 2^31 */
+        if (iy > 0x43f00000) { /* |y| > 2^64: over- or underflow */
+            if (ix <= 0x3fefffff) return hy < 0 ? huge * huge : tiny * tiny;
+            if (ix >= 0x3ff00000) return hy > 0 ? huge * huge : tiny * tiny;
+        }
+        if (ix < 0x3fefffff) return hy < 0 ? s * huge * huge : s * tiny * tiny;
+        if (ix > 0x3ff00000) return hy > 0 ? s * huge * huge : s * tiny * tiny;
+        /* |1 - x| <= 2^-20: log(x) by x - x^2/2 + x^3/3 - x^4/4 */
+        const double t = ax - 1.0;
+        const double w = (t * t) * (0.5 - t * (0.3333333333333333333333 - t * 0.25));
+        const double u = ivln2_h * t;
+        const double v = t * ivln2_l - w * ivln2;
+        t1 = fd_lo_cleared(u + v);
+        t2 = v - (t1 - u);
+    } else {
+        n = 0;
+        if (ix < 0x00100000) { /* subnormal x */
+            ax *= two53;
+            n -= 53;
+            ix = fd_hi(ax);
+        }
+        n += (ix >> 20) - 0x3ff;
+        const int32_t j = ix & 0x000fffff;
+        int k;
+        ix = j | 0x3ff00000; /* normalise */
+        if (j <= 0x3988E)
+            k = 0; /* |x| < sqrt(3/2) */
+        else if (j < 0xBB67A)
+            k = 1; /* |x| < sqrt(3) */
+        else {
+            k = 0;
+            n += 1;
+            ix -= 0x00100000;
+        }
+        ax = fd_with_hi(ax, ix);
+        /* ss = s_h + s_l = (x - 1)/(x + 1) or (x - 1.5)/(x + 1.5) */
+        double u = ax - bp[k];
+        double v = 1.0 / (ax + bp[k]);
+        const double ss = u * v;
+        const double s_h = fd_lo_cleared(ss);
+        double t_h = fd_bits((uint64_t)(uint32_t)(((ix >> 1) | 0x20000000) + 0x00080000 + (k << 18)) << 32);
+        double t_l = ax - (t_h - bp[k]);
+        const double s_l = v * ((u - s_h * t_h) - s_h * t_l);
+        /* log(ax) */
+        double s2 = ss * ss;
+        double r = s2 * s2 * (L1 + s2 * (L2 + s2 * (L3 + s2 * (L4 + s2 * (L5 + s2 * L6)))));
+        r += s_l * (s_h + ss);
+        s2 = s_h * s_h;
+        t_h = fd_lo_cleared(3.0 + s2 + r);
+        t_l = r - ((t_h - 3.0) - s2);
+        u = s_h * t_h;
+        v = s_l * t_h + t_l * ss;
+        const double p_h = fd_lo_cleared(u + v);
+        const double p_l = v - (p_h - u);
+        const double z_h = cp_h * p_h;
+        const double z_l = cp_l * p_h + p_l * cp + dp_l[k];
+        /* log2(ax) = n + dp_h + z_h + z_l */
+        const double t = (double)n;
+        t1 = fd_lo_cleared(((z_h + z_l) + dp_h[k]) + t);
+        t2 = z_l - (((t1 - t) - dp_h[k]) - z_h);
+    }
+    /* (y1 + y2) (t1 + t2) with y1 y's head */
+    const double y1 = fd_lo_cleared(y);
+    const double p_l = (y - y1) * t1 + y * t2;
+    double p_h = y1 * t1;
+    double z = p_l + p_h;
+    int32_t j = fd_hi(z);
+    int32_t i = (int32_t)fd_lo(z);
+    if (j >= 0x40900000) { /* z >= 1024 */
+        if (((j - 0x40900000) | i) != 0) return s * huge * huge;
+        if (p_l + ovt > z - p_h) return s * huge * huge;
+    } else if ((j & 0x7fffffff) >= 0x4090cc00) { /* z <= -1075 */
+        if (((j - (int32_t)0xc090cc00) | i) != 0) return s * tiny * tiny;
+        if (p_l <= z - p_h) return s * tiny * tiny;
+    }
+    /* 2^(p_h + p_l) */
+    i = j & 0x7fffffff;
+    int k = (i >> 20) - 0x3ff;
+    n = 0;
+    if (i > 0x3fe00000) { /* |z| > 0.5: n = [z + 0.5] */
+        n = j + (0x00100000 >> (k + 1));
+        k = ((n & 0x7fffffff) >> 20) - 0x3ff;
+        const double t = fd_bits((uint64_t)(uint32_t)(n & ~(0x000fffff >> k)) << 32);
+        n = ((n & 0x000fffff) | 0x00100000) >> (20 - k);
+        if (j < 0) n = -n;
+        p_h -= t;
+    }
+    const double t = fd_lo_cleared(p_l + p_h);
+    const double u = t * lg2_h;
+    const double v = (p_l - (t - p_h)) * lg2 + t * lg2_l;
+    z = u + v;
+    const double w = v - (z - u);
+    const double tt = z * z;
+    const double tt1 = z - tt * (P1 + tt * (P2 + tt * (P3 + tt * (P4 + tt * P5))));
+    const double r = (z * tt1) / (tt1 - 2.0) - (w + z * w);
+    z = 1.0 - (r - z);
+    j = fd_hi(z);
+    j += (int32_t)((uint32_t)n << 20);
+    if ((j >> 20) <= 0)
+        z = ldexp(z, n); /* subnormal result */
+    else
+        z = fd_with_hi(z, j);
+    return s * z;
+}
+
 }  // namespace tm5

@@ -242,6 +242,7 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
 int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+int launch_libm(tm5_ctx *ctx, int fn, long long n, const double *x, const double *y, double *out);
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);

@@ -215,6 +215,65 @@ def test_fdlibm_exp_within_one_ulp_of_the_c_library():
     assert np.isnan(special[4]) and special[5] == np.inf and special[6] == 0.0
 
 
+def _ulps(got, want):
+    """|got - want| in units in the last place of want (finite, same sign)."""
+    return np.abs(got.view(np.int64) - want.view(np.int64))
+
+
+def test_fdlibm_log_within_one_ulp_of_the_c_library():
+    import math
+
+    rng = np.random.default_rng(11)
+    x = np.concatenate([
+        np.exp(rng.uniform(-744.0, 709.0, 100000)),  # every binade, subnormals included
+        rng.uniform(0.5, 2.0, 100000),  # around 1, both of fdlibm's branches
+        1.0 + rng.uniform(-1e-6, 1e-6, 20000),  # |f| < 2^-20
+        np.ldexp(1.0, np.arange(-1074, 1024)),  # powers of two: k ln2 exactly
+    ])  # fmt: skip
+    got = oracle_libm.log(x)
+    want = np.array([math.log(v) for v in x])
+    assert _ulps(got, want).max() <= 1
+    assert oracle_libm.log(np.array([2.0]))[0] == math.log(2.0)  # ln2_hi + ln2_lo rounds to ln 2
+    special = oracle_libm.log(np.array([1.0, 0.0, -0.0, np.inf, -1.0, np.nan, -np.inf]))
+    assert special[0] == 0.0 and special[1] == -np.inf and special[2] == -np.inf and special[3] == np.inf
+    assert np.isnan(special[4:]).all()
+
+
+def test_fdlibm_pow_within_one_ulp_of_the_c_library_and_c99_special_cases():
+    rng = np.random.default_rng(12)
+    x = np.concatenate([np.exp(rng.uniform(-30.0, 30.0, 100000)), rng.uniform(0.1, 3.0, 100000),
+                        1.0 + rng.uniform(-1e-7, 1e-7, 2000), -np.exp(rng.uniform(-5.0, 5.0, 2000))])  # fmt: skip
+    y = np.concatenate([rng.uniform(-20.0, 20.0, 100000), rng.uniform(-3.0, 3.0, 100000),
+                        rng.uniform(-3e9, 3e9, 2000), np.round(rng.uniform(-20.0, 20.0, 2000))])  # fmt: skip
+    got = oracle_libm.pow(x, y)
+    with np.errstate(all="ignore"):
+        want = np.power(x, y)
+    fine = np.isfinite(want) & (want != 0.0)
+    # |y| > 2^31 with x within 2^-20 of 1 takes fdlibm's short series for log2(x), whose head
+    # ivln2_h * t is not exact (21 x 33 bits): up to a few hundred ulps there, by the method
+    corner = (np.abs(y) > 2.0**31) & (np.abs(x - 1.0) < 2.0**-20)
+    assert _ulps(got[fine & ~corner], want[fine & ~corner]).max() <= 1
+    assert np.all(np.abs(got[fine & corner] / want[fine & corner] - 1.0) < 1e-12)
+    assert np.array_equal(got[~fine], want[~fine])
+    k = np.arange(-1074, 1024, dtype=np.float64)
+    assert np.array_equal(oracle_libm.pow(2.0, k), np.ldexp(1.0, k.astype(int)))  # exact powers of two
+    v = rng.uniform(0.0, 10.0, 1000)
+    assert np.array_equal(oracle_libm.pow(v, 2.0), v * v)
+    assert np.array_equal(oracle_libm.pow(v, 0.5), np.sqrt(v))
+    assert np.array_equal(oracle_libm.pow(v, -1.0), 1.0 / v)
+    inf, nan = np.inf, np.nan
+    cases = [  # C99 Annex F.9.4.4
+        (nan, 0.0, 1.0), (1.0, nan, 1.0), (-1.0, inf, 1.0), (-1.0, -inf, 1.0), (-2.0, 3.0, -8.0),
+        (0.0, -1.0, inf), (-0.0, -1.0, -inf), (-0.0, -2.0, inf), (0.0, 3.0, 0.0), (-0.0, 3.0, -0.0),
+        (0.5, inf, 0.0), (0.5, -inf, inf), (2.0, -inf, 0.0), (2.0, inf, inf), (-inf, -3.0, -0.0),
+        (-inf, 3.0, -inf), (-inf, 2.0, inf), (inf, -1.0, 0.0), (2.0, 1024.0, inf), (2.0, -1075.0, 0.0),
+        (2.0, -1074.0, 5e-324), (-8.0, 1.0 / 3.0, nan), (nan, 1.0, nan), (2.0, nan, nan),
+    ]  # fmt: skip
+    for a, b, want in cases:
+        g = oracle_libm.pow(a, b)[()]
+        assert (np.isnan(g) and np.isnan(want)) or (g == want and np.signbit(g) == np.signbit(want)), (a, b, g)
+
+
 # ------------------------------------------------------------------ the zoom tree
 def _zoom_against_golden(name, limit=None):
     import hashlib
