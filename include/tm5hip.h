@@ -63,9 +63,10 @@ typedef struct tm5_iview {
 #define TM5_REHEARSAL 32 /* a Check_CFL mass-only operator exceeded its cap (the verdict) */
 #define TM5_FAIL_XC 64   /* a child's dynamu: nloop too high (CFLError "dynuamu: nloop too high") */
 #define TM5_FAIL_YC 128  /* a child's dynamv: nloop too high (CFLError "nloop too high in dynamv") */
-#define TM5_ERRORS 223   /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
+#define TM5_ERRORS 735   /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
 #define TM5_SPEC 256     /* tm5_region_run: a queued interval's trial or checked step disagreed with its plan;
                             the kernels queued after it did nothing (cleared before tm5_region_run returns) */
+#define TM5_BAD_LEVELS 512 /* tm5_loss_rates: layers not strictly descending (ValueError) */
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
@@ -184,6 +185,20 @@ int tm5_scale(tm5_ctx *ctx, int n0, int n1, int n2, const tm5_view *a, double fa
    pow(x, y) (fn TM5_LIBM_*; y only for pow), by the fdlibm methods the kernels use.  The
    reference's libm route ops.libm.exp/log/pow (pydyn/ops.py:165-232) with its device twin
    (pydyn/gpu/csrc/pmath.cuh). */
+/* The CH4 loss rates and decay of one month / step (pydyn/chem.py:129-170, ch4_loss_kernel; the
+   reference's device pydyn.gpu.kernels.loss_rates, gpu/kernels.py:545-645 with
+   csrc/chem.cuh): loss (im, jm, lm, 4; stride[3] the reaction axis) and, when decay is not
+   null, decay (im, jm, lm) = exp(-(((L1 + L2) + L3) + L4) dt).  temper, oh_3d (im, jm, lm),
+   phlb (im, jm, lm + 1) descending; trop (jm) the tropopause pressures and edges
+   (jm, nstrat + 1) the zonal columns' edges in Pa, both computed by the caller on the host as
+   the reference does (pydyn/chem.py:106-119); strat_oh/cl/od (jm, nstrat) contiguous; pl_trop and
+   cl_trop (im, jm, ntrop) the tropospheric Cl on its descending pressures, both null for none;
+   soil (im, jm) or null.  Layers not strictly descending set TM5_BAD_LEVELS. */
+int tm5_loss_rates(tm5_ctx *ctx, int im, int jm, int lm, int nstrat, int ntrop, const tm5_view *temper,
+                   const tm5_view *phlb, const double *trop, const double *edges, const double *strat_oh,
+                   const double *strat_cl, const double *strat_od, const tm5_view *oh_3d, const tm5_view *pl_trop,
+                   const tm5_view *cl_trop, const tm5_view *soil, double oh_scale, double cl_scale,
+                   double o1d_scale, double soil_rate, double dt, const tm5_view *loss, const tm5_view *decay);
 #define TM5_LIBM_EXP 0
 #define TM5_LIBM_LOG 1
 #define TM5_LIBM_POW 2

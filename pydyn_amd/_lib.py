@@ -10,7 +10,8 @@ import pathlib
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libtm5hip.so"
 
 FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC, SPEC = 1, 2, 4, 8, 16, 32, 64, 128, 256
-ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC
+BAD_LEVELS = 512
+ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC | BAD_LEVELS
 EINVAL, EFLAGS = -1, -2
 
 
@@ -107,6 +108,8 @@ _SIGNATURES = {
     "tm5_ctx_clear": [_VP, _I],
     "tm5_ctx_set_tile": [_VP, _I, _I],
     "tm5_libm": [_VP, _I, ctypes.c_int64, _VP, _VP, _VP],
+    "tm5_loss_rates": [_VP, _I, _I, _I, _I, _I, _P(View), _P(View), _VP, _VP, _VP, _VP, _VP, _P(View), _P(View),
+                       _P(View), _P(View), _D, _D, _D, _D, _D, _P(View), _P(View)],
     "tm5_ctx_get_tile": [_VP, _I, _P(_I)],
     "tm5_profile_begin": [_VP],
     "tm5_profile_end": [_VP],

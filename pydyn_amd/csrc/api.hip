@@ -447,6 +447,42 @@ extern "C" int tm5_decay(tm5_ctx *c, int im, int jm, int lm, const tm5_view *los
     return launch_decay(c, im, jm, lm, view3(loss), loss->stride[3], dt, tracers(q), t0, count);
 }
 
+extern "C" int tm5_loss_rates(tm5_ctx *c, int im, int jm, int lm, int nstrat, int ntrop, const tm5_view *temper,
+                              const tm5_view *phlb, const double *trop, const double *edges, const double *strat_oh,
+                              const double *strat_cl, const double *strat_od, const tm5_view *oh_3d,
+                              const tm5_view *pl_trop, const tm5_view *cl_trop, const tm5_view *soil,
+                              double oh_scale, double cl_scale, double o1d_scale, double soil_rate, double dt,
+                              const tm5_view *loss, const tm5_view *decay)
+{
+    if (!c || bad_view(temper) || bad_view(phlb) || bad_view(oh_3d) || bad_view(loss)) return TM5_EINVAL;
+    LossArgs a{};
+    a.im = im;
+    a.jm = jm;
+    a.lm = lm;
+    a.nstrat = nstrat;
+    a.ntrop = ntrop;
+    a.temper = view3(temper);
+    a.phlb = view3(phlb);
+    a.oh3d = view3(oh_3d);
+    a.trop = trop;
+    a.edges = edges;
+    a.s_oh = strat_oh;
+    a.s_cl = strat_cl;
+    a.s_od = strat_od;
+    if (pl_trop && pl_trop->data) a.pl = view3(pl_trop);
+    if (cl_trop && cl_trop->data) a.cl = view3(cl_trop);
+    if (soil && soil->data) a.soil = view3(soil);
+    a.oh_scale = oh_scale;
+    a.cl_scale = cl_scale;
+    a.o1d_scale = o1d_scale;
+    a.soil_rate = soil_rate;
+    a.dt = dt;
+    a.loss = view3(loss);
+    a.loss_sr = loss->stride[3];
+    if (decay && decay->data) a.decay = view3(decay);
+    return launch_loss_rates(c, a);
+}
+
 extern "C" int tm5_libm(tm5_ctx *c, int fn, int64_t n, const double *x, const double *y, double *out)
 {
     if (!c) return TM5_EINVAL;

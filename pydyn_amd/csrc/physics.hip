@@ -244,6 +244,100 @@ int launch_decay(tm5_ctx *ctx, int im, int jm, int lm, View3 loss, long long sr,
     return status(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ CH4 loss rates
+// pydyn/chem.py:129-170 (ch4_loss_kernel, then decay) for one column (i, j) per thread: the zonal
+// columns' rebin onto the model layers by the Fortran loop of rebin_levels(ip_mix) -- each layer's
+// overlaps from zero in order, a running j_start, the exit at the first input layer above it, the
+// mean where the layer has thickness; the layers below the first edge take the first value -- the
+// tropospheric Cl likewise on its own edges (pl_trop, 0), the Spivakovsky OH and Cl rates by the
+// fdlibm exp and pow, the choice at the tropopause, the soil sink at the surface, and the decay
+// factor exp(-(((L1 + L2) + L3) + L4) dt).  Layers that are not strictly descending raise
+// TM5_BAD_LEVELS (the reference's ValueError).
+__global__ void __launch_bounds__(256) k_loss_rates(LossArgs a, int *words)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.im * a.jm) return;
+    const int i = col % a.im, j = col / a.im, lm = a.lm, ns = a.nstrat, nt = a.ntrop;
+    const bool trop = a.pl.p != nullptr;
+    bool bad = false;
+    for (int l = 0; l < lm; l++) bad = bad || !(a.phlb(i, j, l + 1) < a.phlb(i, j, l));
+    if (trop) {
+        for (int k = 0; k + 1 < nt; k++) bad = bad || !(a.pl(i, j, k + 1) < a.pl(i, j, k));
+        bad = bad || !(0.0 < a.pl(i, j, nt - 1));
+    }
+    if (bad) {
+        atomicOr(words + W_FLAGS, TM5_BAD_LEVELS);
+        return;
+    }
+    const double *E = a.edges + (size_t)j * (ns + 1);
+    const double *SOH = a.s_oh + (size_t)j * ns, *SCL = a.s_cl + (size_t)j * ns, *SOD = a.s_od + (size_t)j * ns;
+    const double OH_A = 2.45e-12, OH_EA = 1775.0, CL_A = 2.36e-12, CL_T0 = 298.0, CL_N = 1.37, CL_EA = 939.0;
+    int js = 0, jt = 0;
+    for (int l = 0; l < lm; l++) {
+        const double op_bot = a.phlb(i, j, l), op_top = a.phlb(i, j, l + 1), width = op_bot - op_top;
+        while (js < ns && E[js + 1] >= op_bot) js++;
+        double s_oh = 0.0, s_cl = 0.0, s_od = 0.0;
+        for (int k = js; k < ns; k++) {
+            const double ip_bot = E[k], ip_top = E[k + 1];
+            if (ip_bot <= op_top) break;
+            const double thick = (op_bot < ip_bot ? op_bot : ip_bot) - (op_top > ip_top ? op_top : ip_top);
+            if (thick > 0.0) {
+                s_oh = s_oh + SOH[k] * thick;
+                s_cl = s_cl + SCL[k] * thick;
+                s_od = s_od + SOD[k] * thick;
+            }
+        }
+        if (width > 0.0) {
+            s_oh = s_oh / width;
+            s_cl = s_cl / width;
+            s_od = s_od / width;
+        }
+        if (op_bot > E[0]) {  // below the zonal columns' first edge
+            s_oh = SOH[0];
+            s_cl = SCL[0];
+            s_od = SOD[0];
+        }
+        const double T = a.temper(i, j, l);
+        double cl_col = 0.0;
+        if (trop) {
+            while (jt < nt && (jt + 1 < nt ? a.pl(i, j, jt + 1) : 0.0) >= op_bot) jt++;
+            double acc = 0.0;
+            for (int k = jt; k < nt; k++) {
+                const double ip_bot = a.pl(i, j, k), ip_top = k + 1 < nt ? a.pl(i, j, k + 1) : 0.0;
+                if (ip_bot <= op_top) break;
+                const double thick = (op_bot < ip_bot ? op_bot : ip_bot) - (op_top > ip_top ? op_top : ip_top);
+                if (thick > 0.0) acc = acc + a.cl(i, j, k) * thick;
+            }
+            if (width > 0.0) acc = acc / width;
+            if (op_bot > a.pl(i, j, 0)) acc = a.cl(i, j, 0);
+            const double krate = (CL_A * fdlibm_pow(T / CL_T0, CL_N)) * fdlibm_exp(-CL_EA / T);
+            cl_col = (acc * krate) * a.cl_scale;
+        }
+        const double oh_spiv = ((OH_A * fdlibm_exp(-OH_EA / T)) * a.oh3d(i, j, l)) * a.oh_scale;
+        const bool strato = op_bot < a.trop[j];
+        const double L0 = strato ? s_oh : oh_spiv;
+        const double L1 = strato ? s_cl : s_cl + cl_col;
+        const double L2 = s_od * a.o1d_scale;
+        const double L3 = (l == 0 && a.soil.p) ? -(a.soil_rate * a.soil(i, j, 0)) : 0.0;
+        double *L = &a.loss(i, j, l);
+        L[0] = L0;
+        L[a.loss_sr] = L1;
+        L[2 * a.loss_sr] = L2;
+        L[3 * a.loss_sr] = L3;
+        if (a.decay.p) a.decay(i, j, l) = fdlibm_exp(-((((L0 + L1) + L2) + L3) * a.dt));
+    }
+}
+
+int launch_loss_rates(tm5_ctx *ctx, const LossArgs &a)
+{
+    if (a.im < 1 || a.jm < 1 || a.lm < 1 || a.nstrat < 1 || !a.edges || !a.trop || !a.s_oh || !a.s_cl || !a.s_od ||
+        !a.temper.p || !a.phlb.p || !a.oh3d.p || !a.loss.p || (a.pl.p != nullptr) != (a.cl.p != nullptr) ||
+        (a.pl.p && a.ntrop < 1))
+        return TM5_EINVAL;
+    TM5_LAUNCH(ctx, k_loss_rates, blocks((long long)a.im * a.jm), dim3(256), 0, ctx->stream, a, ctx->words);
+    return status(hipGetLastError());
+}
+
 int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor)
 {
     if (n0 < 1 || n1 < 1 || n2 < 1) return TM5_EINVAL;

@@ -243,6 +243,19 @@ int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
 int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
 int launch_libm(tm5_ctx *ctx, int fn, long long n, const double *x, const double *y, double *out);
+// The CH4 loss rates of tm5_loss_rates (physics.hip)
+struct LossArgs {
+    int im, jm, lm, nstrat, ntrop;
+    View3 temper, phlb, oh3d;             // (im, jm, lm), phlb (im, jm, lm + 1)
+    const double *trop, *edges;           // (jm), (jm, nstrat + 1) contiguous
+    const double *s_oh, *s_cl, *s_od;     // (jm, nstrat) contiguous
+    View3 pl, cl, soil;                   // (im, jm, ntrop) twice, (im, jm, 1); p null: not applied
+    double oh_scale, cl_scale, o1d_scale, soil_rate, dt;
+    View3 loss;                           // (im, jm, lm) of reaction 0 ...
+    long long loss_sr;                    // ... and the reaction axis' stride
+    View3 decay;                          // (im, jm, lm); p null: not written
+};
+int launch_loss_rates(tm5_ctx *ctx, const LossArgs &a);
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);

@@ -81,6 +81,8 @@ class Context:
             raise CFLError("dynuamu: nloop too high")  # the reference's spelling (pydyn/zoom/advect.py:330)
         if f & _lib.FAIL_YC:
             raise CFLError("nloop too high in dynamv")
+        if f & _lib.BAD_LEVELS:
+            raise ValueError("rebin: levels not strictly descending (the Fortran keeps the stale column)")
         return f
 
     def nloop(self, lm):

@@ -7,6 +7,8 @@ advection operators check the device flags before returning unless `deferred=Tru
 ValueError with its messages.
 """
 
+import numpy as np
+
 from .. import _lib
 from .. import device as D
 from ..device import CFLError  # noqa: F401  (the reference's exception, re-exported)
@@ -24,6 +26,7 @@ __all__ = [
     "decay",
     "dkg",
     "fluxes_from_meteo",
+    "loss_rates",
     "m2phlb1",
     "mass_x_rows",
     "mass_y",
@@ -283,3 +286,47 @@ def scale(a, factor):
     ctx = D.context()
     n0, n1, n2 = a.shape
     _lib.check(_lib.lib().tm5_scale(ctx.ptr, n0, n1, n2, D.view(a), float(factor)), "tm5_scale")
+
+
+def loss_rates(dt_sec, temper, phlb, lat, strat_press, strat_oh, strat_cl, strat_od, oh_3d, pl_trop=None,
+               cl_trop=None, soil=None, oh_scale=1.0, cl_scale=1.0, o1d_scale=1.0, soil_rate=0.0):  # fmt: skip
+    """chem.loss_rates and chem.decay on the device (pydyn/gpu/kernels.py:545-645): (loss
+    (im, jm, lm, 4), decay (im, jm, lm)).  Host arrays are uploaded, device ones used as they are;
+    lat and strat_press stay host arrays (the tropopause and the zonal columns' edges are derived
+    from them on the host, as the reference does)."""
+    from .. import chem
+
+    torch = D._torch()
+
+    def dev(x):
+        if x is None or isinstance(x, torch.Tensor):
+            return x
+        return D.upload(np.asarray(x, dtype=np.float64))
+
+    def flat(x):  # a contiguous (jm, n) device copy of a zonal column set
+        x = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, dtype=np.float64))
+        return x.to(device="cuda", dtype=torch.float64).contiguous()
+
+    edges = chem.strat_edges(np.asarray(strat_press, dtype=np.float64))
+    if not (np.diff(edges, axis=-1) < 0.0).all():
+        raise ValueError("rebin: input levels not strictly descending (the Fortran keeps the stale column)")
+    t, p, oh = dev(temper), dev(phlb), dev(oh_3d)
+    pl, cl, sl = dev(pl_trop), dev(cl_trop), dev(soil)
+    if (pl is None) != (cl is None):
+        raise ValueError("loss_rates: pl_trop and cl_trop go together")
+    im, jm, lm = t.shape
+    nstrat = edges.shape[-1] - 1
+    ntrop = cl.shape[2] if cl is not None else 0
+    trop = flat(chem.tropopause(np.asarray(lat, dtype=np.float64)))
+    e, s_oh, s_cl, s_od = flat(edges), flat(strat_oh), flat(strat_cl), flat(strat_od)
+    loss, dec = D.zeros((im, jm, lm, 4)), D.empty((im, jm, lm))
+    none = None
+    ctx = D.context()
+    rc = _lib.lib().tm5_loss_rates(
+        ctx.ptr, im, jm, lm, nstrat, ntrop, D.view(t), D.view(p), trop.data_ptr(), e.data_ptr(), s_oh.data_ptr(),
+        s_cl.data_ptr(), s_od.data_ptr(), D.view(oh), D.view(pl) if pl is not None else none,
+        D.view(cl) if cl is not None else none, D.view(sl) if sl is not None else none, float(oh_scale),
+        float(cl_scale), float(o1d_scale), float(soil_rate), float(dt_sec), D.view(loss), D.view(dec))
+    _lib.check(rc, "tm5_loss_rates")
+    ctx.raise_errors()
+    return loss, dec

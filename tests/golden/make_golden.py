@@ -7,6 +7,7 @@ Run in the build container only (the reference is not on the GPU machines):
     python tests/golden/make_golden.py run glb6x4 24
     python tests/golden/make_golden.py run glb3x2 720 # long: written interval by interval
     python tests/golden/make_golden.py zoom tiny2 48  # a global region with one child (zoom tree)
+    python tests/golden/make_golden.py chem           # CH4 loss rates and decay (npz)
 
 It imports the reference package from /root/reference (read-only) and routes the reference's
 libm through the oracle's fdlibm library with the reference's own `ops.use_libm`, so that its
@@ -351,6 +352,43 @@ def zoom_golden(name, n_intervals):
                             **{f"r{r}.{k}": st[r][k].data for r in (1, 2) for k in ZFIELDS})  # fmt: skip
 
 
+def chem_cases():
+    """pydyn.chem.loss_rates and decay on seeded columns: three cases -- the zonal columns alone,
+    with the tropospheric Cl and the soil sink, and with the model's floor above the zonal
+    columns' first edge (no fill from below).  Inputs and outputs, prefixed by the case."""
+    out = {}
+    im, jm, lm, nstrat, ntrop = 6, 5, 12, 8, 5
+    for case, seed in (("plain", 1), ("trop", 2), ("high", 3)):
+        rng = np.random.default_rng(40 + seed)
+        sp = rng.uniform(0.97e5, 1.03e5, (im, jm))
+        frac = np.concatenate([[1.0], np.sort(rng.uniform(0.0, 1.0, lm - 1))[::-1], [0.0]])
+        if case == "high":
+            frac = frac * 0.5  # the floor at half the surface pressure, above the first edge
+        phlb = sp[..., None] * frac[None, None, :]
+        inp = {
+            "temper": rng.uniform(210.0, 300.0, (im, jm, lm)),
+            "phlb": phlb,
+            "lat": np.linspace(-1.4, 1.4, jm),
+            "strat_press": np.sort(rng.uniform(1.0, 700.0, (jm, nstrat)), axis=1)[:, ::-1].copy(),
+            "strat_oh": rng.uniform(1e-9, 1e-8, (jm, nstrat)),
+            "strat_cl": rng.uniform(1e-10, 1e-9, (jm, nstrat)),
+            "strat_od": rng.uniform(1e-11, 1e-10, (jm, nstrat)),
+            "oh_3d": rng.uniform(1e5, 1e6, (im, jm, lm)),
+        }
+        kw = {"oh_scale": 1.1, "cl_scale": 0.9, "o1d_scale": 1.2}
+        if case == "trop":
+            inp["pl_trop"] = np.sort(rng.uniform(2e4, 1.05e5, (im, jm, ntrop)), axis=2)[..., ::-1].copy()
+            inp["cl_trop"] = rng.uniform(1e3, 1e4, (im, jm, ntrop))
+            inp["soil"] = rng.uniform(0.0, 1.0, (im, jm))
+            kw["soil_rate"] = 2e-4
+        loss = chem.loss_rates(**inp, **kw)
+        for k, v in {**inp, **{k: np.array(v) for k, v in kw.items()}}.items():
+            out[f"{case}.{k}"] = np.asarray(v)
+        out[f"{case}.loss"] = loss
+        out[f"{case}.decay"] = chem.decay(loss, 1800.0)
+    return out
+
+
 def main(argv):
     _use_twin()
     if argv[0] == "ops":
@@ -359,6 +397,8 @@ def main(argv):
         run_golden(argv[1], int(argv[2]))
     elif argv[0] == "zoom":
         zoom_golden(argv[1], int(argv[2]))
+    elif argv[0] == "chem":
+        np.savez_compressed(GOLDEN / "chem_loss.npz", **chem_cases())
     else:
         raise SystemExit(__doc__)
 

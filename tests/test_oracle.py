@@ -437,3 +437,45 @@ def test_operator_adjoints_with_sub_cycling(axis):
         for k in range(4):
             fd = (out[0][k] - out[1][k]) / (2.0 * eps)
             assert np.abs(fd - tl[k]).max() <= 1e-6 * np.abs(tl[k]).max(), (axis, k)
+
+
+# ------------------------------------------------------------------ CH4 loss rates
+def _chem_case(case):
+    g = np.load(GOLDEN / "chem_loss.npz")
+    pre = f"{case}."
+    return {k[len(pre):]: g[k] for k in g.files if k.startswith(pre)}
+
+
+def _loss_inputs(c):
+    keys = ("temper", "phlb", "lat", "strat_press", "strat_oh", "strat_cl", "strat_od", "oh_3d", "pl_trop",
+            "cl_trop", "soil")  # fmt: skip
+    kw = {k: c[k] for k in keys if k in c}
+    for k in ("oh_scale", "cl_scale", "o1d_scale", "soil_rate"):
+        if k in c:
+            kw[k] = float(c[k])
+    return kw
+
+
+@pytest.mark.parametrize("case", ["plain", "trop", "high"])
+def test_loss_rates_and_decay_match_the_reference(case):
+    from oracle import chem
+
+    c = _chem_case(case)
+    loss = chem.loss_rates(**_loss_inputs(c))
+    assert np.array_equal(loss.view(np.int64), c["loss"].view(np.int64))
+    assert np.array_equal(chem.decay(loss, 1800.0).view(np.int64), c["decay"].view(np.int64))
+
+
+def test_loss_rate_known_answers():
+    """The reference tests' known answers (tests/test_physics.py:97-101): edges of the zonal
+    columns, the tropopause at the equator and the pole; and a rebin whose means are exact."""
+    from oracle import chem
+
+    assert np.array_equal(chem.strat_edges(np.array([[900.0, 700.0, 300.0]])), [[100000.0, 80000.0, 50000.0, 0.0]])
+    assert chem.tropopause(np.array([0.0]))[0] == 30000.0 - 21500.0
+    assert abs(chem.tropopause(np.array([np.pi / 2]))[0] - 30000.0) < 1e-9
+    # layers [1000, 600] and [600, 0] of values 2 and 5 onto [1000, 800, 400, 0]: means 2, (2*200 + 5*200)/400, 5
+    got = chem.rebin(np.array([1000.0, 600.0, 0.0]), np.array([2.0, 5.0]), np.array([1000.0, 800.0, 400.0, 0.0]))
+    assert np.array_equal(got, [2.0, 3.5, 5.0])
+    with pytest.raises(ValueError):
+        chem.rebin(np.array([0.0, 600.0, 1000.0]), np.array([2.0, 5.0]), np.array([1000.0, 0.0]))
