@@ -199,6 +199,19 @@ int tm5_loss_rates(tm5_ctx *ctx, int im, int jm, int lm, int nstrat, int ntrop, 
                    const double *strat_cl, const double *strat_od, const tm5_view *oh_3d, const tm5_view *pl_trop,
                    const tm5_view *cl_trop, const tm5_view *soil, double oh_scale, double cl_scale,
                    double o1d_scale, double soil_rate, double dt, const tm5_view *loss, const tm5_view *decay);
+/* Calc_Kzz of one dynamic step (pydyn/diffusion.py:266-276 calc_kzz; the reference's device
+   pydyn.gpu.diffusion.calc_kzz, gpu/diffusion.py:38 with csrc/kzz.cuh): dkg (im, jm, lmax) and the
+   interface heights gph (im, jm, lm + 1); ustar (im, jm), pblh (im, jm) and kvh (im, jm, lm - 1)
+   when not null.  spm, sp (im, jm) the surface pressure at the middle and the start of the step;
+   pu (im + 1, jm, lm), pv (im, jm + 1, lm), t, q (im, jm, lm); surface[10] the (im, jm) fields oro,
+   lsm, sr, srols, u10m, v10m, sshf, slhf, ewss, nsss; at, bt (lm + 1), dxyp (jm) device arrays;
+   dxx (jm) the cells' zonal widths AE dx GTOR cos(lat GTOR) and dyy = AE dy GTOR, computed by the
+   caller on the host (the reference evaluates that cos on the host, gpu/diffusion.py:55-57). */
+int tm5_calc_kzz(tm5_ctx *ctx, int im, int jm, int lm, int lmax, const tm5_view *spm, const tm5_view *sp,
+                 const tm5_view *pu, const tm5_view *pv, const tm5_view *t, const tm5_view *q,
+                 const tm5_view *surface, const double *at, const double *bt, const double *dxyp, const double *dxx,
+                 double dyy, const tm5_view *gph, const tm5_view *dkg, const tm5_view *ustar, const tm5_view *pblh,
+                 const tm5_view *kvh);
 #define TM5_LIBM_EXP 0
 #define TM5_LIBM_LOG 1
 #define TM5_LIBM_POW 2

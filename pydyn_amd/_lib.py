@@ -108,6 +108,8 @@ _SIGNATURES = {
     "tm5_ctx_clear": [_VP, _I],
     "tm5_ctx_set_tile": [_VP, _I, _I],
     "tm5_libm": [_VP, _I, ctypes.c_int64, _VP, _VP, _VP],
+    "tm5_calc_kzz": [_VP, _I, _I, _I, _I, _P(View), _P(View), _P(View), _P(View), _P(View), _P(View), _P(View), _VP,
+                     _VP, _VP, _VP, _D, _P(View), _P(View), _P(View), _P(View), _P(View)],
     "tm5_loss_rates": [_VP, _I, _I, _I, _I, _I, _P(View), _P(View), _VP, _VP, _VP, _VP, _VP, _P(View), _P(View),
                        _P(View), _P(View), _D, _D, _D, _D, _D, _P(View), _P(View)],
     "tm5_ctx_get_tile": [_VP, _I, _P(_I)],

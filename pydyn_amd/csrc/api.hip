@@ -483,6 +483,43 @@ extern "C" int tm5_loss_rates(tm5_ctx *c, int im, int jm, int lm, int nstrat, in
     return launch_loss_rates(c, a);
 }
 
+extern "C" int tm5_calc_kzz(tm5_ctx *c, int im, int jm, int lm, int lmax, const tm5_view *spm, const tm5_view *sp,
+                            const tm5_view *pu, const tm5_view *pv, const tm5_view *t, const tm5_view *q,
+                            const tm5_view *surface, const double *at, const double *bt, const double *dxyp,
+                            const double *dxx, double dyy, const tm5_view *gph, const tm5_view *dkg,
+                            const tm5_view *ustar, const tm5_view *pblh, const tm5_view *kvh)
+{
+    if (!c || bad_view(spm) || bad_view(sp) || bad_view(pu) || bad_view(pv) || bad_view(t) || bad_view(q) ||
+        !surface || bad_view(gph) || bad_view(dkg))
+        return TM5_EINVAL;
+    KzzArgs a{};
+    a.im = im;
+    a.jm = jm;
+    a.lm = lm;
+    a.lmax = lmax;
+    a.spm = view3(spm);
+    a.sp = view3(sp);
+    a.pu = view3(pu);
+    a.pv = view3(pv);
+    a.t = view3(t);
+    a.q = view3(q);
+    for (int k = 0; k < 10; k++) {
+        if (bad_view(surface + k)) return TM5_EINVAL;
+        a.surf[k] = view3(surface + k);
+    }
+    a.at = at;
+    a.bt = bt;
+    a.dxyp = dxyp;
+    a.dxx = dxx;
+    a.dyy = dyy;
+    a.gph = view3(gph);
+    a.dkg = view3(dkg);
+    if (ustar && ustar->data) a.ustar = view3(ustar);
+    if (pblh && pblh->data) a.pblh = view3(pblh);
+    if (kvh && kvh->data) a.kvh = view3(kvh);
+    return launch_calc_kzz(c, a);
+}
+
 extern "C" int tm5_libm(tm5_ctx *c, int fn, int64_t n, const double *x, const double *y, double *out)
 {
     if (!c) return TM5_EINVAL;

@@ -338,6 +338,184 @@ int launch_loss_rates(tm5_ctx *ctx, const LossArgs &a)
     return status(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ Calc_Kzz
+// pydyn/diffusion.py:71-276 for one column (i, j) per thread, lanes along i: the friction velocity,
+// the interface heights (written to gph and read back), then three sweeps up the column that
+// recompute each level's quantities from the inputs (the same operations, so the same bits): the
+// first bulk-Richardson sweep, the second with the convective excess, and the interfaces' free
+// and boundary-layer diffusivities with dkg.  Every `if` of the Fortran is a 0/1 mask (jqif)
+// multiplying both branches, max / min propagate NaN as NumPy's, log / pow / exp are fdlibm.
+__device__ __forceinline__ double np_max(double a, double b) { return (a >= b || a != a) ? a : b; }
+__device__ __forceinline__ double np_min(double a, double b) { return (a <= b || a != a) ? a : b; }
+__device__ __forceinline__ double jqif(double x, double y) { return __double_as_longlong(x - y) < 0 ? 0.0 : 1.0; }
+
+__global__ void __launch_bounds__(256) k_calc_kzz(KzzArgs a)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.im * a.jm) return;
+    const int i = col % a.im, j = col / a.im, lm = a.lm;
+    const double GRAV = 9.80665, VKARMAN = 0.4, CP_AIR = 1004.0, LVAP = 2.5e6, RHO_AIR = 1.2, CKMIN = 1.0e-15;
+    const double SFFRAC = 0.1, ONET = 1.0 / 3.0, APZERO = 100000.0, BETAH = 15.0, FAK = 8.5, FAKN = 7.2;
+    const double RAIR = 8.3144 * 1.0e3 / 28.94;
+    const double EPSILO = RAIR / 461.51, ZKAPPA = RAIR / CP_AIR, ZCRDQ = 1.0 / EPSILO - 1.0;
+    const double CCON = SFFRAC * VKARMAN, BINM = 15.0 * SFFRAC;
+    const double TINY = 1.0e-9, ZACB = 100.0, RICR = 0.3, SSFRAC = 0.1, KVHMIN = 0.1;
+    auto S = [&](int k) { return a.surf[k](i, j, 0); };  // oro lsm sr srols u10m v10m sshf slhf ewss nsss
+    const double spm = a.spm(i, j, 0), sp = a.sp(i, j, 0), dxyp = a.dxyp[j], dxx = a.dxx[j], dyy = a.dyy;
+
+    // ustar (pydyn/diffusion.py:76-91)
+    double ust;
+    {
+        const double xland = S(1) / 100.0;
+        const double tau = sqrt(S(8) * S(8) + S(9) * S(9));
+        const double ustar_sea = np_max(0.001, sqrt(tau / RHO_AIR));
+        const double wind10m = sqrt(S(4) * S(4) + S(5) * S(5));
+        const double sr_land = np_max(1e-2, S(3)), sr_help = np_min(S(2), 0.03);
+        double land = ((VKARMAN * wind10m) / fdlibm_log(10.0 / sr_help)) * fdlibm_log(75.0 / sr_help);
+        land = land / fdlibm_log(10.0 / sr_land);
+        const double ustar_land = xland > 0.0 ? land : 0.0;
+        ust = xland * ustar_land + (1.0 - xland) * ustar_sea;
+    }
+    if (a.ustar.p) a.ustar(i, j, 0) = ust;
+    // the interface heights (pydyn/diffusion.py:94-106)
+    {
+        double g = S(0) / GRAV;
+        a.gph(i, j, 0) = g;
+        for (int k = 0; k + 1 < lm; k++) {
+            const double tv = a.t(i, j, k) * (1.0 + 0.608 * a.q(i, j, k));
+            const double ratio = (a.at[k] + a.bt[k] * sp) / (a.at[k + 1] + a.bt[k + 1] * sp);
+            g = g + ((tv * 287.05) * fdlibm_log(ratio)) / GRAV;
+            a.gph(i, j, k + 1) = g;
+        }
+        a.gph(i, j, lm) = 200000.0;
+    }
+    const double G0 = a.gph(i, j, 0);
+    auto PH = [&](int l) { return a.at[l] + a.bt[l] * spm; };
+    auto M = [&](int l) { return ((PH(l) - PH(l + 1)) * dxyp) / GRAV; };
+    auto PF = [&](int l) { return (PH(l) + PH(l + 1)) * 0.5; };
+    auto WTHM = [&](int l) { return a.t(i, j, l) * fdlibm_pow(APZERO / PF(l), ZKAPPA); };
+    auto ZTHVK = [&](int l) { return WTHM(l) * (1.0 + ZCRDQ * a.q(i, j, l)); };
+    auto WHM = [&](int l) {
+        const double lo = PH(l), pf = PF(l), intfac = (lo - pf) / (lo - PH(l + 1));
+        return (a.gph(i, j, l) - G0) * (1.0 - intfac) + (a.gph(i, j, l + 1) - G0) * intfac;
+    };
+    auto UW = [&](int l) { return ((dxx * (a.pu(i + 1, j, l) + a.pu(i, j, l))) * 0.5) / M(l); };
+    auto VW = [&](int l) { return ((dyy * (a.pv(i, j + 1, l) + a.pv(i, j, l))) * 0.5) / M(l); };
+
+    // the surface (pydyn/diffusion.py:160-166, 203-206)
+    const double dz1 = a.gph(i, j, 1) - a.gph(i, j, 0), dp1 = PH(0) - PH(1);
+    const double wheat = ((((-S(6)) * dz1) * GRAV) / dp1) / CP_AIR;
+    const double wqflx = ((((-S(7)) * dz1) * GRAV) / dp1) / LVAP;
+    const double ustr = np_max(ust, 0.01);
+    const double wthm0 = WTHM(0), uw0 = UW(0), vw0 = VW(0), whm0 = WHM(0);
+    const double wtseff = wthm0 * (1.0 + ZCRDQ * a.q(i, j, 0));
+    const double wheatv = wheat + (ZCRDQ * wthm0) * wqflx;
+    const double cube = (ustr * ustr) * ustr;
+    const double wobkl = ((-wtseff) * cube) / ((GRAV * VKARMAN) * (wheatv + copysign(1.0e-10, wheatv)));
+    // the two bulk-Richardson sweeps (pydyn/diffusion.py:127-155)
+    auto sweep = [&](double pblh, bool first, double reference) {
+        double jwork = 1.0, wrino = 0.0;
+        for (int k = first ? 0 : 1; k < lm; k++) {
+            const double whm_k = k == 0 ? whm0 : WHM(k);
+            const double du = UW(k) - uw0, dv = VW(k) - vw0;
+            const double vvk = np_max((du * du + dv * dv) + (ZACB * ustr) * ustr, TINY);
+            const double tkv = WTHM(k) * (1.0 + ZCRDQ * a.q(i, j, k));
+            const double dz = whm_k - whm0;
+            double zrino = first ? ((GRAV * (tkv - wtseff)) * dz) / (wtseff * vvk)
+                                 : ((GRAV * (tkv - reference)) * dz) / (vvk * wtseff);
+            zrino = jwork * (zrino + copysign(1.0e-10, zrino));
+            const double jq = jqif(RICR, zrino);
+            const double frac = (RICR - wrino) / (zrino - wrino);
+            const double below = k == 0 ? 0.0 : WHM(k - 1);
+            const double at_ricr = k == 0 ? frac * whm0 : below + frac * (whm_k - below);
+            pblh = jq * pblh + (1.0 - jq) * at_ricr;
+            jwork = jq * jwork;
+            wrino = zrino + (1.0 - jwork) * 0.1;
+        }
+        const double top = WHM(lm - 1);
+        return (1.0 - jwork) * pblh + (first ? top * jwork : jwork * top);
+    };
+    double pblh = sweep(0.0, true, 0.0);
+    {
+        const double jq = jqif(wheatv, 0.0);
+        const double fmt = fdlibm_pow(jq * (1.0 - (BINM * pblh) / wobkl) + (1.0 - jq), ONET);
+        const double zexcess = (wheatv * FAK) / (ustr * fmt);
+        const double thvparc = wtseff + jq * zexcess;
+        pblh = np_max(sweep(pblh, false, thvparc), 100.0);
+    }
+    if (a.pblh.p) a.pblh(i, j, 0) = pblh;
+    // the interfaces l + 1 (pydyn/diffusion.py:178-253) and dkg (:256-263)
+    const double P = pblh, W = wobkl, HV = wheatv, U = ustr, TS = wtseff;
+    const int n = a.lmax - 1;
+    for (int l = 0; l + 1 < lm; l++) {
+        const double z = a.gph(i, j, l + 1) - G0;
+        double jq = jqif(1000.0, z);
+        const double zlambdac = jq * 300.0 + (1.0 - jq) * (30.0 + 270.0 * fdlibm_exp(1.0 - z / 1000.0));
+        double cml2 = 1.0 / (1.0 / zlambdac + 1.0 / (VKARMAN * z));
+        cml2 = cml2 * cml2;
+        const double du = UW(l + 1) - UW(l), dv = VW(l + 1) - VW(l);
+        const double whm_l = WHM(l), whm_u = WHM(l + 1);
+        const double zdz = whm_u - whm_l;
+        const double zdup2 = np_max(du * du + dv * dv, 1.0e-10) / (zdz * zdz);
+        const double log_pf0 = fdlibm_log(PF(l)), log_pf1 = fdlibm_log(PF(l + 1));
+        const double arg = (fdlibm_log(PH(l + 1)) - log_pf0) / (log_pf1 - log_pf0);
+        const double th0 = ZTHVK(l), dthv = ZTHVK(l + 1) - th0;
+        const double zthva = th0 + arg * dthv;
+        const double zrinub = (((GRAV / zthva) * dthv) / zdz) / zdup2;
+        const double zfunst = sqrt(np_max(1.0 - 18.0 * zrinub, 0.0));
+        double zfstabh = 1.0 / (1.0 + (10.0 * zrinub) * (1.0 + 8.0 * zrinub));
+        const double zkvn = cml2 * sqrt(zdup2);
+        jq = jqif(zrinub, 0.0);
+        const double wkvf = jq * np_max(CKMIN, zkvn * zfstabh) + (1.0 - jq) * np_max(CKMIN, zkvn * zfunst);
+
+        jq = jqif(P, z);
+        const double zh = (jq * z) / P, zl = (jq * z) / W + (1.0 - jq);
+        const double zzh = jq * ((1.0 - zh) * (1.0 - zh));
+        double jq1 = jqif(HV, 0.0);
+        const double jq2 = jq * (1.0 - jq1), jq3 = jq * jq1;
+        const double jq6 = (jq1 * jqif(P, whm_l)) * jqif(whm_u, P);
+        jq1 = jqif(SFFRAC, zh);
+        const double jq4 = jq3 * jq1, jq5 = jq3 * (1.0 - jq1);
+        cml2 = 1.0 / (1.0 / (VKARMAN * z) + 1.0 / 450.0);
+        cml2 = cml2 * cml2;
+        const double jqq = jqif(zrinub, 0.0);
+        zfstabh = jqq / (1.0 + (10.0 * zrinub) * sqrt(1.0 + jqq * zrinub)) + (1.0 - jqq);
+        double zpblk = (cml2 * sqrt(zdup2)) * zfstabh;
+        const double zkvh = jq2 * np_max(zpblk, KVHMIN) + (1.0 - jq2) * wkvf;
+        double zslask = jq4 * (1.0 - BETAH * zl) + (1.0 - jq4);
+        zpblk = (1.0 - jq4) * zkvh + jq4 * ((((U * VKARMAN) * z) * zzh) * fdlibm_pow(zslask, ONET));
+        zslask = jq5 * (1.0 - ((SSFRAC * BETAH) * P) / W) + (1.0 - jq5);
+        const double wsc = U * fdlibm_pow(zslask, ONET);
+        zpblk = (1.0 - jq5) * zpblk + (((jq5 * wsc) * VKARMAN) * z) * zzh;
+        const double zwstr = fdlibm_pow(((fabs(HV) * GRAV) * P) / TS, ONET);
+        const double obukov = jq3 * W - (1.0 - jq3);
+        const double stab = 1.0 - ((SSFRAC * BETAH) * P) / obukov;
+        const double term1 = fdlibm_pow(stab, ONET);
+        const double term3 = ((CCON * FAKN) * zwstr) / (U * term1);
+        const double pr = jq3 * (term1 / sqrt(stab) + term3) + (1.0 - jq3);
+        double kvh = jq3 * np_max(zpblk / pr, KVHMIN) + (1.0 - jq3) * zkvh;
+        const double kvhentr = (0.2 * HV) / (dthv / zdz);
+        kvh = jq6 * kvhentr + (1.0 - jq6) * kvh;
+        if (a.kvh.p) a.kvh(i, j, l) = kvh;
+        if (l < n) {
+            const double dz = a.gph(i, j, l + 2) - a.gph(i, j, l);
+            a.dkg(i, j, l) = ((np_max(0.0, kvh) * 2.0) * (M(l + 1) + M(l))) / (dz * dz);
+        }
+    }
+    a.dkg(i, j, n) = 0.0;
+}
+
+int launch_calc_kzz(tm5_ctx *ctx, const KzzArgs &a)
+{
+    if (a.im < 1 || a.jm < 1 || a.lm < 2 || a.lmax < 1 || a.lmax > a.lm || !a.at || !a.bt || !a.dxyp || !a.dxx ||
+        !a.gph.p || !a.dkg.p)
+        return TM5_EINVAL;
+    for (int k = 0; k < 10; k++)
+        if (!a.surf[k].p) return TM5_EINVAL;
+    TM5_LAUNCH(ctx, k_calc_kzz, blocks((long long)a.im * a.jm), dim3(256), 0, ctx->stream, a);
+    return status(hipGetLastError());
+}
+
 int launch_scale(tm5_ctx *ctx, int n0, int n1, int n2, View3 a, double factor)
 {
     if (n0 < 1 || n1 < 1 || n2 < 1) return TM5_EINVAL;

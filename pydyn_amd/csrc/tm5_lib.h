@@ -256,6 +256,19 @@ struct LossArgs {
     View3 decay;                          // (im, jm, lm); p null: not written
 };
 int launch_loss_rates(tm5_ctx *ctx, const LossArgs &a);
+// Calc_Kzz of tm5_calc_kzz (physics.hip)
+struct KzzArgs {
+    int im, jm, lm, lmax;
+    View3 spm, sp;                // (im, jm)
+    View3 pu, pv, t, q;           // (im + 1, jm, lm), (im, jm + 1, lm), (im, jm, lm) twice
+    View3 surf[10];               // oro lsm sr srols u10m v10m sshf slhf ewss nsss, (im, jm)
+    const double *at, *bt;        // lm + 1
+    const double *dxyp, *dxx;     // jm: the cells' areas and zonal widths
+    double dyy;                   // the cells' meridional width
+    View3 gph, dkg;               // (im, jm, lm + 1), (im, jm, lmax)
+    View3 ustar, pblh, kvh;       // (im, jm), (im, jm), (im, jm, lm - 1); p null: not written
+};
+int launch_calc_kzz(tm5_ctx *ctx, const KzzArgs &a);
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);

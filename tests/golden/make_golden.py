@@ -8,6 +8,7 @@ Run in the build container only (the reference is not on the GPU machines):
     python tests/golden/make_golden.py run glb3x2 720 # long: written interval by interval
     python tests/golden/make_golden.py zoom tiny2 48  # a global region with one child (zoom tree)
     python tests/golden/make_golden.py chem           # CH4 loss rates and decay (npz)
+    python tests/golden/make_golden.py kzz            # Calc_Kzz: dkg and its intermediates (npz)
 
 It imports the reference package from /root/reference (read-only) and routes the reference's
 libm through the oracle's fdlibm library with the reference's own `ops.use_libm`, so that its
@@ -389,6 +390,22 @@ def chem_cases():
     return out
 
 
+def kzz_cases():
+    """pydyn.diffusion.calc_kzz on the reference's own synthetic inputs (pydyn.synth.kzz_inputs)
+    at two shapes: dkg and the intermediates (ustar, gph, pblh, kvh), with the inputs."""
+    out = {}
+    for case, (seed, im, jm, lm, lmax) in {"a": (61, 8, 6, 8, 6), "b": (62, 10, 9, 20, 14)}.items():
+        args, grid = synth.kzz_inputs(np.random.default_rng(seed), im, jm, lm=lm, lmax_conv=lmax)
+        dkg, parts = diffusion.calc_kzz(*args, **grid)
+        spm, sp, pu, pv, t, q, surface, at, bt, dxyp, lmax_conv = args
+        inp = {"spm": spm, "sp": sp, "pu": pu, "pv": pv, "t": t, "q": q, "at": at, "bt": bt, "dxyp": dxyp,
+               "lmax_conv": np.array(lmax_conv), **{f"surface_{k}": v for k, v in surface.items()},
+               **{k: np.array(v) for k, v in grid.items()}}  # fmt: skip
+        for k, v in {**inp, "dkg": dkg, **parts}.items():
+            out[f"{case}.{k}"] = np.asarray(v)
+    return out
+
+
 def main(argv):
     _use_twin()
     if argv[0] == "ops":
@@ -399,6 +416,8 @@ def main(argv):
         zoom_golden(argv[1], int(argv[2]))
     elif argv[0] == "chem":
         np.savez_compressed(GOLDEN / "chem_loss.npz", **chem_cases())
+    elif argv[0] == "kzz":
+        np.savez_compressed(GOLDEN / "kzz.npz", **kzz_cases())
     else:
         raise SystemExit(__doc__)
 

@@ -479,3 +479,55 @@ def test_loss_rate_known_answers():
     assert np.array_equal(got, [2.0, 3.5, 5.0])
     with pytest.raises(ValueError):
         chem.rebin(np.array([0.0, 600.0, 1000.0]), np.array([2.0, 5.0]), np.array([1000.0, 0.0]))
+
+
+# ------------------------------------------------------------------ Calc_Kzz
+def kzz_case(case):
+    g = np.load(GOLDEN / "kzz.npz")
+    c = {k[len(case) + 1:]: g[k] for k in g.files if k.startswith(case + ".")}
+    surface = {k[len("surface_"):]: c[k] for k in c if k.startswith("surface_")}
+    args = (c["spm"], c["sp"], c["pu"], c["pv"], c["t"], c["q"], surface, c["at"], c["bt"], c["dxyp"],
+            int(c["lmax_conv"]))  # fmt: skip
+    grid = {"dx": float(c["dx"]), "dy": float(c["dy"]), "ybeg": float(c["ybeg"])}
+    return args, grid, c
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_calc_kzz_matches_the_reference(case):
+    from oracle import diffusion as OD
+
+    args, grid, c = kzz_case(case)
+    dkg, parts = OD.calc_kzz(*args, **grid)
+    for k, v in {"dkg": dkg, **parts}.items():
+        assert np.array_equal(v.view(np.int64), c[k].view(np.int64)), k
+
+
+def test_calc_kzz_known_answers():
+    """The reference tests' known answers (tests/test_diffusion.py:20-69): jqif on the sign bit,
+    ustar blending land and sea by the land fraction, hydrostatic interface heights, dkg closed
+    at lmax_conv and non-negative."""
+    from oracle import diffusion as OD
+
+    x = np.array([1.0, 0.0, -0.0, -1.0, 1e-300])
+    assert OD.jqif(x, 0.0).tolist() == [1.0, 1.0, 0.0, 0.0, 1.0]
+    assert OD.jqif(0.0, x).tolist() == [0.0, 1.0, 1.0, 1.0, 0.0]
+    rng = np.random.default_rng(16)
+    lsm = np.array([[0.0, 100.0, 40.0]])
+    f = [rng.uniform(0.5, 1.0, (1, 3)) for _ in range(4)]
+    fields = (0.01 * f[0], 0.5 * f[1], 3.0 * f[2], -2.0 * f[3], 0.1 * f[0], 0.2 * f[1])
+    u, sea, land = OD.ustar(lsm, *fields), OD.ustar(np.zeros((1, 3)), *fields), OD.ustar(np.full((1, 3), 100.0), *fields)
+    assert u[0, 0] == sea[0, 0] and u[0, 1] == land[0, 1]
+    assert u[0, 2] == 0.4 * land[0, 2] + (1.0 - 0.4) * sea[0, 2]
+    sp = rng.uniform(9.0e4, 1.05e5, (4, 3))
+    a, b = 1.0e4 * np.sin(np.linspace(0.0, np.pi, 9)), np.linspace(1.0, 0.0, 9)
+    t, q = np.full((4, 3, 8), 250.0), np.zeros((4, 3, 8))
+    oro = rng.uniform(0.0, 1.0e4, (4, 3))
+    gph = OD.geopotential(oro, sp, t, q, a, b)
+    phlb = a + b * sp[..., None]
+    steps = 250.0 * 287.05 * np.log(phlb[..., :-2] / phlb[..., 1:-1]) / OD.GRAV
+    np.testing.assert_allclose(np.diff(gph[..., :-1], axis=-1), steps, rtol=1e-13)
+    assert np.array_equal(gph[..., 0], oro / OD.GRAV) and (gph[..., -1] == 200000.0).all()
+    args, grid, c = kzz_case("b")
+    dkg, _ = OD.calc_kzz(*args, **grid)
+    lmax = dkg.shape[-1]
+    assert np.isfinite(dkg).all() and (dkg >= 0.0).all() and (dkg[..., lmax - 1] == 0.0).all()
