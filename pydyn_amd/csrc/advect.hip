@@ -52,6 +52,19 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         if (bk) bk[brow + i] = v;
     }
     for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
+    // the first tracer's moments of the lane's (at most two) cells, in flight during the search
+    constexpr int PRE = 2;
+    const bool pre = !MASS_ONLY && q.nt > 0 && n <= 64 * PRE;
+    double qp[PRE][4];
+#pragma unroll
+    for (int c = 0; c < PRE; c++) {
+        const int i = lane + 64 * c;
+        if (pre && i < n) {
+            const long long o = q.off(i, j, l, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++) qp[c][a] = q.q[a][o];
+        }
+    }
     __syncthreads();
 
     int nl = 1;
@@ -116,7 +129,19 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
             }
             for (int i = lane; i < n; i += 64) {
                 const long long o = q.off(i, j, l, t);
-                const double rm = q.q[0][o], rx = q.q[1][o], ry = q.q[2][o], rz = q.q[3][o];
+                double rm, rx, ry, rz;
+                if (pre && t == 0) {  // i = lane + 64 c with c < PRE
+                    const int c = (i - lane) / 64;
+                    rm = c == 0 ? qp[0][0] : qp[1][0];
+                    rx = c == 0 ? qp[0][1] : qp[1][1];
+                    ry = c == 0 ? qp[0][2] : qp[1][2];
+                    rz = c == 0 ? qp[0][3] : qp[1][3];
+                } else {
+                    rm = q.q[0][o];
+                    rx = q.q[1][o];
+                    ry = q.q[2][o];
+                    rz = q.q[3][o];
+                }
                 R[i] = rm;
                 R[n + i] = limits ? limit(rx, rm) : rx;
                 R[2 * n + i] = ry;
@@ -416,7 +441,9 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
     if (spec_k >= 0) known_one = true;
     if (im < 1 || jm < 2 || lm < 1 || lm > ctx->y_cap) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = lines_per_block(jm, ctx->tile[TM5_TILE_Y_LINES]);
+    // many tracers: the widest tile, each tracer's rows read in the longest runs along i (the
+    // context's tile is tuned for the one-tracer step)
+    const int TI = lines_per_block(jm, q.nt >= 8 ? 16 : ctx->tile[TM5_TILE_Y_LINES]);
     if (TI < 1) return TM5_EINVAL;
     const int chunks = (im * (jm - 1) + Y_NTH * Y_FPT - 1) / (Y_NTH * Y_FPT);
     if (!known_one)
