@@ -347,6 +347,7 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
                                                   const int *nloop, int limits, int *words, int skip_mask, int spec_k)
 {
     if (skipped(words, skip_mask)) return;
+    extern __shared__ double yst[];  // 4 jm TI: one tracer's moments of the workgroup's cells
     int bx, l;
     xcd_grid2(bx, l);
     const int i0 = bx * TI;
@@ -382,13 +383,23 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
             const double mass_x3 = 3.0 * m_old;
             const bool lim0 = limits && s == 0;
             const bool pole = jj == 0 || jj == jm - 1;
-            // both candidates of each face's upwind cell are read, independently of the flux signs
+            // both candidates of each face's upwind cell are read, independently of the flux signs:
+            // the masses here, each tracer's moments from the workgroup's cells staged in LDS
+            // ([moment][row][ti]; every cell read from HBM once)
             const double m_dn = m(i, jlo, l), m_up = m(i, jhi, l);
+            const int stride = jm * TI, me = jj * TI + ti, dn = jlo * TI + ti, up = jhi * TI + ti;
             for (int t = 0; t < q.nt; t++) {
-                const long long o = q.off(i, jj, l, t), od = q.off(i, jlo, l, t), ou = q.off(i, jhi, l, t);
+                const long long o = q.off(i, jj, l, t);
                 const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
-                const double rm_d = q.q[0][od], rx_d = q.q[1][od], ry_d = q.q[2][od], rz_d = q.q[3][od];
-                const double rm_u = q.q[0][ou], rx_u = q.q[1][ou], ry_u = q.q[2][ou], rz_u = q.q[3][ou];
+                if (mine) {
+                    yst[me] = rm;
+                    yst[stride + me] = rx0;
+                    yst[2 * stride + me] = ry0;
+                    yst[3 * stride + me] = rz0;
+                }
+                __syncthreads();
+                const double rm_d = yst[dn], rx_d = yst[stride + dn], ry_d = yst[2 * stride + dn], rz_d = yst[3 * stride + dn];
+                const double rm_u = yst[up], rx_u = yst[stride + up], ry_u = yst[2 * stride + up], rz_u = yst[3 * stride + up];
                 const bool lo_dn = b_lo >= 0.0, hi_own = b_hi >= 0.0;  // upwind: below / this cell
                 const FaceFlux lo = y_face(b_lo, lo_dn ? m_dn : m_old, lo_dn ? rm_d : rm, lo_dn ? rx_d : rx0,
                                            lo_dn ? ry_d : ry0, lo_dn ? rz_d : rz0, jj, jm, lim0);
@@ -402,8 +413,7 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
                 if (limits) ry = limit(ry, rm_new);
                 const double rx = rx0 - (hi.ft - lo.ft);
                 const double rz = rz0 - (hi.fw - lo.fw);
-                __syncthreads();
-                if (mine) {
+                if (mine) {  // no other thread reads the cell from memory
                     q.q[0][o] = rm_new;
                     q.q[3][o] = rz;
                     if (!pole) {
@@ -413,8 +423,7 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
                         q.q[2][o] = rym;  // the pole keeps its (limited) slope
                     }
                 }
-                __threadfence_block();
-                __syncthreads();
+                __syncthreads();  // the staged values are read before the next tracer's land
             }
         }
         __syncthreads();
@@ -456,8 +465,8 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
         TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
                    ctx->words, skip_mask, spec_k);
     else
-        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
-                   ctx->words, skip_mask, spec_k);
+        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, (size_t)4 * jm * TI * sizeof(double), ctx->stream, im, jm, lm,
+                   TI, m, q, bm, nloop, limits, ctx->words, skip_mask, spec_k);
     return status(hipGetLastError());
 }
 
@@ -579,7 +588,7 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
 // TI whole columns (i0.., j = blockIdx.y), the neighbourhood read into registers, the update
 // written after the workgroup's barrier.  Along-flow slope rzm; cross slopes rxm (ft), rym (fw).
 // A face's flux, its Courant number c / m(upwind) and upwind cell are the same for every tracer
-// of a sub-step and are found once; tracers go two a round, so that their loads overlap.
+// of a sub-step and are found once; each tracer's moments are staged in LDS for the neighbours.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
                                                   int limits, int prelimit, int *words, int skip_mask, int fixed_n,
@@ -589,6 +598,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     // fixed_n > 0: n_advz given by the caller, else the search's (the context's word)
     // spec_k >= 0 (fixed_n 1): a checked step's dynamw, first testing its faces as k_z_mass1 does
     if (skipped(words, skip_mask)) return;
+    extern __shared__ double zst[];  // 4 lm TI: one tracer's moments of the workgroup's cells
     int bx, j;
     xcd_grid2(bx, j);
     const int i0 = bx * TI;
@@ -624,55 +634,42 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
             const bool lim0 = prelimit && s == 0;
             // the Courant numbers of the sub-step, the same for every tracer
             const double g_lo = c_lo / m(i, j, u_lo), g_hi = c_hi / m(i, j, u_hi);
-            // one tracer's update (z_face's fluxes with the Courant numbers above)
-            auto update = [&](int t, long long o, double &rm_new, double &rx, double &ry, double &rz) {
-                FaceFlux lo{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};
-                const long long ol = q.off(i, j, u_lo, t), oh = q.off(i, j, u_hi, t);
-                const double rml = q.q[0][ol], rxl = q.q[1][ol], ryl = q.q[2][ol], rzl = q.q[3][ol];
-                const double rmh = q.q[0][oh], rxh = q.q[1][oh], ryh = q.q[2][oh], rzh = q.q[3][oh];
+            // each tracer's moments of the workgroup's cells pass through LDS ([moment][level][ti]):
+            // every cell is read from HBM once, its upwind neighbours from LDS, and the new values
+            // go straight back (no other thread reads them from memory)
+            const int stride = lm * TI, me = ll * TI + ti;
+            for (int t = 0; t < q.nt; t++) {
+                const long long o = q.off(i, j, ll, t);
                 const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
-                if (has_lo) lo = face_flux(c_lo, g_lo, rml, lim0 ? limit(rzl, rml) : rzl, rxl, ryl);
-                if (has_hi) hi = face_flux(c_hi, g_hi, rmh, lim0 ? limit(rzh, rmh) : rzh, rxh, ryh);
+                if (mine) {
+                    zst[me] = rm;
+                    zst[stride + me] = rx0;
+                    zst[2 * stride + me] = ry0;
+                    zst[3 * stride + me] = rz0;
+                }
+                __syncthreads();
+                FaceFlux lo{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};
+                if (mine && has_lo) {
+                    const int u = u_lo * TI + ti;
+                    const double rmu = zst[u], rzu = zst[3 * stride + u];
+                    lo = face_flux(c_lo, g_lo, rmu, lim0 ? limit(rzu, rmu) : rzu, zst[stride + u], zst[2 * stride + u]);
+                }
+                if (mine && has_hi) {
+                    const int u = u_hi * TI + ti;
+                    const double rmu = zst[u], rzu = zst[3 * stride + u];
+                    hi = face_flux(c_hi, g_hi, rmu, lim0 ? limit(rzu, rmu) : rzu, zst[stride + u], zst[2 * stride + u]);
+                }
                 const double rzm = lim0 ? limit(rz0, rm) : rz0;
-                rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
-                rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
+                const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
+                double rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
                 if (limits) rz = limit(rz, rm_new);
-                rx = (rx0 + lo.ft) - hi.ft;
-                ry = (ry0 + lo.fw) - hi.fw;
-            };
-            int t = 0;
-            for (; t + 1 < q.nt; t += 2) {  // two tracers a round: their loads in flight together
-                const long long o0 = q.off(i, j, ll, t), o1 = q.off(i, j, ll, t + 1);
-                double a0, a1, a2, a3, b0, b1, b2, b3;
-                update(t, o0, a0, a1, a2, a3);
-                update(t + 1, o1, b0, b1, b2, b3);
-                __syncthreads();
                 if (mine) {
-                    q.q[0][o0] = a0;
-                    q.q[1][o0] = a1;
-                    q.q[2][o0] = a2;
-                    q.q[3][o0] = a3;
-                    q.q[0][o1] = b0;
-                    q.q[1][o1] = b1;
-                    q.q[2][o1] = b2;
-                    q.q[3][o1] = b3;
+                    q.q[0][o] = rm_new;
+                    q.q[1][o] = (rx0 + lo.ft) - hi.ft;
+                    q.q[2][o] = (ry0 + lo.fw) - hi.fw;
+                    q.q[3][o] = rz;
                 }
-                __threadfence_block();
-                __syncthreads();
-            }
-            if (t < q.nt) {
-                const long long o0 = q.off(i, j, ll, t);
-                double a0, a1, a2, a3;
-                update(t, o0, a0, a1, a2, a3);
-                __syncthreads();
-                if (mine) {
-                    q.q[0][o0] = a0;
-                    q.q[1][o0] = a1;
-                    q.q[2][o0] = a2;
-                    q.q[3][o0] = a3;
-                }
-                __threadfence_block();
-                __syncthreads();
+                __syncthreads();  // the staged values are read before the next tracer's land
             }
         }
         __syncthreads();
@@ -808,7 +805,8 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
                    ctx->words, skip_mask, fixed_n, spec_k);
     else
-        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
+        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, (size_t)4 * lm * TI * sizeof(double), ctx->stream, si, sj, lm,
+                   TI, ms, qs, cs, limits, limits,
                    ctx->words, skip_mask, fixed_n, spec_k);
     return status(hipGetLastError());
 }
@@ -828,7 +826,8 @@ int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, 
     const int TI = lines_per_block(lm, ctx->tile[TM5_TILE_Z_LINES]);
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
-    TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm), 0, ctx->stream, im, jm, lm, TI,
+    TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm),
+               (size_t)4 * lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI,
                m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, -1);
     return status(hipGetLastError());
 }

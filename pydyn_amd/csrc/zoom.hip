@@ -604,31 +604,43 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
     const int nl = nloop[blockIdx.y];
     // faces below (bm(jc)) and above (bm(jc+1)) an interior cell
     const double b_lo = inner ? C.bm(ii, jc, l) : 0.0, b_hi = inner ? C.bm(ii, jc + 1, l) : 0.0;
+    // each tracer's moments of the workgroup's cells pass through LDS ([moment][cell][ti]): every
+    // cell is read from HBM once and its neighbours from LDS
+    extern __shared__ double yst[];
+    const int stride = n * TI, me = cc * TI + ti;
+    const int u_lo = b_lo >= 0.0 ? cc - 1 : cc, u_hi = b_hi >= 0.0 ? cc : cc + 1;  // cells of the line
     for (int s = 0; s < nl; s++) {
         const double m_old = C.m(ii, jc, l);
         const double m_new = (m_old + b_lo) - b_hi;
         if (!MASS_ONLY) {
             const bool lim0 = limits && s == 0;
+            // the Courant numbers of the sub-step, the same for every tracer
+            const double beta_lo = inner ? b_lo / C.m(ii, j0 + u_lo, l) : 0.0;
+            const double beta_hi = inner ? b_hi / C.m(ii, j0 + u_hi, l) : 0.0;
             for (int t = 1; t <= C.nt; t++) {
-                const double rm = C.q[0](ii, jc, l, t);
-                double rym = C.q[2](ii, jc, l, t);
+                const double rm = C.q[0](ii, jc, l, t), rx0 = C.q[1](ii, jc, l, t);
+                const double ry0 = C.q[2](ii, jc, l, t), rz0 = C.q[3](ii, jc, l, t);
+                if (mine) {
+                    yst[me] = rm;
+                    yst[stride + me] = rx0;
+                    yst[2 * stride + me] = ry0;
+                    yst[3 * stride + me] = rz0;
+                }
+                __syncthreads();
+                double rym = ry0;
                 if (lim0) rym = limit(rym, rm);
-                double out0 = rm, out1 = C.q[1](ii, jc, l, t), out2 = rym, out3 = C.q[3](ii, jc, l, t);
+                double out0 = rm, out1 = rx0, out2 = rym, out3 = rz0;
                 if (inner) {
                     FaceFlux lo, hi;
                     {
-                        const int u = b_lo >= 0.0 ? jc - 1 : jc;
-                        const double beta = b_lo / C.m(ii, u, l);
-                        const double rmu = C.q[0](ii, u, l, t);
-                        const double ryu = lim0 ? limit(C.q[2](ii, u, l, t), rmu) : C.q[2](ii, u, l, t);
-                        lo = face_flux(b_lo, beta, rmu, ryu, C.q[1](ii, u, l, t), C.q[3](ii, u, l, t));
+                        const int u = u_lo * TI + ti;
+                        const double rmu = yst[u], ryu = lim0 ? limit(yst[2 * stride + u], rmu) : yst[2 * stride + u];
+                        lo = face_flux(b_lo, beta_lo, rmu, ryu, yst[stride + u], yst[3 * stride + u]);
                     }
                     {
-                        const int u = b_hi >= 0.0 ? jc : jc + 1;
-                        const double beta = b_hi / C.m(ii, u, l);
-                        const double rmu = C.q[0](ii, u, l, t);
-                        const double ryu = lim0 ? limit(C.q[2](ii, u, l, t), rmu) : C.q[2](ii, u, l, t);
-                        hi = face_flux(b_hi, beta, rmu, ryu, C.q[1](ii, u, l, t), C.q[3](ii, u, l, t));
+                        const int u = u_hi * TI + ti;
+                        const double rmu = yst[u], ryu = lim0 ? limit(yst[2 * stride + u], rmu) : yst[2 * stride + u];
+                        hi = face_flux(b_hi, beta_hi, rmu, ryu, yst[stride + u], yst[3 * stride + u]);
                     }
                     const double inv_mnew = 1.0 / m_new;
                     const double flow_conv = b_lo - b_hi, flow_mean = 3.0 * (b_lo + b_hi), mass_x3 = 3.0 * m_old;
@@ -641,15 +653,13 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
                     out1 = out1 + (lo.ft - hi.ft);
                     out3 = out3 + (lo.fw - hi.fw);
                 }
-                __syncthreads();
-                if (mine && (inner || lim0)) {
+                if (mine && (inner || lim0)) {  // no other thread reads the cell from memory
                     C.q[0](ii, jc, l, t) = out0;
                     C.q[1](ii, jc, l, t) = out1;
                     C.q[2](ii, jc, l, t) = out2;
                     C.q[3](ii, jc, l, t) = out3;
                 }
-                __threadfence_block();
-                __syncthreads();
+                __syncthreads();  // the staged values are read before the next tracer's land
             }
         }
         __syncthreads();
@@ -913,7 +923,8 @@ int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int 
         TM5_LAUNCH(ctx, k_y_child_steps<true>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, 0, ctx->words,
                    skip_mask);
     else
-        TM5_LAUNCH(ctx, k_y_child_steps<false>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, limits,
+        TM5_LAUNCH(ctx, k_y_child_steps<false>, grid, block, (size_t)4 * n * TI * sizeof(double), ctx->stream, C, is,
+                   ie, ls, TI, nloop, limits,
                    ctx->words, skip_mask);
     const int n0 = C.im + 2, n1 = C.jm + 2;
     TM5_LAUNCH(ctx, k_y_child_restore, dim3((unsigned)((n0 * n1 + 255) / 256), (unsigned)nlay), dim3(256), 0,
