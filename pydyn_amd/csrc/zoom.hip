@@ -472,17 +472,45 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
         }
         for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
     } else {
+        // each tracer's moments of the lane's (at most two) cells are loaded during the sub-steps
+        // of the tracer before
+        constexpr int PRE = 2;
+        const bool pre = n <= 64 * PRE;
+        double qp[PRE][4];
+        auto fetch = [&](int t) {
+#pragma unroll
+            for (int k = 0; k < PRE; k++) {
+                const int c = lane + 64 * k;
+                if (c < n)
+#pragma unroll
+                    for (int a = 0; a < 4; a++) qp[k][a] = C.q[a](i0 + c, j, l, t);
+            }
+        };
+        if (pre) fetch(1);
         for (int t = 1; t <= C.nt; t++) {
             if (t > 1) {
                 for (int c = lane; c < n; c += 64) M[c] = M0[c];
             }
             for (int c = lane; c < n; c += 64) {
-                const double rm = C.q[0](i0 + c, j, l, t);
+                double rm, rx, ry, rz;
+                if (pre) {
+                    const int k = (c - lane) / 64;
+                    rm = k == 0 ? qp[0][0] : qp[1][0];
+                    rx = k == 0 ? qp[0][1] : qp[1][1];
+                    ry = k == 0 ? qp[0][2] : qp[1][2];
+                    rz = k == 0 ? qp[0][3] : qp[1][3];
+                } else {
+                    rm = C.q[0](i0 + c, j, l, t);
+                    rx = C.q[1](i0 + c, j, l, t);
+                    ry = C.q[2](i0 + c, j, l, t);
+                    rz = C.q[3](i0 + c, j, l, t);
+                }
                 R[c] = rm;
-                R[n + c] = limits ? limit(C.q[1](i0 + c, j, l, t), rm) : C.q[1](i0 + c, j, l, t);
-                R[2 * n + c] = C.q[2](i0 + c, j, l, t);
-                R[3 * n + c] = C.q[3](i0 + c, j, l, t);
+                R[n + c] = limits ? limit(rx, rm) : rx;
+                R[2 * n + c] = ry;
+                R[3 * n + c] = rz;
             }
+            if (pre && t < C.nt) fetch(t + 1);
             __syncthreads();
             for (int s = 0; s < nl; s++) {
                 for (int k = lane; k < nf; k += 64) {
