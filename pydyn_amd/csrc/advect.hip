@@ -638,9 +638,18 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
             // every cell is read from HBM once, its upwind neighbours from LDS, and the new values
             // go straight back (no other thread reads them from memory)
             const int stride = lm * TI, me = ll * TI + ti;
+            // each tracer's moments are loaded while the tracer before it is worked on
+            double nx[4] = {0.0, 0.0, 0.0, 0.0};
+            if (q.nt > 0) {
+                const long long o0 = q.off(i, j, ll, 0);
+                for (int k = 0; k < 4; k++) nx[k] = q.q[k][o0];
+            }
             for (int t = 0; t < q.nt; t++) {
                 const long long o = q.off(i, j, ll, t);
-                const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
+                const double rm = nx[0], rx0 = nx[1], ry0 = nx[2], rz0 = nx[3];
+                if (t + 1 < q.nt) {
+                    for (int k = 0; k < 4; k++) nx[k] = q.q[k][o + q.st];
+                }
                 if (mine) {
                     zst[me] = rm;
                     zst[stride + me] = rx0;

@@ -131,6 +131,7 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     if (!c) return TM5_EINVAL;
     c->stream = (hipStream_t)stream;
     hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     c->y_cap = 256;
     c->x_cap = 1 << 17;
     c->z_cap = 1 << 13;
@@ -172,6 +173,7 @@ extern "C" int tm5_ctx_destroy(tm5_ctx *c)
     if (c->y_nloop) hipFree(c->y_nloop);
     if (c->x_rows) hipFree(c->x_rows);
     if (c->z_ok) hipFree(c->z_ok);
+    if (c->cbuf) hipFree(c->cbuf);
     if (c->hwords) hipHostFree(c->hwords);
     free(c);
     return 0;

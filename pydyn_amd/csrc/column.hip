@@ -122,6 +122,121 @@ template <int N, int K> __device__ __forceinline__ void lu_steps(double (&A)[N],
     }
 }
 
+// What the flagged sequence needs of a column besides C: the decay factors of the lane's two
+// levels (fdlibm exp of minus the summed loss rates times dt) and the column's emission.
+struct ColumnSeq {
+    double decay, decay1, emis, x;
+};
+
+__device__ __forceinline__ ColumnSeq column_seq_setup(const ColumnArgs &a, int i, int j, int li)
+{
+    ColumnSeq s{1.0, 1.0, 0.0, 0.0};
+    const int l1 = li + 32;
+    if (a.pre || a.twice || a.post) {
+        if (li < a.lm) {
+            const double *L = &a.loss(i, j, li);
+            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
+            s.decay = fdlibm_exp(-(total * a.dt_chem));
+        }
+        if (l1 < a.lm) {
+            const double *L = &a.loss(i, j, l1);
+            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
+            s.decay1 = fdlibm_exp(-(total * a.dt_chem));
+        }
+        s.emis = a.emis(i, j, 0);
+        s.x = ((s.emis * a.factor) * a.ratio) * a.efrac;
+    }
+    return s;
+}
+
+// v on one tracer: rm <- row li of the operator (R) times the column's rm, summed from zero in
+// increasing k over the levels < N, the slopes times the diagonal entry (pydyn/kernels.py:448-457);
+// rv is the half's LDS vector of the column's rm.
+template <int N>
+__device__ __forceinline__ void column_v(const double (&R)[N], double diag, int li, double *rv, double &rm, double &rx,
+                                         double &ry, double &rz)
+{
+    wsync();
+    rv[li] = rm;
+    wsync();
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) acc = acc + R[k] * rv[k];
+    if (li < N) {
+        rm = acc;
+        rx = diag * rx;
+        ry = diag * ry;
+        rz = diag * rz;
+    }
+}
+
+// Whether tracer t takes the emission at the lane's two cells, at levels lev (here) and lev1
+// (here1), and the amount: do_add_2d's rm + x, rzm - x at the level (pydyn/zoom/physics.py:55-64);
+// an ensemble member's emission field is its scale times the region's, scaled first.
+__device__ __forceinline__ double column_emission(const ColumnArgs &a, const ColumnSeq &s, int lev, int lev1, int t,
+                                                  bool &here, bool &here1, double &es)
+{
+    here = lev == a.level && t == a.tracer;
+    here1 = lev1 == a.level && t == a.tracer;
+    es = 1.0;
+    if (!a.escale) return s.x;
+    es = a.escale[t];
+    here = lev == a.level && es != 0.0;
+    here1 = lev1 == a.level && es != 0.0;
+    return (((s.emis * es) * a.factor) * a.ratio) * a.efrac;
+}
+
+// a level no cell has (the second cell of a lane that holds one)
+constexpr int NO_LEVEL = -(1 << 30);
+
+// The decay of every moment of the lane's two cells (chemistry's c).
+__device__ __forceinline__ void column_c(const ColumnSeq &s, double (&q)[8])
+{
+#pragma unroll
+    for (int m = 0; m < 4; m++) q[m] = q[m] * s.decay;
+#pragma unroll
+    for (int m = 4; m < 8; m++) q[m] = q[m] * s.decay1;
+}
+
+// One tracer's flagged sequence around v: "c s" if pre, v, "s c c s v" if twice, "s c" if post,
+// on two cells of the lane: level lev (q[0..3] = rm, rxm, rym, rzm; decay s.decay) and level
+// lev1 (q[4..7]; s.decay1).  V: v acts on the first cell, at level li with R row li of C;
+// without V both cells lie above the convective levels, where v changes nothing.
+template <int N, bool V>
+__device__ __forceinline__ void column_seq(const ColumnArgs &a, const ColumnSeq &s, const double (&R)[N], double diag,
+                                           int li, int lev, int lev1, int t, double *rv, double (&q)[8])
+{
+    bool here, here1;
+    double es;
+    const double xt = column_emission(a, s, lev, lev1, t, here, here1, es);
+    auto s_op = [&]() {
+        if (here) {
+            q[0] = q[0] + xt;
+            q[3] = q[3] - xt;
+        }
+        if (here1) {
+            q[4] = q[4] + xt;
+            q[7] = q[7] - xt;
+        }
+    };
+    if (a.pre) {
+        column_c(s, q);
+        s_op();
+    }
+    if (V) column_v<N>(R, diag, li, rv, q[0], q[1], q[2], q[3]);
+    if (a.twice) {
+        s_op();
+        column_c(s, q);
+        column_c(s, q);
+        s_op();
+        if (V) column_v<N>(R, diag, li, rv, q[0], q[1], q[2], q[3]);
+    }
+    if (a.post) {
+        s_op();
+        column_c(s, q);
+    }
+}
+
 template <int N, bool ADJ>
 __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
 {
@@ -358,6 +473,13 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         for (int k = 0; k < N; k++) crow[k] = Mt[r * LD + k];
     }
     const double diag = li < N ? Mt[li * LD + li] : 0.0;
+    if constexpr (!ADJ) {
+        if (a.cbuf && active && li < N) {
+            double *Cr = a.cbuf + ((long long)col * N + li) * N;
+#pragma unroll
+            for (int k = 0; k < N; k++) Cr[k] = crow[k];
+        }
+    }
 
     // zfused: dynamw's sub-step through LDS staging Z (the matrix's space, free once C is read):
     // Z[0, 64) the masses at the first sub-step's start, then rm, rxm, rym, rzm by level; the
@@ -400,160 +522,83 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         rz = rzn;
     };
 
-    // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
-    const bool sc = a.pre || a.twice || a.post;
-    double decay = 1.0, decay1 = 1.0, x = 0.0, emis_ij = 0.0;
-    if (sc) {
+    // dynamw at n_advz 1 on one tracer: m0 -> m1 (first), m1 -> m2 (second)
+    auto z_op = [&](bool second, double (&q)[8]) {
+        wsync();  // earlier reads of Z and of the matrix precede these writes
+        if (!second) {
+            if (li < lm) Z[li] = zm0;
+            if (l1 < lm) Z[l1] = zm0h;
+        }
         if (li < lm) {
-            const double *L = &a.loss(i, j, li);
-            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
-            decay = fdlibm_exp(-(total * a.dt_chem));
-        }
-        if (l1 < lm) {
-            const double *L = &a.loss(i, j, l1);
-            const double total = ((L[0] + L[a.loss_sr]) + L[2 * a.loss_sr]) + L[3 * a.loss_sr];
-            decay1 = fdlibm_exp(-(total * a.dt_chem));
-        }
-        emis_ij = a.emis(i, j, 0);
-        x = ((emis_ij * a.factor) * a.ratio) * a.efrac;
-    }
-    double lam_e = 0.0;  // ADJ: the emission's lambda from this lane's level
-    for (int t = 0; t < a.q.nt; t++) {
-        const long long o = a.q.off(i, j, li < lm ? li : 0, t), o1 = a.q.off(i, j, l1 < lm ? l1 : 0, t);
-        double rm = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rm1 = 0.0, rx1 = 0.0, ry1 = 0.0, rz1 = 0.0;
-        if (li < lm) {
-            rm = a.q.q[0][o];
-            rx = a.q.q[1][o];
-            ry = a.q.q[2][o];
-            rz = a.q.q[3][o];
-        }
-        if (l1 < lm) {
-            rm1 = a.q.q[0][o1];
-            rx1 = a.q.q[1][o1];
-            ry1 = a.q.q[2][o1];
-            rz1 = a.q.q[3][o1];
-        }
-        // do_add_2d: rm + x, rzm - x at the level (pydyn/zoom/physics.py:55-64); an ensemble member's
-        // emission field is its scale times the region's, scaled first
-        bool here = li == a.level && t == a.tracer, here1 = l1 == a.level && t == a.tracer;
-        double xt = x, es = 1.0;
-        if (a.escale) {
-            es = a.escale[t];
-            here = li == a.level && es != 0.0;
-            here1 = l1 == a.level && es != 0.0;
-            xt = (((emis_ij * es) * a.factor) * a.ratio) * a.efrac;
-        }
-        auto s_op = [&]() {
-            if (here) {
-                rm = rm + xt;
-                rz = rz - xt;
-            }
-            if (here1) {
-                rm1 = rm1 + xt;
-                rz1 = rz1 - xt;
-            }
-        };
-        auto c_op = [&]() {  // the decay of every moment
-            rm = rm * decay;
-            rx = rx * decay;
-            ry = ry * decay;
-            rz = rz * decay;
-            rm1 = rm1 * decay1;
-            rx1 = rx1 * decay1;
-            ry1 = ry1 * decay1;
-            rz1 = rz1 * decay1;
-        };
-        auto z_op = [&](bool second) {  // dynamw at n_advz 1: m0 -> m1 (first), m1 -> m2 (second)
-            wsync();  // earlier reads of Z and of the matrix precede these writes
-            if (!second) {
-                if (li < lm) Z[li] = zm0;
-                if (l1 < lm) Z[l1] = zm0h;
-            }
-            if (li < lm) {
-                Z[64 + li] = rm;
-                Z[128 + li] = rx;
-                Z[192 + li] = ry;
-                Z[256 + li] = rz;
-            }
-            if (l1 < lm) {
-                Z[64 + l1] = rm1;
-                Z[128 + l1] = rx1;
-                Z[192 + l1] = ry1;
-                Z[256 + l1] = rz1;
-            }
-            wsync();
-            const double *Mz = second ? mc : Z;
-            if (li < lm) z_cell(li, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], rm, rx, ry, rz);
-            if (l1 < lm) z_cell(l1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], rm1, rx1, ry1, rz1);
-        };
-        auto v_op = [&]() {  // rm <- C rm over levels < N (ADJ: C^T, lane k holds column k), slopes times diag(C)
-            wsync();
-            rv[li] = rm;
-            wsync();
-            double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < N; k++) acc = acc + (ADJ ? y[k] : crow[k]) * rv[k];
-            if (li < N) {
-                rm = acc;
-                rx = diag * rx;
-                ry = diag * ry;
-                rz = diag * rz;
-            }
-        };
-        if (ADJ) {  // the transposes in reverse: v^T; s^T adds (rm - rzm) times dx/demis at the level
+            for (int m = 0; m < 4; m++) Z[64 * (m + 1) + li] = q[m];
+        }
+        if (l1 < lm) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) Z[64 * (m + 1) + l1] = q[4 + m];
+        }
+        wsync();
+        const double *Mz = second ? mc : Z;
+        if (li < lm) z_cell(li, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
+        if (l1 < lm) z_cell(l1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
+    };
+
+    // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
+    // (the factor pass of the split form stops here: k_column_apply takes the tracers)
+    const ColumnSeq cs = column_seq_setup(a, i, j, li);
+    const int ntl = a.cbuf ? 0 : a.q.nt;
+    double lam_e = 0.0;  // ADJ: the emission's lambda from this lane's level
+    for (int t = 0; t < ntl; t++) {
+        const long long o = a.q.off(i, j, li < lm ? li : 0, t), o1 = a.q.off(i, j, l1 < lm ? l1 : 0, t);
+        double q[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (li < lm) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) q[m] = a.q.q[m][o];
+        }
+        if (l1 < lm) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) q[4 + m] = a.q.q[m][o1];
+        }
+        if constexpr (ADJ) {  // the transposes in reverse: v^T; s^T adds (rm - rzm) times dx/demis at the level
+            bool here, here1;
+            double es;
+            column_emission(a, cs, li, l1, t, here, here1, es);
             const double kt = ((es * a.factor) * a.ratio) * a.efrac;
             auto s_ad = [&]() {
-                if (here) lam_e = lam_e + kt * (rm - rz);
-                if (here1) lam_e = lam_e + kt * (rm1 - rz1);
+                if (here) lam_e = lam_e + kt * (q[0] - q[3]);
+                if (here1) lam_e = lam_e + kt * (q[4] - q[7]);
             };
+            // v^T: lane k holds column k of C
+            auto v_op = [&]() { column_v<N>(y, diag, li, rv, q[0], q[1], q[2], q[3]); };
             if (a.post) {
-                c_op();
+                column_c(cs, q);
                 s_ad();
             }
             if (a.twice) {
                 v_op();
                 s_ad();
-                c_op();
-                c_op();
+                column_c(cs, q);
+                column_c(cs, q);
                 s_ad();
             }
             v_op();
             if (a.pre) {
                 s_ad();
-                c_op();
+                column_c(cs, q);
             }
         } else {
-            if (zf) z_op(false);
-            if (a.pre) {
-                c_op();
-                s_op();
-            }
-            v_op();
-            if (a.twice) {
-                s_op();
-                c_op();
-                c_op();
-                s_op();
-                v_op();
-            }
-            if (a.post) {
-                s_op();
-                c_op();
-            }
-            if (zf) z_op(true);
+            if (zf) z_op(false, q);
+            column_seq<N, true>(a, cs, crow, diag, li, li, l1, t, rv, q);
+            if (zf) z_op(true, q);
         }
         if (active && own) {
             if (li < lm) {
-                a.q.q[0][o] = rm;
-                a.q.q[1][o] = rx;
-                a.q.q[2][o] = ry;
-                a.q.q[3][o] = rz;
+#pragma unroll
+                for (int m = 0; m < 4; m++) a.q.q[m][o] = q[m];
             }
             if (l1 < lm) {
-                a.q.q[0][o1] = rm1;
-                a.q.q[1][o1] = rx1;
-                a.q.q[2][o1] = ry1;
-                a.q.q[3][o1] = rz1;
+#pragma unroll
+                for (int m = 0; m < 4; m++) a.q.q[m][o1] = q[4 + m];
             }
         }
     }
@@ -576,6 +621,105 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     }
 }
 
+// The apply of the split form, for many tracers: k_column's sequence on every tracer, from the
+// rows of C its factor pass stored in cbuf.  A workgroup takes APPLY_COLS neighbouring columns of
+// a row (two columns a wave, lanes over levels as in k_column) and a range of tracers (grid z),
+// so each level's cache line of a tracer is read and written by one workgroup at about the same
+// time: its waves meet at a barrier after every tracer (a rendezvous only; nothing is exchanged
+// between waves), and each wave loads the tracers ahead of the one it works on.  Levels 32 and
+// up (only sources and chemistry act there) follow for the range's tracers at once, cell by cell.
+constexpr int APPLY_COLS = 16;
+template <int N>
+__global__ void __launch_bounds__(APPLY_COLS * 32, 4)
+    k_column_apply(ColumnArgs a, const double *cbuf, int tchunk, int *words, int skip_mask)
+{
+    if (skipped_k(words, skip_mask)) return;
+    constexpr int AHEAD = 2;                     // tracers loaded ahead
+    __shared__ double rvs[APPLY_COLS * 32];      // each column's rm by level, for v
+    __shared__ double dec_hi[APPLY_COLS * 32];   // each column's decay factors at levels 32 and up
+    // tiles renumbered XCD-contiguously (xcd_tile), the tracer range slowest
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int h = lane >> 5, li = lane & 31, lm = a.lm;
+    const int t0 = bz * tchunk, nt = a.q.nt - t0 < tchunk ? a.q.nt - t0 : tchunk;  // tracers t0 .. t0 + nt - 1
+    int i = bx * APPLY_COLS + wave * 2 + h;
+    const int j = by;
+    const bool active = i < a.im;
+    if (!active) i = a.im - 1;
+    const bool store = active && (a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0);
+    double *rv = rvs + (wave * 2 + h) * 32;
+    const double *Cr = cbuf + ((long long)j * a.im + i) * (N * N) + (long long)(li < N ? li : N - 1) * N;
+    double crow[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) crow[k] = Cr[k];
+    const double diag = li < N ? Cr[li] : 0.0;
+    const ColumnSeq cs = column_seq_setup(a, i, j, li);
+    const bool in = li < lm;
+    const long long o = a.q.off(i, j, in ? li : 0, t0), st = a.q.st;
+    double ahead[AHEAD][4];
+#pragma unroll
+    for (int d = 0; d < AHEAD; d++) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) ahead[d][m] = in && d < nt ? a.q.q[m][o + d * st] : 0.0;
+    }
+    for (int t = 0; t < nt; t++) {
+        double q[8] = {ahead[0][0], ahead[0][1], ahead[0][2], ahead[0][3], 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d + 1 < AHEAD; d++) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) ahead[d][m] = ahead[d + 1][m];
+        }
+        const int tn = t + AHEAD;
+#pragma unroll
+        for (int m = 0; m < 4; m++) ahead[AHEAD - 1][m] = in && tn < nt ? a.q.q[m][o + tn * st] : 0.0;
+        column_seq<N, true>(a, cs, crow, diag, li, li, NO_LEVEL, t0 + t, rv, q);
+        if (store && in) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) a.q.q[m][o + t * st] = q[m];
+        }
+        __builtin_amdgcn_s_barrier();
+    }
+    // levels 32 .. lm - 1 of every tracer: cell (t, 32 + c) of the column by lane li + 32 n
+    const int nhi = lm - 32;
+    if (nhi <= 0) return;
+    double *dh = dec_hi + (wave * 2 + h) * 32;
+    if (li < nhi) dh[li] = cs.decay1;
+    wsync();
+    const int cells = nhi * nt;
+    constexpr int B = 4;  // cells per lane in flight
+    for (int c0 = 0; c0 < cells; c0 += 32 * B) {
+        double v[B][8];
+        long long oc[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int c = c0 + u * 32 + li;
+            const int t = c < cells ? c / nhi : 0, lev = c < cells ? 32 + c % nhi : 32;
+            oc[u] = a.q.off(i, j, lev, t0 + t);
+#pragma unroll
+            for (int m = 0; m < 4; m++) v[u][m] = c < cells ? a.q.q[m][oc[u]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int c = c0 + u * 32 + li;
+            if (c >= cells) continue;
+            const int t = c / nhi, lev = 32 + c % nhi;
+            const ColumnSeq ch{dh[lev - 32], 1.0, cs.emis, cs.x};
+#pragma unroll
+            for (int m = 4; m < 8; m++) v[u][m] = 0.0;
+            column_seq<N, false>(a, ch, crow, diag, li, lev, NO_LEVEL, t0 + t, rv, v[u]);
+            if (store) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) a.q.q[m][oc[u]] = v[u][m];
+            }
+        }
+    }
+}
+
+// the split form from this many tracers on (a factor pass, then k_column_apply)
+constexpr int SPLIT_NT = 8;
+
 template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
 {
@@ -590,12 +734,56 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
                                  "k_column<20>", "k_column<21>", "k_column<22>", "k_column<23>", "k_column<24>",
                                  "k_column<25>", "k_column<26>", "k_column<27>", "k_column<28>", "k_column<29>",
                                  "k_column<30>", "k_column<31>"};
-    if (args.adjoint)
+    if (args.adjoint) {
         TM5_LAUNCH_NAMED(ctx, "k_column_adjoint", (k_column<N, true>), grid, block, lds, ctx->stream, args, ctx->words,
                          skip_mask);
-    else
+        return status(hipGetLastError());
+    }
+    if (args.zfused || args.q.nt < SPLIT_NT) {
         TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), grid, block, lds, ctx->stream, args, ctx->words,
                          skip_mask);
+        return status(hipGetLastError());
+    }
+    // the split form: the matrices of every column into the context's buffer, then the tracers
+    const size_t need = (size_t)ncol * N * N;
+    if (ctx->cbuf_n < need) {
+        hipError_t e = hipSuccess;
+        if (ctx->cbuf) {
+            e = hipStreamSynchronize(ctx->stream);  // earlier launches may still read it
+            if (e == hipSuccess) e = hipFree(ctx->cbuf);
+            if (e != hipSuccess) return status(e);
+            ctx->cbuf = nullptr;
+            ctx->cbuf_n = 0;
+        }
+        e = hipMalloc((void **)&ctx->cbuf, need * sizeof(double));
+        if (e != hipSuccess) return status(e);
+        ctx->cbuf_n = need;
+    }
+    ColumnArgs f = args;
+    f.cbuf = ctx->cbuf;
+    TM5_LAUNCH_NAMED(ctx, "k_column_factor", (k_column<N, false>), grid, block, lds, ctx->stream, f, ctx->words,
+                     skip_mask);
+    const int rc = status(hipGetLastError());
+    if (rc) return rc;
+    // tracer ranges enough for the tiles to fill the device's workgroup slots in nearly whole rounds
+    const int tiles = (args.im + APPLY_COLS - 1) / APPLY_COLS * args.jm, slots = 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256);  // 16 waves a CU
+    const int nt = args.q.nt, most = nt / 16 > 1 ? nt / 16 : 1;
+    int ranges = 1;
+    double best = -1.0;
+    for (int r = 1; r <= most; r++) {
+        const int chunk = (nt + r - 1) / r, used = (nt + chunk - 1) / chunk;
+        const double w = (double)tiles * used / slots, eff = w / ceil(w);
+        if (eff > best) {
+            best = eff;
+            ranges = used;
+        }
+        if (eff >= 0.9) break;
+    }
+    const int tchunk = (nt + ranges - 1) / ranges;
+    const dim3 agrid((args.im + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
+    const dim3 ablock(APPLY_COLS * 32);
+    TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, ctx->cbuf, tchunk,
+                     ctx->words, skip_mask);
     return status(hipGetLastError());
 }
 

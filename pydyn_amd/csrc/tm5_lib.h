@@ -34,6 +34,9 @@ struct tm5_ctx {
     int *z_ok;      // device, per-block verdicts of the z check at n = 1
     int z_cap;      // its length
     int tile[TM5_TILES];  // tile shapes (tm5_ctx_set_tile)
+    int ncu;        // the device's compute units
+    double *cbuf;   // device, the split column form's matrices (grown on demand)
+    size_t cbuf_n;  // its length in doubles
     tm5_prof *prof; // non-null while profiling (then prof == prof_store)
     tm5_prof *prof_store;  // owned: the events and the last totals
 };
@@ -234,6 +237,9 @@ struct ColumnArgs {
     // the limiter flag zlimits; spec_k >= 0: a checked step's, testing both at cap 1 first
     int zfused, zlimits, spec_k;
     View3 cm;
+    // non-null: the factor pass of the split form (many tracers), which stores each column's rows
+    // of C there ((im, jm) columns, i fastest, of N x N row-major) and leaves the tracers alone
+    double *cbuf;
 };
 bool column_supported(int n);
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,

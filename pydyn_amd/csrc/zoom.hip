@@ -621,15 +621,17 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
                                                         int limits, int *words, int skip_mask)
 {
     if (zskip(words, skip_mask)) return;
-    const int l = ls + blockIdx.y;
+    int bx, by;  // neighbouring tiles, which share cache lines, on one XCD
+    xcd_grid2(bx, by);
+    const int l = ls + by;
     const int j0 = C.yr - 1, n = C.jm - 2 * C.yr + 4;
     const int ti = threadIdx.x % TI, c = threadIdx.x / TI;
-    const int i = is + blockIdx.x * TI + ti;
+    const int i = is + bx * TI + ti;
     const bool mine = c < n && i <= ie;
     const int ii = mine ? i : is, cc = mine ? c : 0;
     const int jc = j0 + cc;  // Fortran j of the cell
     const bool inner = cc > 0 && cc < n - 1;
-    const int nl = nloop[blockIdx.y];
+    const int nl = nloop[by];
     // faces below (bm(jc)) and above (bm(jc+1)) an interior cell
     const double b_lo = inner ? C.bm(ii, jc, l) : 0.0, b_hi = inner ? C.bm(ii, jc + 1, l) : 0.0;
     // each tracer's moments of the workgroup's cells pass through LDS ([moment][cell][ti]): every
@@ -645,9 +647,16 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
             // the Courant numbers of the sub-step, the same for every tracer
             const double beta_lo = inner ? b_lo / C.m(ii, j0 + u_lo, l) : 0.0;
             const double beta_hi = inner ? b_hi / C.m(ii, j0 + u_hi, l) : 0.0;
+            // each tracer's moments are loaded while the tracer before it is worked on
+            double nx[4] = {0.0, 0.0, 0.0, 0.0};
+            if (C.nt > 0) {
+                for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, 1);
+            }
             for (int t = 1; t <= C.nt; t++) {
-                const double rm = C.q[0](ii, jc, l, t), rx0 = C.q[1](ii, jc, l, t);
-                const double ry0 = C.q[2](ii, jc, l, t), rz0 = C.q[3](ii, jc, l, t);
+                const double rm = nx[0], rx0 = nx[1], ry0 = nx[2], rz0 = nx[3];
+                if (t < C.nt) {
+                    for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, t + 1);
+                }
                 if (mine) {
                     yst[me] = rm;
                     yst[stride + me] = rx0;

@@ -178,6 +178,55 @@ def test_convec_on_a_column_list(dev):
         assert_bitwise(got[~listed], st[k][~listed], f"{k} not listed")
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_convec_many_tracers(dev, masked):
+    """convec on 12 tracers (the split form: every column's matrix first, then the tracers in
+    workgroups of 16 columns) at 34 levels over 26 (levels 32 and 33 above the convective ones),
+    on all columns or a column list, against the oracle; and each tracer as one-tracer calls."""
+    import torch
+
+    from oracle import convection, physics
+    from pydyn_amd import meteo
+
+    D, K = dev
+    met = meteo.Meteo(meteo.grid("glb3x2"), seed=4)
+    sl = (slice(0, 37), slice(40, 47))  # 37 x 7 columns: a partial workgroup of 16 at each row's end
+    lmax, nt = met.g.lmax, 12
+    m = np.ascontiguousarray(met.m[sl])
+    im, jm, lm = m.shape
+    dkg = np.ascontiguousarray(physics.dkg(met.kvh[sl], m, met.gph[sl], lmax))
+    c = {k: np.ascontiguousarray(getattr(met, k)[sl]) for k in ("entu", "detu", "entd", "detd", "cloud_top", "cloud_lfs")}
+    rng = np.random.default_rng(12)
+    rm = rng.uniform(0.5, 1.5, (im, jm, lm, nt)) * m[..., None] * 1.8e-6
+    st = {"m": m, "rm": rm, **{k: rm * rng.uniform(-1.0, 1.0, rm.shape) for k in ("rxm", "rym", "rzm")}}
+    conv = {k: D.upload(c[k]) for k in ("entu", "detu", "entd", "detd")}
+    conv["dkg"] = D.upload(dkg)
+    for k in ("cloud_top", "cloud_lfs"):
+        conv[k] = D.upload(c[k].astype(np.int32)).to(torch.int32)
+    cols = None
+    listed = np.ones((im, jm), dtype=bool)
+    if masked:
+        flat = np.sort(rng.choice(im * jm, size=im * jm // 2, replace=False)).astype(np.int32)
+        cols = torch.as_tensor(flat, device="cuda")
+        listed[...] = False
+        listed[flat // jm, flat % jm] = True
+    dst = {k: D.upload(st[k]) for k in FIELDS}
+    K.convec(*(dst[k] for k in FIELDS), conv, 1800.0, lmax, cols=cols)
+    D.context().raise_errors()
+    full = {k: st[k].copy() for k in FIELDS}
+    convection.convec(*(full[k] for k in FIELDS), c["entu"], c["detu"], c["entd"], c["detd"], c["cloud_top"],
+                      c["cloud_lfs"], dkg, 1800.0, lmax)  # fmt: skip
+    for k in FIELDS[1:]:
+        got = D.download(dst[k])
+        assert_bitwise(got[listed], full[k][listed], f"{k} listed")
+        assert_bitwise(got[~listed], st[k][~listed], f"{k} not listed")
+    for t in (0, 7, 11):  # one tracer a call: the one-kernel form
+        one = {k: D.upload(np.ascontiguousarray(st[k][..., t : t + 1])) for k in FIELDS[1:]}
+        K.convec(D.upload(m), *(one[k] for k in FIELDS[1:]), conv, 1800.0, lmax, cols=cols)
+        for k in FIELDS[1:]:
+            assert_bitwise(D.download(one[k])[..., 0], D.download(dst[k])[..., t], f"{k} tracer {t}")
+
+
 def test_advect_y_on_region1_halo(dev):
     """dynamv on the interior slice of region 1's halo'd bm (0:im+1, 0:jm+1, 0:lm+1): the
     faces outside it through the whole-layer scaling when a layer sub-cycles."""
