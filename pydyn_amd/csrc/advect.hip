@@ -589,18 +589,25 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
 // written after the workgroup's barrier.  Along-flow slope rzm; cross slopes rxm (ft), rym (fw).
 // A face's flux, its Courant number c / m(upwind) and upwind cell are the same for every tracer
 // of a sub-step and are found once; each tracer's moments are staged in LDS for the neighbours.
+// With tracer ranges (tchunk > 0, grid z) several workgroups take the same columns: each moves
+// the masses through the sub-steps on its own (in LDS), none writes them, and the caller runs
+// the mass-only form afterwards.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
                                                   int limits, int prelimit, int *words, int skip_mask, int fixed_n,
-                                                  int spec_k)
+                                                  int spec_k, int tchunk)
 {
     // prelimit: dynamw's limiter on rzm before the first sub-step (advect_z's, not the bare sub-steps')
     // fixed_n > 0: n_advz given by the caller, else the search's (the context's word)
     // spec_k >= 0 (fixed_n 1): a checked step's dynamw, first testing its faces as k_z_mass1 does
     if (skipped(words, skip_mask)) return;
-    extern __shared__ double zst[];  // 4 lm TI: one tracer's moments of the workgroup's cells
-    int bx, j;
-    xcd_grid2(bx, j);
+    extern __shared__ double zst[];  // 4 lm TI: one tracer's moments of the workgroup's cells; lm TI masses
+    // tiles renumbered XCD-contiguously (as xcd_grid2), the tracer range slowest
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
+    const int bx = b % nbx, j = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t0 = tchunk > 0 ? bz * tchunk : 0;
+    const int t1 = tchunk > 0 && t0 + tchunk < q.nt ? t0 + tchunk : q.nt;
     const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, l = threadIdx.x / TI;
     const bool mine = l < lm && i0 + ti < im;
@@ -627,27 +634,32 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     // each face's upwind cell (the closed ends carry nothing)
     const bool has_lo = ll >= 1, has_hi = ll + 1 <= lm - 1;
     const int u_lo = c_lo >= 0.0 ? (has_lo ? ll - 1 : ll) : ll, u_hi = c_hi >= 0.0 ? ll : (has_hi ? ll + 1 : ll);
+    double m_cur = m(i, j, ll);
     for (int s = 0; s < nz; s++) {
-        const double m_old = m(i, j, ll);
+        const double m_old = MASS_ONLY ? m(i, j, ll) : m_cur;
         const double m_new = (m_old + c_lo) - c_hi;
         if (!MASS_ONLY) {
             const bool lim0 = prelimit && s == 0;
-            // the Courant numbers of the sub-step, the same for every tracer
-            const double g_lo = c_lo / m(i, j, u_lo), g_hi = c_hi / m(i, j, u_hi);
+            // the Courant numbers of the sub-step, the same for every tracer, from the column's
+            // masses at the sub-step's start (staged after the tracers' space)
+            double *zm = zst + 4 * lm * TI;
+            if (mine) zm[ll * TI + ti] = m_old;
+            __syncthreads();
+            const double g_lo = c_lo / zm[u_lo * TI + ti], g_hi = c_hi / zm[u_hi * TI + ti];
             // each tracer's moments of the workgroup's cells pass through LDS ([moment][level][ti]):
             // every cell is read from HBM once, its upwind neighbours from LDS, and the new values
             // go straight back (no other thread reads them from memory)
             const int stride = lm * TI, me = ll * TI + ti;
             // each tracer's moments are loaded while the tracer before it is worked on
             double nx[4] = {0.0, 0.0, 0.0, 0.0};
-            if (q.nt > 0) {
-                const long long o0 = q.off(i, j, ll, 0);
+            if (t0 < t1) {
+                const long long o0 = q.off(i, j, ll, t0);
                 for (int k = 0; k < 4; k++) nx[k] = q.q[k][o0];
             }
-            for (int t = 0; t < q.nt; t++) {
+            for (int t = t0; t < t1; t++) {
                 const long long o = q.off(i, j, ll, t);
                 const double rm = nx[0], rx0 = nx[1], ry0 = nx[2], rz0 = nx[3];
-                if (t + 1 < q.nt) {
+                if (t + 1 < t1) {
                     for (int k = 0; k < 4; k++) nx[k] = q.q[k][o + q.st];
                 }
                 if (mine) {
@@ -680,9 +692,10 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
                 }
                 __syncthreads();  // the staged values are read before the next tracer's land
             }
+            m_cur = m_new;
         }
         __syncthreads();
-        if (mine) m(i, j, ll) = m_new;
+        if (mine && (MASS_ONLY || tchunk == 0)) m(i, j, ll) = m_new;
         __threadfence_block();
         __syncthreads();
     }
@@ -810,13 +823,21 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     Tracers qs = q;
     for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (i0 * q.si + j0 * q.sj);
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
-    if (mass_only)
+    if (mass_only) {
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words, skip_mask, fixed_n, spec_k);
-    else
-        TM5_LAUNCH(ctx, k_z_steps<false>, grid, block, (size_t)4 * lm * TI * sizeof(double), ctx->stream, si, sj, lm,
-                   TI, ms, qs, cs, limits, limits,
-                   ctx->words, skip_mask, fixed_n, spec_k);
+                   ctx->words, skip_mask, fixed_n, spec_k, 0);
+        return status(hipGetLastError());
+    }
+    // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
+    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt, 16)
+                                  : q.nt;
+    const int ranges = (q.nt + tchunk - 1) / tchunk;
+    TM5_LAUNCH(ctx, k_z_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)(4 * lm * TI + lm * TI) * sizeof(double),
+               ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits, ctx->words, skip_mask, fixed_n, spec_k,
+               ranges > 1 ? tchunk : 0);
+    if (ranges > 1)
+        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
+                   ctx->words, skip_mask, fixed_n, -1, 0);
     return status(hipGetLastError());
 }
 
@@ -836,8 +857,8 @@ int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, 
     if (TI < 1) return TM5_EINVAL;
     TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm),
-               (size_t)4 * lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI,
-               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, -1);
+               (size_t)(4 * lm * TI + lm * TI) * sizeof(double), ctx->stream, im, jm, lm, TI,
+               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, -1, 0);
     return status(hipGetLastError());
 }
 

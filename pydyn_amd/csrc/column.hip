@@ -720,6 +720,23 @@ __global__ void __launch_bounds__(APPLY_COLS * 32, 4)
 // the split form from this many tracers on (a factor pass, then k_column_apply)
 constexpr int SPLIT_NT = 8;
 
+int tracer_chunk(long long tiles, long long slots, int nt, int min_chunk)
+{
+    const int most = nt / min_chunk > 1 ? nt / min_chunk : 1;
+    int ranges = 1;
+    double best = -1.0;
+    for (int r = 1; r <= most; r++) {
+        const int chunk = (nt + r - 1) / r, used = (nt + chunk - 1) / chunk;
+        const double w = (double)tiles * used / (double)slots, eff = w / ceil(w);
+        if (eff > best) {
+            best = eff;
+            ranges = used;
+        }
+        if (eff >= 0.9) break;
+    }
+    return (nt + ranges - 1) / ranges;
+}
+
 template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
 {
@@ -765,21 +782,10 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
                      skip_mask);
     const int rc = status(hipGetLastError());
     if (rc) return rc;
-    // tracer ranges enough for the tiles to fill the device's workgroup slots in nearly whole rounds
-    const int tiles = (args.im + APPLY_COLS - 1) / APPLY_COLS * args.jm, slots = 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256);  // 16 waves a CU
-    const int nt = args.q.nt, most = nt / 16 > 1 ? nt / 16 : 1;
-    int ranges = 1;
-    double best = -1.0;
-    for (int r = 1; r <= most; r++) {
-        const int chunk = (nt + r - 1) / r, used = (nt + chunk - 1) / chunk;
-        const double w = (double)tiles * used / slots, eff = w / ceil(w);
-        if (eff > best) {
-            best = eff;
-            ranges = used;
-        }
-        if (eff >= 0.9) break;
-    }
-    const int tchunk = (nt + ranges - 1) / ranges;
+    // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
+    const int nt = args.q.nt;
+    const int tiles = (args.im + APPLY_COLS - 1) / APPLY_COLS * args.jm;
+    const int tchunk = tracer_chunk(tiles, 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256), nt, 16);
     const dim3 agrid((args.im + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
     const dim3 ablock(APPLY_COLS * 32);
     TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, ctx->cbuf, tchunk,

@@ -241,6 +241,10 @@ struct ColumnArgs {
     // of C there ((im, jm) columns, i fastest, of N x N row-major) and leaves the tracers alone
     double *cbuf;
 };
+// The tracer ranges a many-tracer kernel splits nt tracers over (grid z): enough for `tiles`
+// workgroups a range to fill the device's `slots` workgroup slots in nearly whole rounds, each
+// range at least min_chunk tracers.  Returns the tracers a range (nt for a single range).
+int tracer_chunk(long long tiles, long long slots, int nt, int min_chunk);
 bool column_supported(int n);
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1);

@@ -129,6 +129,26 @@ def test_advect_z_substeps(dev, n_advz, prelimit):
         assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "cm")
 
 
+@pytest.mark.parametrize("scale", [1.0, 8.0])
+def test_advect_z_many_tracers(dev, scale):
+    """dynamw on 40 tracers (tracer ranges over the grid, each range moving the masses through
+    the sub-steps on its own, then the masses once), with and without sub-cycling, against the
+    oracle."""
+    from oracle import advection
+
+    D, K = dev
+    m, cm, q = records("z")[0]
+    cm = scale * cm
+    rng = np.random.default_rng(40)
+    q = [np.ascontiguousarray(np.repeat(x, 40, axis=3) * rng.uniform(0.5, 1.5, (*x.shape[:3], 40))) for x in q]
+    dm, dq, dc = D.upload(m), [D.upload(x) for x in q], D.upload(cm)
+    n = K.advect_z(dm, *dq, dc, True)
+    want_n = advection.advect_z(m, *q, cm, True)
+    assert n == want_n and (scale == 1.0 or n > 1), (n, want_n)
+    for k, (g, w) in enumerate(zip([dm, *dq, dc], [m, *q, cm])):
+        assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "cm")
+
+
 def test_cluster_sum(dev):
     D, K = dev
     rng = np.random.default_rng(4)
