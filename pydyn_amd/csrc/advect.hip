@@ -342,14 +342,23 @@ __device__ __forceinline__ FaceFlux y_face(double bf, double m_u, double rm, dou
     return ff;
 }
 
+// With tracer ranges (tchunk > 0, grid z) several workgroups take the same lines: each moves the
+// masses through the sub-steps on its own (in LDS), none writes them or restores bm, and the
+// caller runs the mass-only form afterwards.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 bm,
-                                                  const int *nloop, int limits, int *words, int skip_mask, int spec_k)
+                                                  const int *nloop, int limits, int *words, int skip_mask, int spec_k,
+                                                  int tchunk)
 {
     if (skipped(words, skip_mask)) return;
-    extern __shared__ double yst[];  // 4 jm TI: one tracer's moments of the workgroup's cells
-    int bx, l;
-    xcd_grid2(bx, l);
+    extern __shared__ double yst[];  // 4 jm TI: one tracer's moments of the workgroup's cells; jm TI masses
+    // tiles renumbered XCD-contiguously (as xcd_grid2), the tracer range slowest
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
+    const int bx = b % nbx, l = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t0 = tchunk > 0 ? bz * tchunk : 0;
+    const int t1 = tchunk > 0 && t0 + tchunk < q.nt ? t0 + tchunk : q.nt;
+    const bool writes_mass = MASS_ONLY || tchunk == 0;
     const int i0 = bx * TI;
     const int ti = threadIdx.x % TI, j = threadIdx.x / TI;
     const bool mine = j < jm && i0 + ti < im;
@@ -373,8 +382,9 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
     }
     // the cells below and above (clamped at the poles, where their faces carry nothing)
     const int jlo = jj > 0 ? jj - 1 : 0, jhi = jj + 1 < jm ? jj + 1 : jm - 1;
+    double m_cur = m(i, jj, l);
     for (int s = 0; s < nl; s++) {
-        const double m_old = m(i, jj, l), b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
+        const double m_old = MASS_ONLY ? m(i, jj, l) : m_cur, b_lo = bm(i, jj, l), b_hi = bm(i, jj + 1, l);
         const double m_new = (m_old + b_lo) - b_hi;
         if (!MASS_ONLY) {
             const double inv_mnew = 1.0 / m_new;
@@ -386,9 +396,19 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
             // both candidates of each face's upwind cell are read, independently of the flux signs:
             // the masses here, each tracer's moments from the workgroup's cells staged in LDS
             // ([moment][row][ti]; every cell read from HBM once)
-            const double m_dn = m(i, jlo, l), m_up = m(i, jhi, l);
             const int stride = jm * TI, me = jj * TI + ti, dn = jlo * TI + ti, up = jhi * TI + ti;
-            for (int t = 0; t < q.nt; t++) {
+            double m_dn, m_up;
+            if (tchunk > 0) {  // the line's masses at the sub-step's start, from LDS
+                double *ym = yst + 4 * stride;
+                if (mine) ym[me] = m_old;
+                __syncthreads();
+                m_dn = ym[dn];
+                m_up = ym[up];
+            } else {  // as written by the sub-step before
+                m_dn = m(i, jlo, l);
+                m_up = m(i, jhi, l);
+            }
+            for (int t = t0; t < t1; t++) {
                 const long long o = q.off(i, jj, l, t);
                 const double rm = q.q[0][o], rx0 = q.q[1][o], ry0 = q.q[2][o], rz0 = q.q[3][o];
                 if (mine) {
@@ -425,13 +445,14 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
                 }
                 __syncthreads();  // the staged values are read before the next tracer's land
             }
+            m_cur = m_new;
         }
         __syncthreads();
-        if (mine) m(i, jj, l) = m_new;
+        if (mine && writes_mass) m(i, jj, l) = m_new;
         __threadfence_block();
         __syncthreads();
     }
-    if (nl > 1 && mine) {  // "restore old bm": the search left the faces scaled
+    if (nl > 1 && mine && writes_mass) {  // "restore old bm": the search left the faces scaled
         bm(i, jj, l) = bm(i, jj, l) * (double)nl;
         if (jj == jm - 1) bm(i, jm, l) = bm(i, jm, l) * (double)nl;
     }
@@ -461,12 +482,21 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
                    skip_mask);
     const int *nloop = known_one ? nullptr : ctx->y_nloop;
     const dim3 grid((im + TI - 1) / TI, lm), block = line_block(TI, jm);
-    if (mass_only)
+    if (mass_only) {
         TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
-                   ctx->words, skip_mask, spec_k);
-    else
-        TM5_LAUNCH(ctx, k_y_steps<false>, grid, block, (size_t)4 * jm * TI * sizeof(double), ctx->stream, im, jm, lm,
-                   TI, m, q, bm, nloop, limits, ctx->words, skip_mask, spec_k);
+                   ctx->words, skip_mask, spec_k, 0);
+        return status(hipGetLastError());
+    }
+    // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
+    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt, 16)
+                                  : q.nt;
+    const int ranges = (q.nt + tchunk - 1) / tchunk;
+    TM5_LAUNCH(ctx, k_y_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)5 * jm * TI * sizeof(double),
+               ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits, ctx->words, skip_mask, spec_k,
+               ranges > 1 ? tchunk : 0);
+    if (ranges > 1)
+        TM5_LAUNCH(ctx, k_y_steps<true>, grid, block, 0, ctx->stream, im, jm, lm, TI, m, q, bm, nloop, limits,
+                   ctx->words, skip_mask, -1, 0);
     return status(hipGetLastError());
 }
 

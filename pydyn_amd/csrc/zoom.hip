@@ -616,13 +616,20 @@ __global__ void __launch_bounds__(1024) k_y_child_search(ZR C, int is, int ie, i
 // then -- after the workgroup's barrier -- writes it.  The end cells are held; the limiter of
 // the first sub-step is taken on the values as read (the reference limits rym on every cell of
 // the scope, ends included, before the loop; the ends are written back limited).
+// With tracer ranges (tchunk > 0, grid z) several workgroups take the same columns: each moves
+// the masses through the sub-steps on its own (in LDS), none writes them, and the caller runs
+// the mass-only form afterwards.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, int ls, int TI, const int *nloop,
-                                                        int limits, int *words, int skip_mask)
+                                                        int limits, int *words, int skip_mask, int tchunk)
 {
     if (zskip(words, skip_mask)) return;
-    int bx, by;  // neighbouring tiles, which share cache lines, on one XCD
-    xcd_grid2(bx, by);
+    // neighbouring tiles, which share cache lines, on one XCD (as xcd_grid2), the tracer range slowest
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t0 = tchunk > 0 ? bz * tchunk + 1 : 1;  // Fortran tracer indices t0 .. t1
+    const int t1 = tchunk > 0 && t0 - 1 + tchunk < C.nt ? t0 - 1 + tchunk : C.nt;
     const int l = ls + by;
     const int j0 = C.yr - 1, n = C.jm - 2 * C.yr + 4;
     const int ti = threadIdx.x % TI, c = threadIdx.x / TI;
@@ -639,22 +646,32 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
     extern __shared__ double yst[];
     const int stride = n * TI, me = cc * TI + ti;
     const int u_lo = b_lo >= 0.0 ? cc - 1 : cc, u_hi = b_hi >= 0.0 ? cc : cc + 1;  // cells of the line
+    double m_cur = C.m(ii, jc, l);
     for (int s = 0; s < nl; s++) {
-        const double m_old = C.m(ii, jc, l);
+        const double m_old = MASS_ONLY || tchunk == 0 ? C.m(ii, jc, l) : m_cur;
         const double m_new = (m_old + b_lo) - b_hi;
         if (!MASS_ONLY) {
             const bool lim0 = limits && s == 0;
             // the Courant numbers of the sub-step, the same for every tracer
-            const double beta_lo = inner ? b_lo / C.m(ii, j0 + u_lo, l) : 0.0;
-            const double beta_hi = inner ? b_hi / C.m(ii, j0 + u_hi, l) : 0.0;
+            double beta_lo, beta_hi;
+            if (tchunk > 0) {  // the line's masses at the sub-step's start, from LDS
+                double *ym = yst + 4 * stride;
+                if (mine) ym[me] = m_old;
+                __syncthreads();
+                beta_lo = inner ? b_lo / ym[u_lo * TI + ti] : 0.0;
+                beta_hi = inner ? b_hi / ym[u_hi * TI + ti] : 0.0;
+            } else {  // as written by the sub-step before
+                beta_lo = inner ? b_lo / C.m(ii, j0 + u_lo, l) : 0.0;
+                beta_hi = inner ? b_hi / C.m(ii, j0 + u_hi, l) : 0.0;
+            }
             // each tracer's moments are loaded while the tracer before it is worked on
             double nx[4] = {0.0, 0.0, 0.0, 0.0};
-            if (C.nt > 0) {
-                for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, 1);
+            if (t0 <= t1) {
+                for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, t0);
             }
-            for (int t = 1; t <= C.nt; t++) {
+            for (int t = t0; t <= t1; t++) {
                 const double rm = nx[0], rx0 = nx[1], ry0 = nx[2], rz0 = nx[3];
-                if (t < C.nt) {
+                if (t < t1) {
                     for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, t + 1);
                 }
                 if (mine) {
@@ -698,9 +715,10 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
                 }
                 __syncthreads();  // the staged values are read before the next tracer's land
             }
+            m_cur = m_new;
         }
         __syncthreads();
-        if (mine && inner) C.m(ii, jc, l) = m_new;
+        if (mine && inner && (MASS_ONLY || tchunk == 0)) C.m(ii, jc, l) = m_new;
         __threadfence_block();
         __syncthreads();
     }
@@ -956,13 +974,18 @@ int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int 
     const int TI = (1024 / n) < 16 ? (1024 / n) : 16;
     const int ni = ie - is + 1;
     const dim3 grid((unsigned)((ni + TI - 1) / TI), (unsigned)nlay), block((unsigned)((TI * n + 63) / 64 * 64));
-    if (mass_only)
+    // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
+    const int tchunk = !mass_only && C.nt >= 32
+                           ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, C.nt, 16)
+                           : C.nt;
+    const int ranges = C.nt > 0 ? (C.nt + tchunk - 1) / tchunk : 1;
+    if (!mass_only)
+        TM5_LAUNCH(ctx, k_y_child_steps<false>, dim3(grid.x, grid.y, ranges), block,
+                   (size_t)5 * n * TI * sizeof(double), ctx->stream, C, is, ie, ls, TI, nloop, limits, ctx->words,
+                   skip_mask, ranges > 1 ? tchunk : 0);
+    if (mass_only || ranges > 1)
         TM5_LAUNCH(ctx, k_y_child_steps<true>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, 0, ctx->words,
-                   skip_mask);
-    else
-        TM5_LAUNCH(ctx, k_y_child_steps<false>, grid, block, (size_t)4 * n * TI * sizeof(double), ctx->stream, C, is,
-                   ie, ls, TI, nloop, limits,
-                   ctx->words, skip_mask);
+                   skip_mask, 0);
     const int n0 = C.im + 2, n1 = C.jm + 2;
     TM5_LAUNCH(ctx, k_y_child_restore, dim3((unsigned)((n0 * n1 + 255) / 256), (unsigned)nlay), dim3(256), 0,
                ctx->stream, C, ls, nloop, ctx->words, skip_mask);
