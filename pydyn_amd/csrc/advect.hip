@@ -24,10 +24,13 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // k-1 and k, face 0 the same as face n).  The search (kernels.py:169-199): nloop = 1, 2, ...,
 // each trial carrying every face's upwind cell through nloop mass-only sub-steps; on a failure
 // every face is scaled by (nloop-1)/nloop in place.
+// With tracer ranges (tchunk > 0, grid y) several workgroups take the same row: each searches and
+// moves the row's mass on its own, none writes m, am, rows_out or amax_out, and the caller runs
+// the mass-only form afterwards.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
-                                          const int *imred, double *amax_out, int spec_k, double *bk)
+                                          const int *imred, double *amax_out, int spec_k, double *bk, int tchunk)
 {
     // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
     // spec_k >= 0: a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
@@ -35,8 +38,13 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     // tracer's rm, rxm, rym, rzm (each im * jm * lm)
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
-    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    // rows renumbered XCD-contiguously, the tracer range slowest
+    const int b = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int row = b % gridDim.x, bz = b / gridDim.x;
     const int j = row % jm, l = row / jm;
+    const int t0 = tchunk > 0 ? bz * tchunk : 0;
+    const int t1 = tchunk > 0 && t0 + tchunk < q.nt ? t0 + tchunk : q.nt;
+    const bool writes = tchunk == 0;  // m, am, rows_out, amax_out
     const int n = imred ? imred[j] : im;  // a reduced row has imred(j) cells
     const int lane = threadIdx.x;
     double *M = lds;       // n   current mass
@@ -54,13 +62,13 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
     // the first tracer's moments of the lane's (at most two) cells, in flight during the search
     constexpr int PRE = 2;
-    const bool pre = !MASS_ONLY && q.nt > 0 && n <= 64 * PRE;
+    const bool pre = !MASS_ONLY && t0 < t1 && n <= 64 * PRE;
     double qp[PRE][4];
 #pragma unroll
     for (int c = 0; c < PRE; c++) {
         const int i = lane + 64 * c;
         if (pre && i < n) {
-            const long long o = q.off(i, j, l, 0);
+            const long long o = q.off(i, j, l, t0);
 #pragma unroll
             for (int a = 0; a < 4; a++) qp[c][a] = q.q[a][o];
         }
@@ -98,7 +106,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
                     spec_fail(words, spec_k);
                 else
                     atomicOr(words + W_FLAGS, fail_bit);
-                rows_out[row] = nl;
+                if (writes) rows_out[row] = nl;
             }
             if (!MASS_ONLY && bk)  // the row's moments are not read below
                 for (int t = 0; t < q.nt; t++)
@@ -109,8 +117,8 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
             return;
         }
     }
-    if (lane == 0) rows_out[row] = nl;  // one plain store per row: no contended atomic
-    if (amax_out) {  // the row's largest Courant number (a maximum: any order gives the same value)
+    if (lane == 0 && writes) rows_out[row] = nl;  // one plain store per row: no contended atomic
+    if (amax_out && writes) {  // the row's largest Courant number (a maximum: any order gives the same value)
         for (int off = 32; off > 0; off >>= 1) {
             const double o = __shfl_xor(cmax, off);
             if (o > cmax) cmax = o;
@@ -123,14 +131,14 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
             for (int i = lane; i < n; i += 64) M[i] = (M[i] + A[i]) - A[i + 1];
         __syncthreads();
     } else {
-        for (int t = 0; t < q.nt; t++) {
-            if (t > 0) {
+        for (int t = t0; t < t1; t++) {
+            if (t > t0) {
                 for (int i = lane; i < n; i += 64) M[i] = M0[i];
             }
             for (int i = lane; i < n; i += 64) {
                 const long long o = q.off(i, j, l, t);
                 double rm, rx, ry, rz;
-                if (pre && t == 0) {  // i = lane + 64 c with c < PRE
+                if (pre && t == t0) {  // i = lane + 64 c with c < PRE
                     const int c = (i - lane) / 64;
                     rm = c == 0 ? qp[0][0] : qp[1][0];
                     rx = c == 0 ? qp[0][1] : qp[1][1];
@@ -195,6 +203,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
             __syncthreads();
         }
     }
+    if (!writes) return;
     for (int i = lane; i < n; i += 64) m(i, j, l) = M[i];
     for (int k = lane; k <= n; k += 64) am(k, j, l) = A[k] * (double)nl;  // "restore old am"
 }
@@ -208,12 +217,20 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
     int *rows = rows_out ? rows_out : ctx->x_rows;
+    int ranges = 1, tchunk = q.nt;
+    if (q.nt >= 32 && !backup) {  // many tracers: tracer ranges over the grid, then the masses
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x<false>, 64, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), q.nt, 16);
+        ranges = (q.nt + tchunk - 1) / tchunk;
+    }
     if (q.nt > 0)
-        TM5_LAUNCH(ctx, k_x<false>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup);
-    else
+        TM5_LAUNCH(ctx, k_x<false>, dim3(grid.x, ranges), block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax,
+                   ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, ranges > 1 ? tchunk : 0);
+    if (q.nt == 0 || ranges > 1)
         TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup);
+                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0);
     return status(hipGetLastError());
 }
 

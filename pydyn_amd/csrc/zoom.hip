@@ -422,12 +422,18 @@ __global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask
 // row first, the sub-steps update the interior cells, and the faces are multiplied back.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
-                                                int fail_bit, int *words, int skip_mask)
+                                                int fail_bit, int *words, int skip_mask, int tchunk)
 {
+    // tchunk > 0: tracer ranges (grid y), several workgroups a row, each searching and moving the
+    // row's mass on its own; none writes m or am, the caller runs the mass-only form afterwards
     if (zskip(words, skip_mask)) return;
     extern __shared__ double lds[];
     const int nj = je - js + 1;
-    const int j = js + (int)(blockIdx.x % nj), l = ls + (int)(blockIdx.x / nj);
+    const int b = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);  // the range slowest
+    const int row = b % gridDim.x, bz = b / gridDim.x;
+    const int j = js + row % nj, l = ls + row / nj;
+    const int t0 = tchunk > 0 ? bz * tchunk + 1 : 1;  // Fortran tracer indices t0 .. t1
+    const int t1 = tchunk > 0 && t0 - 1 + tchunk < C.nt ? t0 - 1 + tchunk : C.nt;
     const int lane = threadIdx.x;
     const int i0 = C.xr - 1;  // Fortran i of the row's cell 0
     const int n = C.im - 2 * C.xr + 4, nf = n - 1;
@@ -486,9 +492,9 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                     for (int a = 0; a < 4; a++) qp[k][a] = C.q[a](i0 + c, j, l, t);
             }
         };
-        if (pre) fetch(1);
-        for (int t = 1; t <= C.nt; t++) {
-            if (t > 1) {
+        if (pre && t0 <= t1) fetch(t0);
+        for (int t = t0; t <= t1; t++) {
+            if (t > t0) {
                 for (int c = lane; c < n; c += 64) M[c] = M0[c];
             }
             for (int c = lane; c < n; c += 64) {
@@ -510,7 +516,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                 R[2 * n + c] = ry;
                 R[3 * n + c] = rz;
             }
-            if (pre && t < C.nt) fetch(t + 1);
+            if (pre && t < t1) fetch(t + 1);
             __syncthreads();
             for (int s = 0; s < nl; s++) {
                 for (int k = lane; k < nf; k += 64) {
@@ -547,6 +553,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
             }
             __syncthreads();
         }
+        if (tchunk > 0) return;
         for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
     }
     for (int k = lane; k < nf; k += 64) C.am(i0 + k, j, l) = A[k] * (double)nl;  // "restore old am"
@@ -949,12 +956,20 @@ int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int 
     if (n < 3 || n > 512 || je < js || le < ls) return TM5_EINVAL;  // LDS: 11 n doubles
     const size_t lds = (size_t)(2 * n + (n - 1) + 4 * n + 4 * (n - 1)) * sizeof(double);
     const dim3 grid((unsigned)((je - js + 1) * (le - ls + 1)));
-    if (mass_only)
+    int ranges = 1, tchunk = C.nt;
+    if (!mass_only && C.nt >= 32) {  // many tracers: tracer ranges over the grid, then the masses
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false>, 64, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), C.nt, 16);
+        ranges = (C.nt + tchunk - 1) / tchunk;
+    }
+    if (!mass_only)
+        TM5_LAUNCH(ctx, k_x_child<false>, dim3(grid.x, ranges), dim3(64), lds, ctx->stream, C, js, je, ls, le, limits,
+                   nmax, fail_bit, ctx->words, skip_mask, ranges > 1 ? tchunk : 0);
+    if (mass_only || ranges > 1)
         TM5_LAUNCH(ctx, k_x_child<true>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, 0, nmax, fail_bit,
-                   ctx->words, skip_mask);
-    else
-        TM5_LAUNCH(ctx, k_x_child<false>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, limits, nmax, fail_bit,
-                   ctx->words, skip_mask);
+                   ctx->words, skip_mask, 0);
     return status(hipGetLastError());
 }
 

@@ -149,6 +149,26 @@ def test_advect_z_many_tracers(dev, scale):
         assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "cm")
 
 
+@pytest.mark.parametrize("scale", [1.0, 3.0])
+def test_advect_x_many_tracers(dev, scale):
+    """dynamu on 40 tracers (tracer ranges over the rows, each range searching and moving the row's
+    mass on its own, then the masses and am once), with and without sub-cycling rows, against the
+    oracle."""
+    from oracle import advection
+
+    D, K = dev
+    m, a, q = records("x")[0]
+    a = scale * a
+    rng = np.random.default_rng(41)
+    q = [np.ascontiguousarray(np.repeat(x, 40, axis=3) * rng.uniform(0.5, 1.5, (*x.shape[:3], 40))) for x in q]
+    dm, dq, da = D.upload(m), [D.upload(x) for x in q], D.upload(a)
+    K.advect_x(dm, *dq, da, True)
+    rows = advection.advect_x(m, *q, a, True)
+    assert scale == 1.0 or int(np.max(rows)) > 1
+    for k, (g, w) in enumerate(zip([dm, *dq, da], [m, *q, a])):
+        assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "am")
+
+
 def test_cluster_sum(dev):
     D, K = dev
     rng = np.random.default_rng(4)
