@@ -91,11 +91,17 @@ def main(argv=None):
             print(f"{short[:64]:64s} {calls:6d} {avg / 1e3:9.2f} {tot / 1e3:10.1f} {pct:6.2f}")
     else:
         out = traffic(a.fetch_dir, a.write_dir, a.copy_bytes)
-        text = json.dumps(out, indent=1, sort_keys=True)
         if a.json:
             with open(a.json, "w") as f:
-                f.write(text + "\n")
-        print(text)
+                f.write(json.dumps(out, indent=1, sort_keys=True) + "\n")
+        print(f"calibration: {out['_calibration']}")
+        print(f"{'kernel':36s} {'MB/launch':>10s} {'fetch KiB':>10s} {'write KiB':>10s} {'dispatches':>10s}")
+        for k in sorted(out):
+            if k.startswith("_") or "rocclr" in k or "at::" in k:
+                continue
+            v = out[k]
+            print(f"{k[:36]:36s} {v['bytes_per_launch'] / 1e6:10.2f} {v['fetch_kib']:10.1f} {v['write_kib']:10.1f} "
+                  f"{v['dispatches']:10d}")
     return 0
 
 
