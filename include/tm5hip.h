@@ -139,7 +139,10 @@ int tm5_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const d
 
 /* ------------------------------------------------------------------ physics */
 /* convec on every column: the convection + Kzz diffusion matrix, its Crout inverse, slopes
-   times diag, rm = C rm.  pydyn/gpu/kernels.py:351 convec (pydyn/kernels.py:350-457). */
+   times diag, rm = C rm.  pydyn/gpu/kernels.py:351 convec (pydyn/kernels.py:350-457).
+   With nt >= 8 (and in tm5_zoom_physics) the inverses pass through a context-owned buffer of
+   im jm lmax^2 doubles, allocated on first use and grown (after waiting for the stream) when a
+   larger grid needs more. */
 int tm5_convec(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
                const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
                const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax);
