@@ -230,6 +230,46 @@ __global__ void __launch_bounds__(256) k_mix_edges(ZR C, RingBlocks ring, int nf
         for (int a = 0; a < C.xr; a++) v(ib * C.xr + a + 1, jb * C.yr + b + 1, l, t) = acc;
 }
 
+// The same mixing with a workgroup per (level, field): the ring's cells are read in block order
+// (a block's cells i fastest, then its rows), so neighbouring lanes take neighbouring cells of a
+// row instead of one lane a block; each block's mean is summed in LDS in the order above (j outer,
+// i inner, from zero) and written back to its cells.
+__global__ void __launch_bounds__(256) k_mix_edges_lds(ZR C, RingBlocks ring, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    extern __shared__ double mix[];  // every cell's term, then every block's mean
+    const int nb = ring.count(), per = C.xr * C.yr, ncell = nb * per;
+    const int l = (int)(blockIdx.x % C.lm) + 1, f = (int)(blockIdx.x / C.lm);
+    const HV &v = f == 0 ? C.m : C.q[(f - 1) / C.nt];
+    const int t = f == 0 ? 1 : (f - 1) % C.nt + 1;
+    const double xyr = (double)(C.xr * C.yr);
+    double *term = mix, *mean = mix + ncell;
+    auto cell = [&](int c, int &i, int &j) {  // 0-based within the region
+        int ib, jb;
+        ring.at(c / per, ib, jb);
+        const int w = c % per;
+        i = ib * C.xr + w % C.xr;
+        j = jb * C.yr + w / C.xr;
+    };
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        int i, j;
+        cell(c, i, j);
+        term[c] = C.edge[(long long)j * C.im + i] == -1 ? v(i + 1, j + 1, l, t) / xyr : 0.0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        double acc = 0.0;
+        for (int w = 0; w < per; w++) acc = acc + term[k * per + w];
+        mean[k] = acc;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        int i, j;
+        cell(c, i, j);
+        v(i + 1, j + 1, l, t) = mean[c / per];
+    }
+}
+
 // ------------------------------------------------------------------ update_parent
 // pydyn/zoom/exchange.py:235-258: the parent's m and rm over the footprint from the child's
 // block sums (from zero, i outer, j inner); f = 0 m, f = t rm of tracer t.
@@ -878,8 +918,13 @@ int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers)
     const RingBlocks ring = ring_blocks(C, xedges);
     const long long n = (long long)ring.count() * C.lm * nf;
     if (n == 0) return 0;
-    TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, ring, nf, ctx->words,
-               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    const size_t lds = (size_t)ring.count() * (C.xr * C.yr + 1) * sizeof(double);
+    if (nf >= 32 && lds <= 65536)  // many fields: a workgroup per (level, field)
+        TM5_LAUNCH(ctx, k_mix_edges_lds, dim3((unsigned)(C.lm * nf)), dim3(256), lds, ctx->stream, C, ring, ctx->words,
+                   TM5_ERRORS | TM5_REHEARSAL * !tracers);
+    else
+        TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, ring, nf, ctx->words,
+                   TM5_ERRORS | TM5_REHEARSAL * !tracers);
     return status(hipGetLastError());
 }
 
