@@ -63,10 +63,11 @@ typedef struct tm5_iview {
 #define TM5_REHEARSAL 32 /* a Check_CFL mass-only operator exceeded its cap (the verdict) */
 #define TM5_FAIL_XC 64   /* a child's dynamu: nloop too high (CFLError "dynuamu: nloop too high") */
 #define TM5_FAIL_YC 128  /* a child's dynamv: nloop too high (CFLError "nloop too high in dynamv") */
-#define TM5_ERRORS 735   /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
+#define TM5_ERRORS 1759  /* any error bit (all but TM5_REHEARSAL and TM5_SPEC) */
 #define TM5_SPEC 256     /* tm5_region_run: a queued interval's trial or checked step disagreed with its plan;
                             the kernels queued after it did nothing (cleared before tm5_region_run returns) */
 #define TM5_BAD_LEVELS 512 /* tm5_loss_rates: layers not strictly descending (ValueError) */
+#define TM5_SINGULAR 1024  /* the LAPACK kind's DGETRF / DGETRI met an exactly zero pivot (ValueError) */
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
@@ -146,6 +147,19 @@ int tm5_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const d
 int tm5_convec(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
                const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
                const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax);
+/* convec with the LAPACK kind's inverse: reference LAPACK's DGETRF (recursive DGETRF2) and DGETRI
+   (lwork = n) per column with reference BLAS loop orders -- pydyn/gpu/kernels.py:351 convec(...,
+   inverse="lapack") (pydyn/gpu/csrc/convec.cuh:17-206, pydyn/lapack.py:43-66).  mask: null (every
+   column) or as tm5_convec_masked's.  Register-form lmax only (4, 5, 8, 10, 15, 20, 25, 26);
+   the inverses pass through the context's buffer.  A zero pivot sets TM5_SINGULAR. */
+int tm5_convec_lapack(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                      const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
+                      const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax,
+                      const int *mask);
+/* The LAPACK kind's inverse alone: ncol row-major n x n device matrices A to Y (A == Y allowed),
+   n in 4, 5, 8, 10, 15, 20, 25, 26; info (device int, or null: TM5_SINGULAR) set to 1 on a zero
+   pivot.  pydyn/gpu/kernels.py:403 inverse_rows (csrc/convec.cuh pd_inverse_rows). */
+int tm5_inverse_rows(tm5_ctx *ctx, long long ncol, int n, const double *A, double *Y, int *info);
 /* convec on the columns mask (im, jm, i fastest, nonzero = act) marks: pydyn/gpu/kernels.py:351
    convec's `cols`, the flat i * jm + j list, as a mask; register form only (lmax in 4, 5, 8, 10,
    15, 20, 25, 26), else TM5_EINVAL. */

@@ -75,10 +75,11 @@ def crout_inverse(a):
     return np.moveaxis(inv, 2, 0)
 
 
-def convec(m, rm, rxm, rym, rzm, entu, detu, entd, detd, cloud_top, cloud_lfs, dkg, dt, lmax):
+def convec(m, rm, rxm, rym, rzm, entu, detu, entd, detd, cloud_top, cloud_lfs, dkg, dt, lmax, inverse="crout"):
     """Convection plus diffusion on every column of the region, in place (pydyn/kernels.py:
     423-457, on the cells a single region owns: all of them, pydyn/zoom/physics.py:494-513).
-    m (im, jm, lm), tracers (im, jm, lm, nt), the meteo (im, jm, >= lmax), cloud levels (im, jm)."""
+    m (im, jm, lm), tracers (im, jm, lm, nt), the meteo (im, jm, >= lmax), cloud levels (im, jm).
+    inverse: 'crout' (fastminv) or 'lapack' (oracle/lapack.c: DGETRF2 + DGETRI, pydyn/lapack.py)."""
     im, jm = m.shape[:2]
     ncol = im * jm
 
@@ -89,7 +90,12 @@ def convec(m, rm, rxm, rym, rzm, entu, detu, entd, detd, cloud_top, cloud_lfs, d
         col(m)[:, :lmax], col(entu), col(detu), col(entd), col(detd),
         col(cloud_top), col(cloud_lfs), col(dkg), dt, lmax,
     )  # fmt: skip
-    c = crout_inverse(conv1)
+    if inverse == "lapack":
+        from oracle import lapack
+
+        c = lapack.inverse_rows(conv1)
+    else:
+        c = crout_inverse(conv1)
     diag = c[:, np.arange(lmax), np.arange(lmax)][:, :, None]
     for s in (rxm, rym, rzm):
         sc = col(s)

@@ -11,7 +11,8 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "libtm5hip.so"
 
 FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC, SPEC = 1, 2, 4, 8, 16, 32, 64, 128, 256
 BAD_LEVELS = 512
-ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC | BAD_LEVELS
+SINGULAR = 1024
+ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC | BAD_LEVELS | SINGULAR
 EINVAL, EFLAGS = -1, -2
 
 
@@ -138,6 +139,9 @@ _SIGNATURES = {
     "tm5_cluster_sum": [_VP, _I, _I, _I, _I, _P(View), _P(View)],
     "tm5_convec_masked": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(View), _P(View), _P(View), _P(IView),
                           _P(IView), _P(View), _D, _I, _VP],
+    "tm5_convec_lapack": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(View), _P(View), _P(View), _P(IView),
+                          _P(IView), _P(View), _D, _I, _VP],
+    "tm5_inverse_rows": [_VP, ctypes.c_longlong, _I, _VP, _VP, _VP],
     "tm5_region_ops": [_VP, _D, _I, ctypes.c_char_p],
     "tm5_region_run": [_VP, _D, _I, _I, _I, _P(_I), _P(_I)],
     "tm5_advect_adjoint": [_VP, _I, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(Tracers), _I, _I, _VP],

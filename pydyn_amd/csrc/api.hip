@@ -370,6 +370,44 @@ extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m,
                          iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
 }
 
+// convec with the LAPACK-kind inverse (DGETRF2 + DGETRI per column), on every column or (mask
+// non-null) on the columns the mask marks
+extern "C" int tm5_convec_lapack(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                                 const tm5_view *entu, const tm5_view *detu, const tm5_view *entd,
+                                 const tm5_view *detd, const tm5_iview *cloud_top, const tm5_iview *cloud_lfs,
+                                 const tm5_view *dkg, double dt, int lmax, const int *mask)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(entu) || bad_view(detu) || bad_view(entd) || bad_view(detd) ||
+        bad_view(dkg) || !cloud_top || !cloud_lfs || !cloud_top->data || !cloud_lfs->data)
+        return TM5_EINVAL;
+    if (!column_supported(lmax) || lm > 64) return TM5_EINVAL;
+    ColumnArgs a{};
+    a.im = im;
+    a.jm = jm;
+    a.lm = lm;
+    a.m = view3(m);
+    a.q = tracers(q);
+    a.entu = view3(entu);
+    a.detu = view3(detu);
+    a.entd = view3(entd);
+    a.detd = view3(detd);
+    a.dkg = view3(dkg);
+    a.top = iview2(cloud_top);
+    a.lfs = iview2(cloud_lfs);
+    a.dt = dt;
+    a.owned = mask;
+    a.lapack = 1;
+    return launch_column(c, a, lmax, TM5_ERRORS);
+}
+
+// the LAPACK-kind inverse of ncol row-major n x n device matrices A into Y; info: a device int
+// that a zero pivot sets to 1 (null: TM5_SINGULAR in the flag word)
+extern "C" int tm5_inverse_rows(tm5_ctx *c, long long ncol, int n, const double *A, double *Y, int *info)
+{
+    if (!c) return TM5_EINVAL;
+    return launch_lapack_inverse(c, ncol, n, A, Y, info, TM5_ERRORS);
+}
+
 // convec on the columns an (im, jm) mask (i fastest) marks; the others keep their tracers
 extern "C" int tm5_convec_masked(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
                                  const tm5_view *entu, const tm5_view *detu, const tm5_view *entd,

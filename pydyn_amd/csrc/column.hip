@@ -406,6 +406,16 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 #pragma unroll
         for (int c = 0; c < N; c++) A[c] = Mt[r * LD + c];
     }
+    if constexpr (!ADJ) {
+        if (a.build_only) {  // the LAPACK kind: conv1's rows for k_lapack_inverse, nothing else
+            if (active && li < N) {
+                double *Cr = a.cbuf + ((long long)col * N + li) * N;
+#pragma unroll
+                for (int c = 0; c < N; c++) Cr[c] = A[c];
+            }
+            return;
+        }
+    }
 
     // ---- LU, lanes over rows.  Entry (r, c) is non-zero only inside the dense block (r, c < DB)
     // or on the three diagonals, and LU fill-in stays inside that envelope; the products skipped
@@ -756,7 +766,8 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
                          skip_mask);
         return status(hipGetLastError());
     }
-    if (args.zfused || args.q.nt < SPLIT_NT) {
+    if (args.lapack && (args.zfused || args.mid || args.pre || args.twice || args.post)) return TM5_EINVAL;
+    if (!args.lapack && (args.zfused || args.q.nt < SPLIT_NT)) {
         TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), grid, block, lds, ctx->stream, args, ctx->words,
                          skip_mask);
         return status(hipGetLastError());
@@ -778,10 +789,15 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     }
     ColumnArgs f = args;
     f.cbuf = ctx->cbuf;
+    f.build_only = args.lapack;
     TM5_LAUNCH_NAMED(ctx, "k_column_factor", (k_column<N, false>), grid, block, lds, ctx->stream, f, ctx->words,
                      skip_mask);
-    const int rc = status(hipGetLastError());
+    int rc = status(hipGetLastError());
     if (rc) return rc;
+    if (args.lapack) {  // conv1 -> C in place, DGETRF2 + DGETRI per column
+        rc = launch_lapack_inverse(ctx, ncol, N, ctx->cbuf, ctx->cbuf, nullptr, skip_mask);
+        if (rc) return rc;
+    }
     // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
     const int nt = args.q.nt;
     const int tiles = (args.im + APPLY_COLS - 1) / APPLY_COLS * args.jm;

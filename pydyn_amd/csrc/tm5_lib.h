@@ -240,6 +240,9 @@ struct ColumnArgs {
     // non-null: the factor pass of the split form (many tracers), which stores each column's rows
     // of C there ((im, jm) columns, i fastest, of N x N row-major) and leaves the tracers alone
     double *cbuf;
+    // the LAPACK kind (pydyn/lapack.py): the inverse by DGETRF2 + DGETRI (k_lapack_inverse)
+    // instead of the Crout form; build_only: the factor pass stores conv1's rows, not C's
+    int lapack, build_only;
 };
 // The tracer ranges a many-tracer kernel splits nt tracers over (grid z): enough for `tiles`
 // workgroups a range to fill the device's `slots` workgroup slots in nearly whole rounds, each
@@ -252,6 +255,9 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
 int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+// ncol row-major n x n matrices to their inverses, LAPACK kind (lapack.hip); info: a device int
+// or null (then TM5_SINGULAR in the flag word)
+int launch_lapack_inverse(tm5_ctx *ctx, long long ncol, int n, const double *A, double *Y, int *info, int skip_mask);
 int launch_libm(tm5_ctx *ctx, int fn, long long n, const double *x, const double *y, double *out);
 // The CH4 loss rates of tm5_loss_rates (physics.hip)
 struct LossArgs {

@@ -83,6 +83,8 @@ class Context:
             raise CFLError("nloop too high in dynamv")
         if f & _lib.BAD_LEVELS:
             raise ValueError("rebin: levels not strictly descending (the Fortran keeps the stale column)")
+        if f & _lib.SINGULAR:
+            raise ValueError("pydyn.gpu.convec: a singular convection matrix (dgetrf / dgetri info != 0)")
         return f
 
     def nloop(self, lm):

@@ -231,22 +231,44 @@ def convec(m, rm, rxm, rym, rzm, conv, dt, lmax, cols=None, inverse="crout"):
     """convec (pydyn/gpu/kernels.py:351, pydyn/kernels.py:423-457): conv holds device arrays
     entu, detu, entd, detd, dkg (float64) and cloud_top, cloud_lfs (int32).  cols: the columns
     to act on (device int32, flat i * jm + j; None: every column).  inverse: 'crout' (fastminv,
-    the reference's default); the LAPACK kind is not provided."""
-    if inverse != "crout":
-        raise ValueError(f"convec: inverse {inverse!r}: only 'crout' (fastminv) is provided")
+    the reference's default) or 'lapack' (reference LAPACK's DGETRF + DGETRI per column, the
+    with_lapack build; a singular matrix raises ValueError)."""
+    if inverse not in ("crout", "lapack"):
+        raise ValueError(f"pydyn.gpu.convec: inverse {inverse!r}: 'crout' or 'lapack' (an open one)")
     torch = D._torch()
     ctx = D.context()
     im, jm, lm = _dims(m)
     args = (ctx.ptr, im, jm, lm, D.view(m), _tr(rm, rxm, rym, rzm),
             *(D.view(conv[k]) for k in ("entu", "detu", "entd", "detd")),
             D.iview(conv["cloud_top"]), D.iview(conv["cloud_lfs"]), D.view(conv["dkg"]), float(dt), int(lmax))  # fmt: skip
-    if cols is None:
+    mask = None
+    if cols is not None:
+        flat = cols.to(torch.int64).reshape(-1)
+        mask = torch.zeros(jm, im, dtype=torch.int32, device="cuda")  # (j, i): i fastest
+        mask[flat % jm, flat // jm] = 1
+    if inverse == "lapack":
+        _lib.check(_lib.lib().tm5_convec_lapack(*args, None if mask is None else mask.data_ptr()), "tm5_convec_lapack")
+        ctx.raise_errors()  # a zero pivot (the reference checks its info flag here)
+        return
+    if mask is None:
         _lib.check(_lib.lib().tm5_convec(*args), "tm5_convec")
         return
-    flat = cols.to(torch.int64).reshape(-1)
-    mask = torch.zeros(jm, im, dtype=torch.int32, device="cuda")  # (j, i): i fastest
-    mask[flat % jm, flat // jm] = 1
     _lib.check(_lib.lib().tm5_convec_masked(*args, mask.data_ptr()), "tm5_convec_masked")
+
+
+def inverse_rows(A):
+    """The LAPACK kind's inverse of each row-major n x n matrix of the device array A
+    (ncol, n, n), C-contiguous: (Y, info), info a one-element device int32 array set to 1 when
+    a pivot is exactly zero (pydyn/gpu/kernels.py:403-411)."""
+    torch = D._torch()
+    A = A.contiguous()
+    ncol, n, _ = A.shape
+    Y = torch.empty_like(A)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx = D.context()
+    _lib.check(_lib.lib().tm5_inverse_rows(ctx.ptr, ncol, n, A.data_ptr(), Y.data_ptr(), info.data_ptr()),
+               "tm5_inverse_rows")  # fmt: skip
+    return Y, info
 
 
 def dkg(kvh, m, gph, lmax):
