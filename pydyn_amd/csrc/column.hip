@@ -654,10 +654,14 @@ __global__ void __launch_bounds__(APPLY_COLS * 32, 4)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int h = lane >> 5, li = lane & 31, lm = a.lm;
     const int t0 = bz * tchunk, nt = a.q.nt - t0 < tchunk ? a.q.nt - t0 : tchunk;  // tracers t0 .. t0 + nt - 1
-    int i = bx * APPLY_COLS + wave * 2 + h;
+    // the row's tiles start on 64-byte boundaries of the tracer arrays (every level and tracer of
+    // the row has the row's offset modulo 8 doubles when those strides are multiples of 8), so
+    // neighbouring tiles do not share the sectors at their edges
     const int j = by;
-    const bool active = i < a.im;
-    if (!active) i = a.im - 1;
+    const int e = (a.q.sl % 8 == 0 && a.q.st % 8 == 0) ? (int)(((uintptr_t)(a.q.q[0] + a.q.off(0, j, 0, 0)) >> 3) & 7) : 0;
+    int i = bx * APPLY_COLS - e + wave * 2 + h;
+    const bool active = i >= 0 && i < a.im;
+    if (!active) i = i < 0 ? 0 : a.im - 1;
     const bool store = active && (a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0);
     double *rv = rvs + (wave * 2 + h) * 32;
     const double *Cr = cbuf + ((long long)j * a.im + i) * (N * N) + (long long)(li < N ? li : N - 1) * N;
@@ -800,9 +804,9 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     }
     // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
     const int nt = args.q.nt;
-    const int tiles = (args.im + APPLY_COLS - 1) / APPLY_COLS * args.jm;
+    const int tiles = (args.im + 7 + APPLY_COLS - 1) / APPLY_COLS * args.jm;
     const int tchunk = tracer_chunk(tiles, 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256), nt, 16);
-    const dim3 agrid((args.im + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
+    const dim3 agrid((args.im + 7 + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
     const dim3 ablock(APPLY_COLS * 32);
     TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, ctx->cbuf, tchunk,
                      ctx->words, skip_mask);
