@@ -804,6 +804,7 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     }
     // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
     const int nt = args.q.nt;
+    if (nt < 1) return 0;  // no tracers: the matrices only
     const int tiles = (args.im + 7 + APPLY_COLS - 1) / APPLY_COLS * args.jm;
     const int tchunk = tracer_chunk(tiles, 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256), nt, 16);
     const dim3 agrid((args.im + 7 + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
