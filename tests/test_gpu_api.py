@@ -290,3 +290,26 @@ def test_advect_y_on_region1_halo(dev):
     for k, (g, w) in enumerate(zip([dm, *dq], [m, *q])):
         assert_bitwise(D.download(g), w, FIELDS[k])
     assert_bitwise(D.download(dfull), full, "full bm")
+
+
+def test_region_middle_split_form_equals_the_fused_step(dev):
+    """12 tracers on the small grid: an interval through the region driver (z fused into the
+    one-kernel column form) and the same steps operator by operator ('V', the column middle
+    without z, in the split form: the inverses through the context's buffer, then the tracers
+    in 16-column workgroups), bit for bit."""
+    import ctypes
+
+    from pydyn_amd import _lib, meteo
+    from pydyn_amd.region import Region
+
+    D, K = dev
+    met = meteo.Meteo(meteo.grid("small"), seed=3, nt=12)
+    fused, ops = Region(met), Region(met)
+    ndyn, n = fused.interval()
+    for s in range(n):
+        for code, t in (("D", float(s * ndyn) + ndyn / 2.0), ("xyzVzyx", 0.0), ("M", 0.0)):
+            _lib.check(_lib.lib().tm5_region_ops(ops._ptr, ctypes.c_double(t), int(ndyn), code.encode()), "tm5_region_ops")
+    fused.check()
+    ops.check()
+    for k in ("m", "rm", "rxm", "rym", "rzm", "sp", "phlb"):
+        assert_bitwise(D.download(getattr(ops, k)), D.download(getattr(fused, k)), k)
