@@ -169,6 +169,26 @@ def test_advect_x_many_tracers(dev, scale):
         assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "am")
 
 
+@pytest.mark.parametrize("scale", [1.0, 3.0])
+def test_advect_y_many_tracers(dev, scale):
+    """dynamv on 40 tracers (tracer ranges over the grid, each range moving the line masses
+    through the sub-steps in LDS, then the masses and the bm restore once), with and without
+    sub-cycling layers, against the oracle."""
+    from oracle import advection
+
+    D, K = dev
+    m, b, q = records("y")[0]
+    b = scale * b
+    rng = np.random.default_rng(42)
+    q = [np.ascontiguousarray(np.repeat(x, 40, axis=3) * rng.uniform(0.5, 1.5, (*x.shape[:3], 40))) for x in q]
+    dm, dq, db = D.upload(m), [D.upload(x) for x in q], D.upload(b)
+    K.advect_y(dm, *dq, db)
+    layers = advection.advect_y(m, *q, b, True)
+    assert scale == 1.0 or int(np.max(layers)) > 1
+    for k, (g, w) in enumerate(zip([dm, *dq, db], [m, *q, b])):
+        assert_bitwise(D.download(g), w, FIELDS[k] if k < 5 else "bm")
+
+
 def test_cluster_sum(dev):
     D, K = dev
     rng = np.random.default_rng(4)
