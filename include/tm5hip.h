@@ -71,6 +71,7 @@ typedef struct tm5_iview {
 
 #define TM5_EINVAL (-1)  /* bad arguments (shapes beyond the kernels' limits, null pointers) */
 #define TM5_EFLAGS (-2)  /* the region driver stopped: an error bit is set in the flag word */
+#define TM5_ESCRATCH (-3) /* tm5_convec_noalloc: one row of columns needs more than its static buffer holds */
 
 /* ------------------------------------------------------------------ context */
 /* A context: the stream every call is queued on (0 = the default stream) and the device scratch
@@ -156,6 +157,19 @@ int tm5_convec_lapack(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, c
                       const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
                       const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt, int lmax,
                       const int *mask);
+/* tm5_convec that allocates nothing.  No counterpart in the reference: it is for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same arguments, same bits as tm5_convec.  With nt >= 8 tm5_convec
+   keeps every column's inverse in a context buffer it allocates and grows; this one uses a
+   static device buffer of TM5_NOALLOC_DOUBLES doubles (32 MiB, one per GPU, held until the
+   process ends) and runs the columns in bands of whole j rows that fit it.  A row of im columns
+   needing more than the buffer (im lmax^2 > TM5_NOALLOC_DOUBLES) is TM5_ESCRATCH.  Every call on
+   one GPU shares that buffer: calls that overlap on one GPU (two streams, two threads) race. */
+#define TM5_NOALLOC_DOUBLES (4ull << 20)
+int tm5_convec_noalloc(tm5_ctx *ctx, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                       const tm5_view *entu, const tm5_view *detu, const tm5_view *entd, const tm5_view *detd,
+                       const tm5_iview *cloud_top, const tm5_iview *cloud_lfs, const tm5_view *dkg, double dt,
+                       int lmax);
 /* The LAPACK kind's inverse alone: ncol row-major n x n device matrices A to Y (A == Y allowed),
    n in 4, 5, 8, 10, 15, 20, 25, 26; info (device int, or null: TM5_SINGULAR) set to 1 on a zero
    pivot.  pydyn/gpu/kernels.py:403 inverse_rows (csrc/convec.cuh pd_inverse_rows). */

@@ -13,7 +13,8 @@ FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC, SPEC = 1, 2, 
 BAD_LEVELS = 512
 SINGULAR = 1024
 ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC | BAD_LEVELS | SINGULAR
-EINVAL, EFLAGS = -1, -2
+EINVAL, EFLAGS, ESCRATCH = -1, -2, -3
+NOALLOC_DOUBLES = 4 << 20  # TM5_NOALLOC_DOUBLES: tm5_convec_noalloc's static buffer
 
 
 class View(ctypes.Structure):
@@ -137,6 +138,8 @@ _SIGNATURES = {
     "tm5_mass_x_rows": [_VP, _I, _I, _I, _P(View), _P(View), _VP, _I],
     "tm5_advect_z_substeps": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _I, _I, _I],
     "tm5_cluster_sum": [_VP, _I, _I, _I, _I, _P(View), _P(View)],
+    "tm5_convec_noalloc": [_VP, _I, _I, _I, _P(View), _P(Tracers), *([_P(View)] * 4), _P(IView), _P(IView), _P(View),
+                           _D, _I],
     "tm5_convec_masked": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(View), _P(View), _P(View), _P(IView),
                           _P(IView), _P(View), _D, _I, _VP],
     "tm5_convec_lapack": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(View), _P(View), _P(View), _P(View), _P(IView),
@@ -190,5 +193,8 @@ def lib():
 
 
 def check(rc, what):
+    if rc == ESCRATCH:
+        raise ValueError(f"libtm5hip: {what}: one row of columns needs more than the static buffer's "
+                         f"{NOALLOC_DOUBLES} doubles (im lmax^2 too large)")
     if rc != 0:
         raise RuntimeError(f"libtm5hip: {what} failed with code {rc}")

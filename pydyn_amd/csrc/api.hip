@@ -4,9 +4,39 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "tm5_lib.h"
+
+// tm5_convec_noalloc's buffer: one copy in every GPU's memory while the library is loaded.  Host
+// code never passes `scratch` to a kernel or a copy (in host code the name is a host-side stand-in,
+// not a device address): its device address comes from hipGetSymbolAddress, once per device, with
+// that device current, and every use goes through that address.
+static __device__ __align__(256) double scratch[TM5_NOALLOC_DOUBLES];
+static constexpr int MAX_DEVICES = 64;
+static double *g_scratch[MAX_DEVICES];  // the device address of `scratch` per device (null: not yet)
+static std::mutex g_scratch_mu;
+
+// the address of the current device's `scratch` (device: the current device)
+static hipError_t scratch_address(int device, double **out)
+{
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    try {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch[device]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return e;
+            if (!p) return hipErrorInvalidSymbol;
+            g_scratch[device] = (double *)p;
+        }
+        *out = g_scratch[device];
+    } catch (...) {  // the mutex could not be locked: nothing may throw across the C ABI
+        return hipErrorUnknown;
+    }
+    return hipSuccess;
+}
 
 // events of the launches timed while profiling, and the totals of the last profile_end
 struct tm5_prof {
@@ -146,6 +176,7 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     if (e == hipSuccess) e = hipMalloc((void **)&c->x_rows, c->x_cap * sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&c->z_ok, c->z_cap * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hwords, W_WORDS * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = scratch_address(c->device, &c->noalloc);  // the device is current here
     if (e != hipSuccess) {
         tm5_ctx_destroy(c);
         return (int)e;
@@ -368,6 +399,51 @@ extern "C" int tm5_convec(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m,
     }
     return launch_convec(c, im, jm, lm, view3(m), tracers(q), view3(entu), view3(detu), view3(entd), view3(detd),
                          iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
+}
+
+// tm5_convec with the split form's matrices in the static `scratch` (through its device address
+// c->noalloc), band by band of whole j rows: a column's convec reads and writes its own column
+// only, so bands are independent.  Nothing is allocated.
+extern "C" int tm5_convec_noalloc(tm5_ctx *c, int im, int jm, int lm, const tm5_view *m, const tm5_tracers *q,
+                                  const tm5_view *entu, const tm5_view *detu, const tm5_view *entd,
+                                  const tm5_view *detd, const tm5_iview *cloud_top, const tm5_iview *cloud_lfs,
+                                  const tm5_view *dkg, double dt, int lmax)
+{
+    if (!c || bad_view(m) || bad_tracers(q) || bad_view(entu) || bad_view(detu) || bad_view(entd) || bad_view(detd) ||
+        bad_view(dkg) || !cloud_top || !cloud_lfs || !cloud_top->data || !cloud_lfs->data || im < 1 || jm < 1)
+        return TM5_EINVAL;
+    if (!column_supported(lmax) || lm > 64)  // the LDS form: no buffer
+        return launch_convec(c, im, jm, lm, view3(m), tracers(q), view3(entu), view3(detu), view3(entd), view3(detd),
+                             iview2(cloud_top), iview2(cloud_lfs), view3(dkg), dt, lmax);
+    if (!c->noalloc) return TM5_EINVAL;
+    ColumnArgs a{};
+    a.im = im;
+    a.lm = lm;
+    a.dt = dt;
+    const View3 vm = view3(m), ve[5] = {view3(entu), view3(detu), view3(entd), view3(detd), view3(dkg)};
+    const Tracers vq = tracers(q);
+    const IView2 vt = iview2(cloud_top), vl = iview2(cloud_lfs);
+    if (vq.nt < column_split_nt()) {  // the one-kernel form: no buffer at all
+        a.jm = jm;
+        a.m = vm, a.q = vq, a.entu = ve[0], a.detu = ve[1], a.entd = ve[2], a.detd = ve[3], a.dkg = ve[4];
+        a.top = vt, a.lfs = vl;
+        return launch_column(c, a, lmax, TM5_ERRORS);
+    }
+    const size_t per_row = (size_t)im * column_matrix_doubles(lmax);
+    if (per_row > TM5_NOALLOC_DOUBLES) return TM5_ESCRATCH;
+    const int band = (int)(TM5_NOALLOC_DOUBLES / per_row < (size_t)jm ? TM5_NOALLOC_DOUBLES / per_row : (size_t)jm);
+    for (int j0 = 0; j0 < jm; j0 += band) {
+        a.jm = jm - j0 < band ? jm - j0 : band;
+        a.m = View3{vm.p + (long long)j0 * vm.sj, vm.si, vm.sj, vm.sl};
+        View3 *dst[5] = {&a.entu, &a.detu, &a.entd, &a.detd, &a.dkg};
+        for (int k = 0; k < 5; k++) *dst[k] = View3{ve[k].p + (long long)j0 * ve[k].sj, ve[k].si, ve[k].sj, ve[k].sl};
+        a.q = vq;
+        for (int k = 0; k < 4; k++) a.q.q[k] = vq.q[k] + (long long)j0 * vq.sj;
+        a.top = IView2{vt.p + (long long)j0 * vt.sj, vt.si, vt.sj};
+        a.lfs = IView2{vl.p + (long long)j0 * vl.sj, vl.si, vl.sj};
+        TRY(launch_column(c, a, lmax, TM5_ERRORS, c->noalloc, (size_t)TM5_NOALLOC_DOUBLES));
+    }
+    return 0;
 }
 
 // convec with the LAPACK-kind inverse (DGETRF2 + DGETRI per column), on every column or (mask

@@ -751,8 +751,10 @@ int tracer_chunk(long long tiles, long long slots, int nt, int min_chunk)
     return (nt + ranges - 1) / ranges;
 }
 
+// ext (non-null): the split form's matrices go to ext (ext_n doubles) instead of the context's
+// buffer, which is then neither allocated nor grown; too small an ext is TM5_ESCRATCH
 template <int N>
-static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
+static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, double *ext, size_t ext_n)
 {
     if (args.zfused && (args.adjoint || args.owned || !args.cm.p)) return TM5_EINVAL;  // a region's step only
     const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
@@ -778,7 +780,8 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     }
     // the split form: the matrices of every column into the context's buffer, then the tracers
     const size_t need = (size_t)ncol * N * N;
-    if (ctx->cbuf_n < need) {
+    if (ext && ext_n < need) return TM5_ESCRATCH;
+    if (!ext && ctx->cbuf_n < need) {
         hipError_t e = hipSuccess;
         if (ctx->cbuf) {
             e = hipStreamSynchronize(ctx->stream);  // earlier launches may still read it
@@ -791,15 +794,16 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
         if (e != hipSuccess) return status(e);
         ctx->cbuf_n = need;
     }
+    double *buf = ext ? ext : ctx->cbuf;
     ColumnArgs f = args;
-    f.cbuf = ctx->cbuf;
+    f.cbuf = buf;
     f.build_only = args.lapack;
     TM5_LAUNCH_NAMED(ctx, "k_column_factor", (k_column<N, false>), grid, block, lds, ctx->stream, f, ctx->words,
                      skip_mask);
     int rc = status(hipGetLastError());
     if (rc) return rc;
     if (args.lapack) {  // conv1 -> C in place, DGETRF2 + DGETRI per column
-        rc = launch_lapack_inverse(ctx, ncol, N, ctx->cbuf, ctx->cbuf, nullptr, skip_mask);
+        rc = launch_lapack_inverse(ctx, ncol, N, buf, buf, nullptr, skip_mask);
         if (rc) return rc;
     }
     // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
@@ -809,10 +813,16 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask)
     const int tchunk = tracer_chunk(tiles, 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256), nt, 16);
     const dim3 agrid((args.im + 7 + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
     const dim3 ablock(APPLY_COLS * 32);
-    TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, ctx->cbuf, tchunk,
+    TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, buf, tchunk,
                      ctx->words, skip_mask);
     return status(hipGetLastError());
 }
+
+// the split form's working set, in doubles, for lmax N: N x N a column
+size_t column_matrix_doubles(int lmax) { return (size_t)lmax * (size_t)lmax; }
+
+// the split form from this many tracers on; below it the one-kernel form, which needs no buffer
+int column_split_nt() { return SPLIT_NT; }
 
 // the lmax values compiled in register form (two columns a wave, four waves a workgroup); others
 // go through the LDS kernel (convec.hip)
@@ -826,18 +836,18 @@ bool column_supported(int n)
     }
 }
 
-int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask)
+int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask, double *ext, size_t ext_n)
 {
     if (args.lm > 64 || lmax > args.lm) return TM5_EINVAL;
     switch (lmax) {
-    case 4: return launch_column_n<4>(ctx, args, skip_mask);
-    case 5: return launch_column_n<5>(ctx, args, skip_mask);
-    case 8: return launch_column_n<8>(ctx, args, skip_mask);
-    case 10: return launch_column_n<10>(ctx, args, skip_mask);
-    case 15: return launch_column_n<15>(ctx, args, skip_mask);
-    case 20: return launch_column_n<20>(ctx, args, skip_mask);
-    case 25: return launch_column_n<25>(ctx, args, skip_mask);
-    case 26: return launch_column_n<26>(ctx, args, skip_mask);
+    case 4: return launch_column_n<4>(ctx, args, skip_mask, ext, ext_n);
+    case 5: return launch_column_n<5>(ctx, args, skip_mask, ext, ext_n);
+    case 8: return launch_column_n<8>(ctx, args, skip_mask, ext, ext_n);
+    case 10: return launch_column_n<10>(ctx, args, skip_mask, ext, ext_n);
+    case 15: return launch_column_n<15>(ctx, args, skip_mask, ext, ext_n);
+    case 20: return launch_column_n<20>(ctx, args, skip_mask, ext, ext_n);
+    case 25: return launch_column_n<25>(ctx, args, skip_mask, ext, ext_n);
+    case 26: return launch_column_n<26>(ctx, args, skip_mask, ext, ext_n);
     default: return TM5_EINVAL;
     }
 }

@@ -39,6 +39,7 @@ struct tm5_ctx {
     size_t cbuf_n;  // its length in doubles
     tm5_prof *prof; // non-null while profiling (then prof == prof_store)
     tm5_prof *prof_store;  // owned: the events and the last totals
+    double *noalloc;       // tm5_convec_noalloc's static device buffer on this context's device
 };
 
 namespace tm5 {
@@ -254,7 +255,12 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
-int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask);
+// ext (non-null): the split form's buffer of ext_n doubles in place of the context's (nothing is
+// allocated; too small is TM5_ESCRATCH)
+int launch_column(tm5_ctx *ctx, const ColumnArgs &args, int lmax, int skip_mask, double *ext = nullptr,
+                  size_t ext_n = 0);
+size_t column_matrix_doubles(int lmax);  // the split form's buffer per column
+int column_split_nt();                   // the split form from this many tracers on
 // ncol row-major n x n matrices to their inverses, LAPACK kind (lapack.hip); info: a device int
 // or null (then TM5_SINGULAR in the flag word)
 int launch_lapack_inverse(tm5_ctx *ctx, long long ncol, int n, const double *A, double *Y, int *info, int skip_mask);
