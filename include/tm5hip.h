@@ -391,6 +391,54 @@ typedef struct tm5_zphysics {
 
 int tm5_zoom_physics(tm5_ctx *ctx, const tm5_zregion *r, const tm5_zphysics *p, const char *ops);
 
+/* ------------------------------------------------------------------ the zoom tree's native driver */
+/* A zoom tree resident on the device, run interval by interval by a native driver: the
+   reference's zoom driver (pydyn/zoom/steps.py:15-88 do_steps / proces_region / step,
+   pydyn/zoom/massflow.py:62-193 Setup_MassFlow with coarsen_region, pydyn/zoom/check_cfl.py:
+   228-290 Check_CFL of the tree) over the bodies above, with every region's convection, sources
+   (its emission, factor month2dt of the half step ndyn / (2 tref), into `tracer` at `level`, or
+   every tracer's with escale) and chemistry (its decay over the half step) on the cells it owns;
+   the fluxes of a step are the regions' steady pu0, pv0 times the meteo's amplitude a(t) at the
+   step's middle (also stored into each region's pu, pv, as the reference stores the interpolated
+   meteo).  The regions' arrays must be contiguous with i fastest in their declared bounds. */
+typedef struct tm5_tree_desc {
+    int nregions;
+    const tm5_zregion *regions;   /* nregions, region 1 (the globe) first */
+    const int *parent;            /* per region: its parent's 1-based index (0 for region 1) */
+    const tm5_zphysics *physics;  /* per region: the column-physics inputs (dt_conv, factor, dt_chem set per step) */
+    const tm5_view *pu0, *pv0;    /* per region: pu (0:im, 1:jm, 1:lm) and pv (1:im, 1:jm+1, 1:lm) at amplitude 1 */
+    const double *at, *bt;        /* device, lm + 1 */
+    double mod_period, mod_amplitude, sec_month;
+    int limits;
+} tm5_tree_desc;
+
+typedef struct tm5_tree tm5_tree;
+
+#define TM5_ENOMEM (-4)  /* host memory exhausted (nothing throws across the ABI) */
+
+int tm5_tree_create(tm5_ctx *ctx, const tm5_tree_desc *desc, tm5_tree **out);
+int tm5_tree_destroy(tm5_tree *tree);
+/* One interval of nread seconds from t0 the reference's way: Check_CFL (the host reads its verdict
+   once per attempt), then its n steps of ndyn; returns the (ndyn, n) Check_CFL chose. */
+int tm5_tree_interval(tm5_tree *tree, double t0, int nread, int ndyn_max, int *ndyn, int *n);
+/* count intervals from t0, as count calls of tm5_tree_interval would run them, queued: each on the
+   previous interval's step, the trials that plan expects to fail rehearsed and verified on the
+   device, the planned steps run as checked steps (the rehearsal's tests made inside the step's own
+   kernels: x rows at nloop <= 9, y and z faces at nloop 1, so no search runs) with every region's
+   m and moments backed up at the interval's start, the host reading the device once per batch.
+   A step of a given ndyn is captured into a HIP graph the second time it is queued and replayed
+   after that (its time comes from a device clock).  A miss restores the interval's state from the
+   backup and redoes it through tm5_tree_interval.  The reference's queued, graph-replayed
+   submission is Pipeline.submit (pydyn/gpu/zoom.py:944-1041). */
+int tm5_tree_run(tm5_tree *tree, double t0, int nread, int ndyn_max, int count, int *ndyn, int *n);
+/* Options: TM5_TREE_GRAPHS (1: capture and replay steps, the default; 0: issue every launch),
+   TM5_TREE_BATCH (most intervals queued per read of the device, 1..64; 16). */
+#define TM5_TREE_GRAPHS 0
+#define TM5_TREE_BATCH 1
+int tm5_tree_option(tm5_tree *tree, int which, int value);
+/* (queued intervals kept, redone, steps replayed from graphs, graphs captured) since creation */
+int tm5_tree_tally(tm5_tree *tree, long long *kept, long long *redone, long long *replayed, long long *captured);
+
 /* ------------------------------------------------------------------ adjoints */
 /* The adjoint of an advection operator's tracer map (axis 0 dynamu, 1 dynamv, 2 dynamw) at the
    forward trajectory: m, q, flux the operator's forward inputs (unchanged), lam the lambdas after

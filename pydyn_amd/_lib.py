@@ -13,7 +13,7 @@ FAIL_X, FAIL_Y, FAIL_Z, BAD_Y, BAD_Z, REHEARSAL, FAIL_XC, FAIL_YC, SPEC = 1, 2, 
 BAD_LEVELS = 512
 SINGULAR = 1024
 ERRORS = FAIL_X | FAIL_Y | FAIL_Z | BAD_Y | BAD_Z | FAIL_XC | FAIL_YC | BAD_LEVELS | SINGULAR
-EINVAL, EFLAGS, ESCRATCH = -1, -2, -3
+EINVAL, EFLAGS, ESCRATCH, ENOMEM = -1, -2, -3, -4
 NOALLOC_DOUBLES = 4 << 20  # TM5_NOALLOC_DOUBLES: tm5_convec_noalloc's static buffer
 
 
@@ -100,6 +100,25 @@ class RegionDesc(ctypes.Structure):
     ]
 
 
+class TreeDesc(ctypes.Structure):
+    _fields_ = [
+        ("nregions", ctypes.c_int),
+        ("regions", ctypes.POINTER(ZRegion)),
+        ("parent", ctypes.POINTER(ctypes.c_int)),
+        ("physics", ctypes.POINTER(ZPhysics)),
+        ("pu0", ctypes.POINTER(View)),
+        ("pv0", ctypes.POINTER(View)),
+        ("at", ctypes.c_void_p),
+        ("bt", ctypes.c_void_p),
+        ("mod_period", ctypes.c_double),
+        ("mod_amplitude", ctypes.c_double),
+        ("sec_month", ctypes.c_double),
+        ("limits", ctypes.c_int),
+    ]
+
+
+TREE_GRAPHS, TREE_BATCH = 0, 1  # tm5_tree_option
+
 _P = ctypes.POINTER
 _I, _D, _VP = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 _SIGNATURES = {
@@ -170,6 +189,12 @@ _SIGNATURES = {
     "tm5_zoom_decay": [_VP, _VP, _P(View), _I, _I],
     "tm5_column_adjoint": [_VP, _I, _I, _I, _P(View), _P(Tracers), _P(ZPhysics), ctypes.c_char_p, _P(View)],
     "tm5_zoom_scale": [_VP, _VP, ctypes.c_int64, _D],
+    "tm5_tree_create": [_VP, _P(TreeDesc), _P(_VP)],
+    "tm5_tree_destroy": [_VP],
+    "tm5_tree_interval": [_VP, _D, _I, _I, _P(_I), _P(_I)],
+    "tm5_tree_run": [_VP, _D, _I, _I, _I, _P(_I), _P(_I)],
+    "tm5_tree_option": [_VP, _I, _I],
+    "tm5_tree_tally": [_VP, *([_P(ctypes.c_longlong)] * 4)],
 }
 
 _STATE = {}
@@ -196,5 +221,7 @@ def check(rc, what):
     if rc == ESCRATCH:
         raise ValueError(f"libtm5hip: {what}: one row of columns needs more than the static buffer's "
                          f"{NOALLOC_DOUBLES} doubles (im lmax^2 too large)")
+    if rc == ENOMEM:
+        raise MemoryError(f"libtm5hip: {what}: host memory exhausted")
     if rc != 0:
         raise RuntimeError(f"libtm5hip: {what} failed with code {rc}")

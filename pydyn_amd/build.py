@@ -17,7 +17,8 @@ PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libtm5hip.so"
-SOURCES = ["advect.hip", "convec.hip", "column.hip", "lapack.hip", "physics.hip", "zoom.hip", "adjoint.hip", "api.hip"]
+SOURCES = ["advect.hip", "convec.hip", "column.hip", "lapack.hip", "physics.hip", "zoom.hip", "tree.hip", "adjoint.hip",
+           "api.hip"]
 ARCH = os.environ.get("TM5_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

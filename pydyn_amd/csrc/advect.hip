@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
                                           const int *imred, double *amax_out, int spec_k, double *bk, int tchunk)
 {
     // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
-    // spec_k >= 0: a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
+    // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
     // bk (whole rows only): the row's m and moments as read, i fastest, m first and then every
     // tracer's rm, rxm, rym, rzm (each im * jm * lm)
     if (skipped(words, skip_mask)) return;
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         __syncthreads();
         if (nl > nmax) {
             if (lane == 0) {
-                if (spec_k >= 0)
+                if (checked(spec_k))
                     spec_fail(words, spec_k);
                 else
                     atomicOr(words + W_FLAGS, fail_bit);
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(1024) k_y_steps(int im, int jm, int lm, int TI
     const bool mine = j < jm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, jj = mine ? j : 0;
     const int nl = nloop ? nloop[l] : 1;  // no table: every layer at nloop 1
-    if (spec_k >= 0) {
+    if (checked(spec_k)) {
         bool ok = true, bad = false;
         if (mine) {
             const double b_lo = bm(i, jj, l);
@@ -485,7 +485,7 @@ static dim3 line_block(int TI, int len) { return dim3((unsigned)(((TI * len) + 6
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
              int skip_mask, bool known_one, int spec_k)
 {
-    if (spec_k >= 0) known_one = true;
+    if (checked(spec_k)) known_one = true;
     if (im < 1 || jm < 2 || lm < 1 || lm > ctx->y_cap) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
     // many tracers: the widest tile, each tracer's rows read in the longest runs along i (the
@@ -642,11 +642,12 @@ __global__ void __launch_bounds__(256) k_z_search(int im, int jm, int lm, View3 
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI, View3 m, Tracers q, View3 cm,
                                                   int limits, int prelimit, int *words, int skip_mask, int fixed_n,
-                                                  int spec_k, int tchunk)
+                                                  int spec_k, int tchunk, ZScope sc)
 {
+    // sc: the columns the sub-steps move (the grid's columns outside it are only tested, when checked)
     // prelimit: dynamw's limiter on rzm before the first sub-step (advect_z's, not the bare sub-steps')
     // fixed_n > 0: n_advz given by the caller, else the search's (the context's word)
-    // spec_k >= 0 (fixed_n 1): a checked step's dynamw, first testing its faces as k_z_mass1 does
+    // checked(spec_k) (fixed_n 1): a checked step's dynamw, first testing its faces as k_z_mass1 does
     if (skipped(words, skip_mask)) return;
     extern __shared__ double zst[];  // 4 lm TI: one tracer's moments of the workgroup's cells; lm TI masses
     // tiles renumbered XCD-contiguously (as xcd_grid2), the tracer range slowest
@@ -660,7 +661,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
     const bool mine = l < lm && i0 + ti < im;
     const int i = mine ? i0 + ti : i0, ll = mine ? l : 0;
     const int nz = fixed_n > 0 ? fixed_n : words[W_Z_NADV];
-    if (spec_k >= 0) {
+    if (checked(spec_k)) {
         bool ok = true, bad = false;
         if (mine) {
             if (ll >= 1) {
@@ -676,6 +677,8 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
             if (any_fail) spec_fail(words, spec_k);
         }
     }
+    if (j < sc.j0 || j > sc.j1 || i0 > sc.i1 || i0 + TI - 1 < sc.i0) return;  // a workgroup only tested
+    const bool act = i0 + ti >= sc.i0 && i0 + ti <= sc.i1;
     const double dn = (double)nz;
     const double c_lo = cm(i, j, ll) / dn, c_hi = cm(i, j, ll + 1) / dn;
     // each face's upwind cell (the closed ends carry nothing)
@@ -731,7 +734,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
                 const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
                 double rz = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
                 if (limits) rz = limit(rz, rm_new);
-                if (mine) {
+                if (mine && act) {
                     q.q[0][o] = rm_new;
                     q.q[1][o] = (rx0 + lo.ft) - hi.ft;
                     q.q[2][o] = (ry0 + lo.fw) - hi.fw;
@@ -742,7 +745,7 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
             m_cur = m_new;
         }
         __syncthreads();
-        if (mine && (MASS_ONLY || tchunk == 0)) m(i, j, ll) = m_new;
+        if (mine && act && (MASS_ONLY || tchunk == 0)) m(i, j, ll) = m_new;
         __threadfence_block();
         __syncthreads();
     }
@@ -848,7 +851,7 @@ int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask, bool known_one, int spec_k)
 {
-    if (spec_k >= 0) known_one = true;
+    if (checked(spec_k)) known_one = true;
     if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
     if (i0 < 0 || j0 < 0 || i1 >= im || j1 >= jm || i1 < i0 || j1 < j0) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
@@ -864,15 +867,19 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
         TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, scope, nmax,
                    ctx->z_ok, ctx->words, fail_bit, skip_mask);
     const int fixed_n = known_one ? 1 : 0;
-    // the scope's views
-    const int si = i1 - i0 + 1, sj = j1 - j0 + 1;
-    View3 ms{m.p + i0 * m.si + j0 * m.sj, m.si, m.sj, m.sl}, cs{cm.p + i0 * cm.si + j0 * cm.sj, cm.si, cm.sj, cm.sl};
+    // the scope's views, or -- a checked step on a scope smaller than the region, whose test covers
+    // every column of the region as the rehearsal's trial does -- the region's with the scope
+    const bool whole = checked(spec_k) && (i0 > 0 || j0 > 0 || i1 < im - 1 || j1 < jm - 1);
+    const int si = whole ? im : i1 - i0 + 1, sj = whole ? jm : j1 - j0 + 1;
+    const int o0 = whole ? 0 : i0, o1 = whole ? 0 : j0;
+    View3 ms{m.p + o0 * m.si + o1 * m.sj, m.si, m.sj, m.sl}, cs{cm.p + o0 * cm.si + o1 * cm.sj, cm.si, cm.sj, cm.sl};
     Tracers qs = q;
-    for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (i0 * q.si + j0 * q.sj);
+    for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (o0 * q.si + o1 * q.sj);
+    const ZScope moved = whole ? scope : ZScope{0, si - 1, 0, sj - 1};
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
     if (mass_only) {
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words, skip_mask, fixed_n, spec_k, 0);
+                   ctx->words, skip_mask, fixed_n, spec_k, 0, moved);
         return status(hipGetLastError());
     }
     // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
@@ -881,10 +888,10 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     const int ranges = (q.nt + tchunk - 1) / tchunk;
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)(4 * lm * TI + lm * TI) * sizeof(double),
                ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits, ctx->words, skip_mask, fixed_n, spec_k,
-               ranges > 1 ? tchunk : 0);
+               ranges > 1 ? tchunk : 0, moved);
     if (ranges > 1)
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words, skip_mask, fixed_n, -1, 0);
+                   ctx->words, skip_mask, fixed_n, SPEC_NONE, 0, moved);
     return status(hipGetLastError());
 }
 
@@ -905,7 +912,7 @@ int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, 
     TM5_LAUNCH(ctx, k_set_nadv, dim3(1), dim3(64), 0, ctx->stream, ctx->words, n_advz, TM5_ERRORS);
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3((im + TI - 1) / TI, jm), line_block(TI, lm),
                (size_t)(4 * lm * TI + lm * TI) * sizeof(double), ctx->stream, im, jm, lm, TI,
-               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, -1, 0);
+               m, q, cm, limits, prelimit, ctx->words, TM5_ERRORS, 0, SPEC_NONE, 0, ZScope{0, im - 1, 0, jm - 1});
     return status(hipGetLastError());
 }
 

@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         const double m0_l = m_l, m0_h = m_h;
         if (li < lm) m_l = (m0_l + a.cm(i, j, li) / 1.0) - a.cm(i, j, li + 1) / 1.0;
         if (l1 < lm) m_h = (m0_h + a.cm(i, j, l1) / 1.0) - a.cm(i, j, l1 + 1) / 1.0;
-        if (a.spec_k >= 0) {
+        if (checked(a.spec_k)) {
             // a checked step: Check_CFL's dynamwm at cap 1 on both dynamw's masses (m before and
             // after the first), every interior face k tested by lane k and k + 32 of its column
             H[li] = m0_l;

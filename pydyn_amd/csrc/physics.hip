@@ -15,9 +15,9 @@ __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 // interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
 // convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
 // pit summed from the surface and sd accumulated from the top, in the reference's order.
-__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, double a,
+__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, TimeSrc ts,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
-                                                int *words, int skip_mask)
+                                                int *words, int skip_mask, View3 pu_out, View3 pv_out, int am_halo)
 {
     // TI columns i per workgroup (the context's TM5_TILE_DYNAM0)
     if (skipped_p(words, skip_mask)) return;
@@ -25,6 +25,7 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, 
     const int j = blockIdx.y, i0 = blockIdx.x * TI;
     const int ni = min(TI, im - i0);
     const int tid = threadIdx.x, nth = blockDim.x;
+    const double a = time_amplitude(ts, words);
 #pragma unroll 4
     for (int idx = tid; idx < lm * TI; idx += nth) {
         const int ti = idx % TI, l = idx / TI;
@@ -37,8 +38,17 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, 
         const double a1 = dt * p1, b0 = dt * v0, b1 = dt * v1;
         am(i, j, l) = a0;
         if (i == im - 1) am(im, j, l) = a1;
+        if (am_halo && i == 1) am(im + 1, j, l) = a0;
         bm(i, j, l) = b0;
         if (j == jm - 1) bm(i, jm, l) = b1;
+        if (pu_out.p) {
+            pu_out(i, j, l) = p0;
+            if (i == im - 1) pu_out(im, j, l) = p1;
+        }
+        if (pv_out.p) {
+            pv_out(i, j, l) = v0;
+            if (j == jm - 1) pv_out(i, jm, l) = v1;
+        }
     }
     __syncthreads();
     for (int ti = tid; ti < ni; ti += nth) {
@@ -196,13 +206,16 @@ int launch_libm(tm5_ctx *ctx, int fn, long long n, const double *x, const double
 }
 
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic)
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic, const Dynam0Extra *ex)
 {
     const int TI = ctx->tile[TM5_TILE_DYNAM0];
     if (im < 1 || jm < 1 || lm < 1 || lm * TI > 8192 || !bt) return TM5_EINVAL;
     const dim3 grid((im + TI - 1) / TI, jm);
+    Dynam0Extra x{};
+    if (ex) x = *ex;
+    else x.ts.a = a;
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI, pu, pv,
-               a, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask);
+               x.ts, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask, x.pu_out, x.pv_out, x.am_halo);
     return status(hipGetLastError());
 }
 

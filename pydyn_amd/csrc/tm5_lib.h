@@ -16,8 +16,18 @@ enum {
     W_Z_CAND = 4,   // the z search's running candidate
     W_SPEC_AT = 5,  // tm5_region_run: the first queued interval whose rehearsal failed
     W_SPEC_KIND = 6, // 1: that interval's checked step failed a check (its state moved), 0: a trial
-    W_WORDS = 8,
+    // a queued run's clock, kept on the device so that a step captured once can be replayed for any
+    // interval: the interval (counted from the run's start) and the step within it
+    W_CLOCK_K = 8,
+    W_CLOCK_S = 9,
+    W_WORDS = 16,
 };
+
+// spec_k of the step kernels: a checked step of queued interval spec_k (>= 0), of the interval the
+// clock names (SPEC_CLOCK), or not a checked step (SPEC_NONE)
+constexpr int SPEC_NONE = -1;
+constexpr int SPEC_CLOCK = -2;
+__host__ __device__ inline bool checked(int spec_k) { return spec_k != SPEC_NONE; }
 
 struct tm5_prof;  // the launches timed between tm5_profile_begin and tm5_profile_end (api.hip)
 
@@ -40,6 +50,7 @@ struct tm5_ctx {
     tm5_prof *prof; // non-null while profiling (then prof == prof_store)
     tm5_prof *prof_store;  // owned: the events and the last totals
     double *noalloc;       // tm5_convec_noalloc's static device buffer on this context's device
+    int skip_or;           // bits every zoom-tree launcher adds to its skip mask (a queued tree run's TM5_SPEC)
 };
 
 namespace tm5 {
@@ -194,7 +205,7 @@ inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 // skip, and the host restores the interval's state from the backup before redoing it.
 __device__ __forceinline__ void spec_fail(int *words, int spec_k)
 {
-    words[W_SPEC_AT] = spec_k;  // every raiser of one launch writes the same interval
+    words[W_SPEC_AT] = spec_k >= 0 ? spec_k : words[W_CLOCK_K];  // every raiser of one launch writes the same
     words[W_SPEC_KIND] = 1;
     __threadfence();
     atomicOr(words + W_FLAGS, TM5_SPEC);
@@ -250,8 +261,34 @@ struct ColumnArgs {
 // range at least min_chunk tracers.  Returns the tracers a range (nt for a single range).
 int tracer_chunk(long long tiles, long long slots, int nt, int min_chunk);
 bool column_supported(int n);
+// dynam0's meteo amplitude a(t): `a` as given (t0 null), or, for a step of a queued run, from the
+// context's clock words and the run's start *t0 (device) with t = ((t0 + K nread) + S ndyn) +
+// ndyn / 2 -- the host's expression, exact in doubles for whole seconds -- through the triangle
+// wave of the meteo's amplitude (x = t / period, frac = x - floor(x), 1 + amplitude (1 - 4 |frac -
+// 1/2|): the host's double)
+struct TimeSrc {
+    double a;
+    const double *t0;
+    double nread, ndyn, period, amplitude;
+};
+__device__ __forceinline__ double time_amplitude(const TimeSrc &s, const int *words)
+{
+    if (!s.t0) return s.a;
+    const double t = ((*s.t0 + (double)words[W_CLOCK_K] * s.nread) + (double)words[W_CLOCK_S] * s.ndyn) + s.ndyn / 2.0;
+    const double x = t / s.period;
+    const double frac = x - floor(x);
+    return 1.0 + s.amplitude * (1.0 - 4.0 * fabs(frac - 0.5));
+}
+// What a zoom tree's dynam0 adds: the amplitude's source, the fluxes a pu, a pv stored into pu_out,
+// pv_out (p null: not stored) as the reference's interpolated meteo, and am(im) of a cyclic
+// region's halo (am_halo: am(im + 1) = am(1), index im + 1 of the am view)
+struct Dynam0Extra {
+    TimeSrc ts;
+    View3 pu_out, pv_out;
+    int am_halo;
+};
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
-                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1);
+                  View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const Dynam0Extra *ex = nullptr);
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);
@@ -295,6 +332,34 @@ int launch_calc_kzz(tm5_ctx *ctx, const KzzArgs &a);
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
                     int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);
 int launch_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, View3 q, View3 out);
+
+// the zoom tree's launchers (zoom.hip); every one adds ctx->skip_or to its skip mask.  tracers = 0:
+// the mass-only form of Check_CFL's rehearsal (masks with TM5_REHEARSAL)
+int zoom_put_x(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
+int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
+int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, int l1, int tracers, bool un);
+int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers);
+int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
+int zoom_coarsen(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask);
+int zoom_coarsen_ambm(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask);
+// clock_n > 0: the kernel also moves the context's clock on by a step of an interval of clock_n
+int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at, const double *bt, int skip_mask,
+                int clock_n);
+int zoom_x_halo(tm5_ctx *ctx, const ZR &R, int js, int je, int tracers, int skip_mask);
+int zoom_slab(tm5_ctx *ctx, const HV &f, int axis, int k, int n0, int n1, double *save, int restore, int skip_mask);
+int zoom_am_halo(tm5_ctx *ctx, const ZR &R, int skip_mask);
+int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
+                 int fail_bit, int skip_mask, int spec_k);
+int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int limits, int nmax, int mass_only,
+                 int fail_bit, int skip_mask, bool known_one, int spec_k);
+
+}  // namespace tm5
+
+// region 1's dynamu on its reduced grid (zoom.hip)
+int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int limits, int max_nloop, int mass_only, int fail,
+                   int skip, int spec_k);
+
+namespace tm5 {
 int launch_z_substeps(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int n_advz, int limits,
                       int prelimit);
 // Check_CFL's mass-only y and z with no sub-cycling allowed: the check at nloop 1 and its step

@@ -27,6 +27,9 @@ __device__ __forceinline__ bool zskip(const int *words, int mask) { return (word
 
 static dim3 zgrid(long long n, int per = 256) { return dim3((unsigned)((n + per - 1) / per)); }
 
+// a launcher's skip mask with the bits the context adds (a queued tree run's TM5_SPEC)
+static inline int zm(const tm5_ctx *ctx, int mask) { return mask | ctx->skip_or; }
+
 // ------------------------------------------------------------------ exchange: put_x/yedges
 // pydyn/zoom/exchange.py:17-60: for every child row jc and level lc, the parent cells ib-1, ib,
 // ie, ie+1 of the row jbeg + (jc-1)/yref into the child's cells 0, 1..xref, im+1-xref..im,
@@ -371,10 +374,17 @@ __global__ void __launch_bounds__(256) k_coarsen_ambm(ZR P, ZR C, int *words, in
 // pydyn/zoom/massflow.py:24-58: sp = seqsum(m over l) g / dxyp(j) per column, a child's ring
 // cells rescaled p dxyp(j) xref yref / dxyp_parent(jpar); phlb = at + bt sp.  The sp halo stays
 // zero (it is never written).  pdxyp null: region 1 (m2phlb1).
+// clock_n > 0: the last kernel of a queued run's step, which moves the context's clock on to the
+// next step (clock_n steps an interval; thread 0: nothing else in the kernel reads the clock)
 __global__ void __launch_bounds__(256) k_m2phlb_zoom(ZR R, const double *pdxyp, const double *at, const double *bt,
-                                                     int *words, int skip_mask)
+                                                     int *words, int skip_mask, int clock_n)
 {
     if (zskip(words, skip_mask)) return;
+    if (clock_n > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int s = words[W_CLOCK_S] + 1;
+        words[W_CLOCK_S] = s < clock_n ? s : 0;
+        if (s >= clock_n) words[W_CLOCK_K] += 1;
+    }
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= R.im * R.jm) return;
     const int i = col % R.im + 1, j = col / R.im + 1;
@@ -462,10 +472,11 @@ __global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask
 // row first, the sub-steps update the interior cells, and the faces are multiplied back.
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
-                                                int fail_bit, int *words, int skip_mask, int tchunk)
+                                                int fail_bit, int *words, int skip_mask, int tchunk, int spec_k)
 {
     // tchunk > 0: tracer ranges (grid y), several workgroups a row, each searching and moving the
     // row's mass on its own; none writes m or am, the caller runs the mass-only form afterwards
+    // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail)
     if (zskip(words, skip_mask)) return;
     extern __shared__ double lds[];
     const int nj = je - js + 1;
@@ -506,7 +517,12 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
         for (int k = lane; k < nf; k += 64) A[k] = shrink(A[k], nl);
         __syncthreads();
         if (nl > nmax) {
-            if (lane == 0) atomicOr(words + W_FLAGS, fail_bit);
+            if (lane == 0) {
+                if (checked(spec_k))
+                    spec_fail(words, spec_k);
+                else
+                    atomicOr(words + W_FLAGS, fail_bit);
+            }
             return;
         }
     }
@@ -666,9 +682,11 @@ __global__ void __launch_bounds__(1024) k_y_child_search(ZR C, int is, int ie, i
 // With tracer ranges (tchunk > 0, grid z) several workgroups take the same columns: each moves
 // the masses through the sub-steps on its own (in LDS), none writes them, and the caller runs
 // the mass-only form afterwards.
+// nloop null: every layer at nloop 1 (a Check_CFL rehearsal at cap 1 passed, or a checked step:
+// checked(spec_k), which first makes that rehearsal's test of every face of the columns at nloop 1).
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, int ls, int TI, const int *nloop,
-                                                        int limits, int *words, int skip_mask, int tchunk)
+                                                        int limits, int *words, int skip_mask, int tchunk, int spec_k)
 {
     if (zskip(words, skip_mask)) return;
     // neighbouring tiles, which share cache lines, on one XCD (as xcd_grid2), the tracer range slowest
@@ -685,7 +703,17 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
     const int ii = mine ? i : is, cc = mine ? c : 0;
     const int jc = j0 + cc;  // Fortran j of the cell
     const bool inner = cc > 0 && cc < n - 1;
-    const int nl = nloop[by];
+    const int nl = nloop ? nloop[by] : 1;
+    if (checked(spec_k)) {
+        // the rehearsal's dynamvm at cap 1 (pydyn/zoom/check_cfl.py:171-201): the face above each
+        // cell but the last, |b / m(upwind)| < 1 on the masses as they are before the step
+        bool ok = true;
+        if (mine && cc < n - 1) {
+            const double bk = C.bm(ii, jc + 1, l);
+            ok = fabs(bk / C.m(ii, bk >= 0.0 ? jc : jc + 1, l)) < 1.0;
+        }
+        if (__any(!ok) && (threadIdx.x & 63) == 0) spec_fail(words, spec_k);
+    }
     // faces below (bm(jc)) and above (bm(jc+1)) an interior cell
     const double b_lo = inner ? C.bm(ii, jc, l) : 0.0, b_hi = inner ? C.bm(ii, jc + 1, l) : 0.0;
     // each tracer's moments of the workgroup's cells pass through LDS ([moment][cell][ti]): every
@@ -887,7 +915,7 @@ int zoom_put_x(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
     if (C.whole) return 0;
     const int nf = nfields(C, tracers);
     TM5_LAUNCH(ctx, k_put_x, zgrid((long long)C.jm * C.lm * nf), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
-               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+               zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers));
     return status(hipGetLastError());
 }
 
@@ -895,7 +923,7 @@ int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
 {
     const int nf = nfields(C, tracers);
     TM5_LAUNCH(ctx, k_put_y, zgrid((long long)C.im * C.lm * nf), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
-               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+               zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers));
     return status(hipGetLastError());
 }
 
@@ -904,7 +932,7 @@ int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, i
     const int nf = tracers ? 1 + C.nt : 1;
     const long long n = (long long)(a1 - a0 + 1) * (l1 - l0 + 1) * nf;
     if (n <= 0) return 0;
-    const int mask = TM5_ERRORS | TM5_REHEARSAL * !tracers;
+    const int mask = zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers);
     if (un)
         TM5_LAUNCH(ctx, k_uncompress, zgrid(n), dim3(256), 0, ctx->stream, C, axis, a0, a1, l0, l1, nf, ctx->words, mask);
     else
@@ -921,10 +949,10 @@ int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers)
     const size_t lds = (size_t)ring.count() * (C.xr * C.yr + 1) * sizeof(double);
     if (nf >= 32 && lds <= 65536)  // many fields: a workgroup per (level, field)
         TM5_LAUNCH(ctx, k_mix_edges_lds, dim3((unsigned)(C.lm * nf)), dim3(256), lds, ctx->stream, C, ring, ctx->words,
-                   TM5_ERRORS | TM5_REHEARSAL * !tracers);
+                   zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers));
     else
         TM5_LAUNCH(ctx, k_mix_edges, zgrid(n), dim3(256), 0, ctx->stream, C, ring, nf, ctx->words,
-                   TM5_ERRORS | TM5_REHEARSAL * !tracers);
+                   zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers));
     return status(hipGetLastError());
 }
 
@@ -933,7 +961,7 @@ int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
     const int nf = tracers ? 1 + C.nt : 1;
     const long long n = (long long)(C.im / C.xr) * (C.jm / C.yr) * C.lm * nf;
     TM5_LAUNCH(ctx, k_update_parent, zgrid(n), dim3(256), 0, ctx->stream, P, C, nf, ctx->words,
-               TM5_ERRORS | TM5_REHEARSAL * !tracers);
+               zm(ctx, TM5_ERRORS | TM5_REHEARSAL * !tracers));
     return status(hipGetLastError());
 }
 
@@ -941,21 +969,22 @@ int zoom_coarsen(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
 {
     const long long imp = C.im / C.xr, jmp = C.jm / C.yr;
     const long long n = ((imp + 1) * jmp + imp * (jmp + 1) + 2 * imp * jmp) * C.lm;
-    TM5_LAUNCH(ctx, k_coarsen, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, skip_mask);
+    TM5_LAUNCH(ctx, k_coarsen, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
 int zoom_coarsen_ambm(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
 {
     const long long n = (long long)(C.jm + C.im) * C.lm;
-    TM5_LAUNCH(ctx, k_coarsen_ambm, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, skip_mask);
+    TM5_LAUNCH(ctx, k_coarsen_ambm, zgrid(n), dim3(256), 0, ctx->stream, P, C, ctx->words, zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
-int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at, const double *bt, int skip_mask)
+int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at, const double *bt, int skip_mask,
+                int clock_n)
 {
     TM5_LAUNCH(ctx, k_m2phlb_zoom, zgrid((long long)R.im * R.jm), dim3(256), 0, ctx->stream, R, pdxyp, at, bt,
-               ctx->words, skip_mask);
+               ctx->words, zm(ctx, skip_mask), clock_n);
     return status(hipGetLastError());
 }
 
@@ -964,14 +993,14 @@ int zoom_x_halo(tm5_ctx *ctx, const ZR &R, int js, int je, int tracers, int skip
     if (je < js) return 0;
     const int nf = nfields(R, tracers);
     TM5_LAUNCH(ctx, k_x_halo, zgrid((long long)(je - js + 1) * R.lm * nf), dim3(256), 0, ctx->stream, R, js, je, nf,
-               ctx->words, skip_mask);
+               ctx->words, zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
 int zoom_slab(tm5_ctx *ctx, const HV &f, int axis, int k, int n0, int n1, double *save, int restore, int skip_mask)
 {
     TM5_LAUNCH(ctx, k_slab, zgrid((long long)n0 * n1), dim3(256), 0, ctx->stream, f, axis, k, n0, n1, save, restore,
-               ctx->words, skip_mask);
+               ctx->words, zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
@@ -990,12 +1019,13 @@ int zoom_scale(tm5_ctx *ctx, double *a, long long n, double factor)
 int zoom_am_halo(tm5_ctx *ctx, const ZR &R, int skip_mask)
 {
     TM5_LAUNCH(ctx, k_am_halo, zgrid((long long)(R.jm + 2) * (R.lm + 2)), dim3(256), 0, ctx->stream, R, ctx->words,
-               skip_mask);
+               zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
+// spec_k: a checked step's x (the search capped at nmax, a failure spec_fail)
 int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask)
+                 int fail_bit, int skip_mask, int spec_k)
 {
     const int n = C.im - 2 * C.xr + 4;
     if (n < 3 || n > 512 || je < js || le < ls) return TM5_EINVAL;  // LDS: 11 n doubles
@@ -1011,26 +1041,31 @@ int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int 
     }
     if (!mass_only)
         TM5_LAUNCH(ctx, k_x_child<false>, dim3(grid.x, ranges), dim3(64), lds, ctx->stream, C, js, je, ls, le, limits,
-                   nmax, fail_bit, ctx->words, skip_mask, ranges > 1 ? tchunk : 0);
+                   nmax, fail_bit, ctx->words, zm(ctx, skip_mask), ranges > 1 ? tchunk : 0, spec_k);
     if (mass_only || ranges > 1)
         TM5_LAUNCH(ctx, k_x_child<true>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, 0, nmax, fail_bit,
-                   ctx->words, skip_mask, 0);
+                   ctx->words, zm(ctx, skip_mask), 0, spec_k);
     return status(hipGetLastError());
 }
 
+// known_one: every layer at nloop 1 without a search (Check_CFL's rehearsal at cap 1 passed on
+// the same bits); checked(spec_k): a checked step, whose sub-step kernel makes that test itself
 int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask)
+                 int fail_bit, int skip_mask, bool known_one, int spec_k)
 {
+    if (checked(spec_k)) known_one = true;
     const int n = C.jm - 2 * C.yr + 4;
     const int nlay = le - ls + 1;
     if (n < 3 || n > 1024 || ie < is || nlay < 1 || nlay > ctx->y_cap) return TM5_EINVAL;
-    int *nloop = ctx->y_nloop;
-    if (mass_only)
-        TM5_LAUNCH(ctx, k_y_child_search<true>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax, nloop,
-                   fail_bit, ctx->words, skip_mask);
-    else
-        TM5_LAUNCH(ctx, k_y_child_search<false>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax, nloop,
-                   fail_bit, ctx->words, skip_mask);
+    int *nloop = known_one ? nullptr : ctx->y_nloop;
+    if (!known_one) {
+        if (mass_only)
+            TM5_LAUNCH(ctx, k_y_child_search<true>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax, nloop,
+                       fail_bit, ctx->words, zm(ctx, skip_mask));
+        else
+            TM5_LAUNCH(ctx, k_y_child_search<false>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax,
+                       nloop, fail_bit, ctx->words, zm(ctx, skip_mask));
+    }
     const int TI = (1024 / n) < 16 ? (1024 / n) : 16;
     const int ni = ie - is + 1;
     const dim3 grid((unsigned)((ni + TI - 1) / TI), (unsigned)nlay), block((unsigned)((TI * n + 63) / 64 * 64));
@@ -1042,13 +1077,14 @@ int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int 
     if (!mass_only)
         TM5_LAUNCH(ctx, k_y_child_steps<false>, dim3(grid.x, grid.y, ranges), block,
                    (size_t)5 * n * TI * sizeof(double), ctx->stream, C, is, ie, ls, TI, nloop, limits, ctx->words,
-                   skip_mask, ranges > 1 ? tchunk : 0);
+                   zm(ctx, skip_mask), ranges > 1 ? tchunk : 0, spec_k);
     if (mass_only || ranges > 1)
         TM5_LAUNCH(ctx, k_y_child_steps<true>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, 0, ctx->words,
-                   skip_mask, 0);
+                   zm(ctx, skip_mask), 0, mass_only ? spec_k : SPEC_NONE);
+    if (known_one) return status(hipGetLastError());  // times 1 is the identity: nothing to restore
     const int n0 = C.im + 2, n1 = C.jm + 2;
     TM5_LAUNCH(ctx, k_y_child_restore, dim3((unsigned)((n0 * n1 + 255) / 256), (unsigned)nlay), dim3(256), 0,
-               ctx->stream, C, ls, nloop, ctx->words, skip_mask);
+               ctx->stream, C, ls, nloop, ctx->words, zm(ctx, skip_mask));
     return status(hipGetLastError());
 }
 
@@ -1066,8 +1102,9 @@ using namespace tm5;
 // Region 1's dynamu on its reduced grid (pydyn/zoom/advect.py:253-297, pydyn/cfl.py:30-57): the
 // uniform m and am kept, the clusters summed and the faces subsampled, every row advected with
 // its own cell count, the halos, the clusters split with the uniformly advanced mass, am put back.
-static int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int limits, int max_nloop, int mass_only,
-                          int fail, int skip)
+// spec_k: a checked step's (the search capped at max_nloop - 1, a failure spec_fail)
+int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int limits, int max_nloop, int mass_only, int fail,
+                   int skip, int spec_k)
 {
     const ZR R = zr(d);
     if (!d.jred || !d.imred || !d.scratch || R.im > 4096) return TM5_EINVAL;
@@ -1082,7 +1119,8 @@ static int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int 
     TM5_LAUNCH(c, k_uni2red, rows, dim3(64), (size_t)R.im * sizeof(double), c->stream, R, tracers, c->words, skip);
     TM5_LAUNCH(c, k_uni2red_mf, zgrid((long long)R.nred * R.lm, 64), dim3(64), 0, c->stream, R, c->words, skip);
     ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
-                  sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip, R.imred));
+                  sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip, R.imred, nullptr, nullptr,
+                  spec_k));
     const int nf = mass_only ? 1 : 1 + 4 * R.nt;
     const long long nh = (long long)(je - js + 1) * R.lm * nf;
     if (!mass_only && nh > 0)
@@ -1105,7 +1143,10 @@ static int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int 
 
 static bool zbad(const tm5_zregion *r) { return r == nullptr || r->m.data == nullptr || r->im < 1 || r->jm < 1 || r->lm < 1; }
 static bool zglobal(const tm5_zregion *r) { return r->xcyc == 1 && r->touch_sp == 1 && r->touch_np == 1; }
-static int zskipmask(int rehearsal) { return rehearsal ? (TM5_ERRORS | TM5_REHEARSAL) : TM5_ERRORS; }
+static int zskipmask(const tm5_ctx *c, int rehearsal)
+{
+    return zm(c, rehearsal ? (TM5_ERRORS | TM5_REHEARSAL) : TM5_ERRORS);
+}
 
 extern "C" int tm5_zoom_put_xedges(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch, int tracers)
 {
@@ -1164,20 +1205,20 @@ extern "C" int tm5_zoom_update_parent(tm5_ctx *c, const tm5_zregion *p, const tm
 extern "C" int tm5_zoom_coarsen_block(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch)
 {
     if (!c || zbad(p) || zbad(ch) || ch->zref != 1) return TM5_EINVAL;
-    return zoom_coarsen(c, zr(*p), zr(*ch), TM5_ERRORS);
+    return zoom_coarsen(c, zr(*p), zr(*ch), zm(c, TM5_ERRORS));
 }
 
 extern "C" int tm5_zoom_coarsen_ambm(tm5_ctx *c, const tm5_zregion *p, const tm5_zregion *ch)
 {
     if (!c || zbad(p) || zbad(ch)) return TM5_EINVAL;
-    return zoom_coarsen_ambm(c, zr(*p), zr(*ch), TM5_ERRORS);
+    return zoom_coarsen_ambm(c, zr(*p), zr(*ch), zm(c, TM5_ERRORS));
 }
 
 extern "C" int tm5_zoom_m2phlb(tm5_ctx *c, const tm5_zregion *r, const tm5_zregion *p, const double *at,
                                const double *bt)
 {
     if (!c || zbad(r) || !r->dxyp || !at || !bt || (p && !p->dxyp)) return TM5_EINVAL;
-    return zoom_m2phlb(c, zr(*r), p ? p->dxyp : nullptr, at, bt, TM5_ERRORS);
+    return zoom_m2phlb(c, zr(*r), p ? p->dxyp : nullptr, at, bt, zm(c, TM5_ERRORS), 0);
 }
 
 extern "C" int tm5_zoom_dynam0(tm5_ctx *c, const tm5_zregion *r, const double *bt, int ndyn, int rehearsal)
@@ -1185,7 +1226,7 @@ extern "C" int tm5_zoom_dynam0(tm5_ctx *c, const tm5_zregion *r, const double *b
     if (!c || zbad(r) || !bt || r->tref_abs < 1) return TM5_EINVAL;
     const ZR R = zr(*r);
     const double dt = (double)ndyn / (2.0 * (double)r->tref_abs);
-    const int skip = zskipmask(rehearsal);
+    const int skip = zskipmask(c, rehearsal);
     ZTRY(launch_dynam0(c, R.im, R.jm, R.lm, sub3(R.pu, 0, 1, 1), sub3(R.pv, 1, 1, 1), 1.0, bt, dt, sub3(R.am, 0, 1, 1),
                        sub3(R.bm, 1, 1, 1), sub3(R.cm, 1, 1, 0), skip, R.xcyc));
     if (R.xcyc == 1) ZTRY(zoom_am_halo(c, R, skip));
@@ -1197,7 +1238,7 @@ extern "C" int tm5_zoom_outer_faces(tm5_ctx *c, const tm5_zregion *r, int axis, 
 {
     if (!c || zbad(r) || !save || axis < 0 || axis > 1) return TM5_EINVAL;
     const ZR R = zr(*r);
-    const int skip = zskipmask(rehearsal);
+    const int skip = zskipmask(c, rehearsal);
     if (axis == 0) {
         const int n0 = R.jm + 2, n1 = R.lm + 2;
         ZTRY(zoom_slab(c, R.am, 0, R.xr - 1, n0, n1, save, restore, skip));
@@ -1214,7 +1255,7 @@ extern "C" int tm5_zoom_advect_x(tm5_ctx *c, const tm5_zregion *r, int js, int j
 {
     if (!c || zbad(r) || max_nloop < 1) return TM5_EINVAL;
     const ZR R = zr(*r);
-    const int skip = zskipmask(mass_only);
+    const int skip = zskipmask(c, mass_only);
     if (zglobal(r)) {
         const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_X;
         if (r->nred == 0) {
@@ -1222,12 +1263,12 @@ extern "C" int tm5_zoom_advect_x(tm5_ctx *c, const tm5_zregion *r, int js, int j
                           sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip));
             return zoom_x_halo(c, R, js, je, !mass_only, skip);
         }
-        return zoom_x_reduced(c, *r, js, je, limits, max_nloop, mass_only, fail, skip);
+        return zoom_x_reduced(c, *r, js, je, limits, max_nloop, mass_only, fail, skip, SPEC_NONE);
     }
     const int tr = mass_only ? 0 : 1;
     ZTRY(tm5_zoom_compress_xedges(c, r, js, je, 1, R.lm, tr));
     ZTRY(zoom_x_child(c, R, js, je, 1, R.lm, limits, max_nloop - 1, mass_only, mass_only ? TM5_REHEARSAL : TM5_FAIL_XC,
-                      skip));
+                      skip, SPEC_NONE));
     if (mass_only) {  // dynamum's own uncompressm and mixm_edges (pydyn/zoom/check_cfl.py:143-144)
         ZTRY(tm5_zoom_uncompress_xedges(c, r, js, je, 1, R.lm, 0));
         ZTRY(tm5_zoom_mix_edges(c, r, 0));
@@ -1240,7 +1281,7 @@ extern "C" int tm5_zoom_advect_y(tm5_ctx *c, const tm5_zregion *r, int is, int i
 {
     if (!c || zbad(r) || max_nloop < 1) return TM5_EINVAL;
     const ZR R = zr(*r);
-    const int skip = zskipmask(mass_only);
+    const int skip = zskipmask(c, mass_only);
     if (zglobal(r)) {
         const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_Y;
         return launch_y(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
@@ -1249,7 +1290,7 @@ extern "C" int tm5_zoom_advect_y(tm5_ctx *c, const tm5_zregion *r, int is, int i
     const int tr = mass_only ? 0 : 1;
     ZTRY(tm5_zoom_compress_yedges(c, r, is, ie, 1, R.lm, tr));
     ZTRY(zoom_y_child(c, R, is, ie, 1, R.lm, limits, mass_only ? max_nloop : max_nloop - 1, mass_only,
-                      mass_only ? TM5_REHEARSAL : TM5_FAIL_YC, skip));
+                      mass_only ? TM5_REHEARSAL : TM5_FAIL_YC, skip, false, SPEC_NONE));
     if (mass_only) {
         ZTRY(tm5_zoom_uncompress_yedges(c, r, is, ie, 1, R.lm, 0));
         ZTRY(tm5_zoom_mix_edges(c, r, 0));
@@ -1265,7 +1306,7 @@ extern "C" int tm5_zoom_advect_z(tm5_ctx *c, const tm5_zregion *r, int is, int i
     const int fail = mass_only ? TM5_REHEARSAL : TM5_FAIL_Z;
     return launch_z_scoped(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
                            sub3(R.cm, 1, 1, 0), is - 1, ie - 1, js - 1, je - 1, mass_only ? 0 : limits, iter_limit,
-                           fail, zskipmask(mass_only));
+                           fail, zskipmask(c, mass_only));
 }
 
 // ------------------------------------------------------------------ sources and chemistry alone
@@ -1311,7 +1352,7 @@ extern "C" int tm5_zoom_inject(tm5_ctx *c, const tm5_zregion *r, const tm5_view 
         return TM5_EINVAL;
     const ZR R = zr(*r);
     TM5_LAUNCH(c, k_zoom_inject, zgrid((long long)R.im * R.jm), dim3(256), 0, c->stream, R, view3(x), level, tracer,
-               c->words, TM5_ERRORS);
+               c->words, zm(c, TM5_ERRORS));
     return status(hipGetLastError());
 }
 
@@ -1322,7 +1363,7 @@ extern "C" int tm5_zoom_decay(tm5_ctx *c, const tm5_zregion *r, const tm5_view *
     if (count == 0) return 0;
     const ZR R = zr(*r);
     TM5_LAUNCH(c, k_zoom_decay, zgrid((long long)R.im * R.jm * R.lm), dim3(256), 0, c->stream, R, view3(decay), t0,
-               count, c->words, TM5_ERRORS);
+               count, c->words, zm(c, TM5_ERRORS));
     return status(hipGetLastError());
 }
 
@@ -1375,7 +1416,7 @@ extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphy
     a.phlb = sub3(R.phlb, 1, 1, 1);
     if ((a.pre || a.post || a.twice) && (!p->emis.data || !p->loss.data)) return TM5_EINVAL;
     if (a.mid && (!p->at || !p->bt || !r->dxyp)) return TM5_EINVAL;
-    return launch_column(c, a, p->lmax, TM5_ERRORS);
+    return launch_column(c, a, p->lmax, zm(c, TM5_ERRORS));
 }
 
 extern "C" int tm5_zoom_copy(tm5_ctx *c, const double *src, double *dst, int64_t n)

@@ -42,6 +42,7 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
         "tm5_hview": _lib.HView,
         "tm5_zregion": _lib.ZRegion,
         "tm5_zphysics": _lib.ZPhysics,
+        "tm5_tree_desc": _lib.TreeDesc,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tm5hip.h"', "int main(void) {"]
     for cname, cls in checks.items():
@@ -59,7 +60,8 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
             assert int(out[f"{cname}.{fname}"]) == getattr(cls, fname).offset, f"{cname}.{fname}"
 
 
-@pytest.mark.parametrize("source", ["advect.hip", "convec.hip", "column.hip", "physics.hip", "zoom.hip", "adjoint.hip", "api.hip"])
+@pytest.mark.parametrize("source", ["advect.hip", "convec.hip", "column.hip", "lapack.hip", "physics.hip", "zoom.hip", "tree.hip",
+                                    "adjoint.hip", "api.hip"])
 def test_no_contracted_fp64_fma_in_device_code(source, tmp_path):
     """Every v_fma_f64 / v_fmac_f64 in the gfx950 code belongs to a correctly rounded division
     (v_div_scale .. v_div_fmas .. v_div_fixup, five FMAs each) or square root (v_rsq_f64 and

@@ -1,0 +1,98 @@
+"""The zoom tree's queued native run (tm5_tree_run) against the reference, bit for bit: intervals
+queued in runs of `chunk` as checked steps, each step of a given ndyn captured into a HIP graph
+once and replayed, the intervals whose plan fails (Check_CFL cuts) redone from the backup.  Every
+interval's (ndyn, n) and, at the end of every chunk (after the host has read the state, so graph
+replays alternate with eager work and synchronisation), every array of every region against the
+sha1s the reference's zoom path wrote (tests/golden/zoom_*_sha1.json)."""
+
+import hashlib
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pydyn_amd import device as D
+
+    D.context().flags(clear=True)
+    return D
+
+
+def sha(a):
+    return hashlib.sha1(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()[:20]
+
+
+def _compare(host, want, what):
+    bad = [k for k in want if "." in k and sha(host[int(k[1])][k.split(".")[1]]) != want[k]]
+    assert not bad, f"{what}: differ from the reference: {bad}"
+
+
+def _queued(name, chunk, limit=None, graphs=True, ensemble=None):
+    from pydyn_amd import _lib, meteo
+    from pydyn_amd.zoom.run import ZoomRun
+
+    golden = f"zoom_{name}_sha1.json" if ensemble is None else "zoom_tiny2e_sha1.json"
+    ref = json.loads((GOLDEN / golden).read_text())["intervals"][:limit]
+    if ensemble is None:
+        pmet, cmet = meteo.zoom(name)
+        run = ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name))
+    else:
+        pmet, cmet = meteo.zoom(name, nt=ensemble[0])
+        run = ZoomRun(pmet, cmet, ensemble=meteo.Ensemble(pmet, ensemble[1]))
+    if not graphs:
+        run.native().option(_lib.TREE_GRAPHS, 0)
+    i = 0
+    while i < len(ref):
+        k = min(chunk, len(ref) - i)
+        print(f"{name}: intervals {i}..{i + k - 1} queued (graphs {graphs})", flush=True)
+        out = run.run(k)
+        for j, got in enumerate(out):
+            assert got == (ref[i + j]["ndyn"], ref[i + j]["n"]), f"{name} interval {i + j}: (ndyn, n) {got}"
+        i += k
+        _compare(run.host(), ref[i - 1], f"{name} after interval {i - 1}")
+    return run
+
+
+@pytest.mark.parametrize(("name", "chunk", "limit"), [
+    ("tiny2", 12, None),     # 96 intervals, 14 cut
+    ("small2", 8, None),     # 72 intervals, 46 cut
+    ("tiny2r", 16, None),    # region 1 on a reduced grid
+    ("small2r", 12, None),
+    ("c2r", 24, None),       # the production tree on the glb3x2 reduced grid
+])  # fmt: skip
+def test_queued_tree_matches_reference(dev, name, chunk, limit):
+    run = _queued(name, chunk, limit)
+    tally = run.native().tally()
+    assert tally["kept"] + tally["redone"] == len(run.log)
+    assert tally["replayed"] > 0 and tally["captured"] >= 1, tally
+
+
+def test_queued_production_tree_matches_reference(dev):
+    """The production tree (glb3x2 x 34 with a 120 x 70 x 34 child) for all 168 intervals of the
+    golden file (7 days, 60 of them cut by Check_CFL), queued in days: the checked steps of both
+    step lengths replayed from their graphs and every plan that fails redone."""
+    run = _queued("c2", 24)
+    tally = run.native().tally()
+    assert tally["redone"] >= 1 and tally["replayed"] >= 100 and tally["captured"] >= 2, tally
+
+
+def test_queued_tree_without_graphs_matches_reference(dev):
+    """The same queue with every launch issued (no graph): the reference's bits too."""
+    run = _queued("small2", 12, graphs=False)
+    assert run.native().tally()["replayed"] == 0
+
+
+def test_queued_ensemble_tree_matches_reference(dev):
+    """4 members x 4 tracers on the tiny tree, queued, against the reference's ensemble run."""
+    _queued("tiny2", 8, ensemble=(4, 4))

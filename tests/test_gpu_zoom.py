@@ -45,14 +45,14 @@ def _first_difference(got, oracle_st):
     return None
 
 
-def _run(name, limit=None, oracle_every=None):
+def _run(name, limit=None, oracle_every=None, driver="native"):
     from oracle import zoom as OZ
     from pydyn_amd import meteo
     from pydyn_amd.zoom.run import ZoomRun
 
     ref = json.loads((GOLDEN / f"zoom_{name}_sha1.json").read_text())["intervals"][:limit]
     pmet, cmet = meteo.zoom(name)
-    run = ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name))
+    run = ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name), driver=driver)
     orun = OZ.ZoomRun(pmet, cmet, redgrid=meteo.redgrid(name)) if oracle_every else None
     for it, want in enumerate(ref):
         got_nd = run.interval()
@@ -70,12 +70,15 @@ def _run(name, limit=None, oracle_every=None):
     return run
 
 
-def test_zoom_tiny_tree_matches_reference(dev):
-    _run("tiny2", oracle_every=1)
+@pytest.mark.parametrize("driver", ["native", "python"])
+def test_zoom_tiny_tree_matches_reference(dev, driver):
+    """The library's tree driver (tm5_tree_interval) and the Python restatement over the bodies."""
+    _run("tiny2", oracle_every=1, driver=driver)
 
 
-def test_zoom_small_tree_matches_reference(dev):
-    _run("small2", limit=24, oracle_every=6)
+@pytest.mark.parametrize("driver", ["native", "python"])
+def test_zoom_small_tree_matches_reference(dev, driver):
+    _run("small2", limit=24, oracle_every=6, driver=driver)
 
 
 def test_zoom_reduced_grid_tiny_tree_matches_reference(dev):
