@@ -30,8 +30,11 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
-                                          const int *imred, double *amax_out, int spec_k, double *bk, int tchunk)
+                                          const int *imred, double *amax_out, int spec_k, double *bk, int tchunk,
+                                          int hj0, int hj1)
 {
+    // rows j in hj0..hj1 (uniform rows only) also get their periodic halo, index -1 = index n - 1 and
+    // index n = index 0, for m and every moment (a zoom tree's region 1, pydyn/zoom/advect.py:283-288)
     // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
     // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
     // bk (whole rows only): the row's m and moments as read, i fastest, m first and then every
@@ -200,19 +203,29 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
                 q.q[2][o] = R[2 * n + i];
                 q.q[3][o] = R[3 * n + i];
             }
+            if (lane < 8 && j >= hj0 && j <= hj1) {  // lane a < 4: q(-1) of moment a; 4 + a: q(n)
+                const int a = lane & 3;
+                q.q[a][q.off(lane < 4 ? -1 : n, j, l, t)] = R[a * n + (lane < 4 ? n - 1 : 0)];
+            }
             __syncthreads();
         }
     }
     if (!writes) return;
     for (int i = lane; i < n; i += 64) m(i, j, l) = M[i];
+    if (lane == 0 && j >= hj0 && j <= hj1) {
+        m(-1, j, l) = M[n - 1];
+        m(n, j, l) = M[0];
+    }
     for (int k = lane; k <= n; k += 64) am(k, j, l) = A[k] * (double)nl;  // "restore old am"
 }
 
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
-             int skip_mask, const int *imred, int *rows_out, double *amax_out, int spec_k, double *backup)
+             int skip_mask, const int *imred, int *rows_out, double *amax_out, int spec_k, double *backup, int halo_j0,
+             int halo_j1)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
     if (backup && imred) return TM5_EINVAL;  // the backup holds whole rows
+    if (imred && halo_j1 >= halo_j0) return TM5_EINVAL;  // a reduced row's halo is the reduced grid's
     const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
@@ -227,10 +240,11 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
     }
     if (q.nt > 0)
         TM5_LAUNCH(ctx, k_x<false>, dim3(grid.x, ranges), block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax,
-                   ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, ranges > 1 ? tchunk : 0);
+                   ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, ranges > 1 ? tchunk : 0,
+                   halo_j0, halo_j1);
     if (q.nt == 0 || ranges > 1)
         TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0);
+                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0, halo_j0, halo_j1);
     return status(hipGetLastError());
 }
 

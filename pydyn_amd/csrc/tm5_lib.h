@@ -24,9 +24,11 @@ enum {
 };
 
 // spec_k of the step kernels: a checked step of queued interval spec_k (>= 0), of the interval the
-// clock names (SPEC_CLOCK), or not a checked step (SPEC_NONE)
+// clock names (SPEC_CLOCK), or not a checked step (SPEC_NONE); SPEC_REHEARSAL: Check_CFL's
+// mass-only rehearsal at its caps, whose failed test raises TM5_REHEARSAL (the verdict) instead
 constexpr int SPEC_NONE = -1;
 constexpr int SPEC_CLOCK = -2;
+constexpr int SPEC_REHEARSAL = -3;
 __host__ __device__ inline bool checked(int spec_k) { return spec_k != SPEC_NONE; }
 
 struct tm5_prof;  // the launches timed between tm5_profile_begin and tm5_profile_end (api.hip)
@@ -205,6 +207,10 @@ inline int status(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 // skip, and the host restores the interval's state from the backup before redoing it.
 __device__ __forceinline__ void spec_fail(int *words, int spec_k)
 {
+    if (spec_k == SPEC_REHEARSAL) {
+        atomicOr(words + W_FLAGS, TM5_REHEARSAL);
+        return;
+    }
     words[W_SPEC_AT] = spec_k >= 0 ? spec_k : words[W_CLOCK_K];  // every raiser of one launch writes the same
     words[W_SPEC_KIND] = 1;
     __threadfence();
@@ -213,9 +219,11 @@ __device__ __forceinline__ void spec_fail(int *words, int spec_k)
 
 // launchers (advect.hip)
 // rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
+// halo_j0..halo_j1 (0-based rows, uniform rows only): those rows also get their periodic halo cells,
+// index -1 and im of the views (a zoom tree's region 1)
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr,
-             int spec_k = -1, double *backup = nullptr);
+             int spec_k = -1, double *backup = nullptr, int halo_j0 = 0, int halo_j1 = -1);
 // known_one: the caller knows the sub-cycle count is 1 everywhere (no search launched); spec_k >= 0:
 // a checked step of queued interval spec_k, which tests Check_CFL's cap 1 itself
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
@@ -340,6 +348,8 @@ int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
 int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, int l1, int tracers, bool un);
 int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers);
 int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
+// update_parent (with tracers) and the child's m2phlb, one launch
+int zoom_update_m2phlb(tm5_ctx *ctx, const ZR &P, const ZR &C, const double *at, const double *bt, int skip_mask);
 int zoom_coarsen(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask);
 int zoom_coarsen_ambm(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask);
 // clock_n > 0: the kernel also moves the context's clock on by a step of an interval of clock_n
@@ -348,10 +358,13 @@ int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at
 int zoom_x_halo(tm5_ctx *ctx, const ZR &R, int js, int je, int tracers, int skip_mask);
 int zoom_slab(tm5_ctx *ctx, const HV &f, int axis, int k, int n0, int n1, double *save, int restore, int skip_mask);
 int zoom_am_halo(tm5_ctx *ctx, const ZR &R, int skip_mask);
+constexpr int FUSE_COMPRESS = 1, FUSE_OUTER = 2;  // zoom_x_child / zoom_y_child's fuse bits
+// fuse: 1 the compress before and the uncompress after made in the kernel, 2 the outer interface faces
+// taken as zero (the kernel's rows / columns only; for y with known_one only)
 int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask, int spec_k);
+                 int fail_bit, int skip_mask, int spec_k, int fuse);
 int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask, bool known_one, int spec_k);
+                 int fail_bit, int skip_mask, bool known_one, int spec_k, int fuse);
 
 }  // namespace tm5
 

@@ -345,6 +345,13 @@ void z_scope(const tm5_tree *t, int r, const std::string &blk, int &is, int &ie,
 }
 
 // ------------------------------------------------------------------ the step's bodies
+// where compress / uncompress apply (as tm5_zoom_compress_x/yedges decide)
+bool xedges_on(const tm5_zregion &d) { return !(d.xref == 1 || d.whole); }
+bool yedges_on(const tm5_zregion &d)
+{
+    return !(d.yref == 1 || (d.xcyc == 1 && d.touch_sp == 1 && d.touch_np == 1));
+}
+
 int put_edges(tm5_tree *t, int r, int axis, int tracers)
 {
     for (int ch : t->reg[r].children) {
@@ -372,23 +379,21 @@ int advect_x(tm5_tree *t, int r)
     const bool zero = second && !g.d.whole;
     const bool chk = checked(t->spec);
     const int mk = mask(t, false);
-    if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 0, 0, g.slab[0], 0));
     if (r == 1) {
-        if (g.d.nred == 0) {
+        if (g.d.nred == 0) {  // the periodic halo of rows js..je written by the x kernel itself
             TTRY(launch_x(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.am, 0, 1, 1),
-                          t->limits, chk ? CFL_CAP[0] - 1 : 50 - 1, TM5_FAIL_X, mk, nullptr, nullptr, nullptr, t->spec));
-            TTRY(zoom_x_halo(c, Rr, js, je, 1, mk));
+                          t->limits, chk ? CFL_CAP[0] - 1 : 50 - 1, TM5_FAIL_X, mk, nullptr, nullptr, nullptr, t->spec,
+                          nullptr, js - 1, je - 1));
         } else {
             TTRY(zoom_x_reduced(c, g.d, js, je, t->limits, chk ? CFL_CAP[0] : 50, 0, TM5_FAIL_X, mk, t->spec));
         }
-    } else {
-        TTRY(tm5_zoom_compress_xedges(c, &g.d, js, je, 1, Rr.lm, 1));
-        TTRY(zoom_x_child(c, Rr, js, je, 1, Rr.lm, t->limits, chk ? CFL_CAP[0] - 1 : 50 - 1, 0, TM5_FAIL_XC, mk,
-                          t->spec));
-        TTRY(tm5_zoom_uncompress_xedges(c, &g.d, js, je, 1, Rr.lm, 1));
-        TTRY(tm5_zoom_mix_edges(c, &g.d, 1));
+        return 0;
     }
-    if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 0, 1, g.slab[0], 0));
+    // a child: the compress / uncompress and the outer faces' zeroing inside the x kernel
+    const int fuse = (xedges_on(g.d) ? FUSE_COMPRESS : 0) | (zero ? FUSE_OUTER : 0);
+    TTRY(zoom_x_child(c, Rr, js, je, 1, Rr.lm, t->limits, chk ? CFL_CAP[0] - 1 : 50 - 1, 0, TM5_FAIL_XC, mk, t->spec,
+                      fuse));
+    TTRY(tm5_zoom_mix_edges(c, &g.d, 1));
     return 0;
 }
 
@@ -406,16 +411,22 @@ int advect_y(tm5_tree *t, int r)
     y_scope(t, r, blk, is, ie);
     const bool zero = second && (g.d.touch_sp != 1 || g.d.touch_np != 1);
     const int mk = mask(t, false);
-    if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 1, 0, g.slab[1], 0));
-    if (r == 1) {
-        TTRY(launch_y(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.bm, 1, 1, 1), t->limits,
-                      10 - 1, TM5_FAIL_Y, mk, t->known_one, t->spec));
-    } else {
-        TTRY(tm5_zoom_compress_yedges(c, &g.d, is, ie, 1, Rr.lm, 1));
-        TTRY(zoom_y_child(c, Rr, is, ie, 1, Rr.lm, t->limits, 10 - 1, 0, TM5_FAIL_YC, mk, t->known_one, t->spec));
-        TTRY(tm5_zoom_uncompress_yedges(c, &g.d, is, ie, 1, Rr.lm, 1));
-        TTRY(tm5_zoom_mix_edges(c, &g.d, 1));
+    if (r == 1)
+        return launch_y(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.bm, 1, 1, 1),
+                        t->limits, 10 - 1, TM5_FAIL_Y, mk, t->known_one, t->spec);
+    if (t->known_one || checked(t->spec)) {
+        // no search: the compress / uncompress and the outer faces' zeroing (a child touches
+        // neither pole: both faces) inside the sub-step kernel
+        const int fuse = (yedges_on(g.d) ? FUSE_COMPRESS : 0) | (zero ? FUSE_OUTER : 0);
+        if (zero && (g.d.touch_sp == 1 || g.d.touch_np == 1)) return TM5_EINVAL;
+        TTRY(zoom_y_child(c, Rr, is, ie, 1, Rr.lm, t->limits, 10 - 1, 0, TM5_FAIL_YC, mk, true, t->spec, fuse));
+        return tm5_zoom_mix_edges(c, &g.d, 1);
     }
+    if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 1, 0, g.slab[1], 0));
+    TTRY(tm5_zoom_compress_yedges(c, &g.d, is, ie, 1, Rr.lm, 1));
+    TTRY(zoom_y_child(c, Rr, is, ie, 1, Rr.lm, t->limits, 10 - 1, 0, TM5_FAIL_YC, mk, false, t->spec, 0));
+    TTRY(tm5_zoom_uncompress_yedges(c, &g.d, is, ie, 1, Rr.lm, 1));
+    TTRY(tm5_zoom_mix_edges(c, &g.d, 1));
     if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 1, 1, g.slab[1], 0));
     return 0;
 }
@@ -524,9 +535,8 @@ int proces_region(tm5_tree *t, int r, int ndyn)
         if (t->status[r] % NOPS != 0) TTRY(do_steps(t, r, ndyn));
     }
     const int mk = mask(t, false);
-    if (r != 1) {
-        TTRY(zoom_update_parent(c, R(t, g.parent), g.R, 1));
-        TTRY(zoom_m2phlb(c, g.R, t->reg[g.parent].d.dxyp, t->at, t->bt, mk, 0));
+    if (r != 1) {  // update_parent and m2phlb: disjoint arrays, one launch
+        TTRY(zoom_update_m2phlb(c, R(t, g.parent), g.R, t->at, t->bt, mk));
     } else if (t->m2phlb1_done) {
         t->m2phlb1_done = false;  // done inside the fused column pass
     } else {
@@ -595,29 +605,43 @@ int mass_op(tm5_tree *t, int r, char op)
     std::string blk;
     bool second;
     block_of(t, r, t->cstatus[r], g.cfl_order, CFL_NOPS, blk, second);
+    // The mass-only operators at their caps as single kernels: x at cap 10 with its own search (and
+    // region 1's halo, a child's compress / uncompress and outer faces inside the kernel), y and z
+    // at cap 1 as the test of every face at nloop 1 and the one sub-step (SPEC_REHEARSAL: a failed
+    // test raises the verdict; what the failing kernel still moves is discarded with the trial).
+    const ZR &Rr = g.R;
+    const int mk = mask(t, true);
     if (op == 'x') {
         TTRY(put_edges(t, r, 0, 0));
         int js, je;
         x_scope(t, r, blk, js, je);
         const bool zero = second && !g.d.whole;
-        if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 0, 0, g.slab[0], 1));
-        TTRY(tm5_zoom_advect_x(c, &g.d, js, je, 0, CFL_CAP[0], 1));
-        if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 0, 1, g.slab[0], 1));
-    } else if (op == 'y') {
+        if (r == 1 && g.d.nred == 0)
+            return launch_x(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), no_tracers(), sub3(Rr.am, 0, 1, 1), 0,
+                            CFL_CAP[0] - 1, TM5_REHEARSAL, mk, nullptr, nullptr, nullptr, SPEC_NONE, nullptr, js - 1,
+                            je - 1);
+        if (r == 1) return tm5_zoom_advect_x(c, &g.d, js, je, 0, CFL_CAP[0], 1);
+        const int fuse = (xedges_on(g.d) ? FUSE_COMPRESS : 0) | (zero ? FUSE_OUTER : 0);
+        TTRY(zoom_x_child(c, Rr, js, je, 1, Rr.lm, 0, CFL_CAP[0] - 1, 1, TM5_REHEARSAL, mk, SPEC_NONE, fuse));
+        return tm5_zoom_mix_edges(c, &g.d, 0);
+    }
+    if (op == 'y') {
         TTRY(put_edges(t, r, 1, 0));
         int is, ie;
         y_scope(t, r, blk, is, ie);
         const bool zero = second && (g.d.touch_sp != 1 || g.d.touch_np != 1);
-        if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 1, 0, g.slab[1], 1));
-        TTRY(tm5_zoom_advect_y(c, &g.d, r == 1 ? 1 : is, r == 1 ? g.d.im : ie, 0, CFL_CAP[1], 1));
-        if (zero) TTRY(tm5_zoom_outer_faces(c, &g.d, 1, 1, g.slab[1], 1));
-    } else {
-        int is, ie, js, je;
-        z_scope(t, r, blk, is, ie, js, je);
-        TTRY(tm5_zoom_advect_z(c, &g.d, is, ie, js, je, 0, CFL_CAP[2], 1));
-        if (r != 1) TTRY(tm5_zoom_mix_edges(c, &g.d, 0));
+        if (r == 1) return launch_y_mass1(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), sub3(Rr.bm, 1, 1, 1), TM5_REHEARSAL, mk);
+        if (zero && (g.d.touch_sp == 1 || g.d.touch_np == 1)) return TM5_EINVAL;
+        const int fuse = (yedges_on(g.d) ? FUSE_COMPRESS : 0) | (zero ? FUSE_OUTER : 0);
+        TTRY(zoom_y_child(c, Rr, is, ie, 1, Rr.lm, 0, CFL_CAP[1], 1, TM5_REHEARSAL, mk, true, SPEC_REHEARSAL, fuse));
+        return tm5_zoom_mix_edges(c, &g.d, 0);
     }
-    return 0;
+    int is, ie, js, je;
+    z_scope(t, r, blk, is, ie, js, je);
+    if (r == 1) return launch_z_mass1(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), sub3(Rr.cm, 1, 1, 0), TM5_REHEARSAL, mk);
+    TTRY(launch_z_scoped(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), no_tracers(), sub3(Rr.cm, 1, 1, 0), is - 1, ie - 1,
+                         js - 1, je - 1, 0, CFL_CAP[2], TM5_REHEARSAL, mk, true, SPEC_REHEARSAL));
+    return tm5_zoom_mix_edges(c, &g.d, 0);
 }
 
 int cfl_do_steps(tm5_tree *t, int r)

@@ -19,6 +19,8 @@
 // start from 0.0 here too (0.0 + x can change the sign of a zero).
 #include <string.h>
 
+#include <type_traits>
+
 #include "tm5_lib.h"
 
 namespace tm5 {
@@ -276,12 +278,10 @@ __global__ void __launch_bounds__(256) k_mix_edges_lds(ZR C, RingBlocks ring, in
 // ------------------------------------------------------------------ update_parent
 // pydyn/zoom/exchange.py:235-258: the parent's m and rm over the footprint from the child's
 // block sums (from zero, i outer, j inner); f = 0 m, f = t rm of tracer t.
-__global__ void __launch_bounds__(256) k_update_parent(ZR P, ZR C, int nf, int *words, int skip_mask)
+__device__ __forceinline__ void update_parent_at(const ZR &P, const ZR &C, int nf, long long idx)
 {
-    if (zskip(words, skip_mask)) return;
     const int imp = C.im / C.xr, jmp = C.jm / C.yr;
     const long long nb = (long long)imp * jmp;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb * C.lm * nf) return;
     const int ip = (int)(idx % imp), jp = (int)((idx / imp) % jmp);
     const int l = (int)((idx / nb) % C.lm) + 1, f = (int)(idx / (nb * C.lm));
@@ -292,6 +292,12 @@ __global__ void __launch_bounds__(256) k_update_parent(ZR P, ZR C, int nf, int *
     for (int a = 0; a < C.xr; a++)
         for (int b = 0; b < C.yr; b++) acc = acc + src(ip * C.xr + a + 1, jp * C.yr + b + 1, l, t);
     dst(C.ibeg + ip, C.jbeg + jp, l, t) = acc;
+}
+
+__global__ void __launch_bounds__(256) k_update_parent(ZR P, ZR C, int nf, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    update_parent_at(P, C, nf, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ coarsen_block
@@ -376,33 +382,76 @@ __global__ void __launch_bounds__(256) k_coarsen_ambm(ZR P, ZR C, int *words, in
 // zero (it is never written).  pdxyp null: region 1 (m2phlb1).
 // clock_n > 0: the last kernel of a queued run's step, which moves the context's clock on to the
 // next step (clock_n steps an interval; thread 0: nothing else in the kernel reads the clock)
+// One workgroup's tile: TI columns i0 + 1 .. (0-based bx TI ..) of row j + 1, as k_m2phlb1 tiles
+// them: the columns' masses staged in LDS (lm TI doubles, then TI for sp), each column's sum from
+// level 1 up by one thread, then phlb of every level in parallel.
+__device__ __forceinline__ void m2phlb_tile(const ZR &R, const double *pdxyp, const double *at, const double *bt,
+                                            int bx, int j, int TI, double *sh)
+{
+    double *M = sh, *P = sh + R.lm * TI;
+    const int i0 = bx * TI, lm = R.lm;
+    const int ni = min(TI, R.im - i0);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int idx = tid; idx < lm * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti < ni) M[idx] = R.m(i0 + ti + 1, j + 1, l + 1);
+    }
+    __syncthreads();
+    for (int ti = tid; ti < ni; ti += nth) {
+        const int i = i0 + ti + 1, jj = j + 1;  // Fortran
+        double p = M[ti];
+        for (int l = 1; l < lm; l++) p = p + M[l * TI + ti];
+        p = p * 9.80665 / R.dxyp[jj - 1];
+        if (pdxyp) {
+            const int xr = R.xr, yr = R.yr;
+            const bool inner_i = xr < i && i < R.im - xr + 1, inner_j = yr < jj && jj < R.jm - yr + 1;
+            const bool keep = (inner_i && inner_j) || (R.xcyc == 1 && inner_j) || (R.touch_np == 1 && jj > R.jm - yr) ||
+                              (R.touch_sp == 1 && jj < yr + 1);
+            if (!keep) {
+                const int jpar = R.jbeg + (jj - 1) / yr;
+                p = p * R.dxyp[jj - 1] * (double)xr * (double)yr / pdxyp[jpar - 1];
+            }
+        }
+        R.sp(i, jj, 1) = p;
+        P[ti] = p;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < (lm + 1) * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti < ni) R.phlb(i0 + ti + 1, j + 1, l + 1) = at[l] + bt[l] * P[ti];
+    }
+}
+
+// grid (tiles of TI columns, jm); clock_n > 0: the last kernel of a queued run's step, which moves
+// the context's clock on to the next step (clock_n steps an interval; thread 0: nothing else in the
+// kernel reads the clock)
 __global__ void __launch_bounds__(256) k_m2phlb_zoom(ZR R, const double *pdxyp, const double *at, const double *bt,
-                                                     int *words, int skip_mask, int clock_n)
+                                                     int TI, int *words, int skip_mask, int clock_n)
 {
     if (zskip(words, skip_mask)) return;
-    if (clock_n > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    extern __shared__ double msh[];
+    if (clock_n > 0 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         const int s = words[W_CLOCK_S] + 1;
         words[W_CLOCK_S] = s < clock_n ? s : 0;
         if (s >= clock_n) words[W_CLOCK_K] += 1;
     }
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= R.im * R.jm) return;
-    const int i = col % R.im + 1, j = col / R.im + 1;
-    double p = R.m(i, j, 1);
-    for (int l = 2; l <= R.lm; l++) p = p + R.m(i, j, l);
-    p = p * 9.80665 / R.dxyp[j - 1];
-    if (pdxyp) {
-        const int xr = R.xr, yr = R.yr;
-        const bool inner_i = xr < i && i < R.im - xr + 1, inner_j = yr < j && j < R.jm - yr + 1;
-        const bool keep = (inner_i && inner_j) || (R.xcyc == 1 && inner_j) || (R.touch_np == 1 && j > R.jm - yr) ||
-                          (R.touch_sp == 1 && j < yr + 1);
-        if (!keep) {
-            const int jpar = R.jbeg + (j - 1) / yr;
-            p = p * R.dxyp[j - 1] * (double)xr * (double)yr / pdxyp[jpar - 1];
-        }
+    m2phlb_tile(R, pdxyp, at, bt, blockIdx.x, blockIdx.y, TI, msh);
+}
+
+// the end of a child's Proces_Region in one launch: update_parent (the parent's m, rm over the
+// footprint from the child's sums) and the child's m2phlb (its own sp, phlb from its m), which
+// touch disjoint arrays -- the first nub workgroups the one, the rest the tiles of the other
+__global__ void __launch_bounds__(256) k_update_m2phlb(ZR P, ZR C, int nf, int nub, const double *at,
+                                                       const double *bt, int TI, int *words, int skip_mask)
+{
+    if (zskip(words, skip_mask)) return;
+    extern __shared__ double msh[];
+    if ((int)blockIdx.x < nub) {
+        update_parent_at(P, C, nf, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+        return;
     }
-    R.sp(i, j, 1) = p;
-    for (int l = 1; l <= R.lm + 1; l++) R.phlb(i, j, l) = at[l - 1] + bt[l - 1] * p;
+    const int t = blockIdx.x - nub, nbx = (C.im + TI - 1) / TI;
+    m2phlb_tile(C, P.dxyp, at, bt, t % nbx, t / nbx, TI, msh);
 }
 
 // ------------------------------------------------------------------ region 1's x halos
@@ -470,13 +519,61 @@ __global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask
 // n-1 faces am(xref-1..im-xref+1).  The row's sub-cycle search scales its faces by (n-1)/n in
 // place per retry (kernels.py:169-199 with the ends held), the limiter runs on every cell of the
 // row first, the sub-steps update the interior cells, and the faces are multiplied back.
-template <bool MASS_ONLY>
+// X_COMPRESS: compress_xedges before and uncompress_xedges after (pydyn/zoom/exchange.py:109-144) made
+// on the row itself: the row's end cells read as compress leaves them (cell 0 the halo cell, cell 1
+// the sum of cells 1..xref for m and rm, the east end alike), and cells 1..xref (im-xref+1..im)
+// written as uncompress leaves them, the cell's value over xref -- the cells the two kernels write
+// and nothing reads in between.  X_OUTER: outer_x_faces_zeroed (pydyn/zoom/advect.py:121-136): the
+// row's first and last faces taken as zero and left in memory as they are (zeroed, then restored).
+constexpr int X_COMPRESS = FUSE_COMPRESS, X_OUTER = FUSE_OUTER;
+
+// cell c of a child's x row (Fortran i0 + c, i0 = xref - 1) as compress_xedges leaves it
+__device__ __forceinline__ double xrow_cell(const HV &v, int c, int n, int i0, int r, int im, int j, int l, int t,
+                                            bool compress, bool sum)
+{
+    if (compress) {
+        if (c == 0) return v(0, j, l, t);
+        if (c == n - 1) return v(im + 1, j, l, t);
+        if (sum && c == 1) {
+            double s = v(1, j, l, t);
+            for (int k = 2; k <= r; k++) s = s + v(k, j, l, t);
+            return s;
+        }
+        if (sum && c == n - 2) {
+            double s = v(im - r + 1, j, l, t);
+            for (int k = im - r + 2; k <= im; k++) s = s + v(k, j, l, t);
+            return s;
+        }
+    }
+    return v(i0 + c, j, l, t);
+}
+
+// cell c of a child's x row written back: as uncompress_xedges leaves it, or (uncompressed) as is;
+// the ends (the parent's cells) only when held is false
+__device__ __forceinline__ void xrow_store(const HV &v, int c, int n, int i0, int r, int im, int j, int l, int t,
+                                           bool compress, bool ends, double x)
+{
+    if (compress) {
+        if (c == 0 || c == n - 1) return;  // inside 1..xref / im-xref+1..im: uncompress writes them
+        if (c == 1 || c == n - 2) {
+            const double lo = x / (double)r;
+            const int k0 = c == 1 ? 1 : im - r + 1;
+            for (int k = k0; k < k0 + r; k++) v(k, j, l, t) = lo;
+            return;
+        }
+    }
+    if (!ends && (c == 0 || c == n - 1)) return;
+    v(i0 + c, j, l, t) = x;
+}
+
+template <bool MASS_ONLY, int FUSE>
 __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
                                                 int fail_bit, int *words, int skip_mask, int tchunk, int spec_k)
 {
     // tchunk > 0: tracer ranges (grid y), several workgroups a row, each searching and moving the
     // row's mass on its own; none writes m or am, the caller runs the mass-only form afterwards
     // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail)
+    // FUSE: X_COMPRESS, X_OUTER
     if (zskip(words, skip_mask)) return;
     extern __shared__ double lds[];
     const int nj = je - js + 1;
@@ -488,10 +585,12 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
     const int lane = threadIdx.x;
     const int i0 = C.xr - 1;  // Fortran i of the row's cell 0
     const int n = C.im - 2 * C.xr + 4, nf = n - 1;
+    constexpr bool cz = (FUSE & X_COMPRESS) != 0, oz = (FUSE & X_OUTER) != 0;
+    const int xr = C.xr, im = C.im;
     double *M = lds, *M0 = M + n, *A = M0 + n, *R = A + nf, *F = R + 4 * n;
 
-    for (int c = lane; c < n; c += 64) M0[c] = M[c] = C.m(i0 + c, j, l);
-    for (int k = lane; k < nf; k += 64) A[k] = C.am(i0 + k, j, l);
+    for (int c = lane; c < n; c += 64) M0[c] = M[c] = xrow_cell(C.m, c, n, i0, xr, im, j, l, 1, cz, true);
+    for (int k = lane; k < nf; k += 64) A[k] = oz && (k == 0 || k == nf - 1) ? 0.0 : C.am(i0 + k, j, l);
     __syncthreads();
 
     int nl = 1;
@@ -532,7 +631,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
             for (int c = lane + 1; c < n - 1; c += 64) M[c] = (M[c] + A[c - 1]) - A[c];
             __syncthreads();
         }
-        for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
+        for (int c = lane + 1; c < n - 1; c += 64) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
     } else {
         // each tracer's moments of the lane's (at most two) cells are loaded during the sub-steps
         // of the tracer before
@@ -545,7 +644,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                 const int c = lane + 64 * k;
                 if (c < n)
 #pragma unroll
-                    for (int a = 0; a < 4; a++) qp[k][a] = C.q[a](i0 + c, j, l, t);
+                    for (int a = 0; a < 4; a++) qp[k][a] = xrow_cell(C.q[a], c, n, i0, xr, im, j, l, t, cz, a == 0);
             }
         };
         if (pre && t0 <= t1) fetch(t0);
@@ -562,10 +661,10 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                     ry = k == 0 ? qp[0][2] : qp[1][2];
                     rz = k == 0 ? qp[0][3] : qp[1][3];
                 } else {
-                    rm = C.q[0](i0 + c, j, l, t);
-                    rx = C.q[1](i0 + c, j, l, t);
-                    ry = C.q[2](i0 + c, j, l, t);
-                    rz = C.q[3](i0 + c, j, l, t);
+                    rm = xrow_cell(C.q[0], c, n, i0, xr, im, j, l, t, cz, true);
+                    rx = xrow_cell(C.q[1], c, n, i0, xr, im, j, l, t, cz, false);
+                    ry = xrow_cell(C.q[2], c, n, i0, xr, im, j, l, t, cz, false);
+                    rz = xrow_cell(C.q[3], c, n, i0, xr, im, j, l, t, cz, false);
                 }
                 R[c] = rm;
                 R[n + c] = limits ? limit(rx, rm) : rx;
@@ -602,17 +701,16 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                 __syncthreads();
             }
             for (int c = lane; c < n; c += 64) {
-                C.q[0](i0 + c, j, l, t) = R[c];
-                C.q[1](i0 + c, j, l, t) = R[n + c];
-                C.q[2](i0 + c, j, l, t) = R[2 * n + c];
-                C.q[3](i0 + c, j, l, t) = R[3 * n + c];
+#pragma unroll
+                for (int a = 0; a < 4; a++) xrow_store(C.q[a], c, n, i0, xr, im, j, l, t, cz, true, R[a * n + c]);
             }
             __syncthreads();
         }
         if (tchunk > 0) return;
-        for (int c = lane + 1; c < n - 1; c += 64) C.m(i0 + c, j, l) = M[c];
+        for (int c = lane + 1; c < n - 1; c += 64) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
     }
-    for (int k = lane; k < nf; k += 64) C.am(i0 + k, j, l) = A[k] * (double)nl;  // "restore old am"
+    for (int k = lane; k < nf; k += 64)
+        if (!(oz && (k == 0 || k == nf - 1))) C.am(i0 + k, j, l) = A[k] * (double)nl;  // "restore old am"
 }
 
 // ------------------------------------------------------------------ a child's y columns
@@ -674,6 +772,49 @@ __global__ void __launch_bounds__(1024) k_y_child_search(ZR C, int is, int ie, i
     if (tid == 0) nloop[blockIdx.x] = nl;
 }
 
+// Where a cell of a child's y column (Fortran j0 + c, j0 = yref - 1) comes from and goes to when
+// compress_yedges runs before and uncompress_yedges after (pydyn/zoom/exchange.py:147-177): the
+// halo cell moved next to the interface (row 0 or jm + 1), the interface's cells 1..yref
+// (jm-yref+1..jm) summed into one for m and rm, and afterwards the interface cell's value over
+// yref on each of those cells; the moved halo cells lie inside them and are not stored.
+struct YCell {
+    int row;      // the row read for rxm, rym, rzm (and m, rm when sum == 1)
+    int sum0;     // the first row summed for m and rm ...
+    int sum;      // ... and how many
+    int store0;   // the first row written ...
+    int stores;   // ... how many (0: none), each the value over `div`
+    double div;
+};
+
+__device__ __forceinline__ YCell ycell(int c, int n, int j0, int r, int jm, bool compress)
+{
+    YCell y{j0 + c, j0 + c, 1, j0 + c, 1, 1.0};
+    if (!compress) return y;
+    if (c == 0 || c == n - 1) {
+        y.row = y.sum0 = c == 0 ? 0 : jm + 1;
+        y.stores = 0;
+    } else if (c == 1 || c == n - 2) {
+        y.sum0 = y.store0 = c == 1 ? 1 : jm - r + 1;
+        y.sum = y.stores = r;
+        y.div = (double)r;
+    }
+    return y;
+}
+
+__device__ __forceinline__ double ycol_load(const HV &v, const YCell &y, int i, int l, int t, bool sum)
+{
+    if (!sum) return v(i, y.row, l, t);
+    double s = v(i, y.sum0, l, t);
+    for (int k = 1; k < y.sum; k++) s = s + v(i, y.sum0 + k, l, t);
+    return s;
+}
+
+__device__ __forceinline__ void ycol_store(const HV &v, const YCell &y, int i, int l, int t, double x)
+{
+    const double o = y.stores > 1 ? x / y.div : x;
+    for (int k = 0; k < y.stores; k++) v(i, y.store0 + k, l, t) = o;
+}
+
 // The sub-steps of the layer's columns: one thread per cell of TI whole columns; each sub-step
 // every thread reads its cell's neighbourhood, computes its two faces' fluxes and its update,
 // then -- after the workgroup's barrier -- writes it.  The end cells are held; the limiter of
@@ -684,7 +825,12 @@ __global__ void __launch_bounds__(1024) k_y_child_search(ZR C, int is, int ie, i
 // the mass-only form afterwards.
 // nloop null: every layer at nloop 1 (a Check_CFL rehearsal at cap 1 passed, or a checked step:
 // checked(spec_k), which first makes that rehearsal's test of every face of the columns at nloop 1).
-template <bool MASS_ONLY>
+// fuse (nloop null only): Y_COMPRESS -- compress_yedges before and uncompress_yedges after made on
+// the columns themselves (a workgroup holds whole columns); Y_OUTER -- outer_y_faces_zeroed
+// (pydyn/zoom/advect.py:139-157): the columns' first and last faces taken as zero, left in memory.
+constexpr int Y_COMPRESS = FUSE_COMPRESS, Y_OUTER = FUSE_OUTER;
+
+template <bool MASS_ONLY, int FUSE>
 __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, int ls, int TI, const int *nloop,
                                                         int limits, int *words, int skip_mask, int tchunk, int spec_k)
 {
@@ -704,32 +850,39 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
     const int jc = j0 + cc;  // Fortran j of the cell
     const bool inner = cc > 0 && cc < n - 1;
     const int nl = nloop ? nloop[by] : 1;
+    constexpr bool cz = (FUSE & Y_COMPRESS) != 0, oz = (FUSE & Y_OUTER) != 0;
+    const int yr = C.yr, jm = C.jm;
+    // the column's masses as the step starts (compressed when cz)
+    const YCell yc = ycell(cc, n, j0, yr, jm, cz);
+    auto mass = [&](int k) { return ycol_load(C.m, ycell(k, n, j0, yr, jm, cz), ii, l, 1, true); };
     if (checked(spec_k)) {
         // the rehearsal's dynamvm at cap 1 (pydyn/zoom/check_cfl.py:171-201): the face above each
         // cell but the last, |b / m(upwind)| < 1 on the masses as they are before the step
         bool ok = true;
         if (mine && cc < n - 1) {
-            const double bk = C.bm(ii, jc + 1, l);
-            ok = fabs(bk / C.m(ii, bk >= 0.0 ? jc : jc + 1, l)) < 1.0;
+            const double bk = oz && (cc == 0 || cc == n - 2) ? 0.0 : C.bm(ii, jc + 1, l);
+            ok = fabs(bk / mass(bk >= 0.0 ? cc : cc + 1)) < 1.0;
         }
         if (__any(!ok) && (threadIdx.x & 63) == 0) spec_fail(words, spec_k);
     }
     // faces below (bm(jc)) and above (bm(jc+1)) an interior cell
-    const double b_lo = inner ? C.bm(ii, jc, l) : 0.0, b_hi = inner ? C.bm(ii, jc + 1, l) : 0.0;
+    const double b_lo = inner && !(oz && cc == 1) ? C.bm(ii, jc, l) : 0.0;
+    const double b_hi = inner && !(oz && cc == n - 2) ? C.bm(ii, jc + 1, l) : 0.0;
     // each tracer's moments of the workgroup's cells pass through LDS ([moment][cell][ti]): every
     // cell is read from HBM once and its neighbours from LDS
     extern __shared__ double yst[];
     const int stride = n * TI, me = cc * TI + ti;
     const int u_lo = b_lo >= 0.0 ? cc - 1 : cc, u_hi = b_hi >= 0.0 ? cc : cc + 1;  // cells of the line
-    double m_cur = C.m(ii, jc, l);
+    const bool staged = tchunk > 0 || cz;  // the masses of the sub-step from LDS, not memory
+    double m_cur = ycol_load(C.m, yc, ii, l, 1, true);
     for (int s = 0; s < nl; s++) {
-        const double m_old = MASS_ONLY || tchunk == 0 ? C.m(ii, jc, l) : m_cur;
+        const double m_old = (MASS_ONLY || tchunk == 0) && !cz ? C.m(ii, jc, l) : m_cur;
         const double m_new = (m_old + b_lo) - b_hi;
         if (!MASS_ONLY) {
             const bool lim0 = limits && s == 0;
             // the Courant numbers of the sub-step, the same for every tracer
             double beta_lo, beta_hi;
-            if (tchunk > 0) {  // the line's masses at the sub-step's start, from LDS
+            if (staged) {  // the line's masses at the sub-step's start, from LDS
                 double *ym = yst + 4 * stride;
                 if (mine) ym[me] = m_old;
                 __syncthreads();
@@ -742,12 +895,12 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
             // each tracer's moments are loaded while the tracer before it is worked on
             double nx[4] = {0.0, 0.0, 0.0, 0.0};
             if (t0 <= t1) {
-                for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, t0);
+                for (int k = 0; k < 4; k++) nx[k] = ycol_load(C.q[k], yc, ii, l, t0, k == 0);
             }
             for (int t = t0; t <= t1; t++) {
                 const double rm = nx[0], rx0 = nx[1], ry0 = nx[2], rz0 = nx[3];
                 if (t < t1) {
-                    for (int k = 0; k < 4; k++) nx[k] = C.q[k](ii, jc, l, t + 1);
+                    for (int k = 0; k < 4; k++) nx[k] = ycol_load(C.q[k], yc, ii, l, t + 1, k == 0);
                 }
                 if (mine) {
                     yst[me] = rm;
@@ -783,17 +936,19 @@ __global__ void __launch_bounds__(1024) k_y_child_steps(ZR C, int is, int ie, in
                     out3 = out3 + (lo.fw - hi.fw);
                 }
                 if (mine && (inner || lim0)) {  // no other thread reads the cell from memory
-                    C.q[0](ii, jc, l, t) = out0;
-                    C.q[1](ii, jc, l, t) = out1;
-                    C.q[2](ii, jc, l, t) = out2;
-                    C.q[3](ii, jc, l, t) = out3;
+                    ycol_store(C.q[0], yc, ii, l, t, out0);
+                    ycol_store(C.q[1], yc, ii, l, t, out1);
+                    ycol_store(C.q[2], yc, ii, l, t, out2);
+                    ycol_store(C.q[3], yc, ii, l, t, out3);
                 }
                 __syncthreads();  // the staged values are read before the next tracer's land
             }
             m_cur = m_new;
+        } else {
+            m_cur = m_new;
         }
         __syncthreads();
-        if (mine && inner && (MASS_ONLY || tchunk == 0)) C.m(ii, jc, l) = m_new;
+        if (mine && inner && (MASS_ONLY || tchunk == 0)) ycol_store(C.m, yc, ii, l, 1, m_new);
         __threadfence_block();
         __syncthreads();
     }
@@ -965,6 +1120,26 @@ int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers)
     return status(hipGetLastError());
 }
 
+// the tile of the m2phlb kernels (the context's m2phlb1 tile)
+static int m2phlb_ti(const tm5_ctx *ctx, int lm)
+{
+    int TI = ctx->tile[TM5_TILE_M2PHLB];
+    while (TI > 1 && (lm + 1) * TI > 8192) TI /= 2;
+    return TI;
+}
+
+// update_parent (with tracers) and the child's m2phlb in one launch
+int zoom_update_m2phlb(tm5_ctx *ctx, const ZR &P, const ZR &C, const double *at, const double *bt, int skip_mask)
+{
+    const int nf = 1 + C.nt;
+    const long long nu = (long long)(C.im / C.xr) * (C.jm / C.yr) * C.lm * nf;
+    const int TI = m2phlb_ti(ctx, C.lm), nub = (int)((nu + 255) / 256);
+    const long long tiles = (long long)((C.im + TI - 1) / TI) * C.jm;
+    TM5_LAUNCH(ctx, k_update_m2phlb, dim3((unsigned)(nub + tiles)), dim3(256), (size_t)(C.lm + 1) * TI * sizeof(double),
+               ctx->stream, P, C, nf, nub, at, bt, TI, ctx->words, zm(ctx, skip_mask));
+    return status(hipGetLastError());
+}
+
 int zoom_coarsen(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
 {
     const long long imp = C.im / C.xr, jmp = C.jm / C.yr;
@@ -983,8 +1158,10 @@ int zoom_coarsen_ambm(tm5_ctx *ctx, const ZR &P, const ZR &C, int skip_mask)
 int zoom_m2phlb(tm5_ctx *ctx, const ZR &R, const double *pdxyp, const double *at, const double *bt, int skip_mask,
                 int clock_n)
 {
-    TM5_LAUNCH(ctx, k_m2phlb_zoom, zgrid((long long)R.im * R.jm), dim3(256), 0, ctx->stream, R, pdxyp, at, bt,
-               ctx->words, zm(ctx, skip_mask), clock_n);
+    const int TI = m2phlb_ti(ctx, R.lm);
+    TM5_LAUNCH(ctx, k_m2phlb_zoom, dim3((unsigned)((R.im + TI - 1) / TI), (unsigned)R.jm), dim3(256),
+               (size_t)(R.lm + 1) * TI * sizeof(double), ctx->stream, R, pdxyp, at, bt, TI, ctx->words,
+               zm(ctx, skip_mask), clock_n);
     return status(hipGetLastError());
 }
 
@@ -1023,9 +1200,11 @@ int zoom_am_halo(tm5_ctx *ctx, const ZR &R, int skip_mask)
     return status(hipGetLastError());
 }
 
-// spec_k: a checked step's x (the search capped at nmax, a failure spec_fail)
-int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask, int spec_k)
+// spec_k: a checked step's x (the search capped at nmax, a failure spec_fail); FUSE: X_COMPRESS,
+// X_OUTER (the compress / uncompress and the outer faces' zeroing made inside the kernel)
+template <int FUSE>
+static int x_child_f(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
+                     int fail_bit, int skip_mask, int spec_k)
 {
     const int n = C.im - 2 * C.xr + 4;
     if (n < 3 || n > 512 || je < js || le < ls) return TM5_EINVAL;  // LDS: 11 n doubles
@@ -1034,26 +1213,42 @@ int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int 
     int ranges = 1, tchunk = C.nt;
     if (!mass_only && C.nt >= 32) {  // many tracers: tracer ranges over the grid, then the masses
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false>, 64, lds) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false, FUSE>, 64, lds) != hipSuccess ||
+            per_cu < 1)
             per_cu = 1;
         tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), C.nt, 16);
         ranges = (C.nt + tchunk - 1) / tchunk;
     }
     if (!mass_only)
-        TM5_LAUNCH(ctx, k_x_child<false>, dim3(grid.x, ranges), dim3(64), lds, ctx->stream, C, js, je, ls, le, limits,
-                   nmax, fail_bit, ctx->words, zm(ctx, skip_mask), ranges > 1 ? tchunk : 0, spec_k);
+        TM5_LAUNCH_NAMED(ctx, "k_x_child<false>", (k_x_child<false, FUSE>), dim3(grid.x, ranges), dim3(64), lds,
+                         ctx->stream, C, js, je, ls, le, limits, nmax, fail_bit, ctx->words, zm(ctx, skip_mask),
+                         ranges > 1 ? tchunk : 0, spec_k);
     if (mass_only || ranges > 1)
-        TM5_LAUNCH(ctx, k_x_child<true>, grid, dim3(64), lds, ctx->stream, C, js, je, ls, le, 0, nmax, fail_bit,
-                   ctx->words, zm(ctx, skip_mask), 0, spec_k);
+        TM5_LAUNCH_NAMED(ctx, "k_x_child<true>", (k_x_child<true, FUSE>), grid, dim3(64), lds, ctx->stream, C, js, je,
+                         ls, le, 0, nmax, fail_bit, ctx->words, zm(ctx, skip_mask), 0, spec_k);
     return status(hipGetLastError());
 }
 
+int zoom_x_child(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, int limits, int nmax, int mass_only,
+                 int fail_bit, int skip_mask, int spec_k, int fuse)
+{
+    switch (fuse) {
+    case 0: return x_child_f<0>(ctx, C, js, je, ls, le, limits, nmax, mass_only, fail_bit, skip_mask, spec_k);
+    case 1: return x_child_f<1>(ctx, C, js, je, ls, le, limits, nmax, mass_only, fail_bit, skip_mask, spec_k);
+    case 2: return x_child_f<2>(ctx, C, js, je, ls, le, limits, nmax, mass_only, fail_bit, skip_mask, spec_k);
+    case 3: return x_child_f<3>(ctx, C, js, je, ls, le, limits, nmax, mass_only, fail_bit, skip_mask, spec_k);
+    default: return TM5_EINVAL;
+    }
+}
+
 // known_one: every layer at nloop 1 without a search (Check_CFL's rehearsal at cap 1 passed on
-// the same bits); checked(spec_k): a checked step, whose sub-step kernel makes that test itself
+// the same bits); checked(spec_k): a checked step, whose sub-step kernel makes that test itself;
+// fuse (known_one only): Y_COMPRESS, Y_OUTER made inside the sub-step kernel
 int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int limits, int nmax, int mass_only,
-                 int fail_bit, int skip_mask, bool known_one, int spec_k)
+                 int fail_bit, int skip_mask, bool known_one, int spec_k, int fuse)
 {
     if (checked(spec_k)) known_one = true;
+    if (fuse && !known_one) return TM5_EINVAL;  // a search reads the faces and masses from memory
     const int n = C.jm - 2 * C.yr + 4;
     const int nlay = le - ls + 1;
     if (n < 3 || n > 1024 || ie < is || nlay < 1 || nlay > ctx->y_cap) return TM5_EINVAL;
@@ -1074,13 +1269,24 @@ int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int 
                            ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, C.nt, 16)
                            : C.nt;
     const int ranges = C.nt > 0 ? (C.nt + tchunk - 1) / tchunk : 1;
-    if (!mass_only)
-        TM5_LAUNCH(ctx, k_y_child_steps<false>, dim3(grid.x, grid.y, ranges), block,
-                   (size_t)5 * n * TI * sizeof(double), ctx->stream, C, is, ie, ls, TI, nloop, limits, ctx->words,
-                   zm(ctx, skip_mask), ranges > 1 ? tchunk : 0, spec_k);
-    if (mass_only || ranges > 1)
-        TM5_LAUNCH(ctx, k_y_child_steps<true>, grid, block, 0, ctx->stream, C, is, ie, ls, TI, nloop, 0, ctx->words,
-                   zm(ctx, skip_mask), 0, mass_only ? spec_k : SPEC_NONE);
+    auto steps = [&](auto fz) {
+        constexpr int F = decltype(fz)::value;
+        if (!mass_only)
+            TM5_LAUNCH_NAMED(ctx, "k_y_child_steps<false>", (k_y_child_steps<false, F>), dim3(grid.x, grid.y, ranges),
+                             block, (size_t)5 * n * TI * sizeof(double), ctx->stream, C, is, ie, ls, TI, nloop, limits,
+                             ctx->words, zm(ctx, skip_mask), ranges > 1 ? tchunk : 0, spec_k);
+        if (mass_only || ranges > 1)
+            TM5_LAUNCH_NAMED(ctx, "k_y_child_steps<true>", (k_y_child_steps<true, F>), grid, block, 0, ctx->stream, C,
+                             is, ie, ls, TI, nloop, 0, ctx->words, zm(ctx, skip_mask), 0,
+                             mass_only ? spec_k : SPEC_NONE);
+    };
+    switch (fuse) {
+    case 0: steps(std::integral_constant<int, 0>{}); break;
+    case 1: steps(std::integral_constant<int, 1>{}); break;
+    case 2: steps(std::integral_constant<int, 2>{}); break;
+    case 3: steps(std::integral_constant<int, 3>{}); break;
+    default: return TM5_EINVAL;
+    }
     if (known_one) return status(hipGetLastError());  // times 1 is the identity: nothing to restore
     const int n0 = C.im + 2, n1 = C.jm + 2;
     TM5_LAUNCH(ctx, k_y_child_restore, dim3((unsigned)((n0 * n1 + 255) / 256), (unsigned)nlay), dim3(256), 0,
@@ -1268,7 +1474,7 @@ extern "C" int tm5_zoom_advect_x(tm5_ctx *c, const tm5_zregion *r, int js, int j
     const int tr = mass_only ? 0 : 1;
     ZTRY(tm5_zoom_compress_xedges(c, r, js, je, 1, R.lm, tr));
     ZTRY(zoom_x_child(c, R, js, je, 1, R.lm, limits, max_nloop - 1, mass_only, mass_only ? TM5_REHEARSAL : TM5_FAIL_XC,
-                      skip, SPEC_NONE));
+                      skip, SPEC_NONE, 0));
     if (mass_only) {  // dynamum's own uncompressm and mixm_edges (pydyn/zoom/check_cfl.py:143-144)
         ZTRY(tm5_zoom_uncompress_xedges(c, r, js, je, 1, R.lm, 0));
         ZTRY(tm5_zoom_mix_edges(c, r, 0));
@@ -1290,7 +1496,7 @@ extern "C" int tm5_zoom_advect_y(tm5_ctx *c, const tm5_zregion *r, int is, int i
     const int tr = mass_only ? 0 : 1;
     ZTRY(tm5_zoom_compress_yedges(c, r, is, ie, 1, R.lm, tr));
     ZTRY(zoom_y_child(c, R, is, ie, 1, R.lm, limits, mass_only ? max_nloop : max_nloop - 1, mass_only,
-                      mass_only ? TM5_REHEARSAL : TM5_FAIL_YC, skip, false, SPEC_NONE));
+                      mass_only ? TM5_REHEARSAL : TM5_FAIL_YC, skip, false, SPEC_NONE, 0));
     if (mass_only) {
         ZTRY(tm5_zoom_uncompress_yedges(c, r, is, ie, 1, R.lm, 0));
         ZTRY(tm5_zoom_mix_edges(c, r, 0));

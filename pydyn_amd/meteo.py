@@ -333,6 +333,9 @@ _TINY_CHILD = Child("tiny_child", 8, 8, -90.0, 30.0, -45.0, 45.0, 2, 2, 2, 4, 3)
 _SMALL_CHILD = Child("small_child", 24, 18, -120.0, 60.0, -56.25, 45.0, 2, 2, 2, 5, 4)
 # glb3x2 with a 1x1 degree child over 170W-50W, 10N-80N: parent cells 4..43 x 51..85
 _NAM_CHILD = Child("nam1x1", 120, 70, -170.0, -50.0, 10.0, 80.0, 3, 2, 2, 4, 51)
+# the reference's benchmarked tree's child (pydyn/gpu/bench.py:34-37): nam1x1 over 132W-60W,
+# 20N-64N, 72 x 44, parent cells 17..40 x 56..77
+_NAM72_CHILD = Child("nam1x1_72x44", 72, 44, -132.0, -60.0, 20.0, 64.0, 3, 2, 2, 17, 56)
 # glb3x2's reduced grid as production runs it: rows (1-based) and their cluster sizes
 GLB3X2_REDGRID = ((1, 2, 3, 4, 5, 6, 7, 84, 85, 86, 87, 88, 89, 90), (40, 8, 8, 4, 4, 4, 2, 2, 4, 4, 4, 8, 8, 40))
 
@@ -344,6 +347,8 @@ ZOOMS = {
     "tiny2r": ("tiny", 3, _TINY_CHILD, ((1, 2, 8), (12, 4, 3))),
     "small2r": ("small", 5, _SMALL_CHILD, ((1, 2, 3, 14, 15, 16), (24, 8, 4, 4, 8, 24))),
     "c2r": ("glb3x2", 1, _NAM_CHILD, GLB3X2_REDGRID),
+    # the tree the reference publishes its device timings on: glb3x2 on its reduced grid + nam1x1
+    "pub": ("glb3x2", 1, _NAM72_CHILD, GLB3X2_REDGRID),
 }
 
 
