@@ -432,9 +432,17 @@ int tm5_tree_interval(tm5_tree *tree, double t0, int nread, int ndyn_max, int *n
    submission is Pipeline.submit (pydyn/gpu/zoom.py:944-1041). */
 int tm5_tree_run(tm5_tree *tree, double t0, int nread, int ndyn_max, int count, int *ndyn, int *n);
 /* Options: TM5_TREE_GRAPHS (1: capture and replay steps, the default; 0: issue every launch),
-   TM5_TREE_BATCH (most intervals queued per read of the device, 1..64; 16). */
+   TM5_TREE_BATCH (most intervals queued per read of the device, 1..64; 16), TM5_TREE_MODE: how a
+   queued interval is verified -- TM5_TREE_CHECKED (checked steps, a backup of every region's m
+   and moments per interval), TM5_TREE_REHEARSED (the planned trial rehearsed mass-only and its
+   verdict checked on the device before plain steps; no backup) or TM5_TREE_AUTO (checked while
+   the backup is at most 512 MB, the default). */
 #define TM5_TREE_GRAPHS 0
 #define TM5_TREE_BATCH 1
+#define TM5_TREE_MODE 2
+#define TM5_TREE_AUTO 0
+#define TM5_TREE_CHECKED 1
+#define TM5_TREE_REHEARSED 2
 int tm5_tree_option(tm5_tree *tree, int which, int value);
 /* (queued intervals kept, redone, steps replayed from graphs, graphs captured) since creation */
 int tm5_tree_tally(tm5_tree *tree, long long *kept, long long *redone, long long *replayed, long long *captured);

@@ -117,7 +117,8 @@ class TreeDesc(ctypes.Structure):
     ]
 
 
-TREE_GRAPHS, TREE_BATCH = 0, 1  # tm5_tree_option
+TREE_GRAPHS, TREE_BATCH, TREE_MODE = 0, 1, 2  # tm5_tree_option
+TREE_AUTO, TREE_CHECKED, TREE_REHEARSED = 0, 1, 2  # its TREE_MODE values
 
 _P = ctypes.POINTER
 _I, _D, _VP = ctypes.c_int, ctypes.c_double, ctypes.c_void_p

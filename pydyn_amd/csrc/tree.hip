@@ -109,14 +109,14 @@ __global__ void k_tree_word(int *words, int index, int value, int mask)
     if (threadIdx.x == 0) words[index] = (words[index] & ~mask) | (value & mask);
 }
 
-// A queued trial's verdict against the plan (the trial is expected to fail): a passing trial makes
-// every kernel queued after it skip.  The rehearsal bit is cleared.
-__global__ void k_tree_spec_check(int *words, int k)
+// A queued trial's verdict against the plan (expect_fail: the plan cuts the step after it): another
+// verdict makes every kernel queued after it skip.  The rehearsal bit is cleared.
+__global__ void k_tree_spec_check(int *words, int k, int expect_fail)
 {
     if (threadIdx.x != 0) return;
     const int f = words[W_FLAGS];
     int out = f & ~TM5_REHEARSAL;
-    if (!(f & TM5_SPEC) && !(f & TM5_REHEARSAL)) {
+    if (!(f & TM5_SPEC) && ((f & TM5_REHEARSAL) != 0) != (expect_fail != 0)) {
         out |= TM5_SPEC;
         words[W_SPEC_AT] = k;
         words[W_SPEC_KIND] = 0;
@@ -137,7 +137,7 @@ struct TReg {
 };
 
 struct Graph {
-    int ndyn = 0, nread = 0, uses = 0;
+    int ndyn = 0, nread = 0, checked = 0, uses = 0;
     hipGraphExec_t exec = nullptr;
 };
 
@@ -165,6 +165,7 @@ struct tm5_tree {
     double *t0dev = nullptr;   // the queued run's start (the clock's origin)
     // the queue
     int plan = 0, batch = 2, batch_max = 16, graphs_on = 1;
+    int mode = TM5_TREE_AUTO;  // TM5_TREE_MODE: checked steps with a backup, or rehearsed plans
     long long kept = 0, redone = 0, replayed = 0, captured = 0;
     std::vector<Graph> graphs;
     hipStream_t cap = nullptr;  // the capture stream
@@ -719,11 +720,12 @@ int check_cfl(tm5_tree *t, double t0, int nread, int &ndyn, int &n)
     return TM5_EINVAL;  // "check_cfl_zoom: exceeded maximum number of time steps"
 }
 
-// the planned steps of a queued interval, checked: the backup at its first step, Setup_MassFlow at
-// the clock's time, the step; its last kernel moves the clock on
-int checked_step(tm5_tree *t, int ndyn, int nread)
+// a planned step of a queued interval: checked (the backup at its first step, the rehearsal's tests
+// in the kernels) or, when the interval's plan was rehearsed and verified, plain at nloop 1;
+// Setup_MassFlow at the clock's time, the step; its last kernel moves the clock on
+int checked_step(tm5_tree *t, int ndyn, int nread, bool checked_ = true)
 {
-    t->spec = SPEC_CLOCK;
+    t->spec = checked_ ? SPEC_CLOCK : SPEC_NONE;
     t->known_one = true;
     t->clock_n = nread / ndyn;
     t->ts = TimeSrc{};
@@ -731,7 +733,7 @@ int checked_step(tm5_tree *t, int ndyn, int nread)
     t->ts.nread = (double)nread;
     t->ts.period = t->period;
     t->ts.amplitude = t->amp;
-    int rc = backup_restore(t, false);
+    int rc = checked_ ? backup_restore(t, false) : 0;
     if (rc == 0) rc = setup_massflow(t, ndyn, false);
     if (rc == 0) rc = step(t, ndyn);
     t->spec = SPEC_NONE;
@@ -743,15 +745,15 @@ int checked_step(tm5_tree *t, int ndyn, int nread)
 
 // a checked step from its graph: captured the second time its (ndyn, nread) is queued (the first
 // issues every launch, which also sizes every buffer the launches use), replayed after that
-int graph_step(tm5_tree *t, int ndyn, int nread)
+int graph_step(tm5_tree *t, int ndyn, int nread, bool chk)
 {
     tm5_ctx *c = t->ctx;
-    if (!t->graphs_on || c->prof) return checked_step(t, ndyn, nread);
+    if (!t->graphs_on || c->prof) return checked_step(t, ndyn, nread, chk);
     Graph *g = nullptr;
     for (Graph &x : t->graphs)
-        if (x.ndyn == ndyn && x.nread == nread) g = &x;
+        if (x.ndyn == ndyn && x.nread == nread && x.checked == (int)chk) g = &x;
     if (!g) {
-        t->graphs.push_back(Graph{ndyn, nread, 0, nullptr});
+        t->graphs.push_back(Graph{ndyn, nread, (int)chk, 0, nullptr});
         g = &t->graphs.back();
     }
     if (g->exec) {
@@ -759,7 +761,7 @@ int graph_step(tm5_tree *t, int ndyn, int nread)
         t->replayed++;
         return 0;
     }
-    if (g->uses++ == 0) return checked_step(t, ndyn, nread);
+    if (g->uses++ == 0) return checked_step(t, ndyn, nread, chk);
     hipStream_t keep = c->stream;
     c->stream = t->cap;
     hipError_t e = hipStreamBeginCapture(t->cap, hipStreamCaptureModeThreadLocal);
@@ -767,7 +769,7 @@ int graph_step(tm5_tree *t, int ndyn, int nread)
         c->stream = keep;
         return status(e);
     }
-    const int rc = checked_step(t, ndyn, nread);
+    const int rc = checked_step(t, ndyn, nread, chk);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(t->cap, &graph);
     c->stream = keep;
@@ -829,12 +831,18 @@ int tree_interval(tm5_tree *t, double t0, int nread, int ndyn_max, int *ndyn_out
     return 0;
 }
 
+// The queue's mode: checked steps need a backup of every region's m and moments per interval, a
+// rehearsed plan a mass-only rehearsal (whose cost does not grow with the tracers): the backup up
+// to BACKUP_MAX doubles, the rehearsal beyond.
+constexpr size_t BACKUP_MAX = (size_t)64 << 20;  // 512 MB
+
 int tree_run(tm5_tree *t, double t0, int nread, int ndyn_max, int count, int *ndyn_out, int *n_out)
 {
     tm5_ctx *c = t->ctx;
     size_t need = 0;
     for (int r = 1; r <= t->n; r++) need += (size_t)t->reg[r].nm + 4 * (size_t)t->reg[r].nq;
-    if (t->backup_n < need) {
+    const bool chk = t->mode == TM5_TREE_CHECKED || (t->mode == TM5_TREE_AUTO && need <= BACKUP_MAX);
+    if (chk && t->backup_n < need) {
         TTRY(status(hipStreamSynchronize(c->stream)));
         if (t->backup) TTRY(status(hipFree(t->backup)));
         t->backup = nullptr;
@@ -860,15 +868,16 @@ int tree_run(tm5_tree *t, double t0, int nread, int ndyn_max, int count, int *nd
         int rc = 0;
         for (int k = i; k < end && rc == 0; k++) {
             const double tk = t0 + (double)k * (double)nread;
-            // the trials the plan expects to fail are rehearsed and their verdicts checked
-            for (int q = 0; q + 1 < trials && rc == 0; q++) {
+            // the trials the plan expects to fail are rehearsed and their verdicts checked; without
+            // checked steps the planned trial too, expected to pass
+            for (int q = 0; q < trials - (chk ? 1 : 0) && rc == 0; q++) {
                 rc = rehearse(t, tk, nd[q], nn[q], false);  // k_tree_spec_check clears the bit
                 if (rc == 0) {
-                    TM5_LAUNCH(c, k_tree_spec_check, dim3(1), dim3(64), 0, c->stream, c->words, k);
+                    TM5_LAUNCH(c, k_tree_spec_check, dim3(1), dim3(64), 0, c->stream, c->words, k, q + 1 < trials);
                     rc = status(hipGetLastError());
                 }
             }
-            for (int s = 0; s < nn[trials - 1] && rc == 0; s++) rc = graph_step(t, nd[trials - 1], nread);
+            for (int s = 0; s < nn[trials - 1] && rc == 0; s++) rc = graph_step(t, nd[trials - 1], nread, chk);
         }
         c->skip_or = 0;
         TTRY(rc);
@@ -1053,6 +1062,10 @@ extern "C" int tm5_tree_option(tm5_tree *t, int which, int value)
         if (value < 1 || value > 64) return TM5_EINVAL;
         t->batch_max = value;
         if (t->batch > value) t->batch = value;
+        return 0;
+    case TM5_TREE_MODE:
+        if (value != TM5_TREE_AUTO && value != TM5_TREE_CHECKED && value != TM5_TREE_REHEARSED) return TM5_EINVAL;
+        t->mode = value;
         return 0;
     default:
         return TM5_EINVAL;

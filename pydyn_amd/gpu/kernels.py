@@ -15,6 +15,7 @@ from ..device import CFLError  # noqa: F401  (the reference's exception, re-expo
 
 __all__ = [
     "CFLError",
+    "ok_flag",
     "add_2d",
     "advect_x",
     "advect_x_rows",
@@ -352,3 +353,11 @@ def loss_rates(dt_sec, temper, phlb, lat, strat_press, strat_oh, strat_cl, strat
     _lib.check(rc, "tm5_loss_rates")
     ctx.raise_errors()
     return loss, dec
+
+
+def ok_flag(fail):
+    """Check_CFL's verdict on a device fail flag (pydyn/gpu/kernels.py:216-223): True when the
+    flag (a device int array whose first element a failed trial set) is zero, read now.  The
+    reference also defers the read while it speculates; here a queued run verifies its trials
+    on the device itself (tm5_tree_run, tm5_region_run)."""
+    return not int(fail.reshape(-1)[0].item())

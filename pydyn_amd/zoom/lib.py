@@ -1,5 +1,7 @@
 """The bodies the zoom driver runs on a device state: pydyn.gpu.zoom's names and arguments
-(pydyn/gpu/zoom.py:38-74, the list of pydyn/zoom/host.py:42-48), each a call into libtm5hip.
+(pydyn/gpu/zoom.py:38-74, the list of pydyn/zoom/host.py:42-48), each a call into libtm5hip,
+and the state's zeros / upload / download and the queued submission Pipeline (pydyn/gpu/zoom.py:
+77-101, 917-1041).
 
 Every body is asynchronous on the context's stream.  The rehearsal's mass-only bodies do not
 return a verdict: a failure sets TM5_REHEARSAL in the context's flag word, after which every
@@ -12,15 +14,118 @@ from contextlib import contextmanager
 
 from .. import _lib
 from .. import device as D
+from ..dump import FArray
+from .state import HostState, ZoomState
 
 __all__ = [
-    "_coarsen_block", "_dynamu_child", "_dynamu_region1", "_dynamum_child", "_dynamum_region1",
+    "Pipeline", "_coarsen_block", "_dynamu_child", "_dynamu_region1", "_dynamum_child", "_dynamum_region1",
     "_dynamv_child", "_dynamv_region1", "_dynamvm_child", "_dynamvm_region1", "_dynamw_child",
     "_dynamw_region1", "_dynamwm", "_uncompress_xedges", "_uncompress_yedges", "assign", "chemistry_region",
-    "coarsen_ambm", "column_physics", "convec_region", "dynam0", "m2phlb", "m2phlb1", "mix_edges",
+    "coarsen_ambm", "column_physics", "convec_region", "download", "dynam0", "m2phlb", "m2phlb1", "mix_edges",
     "outer_x_faces_zeroed", "outer_y_faces_zeroed", "put_xedges", "put_yedges", "restore", "save", "scale",
-    "sources_region", "update_parent",
+    "sources_region", "update_parent", "upload", "zeros",
 ]  # fmt: skip
+
+
+# ------------------------------------------------------------------ the state on the device
+def zeros(zoom, nt):
+    """Every region's arrays on the device, zero (ZoomState.zeros there; pydyn/gpu/zoom.py:77-88)."""
+    return ZoomState(zoom, nt)
+
+
+def upload(st, tag="seed", zoom=None):
+    """A host state -- fields[r][name] with .data in the declared shape and .lower, as download
+    returns it or the reference's host ZoomState holds it -- on the device (pydyn/gpu/zoom.py:
+    91-98).  zoom: the tree the state belongs to (default st.zoom).  tag is the reference's
+    transfer label and changes nothing."""
+    zoom = zoom if zoom is not None else getattr(st, "zoom", None)
+    if zoom is None:
+        raise ValueError("upload: the state's Zoom is needed (pass zoom=)")
+    dev = ZoomState(zoom, st.fields[1]["rm"].data.shape[3])
+    for r, fields in st.fields.items():
+        for k, v in fields.items():
+            f = dev[r][k]
+            if tuple(v.lower) != f.lower or tuple(v.data.shape) != tuple(f.t.shape):
+                raise ValueError(f"upload: region {r} {k}: bounds {tuple(v.lower)} shape {v.data.shape}, "
+                                 f"declared {f.lower} {tuple(f.t.shape)}")  # fmt: skip
+            f.t.copy_(D.upload(v.data))
+    return dev
+
+
+def download(st, tag="verify"):
+    """The device state's arrays on the host as a HostState of FArrays (pydyn/gpu/zoom.py:
+    101-104); tag as upload's."""
+    return HostState({r: {k: FArray(v.host(), v.lower) for k, v in f.items()} for r, f in st.fields.items()},
+                     st.zoom)  # fmt: skip
+
+
+# ------------------------------------------------------------------ queued submission
+def _interval_of(fn):
+    """The ZoomRun whose own interval fn is (submit(run.interval)), else None."""
+    from .run import ZoomRun
+
+    owner = getattr(fn, "__self__", None)
+    ok = isinstance(owner, ZoomRun) and owner.driver == "native" and getattr(fn, "__name__", "") == "interval"
+    return owner if ok else None
+
+
+class Pipeline:
+    """Calls fn() -- any sequence of the driver's bodies on st -- in submission order, up to
+    `depth` in flight (pydyn/gpu/zoom.py:917-1041); drain() returns the calls' results.
+
+    A ZoomRun's own interval (submit(run.interval, repeat=...)) goes to the library's queue
+    (tm5_tree_run): each interval planned on the previous one's step, its steps checked on the
+    device or its planned trial rehearsed and verified there, the host reading the device once per
+    `depth` intervals, an interval whose plan fails redone the reference's way -- the results are
+    always the careful run's bits.  With `repeat` given, every step of a given length is captured
+    into a HIP graph once and replayed (its time comes from a device clock, so every interval
+    replays the same graph).  Any other fn runs when submitted: the bodies are asynchronous on the
+    context's stream and read the device only where the reference's careful run decides (Check_CFL's
+    verdict).  `tally`: the intervals kept and redone, and the steps replayed from graphs."""
+
+    def __init__(self, zoom, st, depth=2):
+        if depth < 1:
+            raise ValueError("Pipeline: depth >= 1")
+        self.zoom, self.st, self.depth = zoom, st, depth
+        self.done = []
+        self.tally = {"kept": 0, "redone": 0, "replayed": 0}
+        self._pending = None  # [run, intervals, repeat]
+
+    def submit(self, fn, repeat=None):
+        run = _interval_of(fn)
+        if run is None:
+            self._flush()
+            self.done.append(fn())
+            self.tally["kept"] += 1
+            return
+        if self._pending is not None and (self._pending[0] is not run or self._pending[2] != repeat):
+            self._flush()
+        if self._pending is None:
+            self._pending = [run, 0, repeat]
+        self._pending[1] += 1
+        if self._pending[1] >= self.depth:
+            self._flush()
+
+    def _flush(self):
+        if self._pending is None:
+            return
+        run, count, repeat = self._pending
+        self._pending = None
+        tree = run.native()
+        tree.option(_lib.TREE_GRAPHS, 0 if repeat is None else 1)
+        before = tree.tally()
+        self.done.extend(run.run(count))
+        after = tree.tally()
+        for k in ("kept", "redone", "replayed"):
+            self.tally[k] += after[k] - before[k]
+
+    def drain(self):
+        """Settle every call in flight (raising the reference's exception for an error the device
+        recorded); the results so far."""
+        self._flush()
+        D.context().raise_errors()
+        out, self.done = self.done, []
+        return out
 
 
 def _call(name, *args):

@@ -13,7 +13,7 @@ import numpy as np
 from .. import _lib
 from .. import device as D
 
-__all__ = ["FIELDS", "Field", "ZoomState", "declared"]
+__all__ = ["FIELDS", "Field", "HostState", "ZoomState", "declared"]
 
 FIELDS = ("m", "rm", "rxm", "rym", "rzm", "am", "bm", "cm", "pu", "pv", "sp", "phlb", "dxyp")
 
@@ -131,3 +131,19 @@ class ZoomState:
     def host(self):
         """{r: {name: C-ordered NumPy array with the declared shape}}."""
         return {r: {k: v.host() for k, v in f.items()} for r, f in self.fields.items()}
+
+
+class HostState:
+    """Every region's arrays on the host: `fields[r][name]` a pydyn_amd.dump.FArray (the declared
+    shape in C order, its Fortran lower bounds), as the reference's host ZoomState holds FViews
+    (pydyn/zoom/state.py:16-45); `zoom` the tree they belong to."""
+
+    def __init__(self, fields, zoom=None):
+        self.fields, self.zoom = fields, zoom
+
+    def __getitem__(self, r):
+        return self.fields[r]
+
+    @property
+    def nt(self):
+        return self.fields[1]["rm"].data.shape[3]

@@ -38,7 +38,7 @@ def _compare(host, want, what):
     assert not bad, f"{what}: differ from the reference: {bad}"
 
 
-def _queued(name, chunk, limit=None, graphs=True, ensemble=None):
+def _queued(name, chunk, limit=None, graphs=True, ensemble=None, mode=None):
     from pydyn_amd import _lib, meteo
     from pydyn_amd.zoom.run import ZoomRun
 
@@ -52,10 +52,12 @@ def _queued(name, chunk, limit=None, graphs=True, ensemble=None):
         run = ZoomRun(pmet, cmet, ensemble=meteo.Ensemble(pmet, ensemble[1]))
     if not graphs:
         run.native().option(_lib.TREE_GRAPHS, 0)
+    if mode is not None:
+        run.native().option(_lib.TREE_MODE, mode)
     i = 0
     while i < len(ref):
         k = min(chunk, len(ref) - i)
-        print(f"{name}: intervals {i}..{i + k - 1} queued (graphs {graphs})", flush=True)
+        print(f"{name}: intervals {i}..{i + k - 1} queued (graphs {graphs}, mode {mode})", flush=True)
         out = run.run(k)
         for j, got in enumerate(out):
             assert got == (ref[i + j]["ndyn"], ref[i + j]["n"]), f"{name} interval {i + j}: (ndyn, n) {got}"
@@ -96,3 +98,20 @@ def test_queued_tree_without_graphs_matches_reference(dev):
 def test_queued_ensemble_tree_matches_reference(dev):
     """4 members x 4 tracers on the tiny tree, queued, against the reference's ensemble run."""
     _queued("tiny2", 8, ensemble=(4, 4))
+
+
+@pytest.mark.parametrize("name", ["small2", "c2r"])
+def test_queued_tree_rehearsed_plans_match_reference(dev, name):
+    """The queue's other mode (the one many tracers take): every interval's planned trial rehearsed
+    mass-only and verified on the device, then plain steps with no backup; the reference's bits."""
+    from pydyn_amd import _lib
+
+    run = _queued(name, 12, mode=_lib.TREE_REHEARSED)
+    tally = run.native().tally()
+    assert tally["replayed"] > 0, tally
+
+
+def test_queued_ensemble_rehearsed_plans_match_reference(dev):
+    from pydyn_amd import _lib
+
+    _queued("tiny2", 12, ensemble=(4, 4), mode=_lib.TREE_REHEARSED)
