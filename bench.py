@@ -8,13 +8,20 @@ each Setup_MassFlow + advection x/y/z with sub-cycling, convection with Kzz diff
 injection and chemistry, m2phlb1 -- the state resident in HBM, inputs synthetic (seeded,
 pydyn_amd.meteo).  Every interval is bit-identical to the reference (tests/test_gpu_parity.py).
 
+The timed K intervals are the first K of the 30-day run (interval 0 onwards, on a region built
+for the timing; the W warm-up intervals run on another region of the same meteo).  With the
+default K = 720 that is the whole 30-day configuration, Check_CFL's cut intervals included; with
+a smaller K the line also carries the whole 30 days as `full_config`.
+
 value = simulated seconds advanced / wall seconds x 3600 / (365 x 86400), the reference's
 years/hour (pydyn/gpu/bench.py:26, 262) with the ndyn Check_CFL actually chose.
 
-Also reported, in the same line: the zoom tree's rate ("zoom"), the 64-member x 4-tracer batched
-form on it against one member's plain run ("batched"), the cost of one gradient over 24 hourly
-intervals through the adjoint ("adjoint"), the dominant kernel's roofline (its algorithmic bytes or FP64 operations per
-launch over its average launch time, from HIP events the library records around each launch on
+Also reported, in the same line: the step's algorithmic HBM bytes over its time ("hbm"), the zoom
+tree's rate ("zoom", 7 days), the reference's own benchmarked tree ("published_tree"), the
+64-member x 4-tracer batched form on the zoom tree against one member's plain run ("batched"),
+the cost of one gradient over 24 hourly intervals through the adjoint ("adjoint"), the dominant
+kernel's roofline and the step's other kernels (their algorithmic bytes or FP64 operations per
+launch over their average launch time, from HIP events the library records around each launch on
 its stream, over intervals run after the timed ones) and the oracle (NumPy, one core) timed on a
 bounded sample of the same workload on this host.
 """
@@ -25,36 +32,34 @@ import os
 import sys
 import time
 
-import numpy as np
-
 SECONDS_A_YEAR = 365.0 * 86400.0
 METRIC = "simulated years/hour at glb3x2x34 (FP64 bit-exact); 64-member member-steps/s; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix); it counts an FMA as two operations
+C1_INTERVALS = 720  # the 30-day configuration
 
 
 def parse():
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=48, help="timed intervals of 3600 s")
+    p.add_argument("--steps", type=int, default=C1_INTERVALS, help="timed intervals of 3600 s (from t = 0)")
     p.add_argument("--warmup", type=int, default=4)
     p.add_argument("--grid", default="glb3x2")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-intervals", type=int, default=24)
     p.add_argument("--profile-intervals", type=int, default=12, help="intervals timed per kernel after the run")
-    p.add_argument("--no-zoom", action="store_true", help="skip the zoom tree's rate")
+    p.add_argument("--no-zoom", action="store_true", help="skip the zoom trees' rates")
     p.add_argument("--no-batched", action="store_true", help="skip the 64-member batched rate")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window's cost")
     p.add_argument("--members", type=int, default=64)
-    p.add_argument("--only", choices=("zoom", "batched", "adjoint"),
+    p.add_argument("--only", choices=("zoom", "published_tree", "batched", "adjoint"),
                    help="run that section alone and print its object (for profiling one workload)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="end with three plain device copies of MIB MiB (a known size for rocprofv3 --pmc)")
     return p.parse_args()
 
 
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-
-
+# ------------------------------------------------------------------ algorithmic work
 def column_flops(n, nt):
     """Algorithmic FP64 operations of the fused column kernel per column: the dense Crout LU
     of the n x n convection matrix (n-1-k divisions and (n-1-k)^2 multiply-subtracts at step
@@ -67,16 +72,52 @@ def column_flops(n, nt):
     return lu + inverse + apply
 
 
+def _shapes(im, jm, lm):
+    cells, cols = im * jm * lm, im * jm
+    return cells, cols, (im + 1) * jm * lm, im * (jm + 1) * lm, im * jm * (lm + 1)
+
+
+def operator_bytes(im, jm, lm, lmax, nt):
+    """Algorithmic HBM bytes of each operator of a step on an im x jm x lm region with nt tracers:
+    one read and one write of every array the operator changes, one read of every array it only
+    reads (sub-cycles stay on chip)."""
+    cells, cols, fx, fy, fz = _shapes(im, jm, lm)
+    q = 4 * nt  # rm, rxm, rym, rzm of every tracer
+    return {
+        "x": 8 * (2 * cells * (1 + q) + 2 * fx),  # m and moments in and out, am scaled in place
+        "y": 8 * (2 * cells * (1 + q) + 2 * fy),
+        "z": 8 * (2 * cells * (1 + q) + fz),  # cm read (restored exactly)
+        "convec": 8 * (2 * cells * q + cells + 5 * cols * lmax) + 4 * 2 * cols,  # + m, 4 fluxes, dkg, 2 levels
+        "sources": 8 * (cols + 2 * 2 * cols),  # emission; rm, rzm of the emitting level
+        "chemistry": 8 * (4 * cells + 2 * cells * q),  # the four loss rates; the moments
+        "m2phlb1": 8 * (cells + cols * (lm + 2)),  # m in; sp, phlb out
+        "dynam0": 8 * (2 * fx + 2 * fy + fz),  # pu, pv in; am, bm, cm out
+    }
+
+
+def step_bytes(im, jm, lm, lmax, nt):
+    """A step's algorithmic bytes: x, y, z, convection, sources, chemistry and m2phlb1 twice
+    ("xyzvsc m2phlb1 csvzyx m2phlb1"), Setup_MassFlow's dynam0 once."""
+    b = operator_bytes(im, jm, lm, lmax, nt)
+    return 2 * sum(v for k, v in b.items() if k != "dynam0") + b["dynam0"]
+
+
+def tree_step_bytes(zoom, lmax, nt):
+    """A parent step of a tree: every region's step bytes, a child's once per its time refinement
+    (the exchange and coarsening, a few cells per wall, not counted)."""
+    total = 0
+    for r in zoom.regions():
+        g = zoom.g(r)
+        total += g.tref * step_bytes(g.im, g.jm, zoom.lm, lmax, nt)
+    return total
+
+
 def kernel_units(g, nt):
-    """Algorithmic work per launch of the step's kernels at grid g: (bound, amount, unit).
-    Bytes are one read of every input and one write of every output array the kernel touches
-    (sub-cycles stay on chip)."""
-    cells = g.im * g.jm * g.lm
-    fx, fy, fz = (g.im + 1) * g.jm * g.lm, g.im * (g.jm + 1) * g.lm, g.im * g.jm * (g.lm + 1)
-    cols = g.im * g.jm
+    """Algorithmic work per launch of the region step's kernels at grid g: (bound, amount, unit)."""
+    cells, cols, fx, fy, fz = _shapes(g.im, g.jm, g.lm)
     q = 4 * nt
     return {
-        "k_column<%d>" % g.lmax: ("mfma", cols * column_flops(g.lmax, nt), "flop"),
+        "k_column<%d>" % g.lmax: ("fp64_valu", cols * column_flops(g.lmax, nt), "flop"),
         "k_x<false>": ("hbm", 8 * (2 * cells * (1 + q) + 2 * fx), "B"),
         "k_y_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fy), "B"),
         "k_z_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fz), "B"),
@@ -85,7 +126,8 @@ def kernel_units(g, nt):
         "k_z_steps<true>": ("hbm", 8 * (2 * cells + fz), "B"),
         "k_z_search": ("hbm", 8 * (cells + fz), "B"),
         "k_y_search": ("hbm", 8 * (cells + fy), "B"),
-        "k_dynam0": ("hbm", 8 * (fx + fy + fx + fy + fz), "B"),
+        "k_dynam0": ("hbm", 8 * (2 * fx + 2 * fy + fz), "B"),
+        "k_m2phlb1": ("hbm", 8 * (cells + cols * (g.lm + 2)), "B"),
     }
 
 
@@ -103,88 +145,160 @@ def traffic_of(kernel):
 
 def roofline(reg, intervals):
     """Per-kernel GPU time of `intervals` more intervals, from events the library records around
-    every launch on its stream; the roofline of the kernel with the largest total."""
+    every launch on its stream: the roofline of the kernel with the largest total, and every
+    kernel's achieved rate."""
     from pydyn_amd import device as D
 
     if intervals < 1:
-        return None
+        return None, None
     ctx = D.context()
     ctx.profile_begin()
-    for _ in range(intervals):
-        reg.interval()
+    reg.run(intervals)
     prof = ctx.profile_end()
     total = sum(s for _, s in prof.values())
-    kernel, (count, secs) = max(prof.items(), key=lambda kv: kv[1][1])
     units = kernel_units(reg.g, reg.rm.shape[3])
-    if kernel not in units:
-        return {"kernel": kernel, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
-                "traffic": None, "launch_us": round(secs / count * 1e6, 2)}
-    bound, amount, unit = units[kernel]
-    avg = secs / count
+    kernels = []
+    for name, (count, secs) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        avg = secs / count
+        row = {"kernel": name, "launches": count, "launch_us": round(avg * 1e6, 2), "share_of_gpu_time": round(secs / total, 3)}
+        if name in units:
+            bound, amount, unit = units[name]
+            row["algorithmic_per_launch"], row["algorithmic_unit"] = amount, unit
+            if bound == "hbm":
+                row["achieved_GBs"] = round(amount / avg / 1e9, 1)
+                row["frac_of_hbm_peak"] = round(amount / avg / 1e9 / HBM_PEAK_GBS, 4)
+            else:
+                row["achieved_TFLOPs"] = round(amount / avg / 1e12, 3)
+        traffic = traffic_of(name)
+        if traffic is not None:
+            row["traffic_B"] = traffic
+            row["traffic_GBs"] = round(traffic / avg / 1e9, 1)
+        kernels.append(row)
+    top = kernels[0]
+    if top["kernel"] not in units:
+        return {"kernel": top["kernel"], "bound": None, "launch_us": top["launch_us"]}, kernels
+    bound, amount, unit = units[top["kernel"]]
+    avg = top["launch_us"] * 1e-6
     if bound == "hbm":
         achieved, peak, runit = amount / avg / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
         achieved, peak, runit = amount / avg / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
     out = {
-        "kernel": kernel,
+        "kernel": top["kernel"],
         "bound": bound,
         "achieved": round(achieved, 3),
         "peak": peak,
         "unit": runit,
         "frac": round(achieved / peak, 4),
-        "traffic": traffic_of(kernel),
+        "traffic": traffic_of(top["kernel"]),
         "algorithmic_per_launch": amount,
         "algorithmic_unit": unit,
-        "launch_us": round(avg * 1e6, 2),
-        "launches": count,
-        "share_of_gpu_time": round(secs / total, 3),
+        "launch_us": top["launch_us"],
+        "launches": top["launches"],
+        "share_of_gpu_time": top["share_of_gpu_time"],
     }
-    if bound == "mfma":
-        out["note"] = ("FP64 vector adds and multiplies, no FMA (bit-exactness); peak is the published FP64 "
-                       "figure, which counts an FMA as two operations")
-    return out
+    if bound == "fp64_valu":
+        out["frac_no_fma"] = round(achieved / (peak / 2.0), 4)
+        out["note"] = ("FP64 vector adds, multiplies and divisions, no FMA (bit-exactness) and no matrix core; "
+                       "peak is the published FP64 figure, which counts an FMA as two operations: frac_no_fma "
+                       "is against half of it")
+    return out, kernels
 
 
-def _timed(run, intervals, warmup=2):
-    """Wall seconds of `intervals` intervals after `warmup`, and the parent steps and simulated
-    seconds they advanced."""
+# ------------------------------------------------------------------ sections
+def _years(sim, wall):
+    return sim / wall * 3600.0 / SECONDS_A_YEAR
+
+
+def _queued(run, count):
+    """count intervals of a ZoomRun queued on the native driver: (wall s, [(ndyn, n)])."""
     import torch
 
-    for _ in range(warmup):
-        run.interval()
     torch.cuda.synchronize()
     t = time.perf_counter()
-    steps = sim = 0
-    for _ in range(intervals):
-        ndyn, n = run.interval()
-        steps += n
-        sim += ndyn * n
+    out = run.run(count)
     torch.cuda.synchronize()
-    return time.perf_counter() - t, steps, sim
+    return time.perf_counter() - t, out
 
 
-def zoom_section(intervals=12):
-    """The zoom tree (glb3x2x34 parent + the 1x1 degree child over 170W-50W, 10N-80N,
-    120 x 70 x 34, tref 2), one CH4 tracer: simulated years/hour."""
+def _launches(run, intervals):
+    """Kernel launches per interval over `intervals` more queued intervals (the library's events;
+    graphs are not replayed while they are recorded)."""
+    from pydyn_amd import device as D
+
+    ctx = D.context()
+    ctx.profile_begin()
+    run.run(intervals)
+    prof = ctx.profile_end()
+    return round(sum(c for c, _ in prof.values()) / intervals, 1)
+
+
+def tree_section(name, workload, days=7, warmup=4, published=None):
+    """A zoom tree, one CH4 tracer, queued on the native driver: after `warmup` intervals (which
+    capture the step graphs), 24 intervals that Check_CFL does not cut, then `days` days."""
     from pydyn_amd import meteo
     from pydyn_amd.zoom.run import ZoomRun
 
-    p, c = meteo.zoom("c2")
-    run = ZoomRun(p, c)
-    wall, steps, sim = _timed(run, intervals)
-    return {
-        "workload": "glb3x2x34 + child 120x70x34 (xref 3, yref 2, tref 2), two-way exchange, 1 CH4 tracer",
-        "value": round(sim / wall * 3600.0 / SECONDS_A_YEAR, 3),
+    p, c = meteo.zoom(name)
+    run = ZoomRun(p, c, redgrid=meteo.redgrid(name))
+    run.run(warmup)
+    wall_u, out_u = _queued(run, 24)
+    wall, out = _queued(run, 24 * days)
+    sim = sum(a * b for a, b in out)
+    uncut = sum(1 for _, n in out_u if n == 1) == len(out_u)
+    bytes_step = tree_step_bytes(run.zoom, p.g.lmax, 1)
+    steps = sum(n for _, n in out)
+    rec = {
+        "workload": workload,
+        "value": round(_years(sim, wall), 3),
         "unit": "simulated years/hour",
-        "ms_per_interval": round(wall / intervals * 1e3, 3),
-        "intervals": intervals,
+        "intervals": len(out),
+        "cut_intervals": sum(1 for _, n in out if n != 1),
+        "ms_per_interval": round(wall / len(out) * 1e3, 3),
+        "uncut_window": {
+            "intervals": len(out_u),
+            "all_uncut": uncut,
+            "ms_per_interval": round(wall_u / len(out_u) * 1e3, 3),
+            "value": round(_years(sum(a * b for a, b in out_u), wall_u), 3),
+            "launches_per_interval": _launches(run, 2) if uncut else None,
+        },
+        "hbm": {
+            "algorithmic_bytes_per_parent_step": bytes_step,
+            "achieved_GBs": round(bytes_step * steps / wall / 1e9, 1),
+            "frac_of_peak": round(bytes_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "tally": run.native().tally(),
     }
+    if published:
+        rec["published"] = published
+        rec["vs_published"] = {k: round(v["ms_per_interval"] / rec["uncut_window"]["ms_per_interval"], 2)
+                               for k, v in published.items()}  # fmt: skip
+    run.close()
+    return rec
 
 
-def batched_section(members, intervals=6):
+def zoom_section():
+    return tree_section("c2", "glb3x2x34 + child 120x70x34 (xref 3, yref 2, tref 2), two-way exchange, 1 CH4 tracer, "
+                        "7 days, queued on the native tree driver (checked steps replayed from HIP graphs)")
+
+
+def published_tree_section():
+    """The tree the reference publishes its device timings on (pydyn/gpu/bench.py:28-41):
+    glb3x2 on its reduced grid + nam1x1 72 x 44 (xref 3, yref 2, tref 2), 34 levels, convection
+    over 26, one tracer -- on this build's synthetic meteo (the reference's interval: Check_CFL +
+    Setup_MassFlow + step, as the uncut window here)."""
+    published = {  # ms per interval, replayed as a graph (docs/gpu_port.md:196-202, 512-518)
+        "A100_80GB": {"ms_per_interval": 6.8, "years_per_hour": 60.1},
+        "MI100": {"ms_per_interval": 13.9, "years_per_hour": 29.6},
+    }
+    return tree_section("pub", "glb3x2 (reduced grid) x 34 + nam1x1 72x44 (xref 3, yref 2, tref 2), 1 CH4 tracer, "
+                        "7 days, queued on the native tree driver", published=published)
+
+
+def batched_section(members, intervals=24):
     """members x 4 tracers on the zoom tree in one run (each member's emission scaled, each
     bit-identical to its own plain run) against one member's plain run: member-steps/s, a step
-    being one parent step of the tree."""
+    being one parent step of the tree; both queued on the native driver."""
     import torch
 
     from pydyn_amd import meteo
@@ -193,22 +307,37 @@ def batched_section(members, intervals=6):
     p, c = meteo.zoom("c2", nt=4)
     ens = meteo.Ensemble(p, members)
     plain = ZoomRun(p, c, ensemble=ens.subset([0]))
-    wall_p, steps_p, _ = _timed(plain, 2 * intervals)
+    plain.run(4)
+    wall_p, out_p = _queued(plain, 2 * intervals)
+    plain.close()
     del plain
     torch.cuda.empty_cache()
     batched = ZoomRun(p, c, ensemble=ens)
-    wall_b, steps_b, _ = _timed(batched, intervals)
+    batched.run(2)
+    wall_b, out_b = _queued(batched, intervals)
+    steps_p, steps_b = sum(n for _, n in out_p), sum(n for _, n in out_b)
     sequential = steps_p / wall_p  # one member's steps/s = members run one after another
     value = steps_b * members / wall_b
-    return {
-        "workload": f"{members} members x 4 tracers on the zoom tree (nt={members * 4}), FP64",
+    bytes_step = tree_step_bytes(batched.zoom, p.g.lmax, ens.nt)
+    rec = {
+        "workload": f"{members} members x 4 tracers on the zoom tree (nt={members * 4}), FP64, queued on the native "
+                    "tree driver (plans rehearsed and verified on the device, steps replayed from HIP graphs)",
         "value": round(value, 2),
         "unit": "member-steps/s",
         "sequential_member_steps_per_s": round(sequential, 2),
         "speedup_vs_sequential": round(value / sequential, 3),
         "ms_per_interval": round(wall_b / intervals * 1e3, 3),
         "intervals": intervals,
+        "cut_intervals": sum(1 for _, n in out_b if n != 1),
+        "hbm": {
+            "algorithmic_bytes_per_parent_step": bytes_step,
+            "achieved_GBs": round(bytes_step * steps_b / wall_b / 1e9, 1),
+            "frac_of_peak": round(bytes_step * steps_b / wall_b / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "tally": batched.native().tally(),
     }
+    batched.close()
+    return rec
 
 
 def adjoint_section(met):
@@ -258,7 +387,7 @@ def cpu_baseline(met, intervals):
         sim += ndyn * n
     wall = time.perf_counter() - t
     return {
-        "value": round(sim / wall * 3600.0 / SECONDS_A_YEAR, 6),
+        "value": round(_years(sim, wall), 6),
         "unit": "simulated years/hour",
         "cores": 1,
         "kind": "port",
@@ -266,6 +395,23 @@ def cpu_baseline(met, intervals):
         f"single-threaded (os.cpu_count()={os.cpu_count()}, affinity {len(os.sched_getaffinity(0))})",
         "wall_s": round(wall, 2),
     }
+
+
+def _region_run(met, count):
+    """A fresh region from t = 0, count intervals queued (one read of the device per batch):
+    (wall s, [(ndyn, n)], the region)."""
+    import torch
+
+    from pydyn_amd.region import Region
+
+    reg = Region(met)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = reg.run(count)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    reg.check()
+    return wall, out, reg
 
 
 def main():
@@ -281,23 +427,22 @@ def main():
 
     met = meteo.Meteo(meteo.grid(a.grid), seed=1)
     if a.only:
-        out = {"zoom": zoom_section, "batched": lambda: batched_section(a.members),
-               "adjoint": lambda: adjoint_section(met)}[a.only]()
+        out = {"zoom": zoom_section, "published_tree": published_tree_section,
+               "batched": lambda: batched_section(a.members), "adjoint": lambda: adjoint_section(met)}[a.only]()  # fmt: skip
         print(json.dumps({a.only: out}), flush=True)
         return 0
-    reg = Region(met)
-    for _ in range(a.warmup):
-        reg.interval()
+    warm = Region(met)
+    warm.run(max(a.warmup, 1))  # code loaded and every buffer sized, on a region of its own
+    warm.close()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim = sum(ndyn * n for ndyn, n in reg.run(a.steps))  # one read of the device for the K intervals
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    reg.check()
-    value = sim / wall * 3600.0 / SECONDS_A_YEAR
+    wall, out, reg = _region_run(met, a.steps)
+    sim = sum(ndyn * n for ndyn, n in out)
+    steps = sum(n for _, n in out)
+    g = met.g
+    b_step = step_bytes(g.im, g.jm, g.lm, g.lmax, met.nt)
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(_years(sim, wall), 3),
         "unit": "simulated years/hour",
         "n_gpus": 1,
         "steps": a.steps,
@@ -309,17 +454,39 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded pydyn_amd.meteo: steady balanced circulation, CFL 0.3-1.5)",
         "config": {
-            "workload": f"{a.grid} x {met.g.lm} plain FP64 interval (Check_CFL + steps), 1 CH4 tracer",
-            "grid": [met.g.im, met.g.jm, met.g.lm],
-            "lmax": met.g.lmax,
+            "workload": f"{a.grid} x {g.lm} plain FP64 interval (Check_CFL + steps), 1 CH4 tracer, intervals "
+                        f"0..{a.steps - 1} of the 30-day run",
+            "grid": [g.im, g.jm, g.lm],
+            "lmax": g.lmax,
             "nt": met.nt,
             "interval_s": 3600,
             "simulated_s": sim,
+            "dynamic_steps": steps,
+            "cut_intervals": sum(1 for _, n in out if n != 1),
         },
-        "roofline": roofline(reg, a.profile_intervals),
+        "ms_per_dynamic_step": round(wall / steps * 1e3, 4),
+        "hbm": {
+            "algorithmic_bytes_per_step": b_step,
+            "achieved_GBs": round(b_step * steps / wall / 1e9, 1),
+            "frac_of_peak": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        },
     }
+    if a.steps < C1_INTERVALS:
+        wall_f, out_f, reg_f = _region_run(met, C1_INTERVALS)
+        reg_f.close()
+        sim_f, steps_f = sum(x * y for x, y in out_f), sum(n for _, n in out_f)
+        line["full_config"] = {
+            "intervals": C1_INTERVALS,
+            "cut_intervals": sum(1 for _, n in out_f if n != 1),
+            "value": round(_years(sim_f, wall_f), 3),
+            "ms_per_interval": round(wall_f / C1_INTERVALS * 1e3, 4),
+            "hbm_achieved_GBs": round(b_step * steps_f / wall_f / 1e9, 1),
+        }
+    line["roofline"], line["kernels"] = roofline(reg, a.profile_intervals)
+    reg.close()
     if not a.no_zoom:
         line["zoom"] = zoom_section()
+        line["published_tree"] = published_tree_section()
     if not a.no_batched:
         line["batched"] = batched_section(a.members)
     if not a.no_adjoint:
