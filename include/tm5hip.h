@@ -99,7 +99,9 @@ int tm5_ctx_set_tile(tm5_ctx *ctx, int which, int value);
 int tm5_ctx_get_tile(tm5_ctx *ctx, int which, int *value);
 /* Per-kernel GPU time: from tm5_profile_begin every launch on the context is bracketed by
    events on its stream; tm5_profile_end waits for the stream and totals them per kernel, and
-   tm5_profile_entry reads the idx-th total (TM5_EINVAL past the last).  The reference's
+   tm5_profile_entry reads the idx-th total (TM5_EINVAL past the last).  Profiling never fails a
+   launch: if an event cannot be made the span stops recording and tm5_profile_end returns
+   TM5_ENOMEM with no totals.  The reference's
    Backend.profile_begin / profile_end (pydyn/gpu/backend.py:633-649). */
 int tm5_profile_begin(tm5_ctx *ctx);
 int tm5_profile_end(tm5_ctx *ctx);

@@ -14,6 +14,8 @@ Each operator follows the forward restatement it linearises (oracle/advection.py
 physics.py; pydyn/kernels.py:129-457, pydyn/emis.py:52-54, pydyn/chem.py:168-178).
 """
 
+import hashlib
+
 import numpy as np
 
 from . import physics
@@ -436,6 +438,9 @@ def z_op(m, cm, Q, D, limits, mode, iter_limit=15):
 
 
 # ------------------------------------------------------------------ column physics
+_INVERSES = {}  # the last convection inverses, by the bytes of m and dt (a step's two v share them)
+
+
 def v_op(m, met, dkg, dt, lmax, D, mode):
     """convec: rm <- C rm over the lowest lmax levels, slopes times diag(C); the ADM uses C^T."""
     im, jm = m.shape[:2]
@@ -444,9 +449,14 @@ def v_op(m, met, dkg, dt, lmax, D, mode):
     def col(q):
         return q.reshape(ncol, *q.shape[2:])
 
-    conv1 = convection_matrix(col(m)[:, :lmax], col(met.entu), col(met.detu), col(met.entd), col(met.detd),
-                              col(met.cloud_top), col(met.cloud_lfs), col(dkg), dt, lmax)  # fmt: skip
-    C = crout_inverse(conv1)  # (ncol, lmax, lmax)
+    key = (hashlib.sha1(np.ascontiguousarray(m).tobytes()).hexdigest(), float(dt), lmax, id(met))
+    C = _INVERSES.get(key)
+    if C is None:
+        conv1 = convection_matrix(col(m)[:, :lmax], col(met.entu), col(met.detu), col(met.entd), col(met.detd),
+                                  col(met.cloud_top), col(met.cloud_lfs), col(dkg), dt, lmax)  # fmt: skip
+        C = crout_inverse(conv1)  # (ncol, lmax, lmax)
+        _INVERSES.clear()
+        _INVERSES[key] = C
     out = [x.copy() for x in D]
     diag = C[:, np.arange(lmax), np.arange(lmax)][:, :, None]
     for k in (1, 2, 3):
@@ -463,7 +473,10 @@ class Window:
     """A forward window of the single-region run (oracle.driver.Run) with every tracer
     operator's inputs recorded: [(op, ndyn, m, flux, tracers)]."""
 
-    def __init__(self, met, intervals, limits=True, emis=None, init=None):
+    def __init__(self, met, intervals, limits=True, emis=None, init=None, on_op=None, progress=None):
+        """on_op: called with every operator's record as the forward reaches it, instead of the
+        record being kept in ops (a tangent run alongside the forward, tangent()); progress:
+        called with each interval's index once it has run."""
         self.met, self.limits = met, limits
         self.run = Run(met, limits)
         if init is not None:
@@ -476,8 +489,11 @@ class Window:
         def operator(op, ndyn):
             st = run.st
             flux = {"x": "am", "y": "bm", "z": "cm"}.get(op)
-            window.ops.append((op, ndyn, st["m"].copy(), None if flux is None else st[flux].copy(),
-                               [st[k].copy() for k in FIELDS]))  # fmt: skip
+            rec = (op, ndyn, st["m"].copy(), None if flux is None else st[flux].copy(), [st[k].copy() for k in FIELDS])
+            if on_op is None:
+                window.ops.append(rec)
+            else:
+                on_op(rec)
             if op == "s":
                 factor = physics.month2dt(ndyn, SEC_MONTH, 1)
                 x = physics.inject(window.emis, factor, XMCH4, XMCH4, 1.0)
@@ -486,10 +502,28 @@ class Window:
                 Run.operator(run, op, ndyn)
 
         run.operator = operator
-        for _ in range(intervals):
+        for k in range(intervals):
             run.interval()
+            if progress is not None:
+                progress(k)
         self.final = [run.st[k].copy() for k in FIELDS]
         self.m_final = run.st["m"].copy()
+
+    @classmethod
+    def tangent(cls, met, intervals, d_init, d_emis, limits=True, emis=None, init=None, progress=None):
+        """The TLM run alongside the forward, operator by operator, nothing recorded: (the
+        perturbations of the final tracers, the window, whose final and m_final are the
+        forward's)."""
+        w = cls.__new__(cls)
+        D = [x.copy() for x in d_init]
+
+        def step(rec):  # _op needs met, limits, emis and run: the constructor sets them first
+            nonlocal D
+            op, ndyn, m, flux, Q = rec
+            D = w._op(op, ndyn, m, flux, Q, D, "tl", d_emis)
+
+        cls.__init__(w, met, intervals, limits=limits, emis=emis, init=init, on_op=step, progress=progress)
+        return D, w
 
     def tlm(self, d_init, d_emis):
         """Perturbations of the final tracers from those of the initial tracers and the emission."""

@@ -66,6 +66,8 @@ class Window:
         self.work = D.empty(((g.im + 1) * (g.jm + 1) * (g.lm + 1) + g.im * g.jm * g.lm,))
         self.checkpoints = []
         self.m_final = None
+        self.generation = 0  # forwards run: a backward replays the trajectory of its own forward
+        self._arena = []  # per step of an interval: its records' buffers, reused interval after interval
 
     # ------------------------------------------------------------------ forward
     def forward(self, init, emis):
@@ -78,7 +80,8 @@ class Window:
             getattr(r, k).copy_(x)
         r.emis.copy_(emis)
         r.t, r.log = 0.0, []
-        self.checkpoints = []
+        self.checkpoints = []  # a new list: a graph that kept the last one keeps its trajectory
+        self.generation += 1
         for _ in range(self.intervals):
             saved = {k: getattr(r, k).clone() for k in ("m", *TRACERS)}
             t0 = r.t
@@ -92,21 +95,43 @@ class Window:
     def _ops(self, t, ndyn, ops):
         _lib.check(_lib.lib().tm5_region_ops(self.region._ptr, float(t), int(ndyn), ops.encode()), "tm5_region_ops")
 
-    def _replay(self, k):
-        """Interval k again from its checkpoint: the records the adjoint needs, in forward order."""
+    def _buffers(self, s):
+        """The record buffers of step s of an interval (made on first use, then reused): per
+        advection operator of "xyzVzyx" its m, faces and tracers, for V its m."""
         r = self.region
-        t0, ndyn, n, saved = self.checkpoints[k]
+        while len(self._arena) <= s:
+            step = []
+            for op in "xyzVzyx":
+                m = D.empty(tuple(r.m.shape))
+                if op == "V":
+                    step.append((m, None, None))
+                else:
+                    flux = getattr(r, AXES[op][1])
+                    step.append((m, D.empty(tuple(flux.shape)), [D.empty(tuple(r.rm.shape)) for _ in TRACERS]))
+            self._arena.append(step)
+        return self._arena[s]
+
+    def _replay(self, k, checkpoints=None):
+        """Interval k of a forward's checkpoints (default the last forward's) again: the records
+        the adjoint needs, in forward order, copied into the window's buffers (one multi-tensor
+        copy per record)."""
+        torch = __import__("torch")
+        r = self.region
+        t0, ndyn, n, saved = (self.checkpoints if checkpoints is None else checkpoints)[k]
         for name, x in saved.items():
             getattr(r, name).copy_(x)
         records = []
         for s in range(n):
+            bufs = self._buffers(s)
             self._ops((t0 + float(s * ndyn)) + ndyn / 2.0, ndyn, "D")
-            for op in "xyzVzyx":
+            for op, (m, flux, q) in zip("xyzVzyx", bufs):
                 if op == "V":
-                    records.append(("V", ndyn, r.m.clone(), None, None))
+                    m.copy_(r.m)
+                    records.append(("V", ndyn, m, None, None))
                 else:
-                    flux = getattr(r, AXES[op][1])
-                    records.append((op, ndyn, r.m.clone(), flux.clone(), [getattr(r, q).clone() for q in TRACERS]))
+                    src = [r.m, getattr(r, AXES[op][1]), *(getattr(r, name) for name in TRACERS)]
+                    torch._foreach_copy_([m, flux, *q], src)
+                    records.append((op, ndyn, m, flux, q))
                 self._ops(0.0, ndyn, op)
             self._ops(0.0, ndyn, "M")
         r.check()
@@ -143,13 +168,15 @@ class Window:
                                            ctypes.byref(D.view(lam_emis)))  # fmt: skip
         _lib.check(rc, "tm5_column_adjoint")
 
-    def backward(self, lam_final):
+    def backward(self, lam_final, checkpoints=None):
         """(lambdas of the initial tracers, lambda of the emission) from those of the final
-        tracers (four (im, jm, lm, nt) device tensors, consumed)."""
+        tracers (four (im, jm, lm, nt) device tensors, consumed), along the trajectory of
+        `checkpoints` (a forward's; default the last forward's)."""
+        checkpoints = self.checkpoints if checkpoints is None else checkpoints
         lam = [D.empty(tuple(x.shape)).copy_(x) for x in lam_final]  # i fastest, one stride set
         lam_emis = D.zeros(tuple(self.region.emis.shape))
-        for k in reversed(range(len(self.checkpoints))):
-            for op, ndyn, m, flux, q in reversed(self._replay(k)):
+        for k in reversed(range(len(checkpoints))):
+            for op, ndyn, m, flux, q in reversed(self._replay(k, checkpoints)):
                 if op == "V":
                     self.column_adjoint(m, ndyn, lam, lam_emis)
                 else:
@@ -166,6 +193,10 @@ class Transport(__import__("torch").autograd.Function):
     def forward(ctx, rm, rxm, rym, rzm, emis, window):
         ctx.window = window
         out = window.forward((rm, rxm, rym, rzm), emis)
+        # this forward's trajectory: a later forward on the same window makes new checkpoints
+        # (the air mass, m_final, does not depend on the controls and is the same for every forward)
+        ctx.checkpoints = window.checkpoints
+        ctx.generation = window.generation
         return tuple(out)
 
     @staticmethod
@@ -174,7 +205,7 @@ class Transport(__import__("torch").autograd.Function):
         w = ctx.window
         shape = tuple(w.region.rm.shape)
         lam = [torch.zeros(shape, dtype=torch.float64, device="cuda") if gr is None else gr for gr in grads]
-        lam_init, lam_emis = w.backward(lam)
+        lam_init, lam_emis = w.backward(lam, ctx.checkpoints)
         return (*lam_init, lam_emis, None)
 
 

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "tm5_lib.h"
@@ -46,20 +47,36 @@ struct tm5_prof {
     std::vector<const char *> total_names;
     std::vector<long long> counts;
     std::vector<double> seconds;
+    bool failed = false;  // an event or a record could not be made: profiling stopped, no totals
 };
 
 namespace tm5 {
 
+// Never throws and never fails the launch it brackets: when an event cannot be created or
+// recorded (or a list cannot grow) profiling stops for the rest of the span, and the next
+// tm5_profile_end reports TM5_ENOMEM instead of totals that would pair the wrong events.
 void prof_mark(tm5_ctx *ctx, const char *name, bool stop)
 {
     tm5_prof *p = ctx->prof;
-    if (p->used == p->events.size()) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        p->events.push_back(e);
+    try {
+        if (p->used == p->events.size()) {
+            p->events.reserve(p->events.size() + 1);  // may throw, before the event exists
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) throw std::bad_alloc();
+            p->events.push_back(e);
+        }
+        if (!stop) p->names.push_back(name);
+    } catch (...) {
+        p->failed = true;
+        ctx->prof = nullptr;
+        return;
     }
-    if (!stop) p->names.push_back(name);
-    hipEventRecord(p->events[p->used++], ctx->stream);
+    if (hipEventRecord(p->events[p->used], ctx->stream) != hipSuccess) {
+        p->failed = true;
+        ctx->prof = nullptr;
+        return;
+    }
+    p->used++;
 }
 
 int launch_convec(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 entu, View3 detu, View3 entd,
@@ -229,35 +246,48 @@ extern "C" int tm5_profile_begin(tm5_ctx *c)
 {
     if (!c) return TM5_EINVAL;
     TRY(status(hipStreamSynchronize(c->stream)));
-    if (!c->prof_store) c->prof_store = new tm5_prof();
+    if (!c->prof_store) c->prof_store = new (std::nothrow) tm5_prof();
+    if (!c->prof_store) return TM5_ENOMEM;
     c->prof = c->prof_store;
     c->prof->used = 0;
+    c->prof->failed = false;
     c->prof->names.clear();
     return 0;
 }
 
 extern "C" int tm5_profile_end(tm5_ctx *c)
 {
-    if (!c || !c->prof) return TM5_EINVAL;
-    tm5_prof *p = c->prof;
+    if (!c || !c->prof_store || (!c->prof && !c->prof_store->failed)) return TM5_EINVAL;  // not profiling
+    tm5_prof *p = c->prof_store;
+    c->prof = nullptr;  // stop recording; the totals stay until the next begin
     TRY(status(hipStreamSynchronize(c->stream)));
     p->total_names.clear();
     p->counts.clear();
     p->seconds.clear();
-    for (size_t s = 0; s < p->names.size() && 2 * s + 1 < p->used; s++) {
-        float ms = 0.0f;
-        TRY(status(hipEventElapsedTime(&ms, p->events[2 * s], p->events[2 * s + 1])));
-        size_t k = 0;
-        while (k < p->total_names.size() && strcmp(p->total_names[k], p->names[s]) != 0) k++;
-        if (k == p->total_names.size()) {
-            p->total_names.push_back(p->names[s]);
-            p->counts.push_back(0);
-            p->seconds.push_back(0.0);
-        }
-        p->counts[k] += 1;
-        p->seconds[k] += ms * 1e-3;
+    if (p->failed) {
+        p->failed = false;
+        return TM5_ENOMEM;
     }
-    c->prof = nullptr;  // stop recording; the totals stay until the next begin
+    try {
+        for (size_t s = 0; s < p->names.size() && 2 * s + 1 < p->used; s++) {
+            float ms = 0.0f;
+            TRY(status(hipEventElapsedTime(&ms, p->events[2 * s], p->events[2 * s + 1])));
+            size_t k = 0;
+            while (k < p->total_names.size() && strcmp(p->total_names[k], p->names[s]) != 0) k++;
+            if (k == p->total_names.size()) {
+                p->total_names.push_back(p->names[s]);
+                p->counts.push_back(0);
+                p->seconds.push_back(0.0);
+            }
+            p->counts[k] += 1;
+            p->seconds[k] += ms * 1e-3;
+        }
+    } catch (const std::bad_alloc &) {
+        p->total_names.clear();
+        p->counts.clear();
+        p->seconds.clear();
+        return TM5_ENOMEM;
+    }
     return 0;
 }
 

@@ -105,12 +105,18 @@ def test_column_adjoint_matches_oracle(dev):
 
 
 # ------------------------------------------------------------------ windows
-@pytest.mark.parametrize("limits", [True, False])
-def test_window_adjoint_matches_oracle(dev, limits):
+@pytest.mark.parametrize(("grid", "intervals", "limits"), [
+    ("small", 3, True),
+    ("small", 3, False),
+    ("glb6x4", 24, True),  # 60 x 45 x 25 over a day, slopes the limiter clips
+])  # fmt: skip
+def test_window_adjoint_matches_oracle(dev, grid, intervals, limits):
+    """The device adjoint of a window against the NumPy adjoint of the same (bit-identical)
+    forward, to 1e-11 relative."""
     D, AD = dev
-    print(f"window adjoint small limits {limits}", flush=True)
-    met, W = oracle_window("small", 3, limits=limits)
-    window = AD.Window(met, 3, limits=limits)
+    print(f"window adjoint {grid} x {intervals} limits {limits}", flush=True)
+    met, W = oracle_window(grid, intervals, limits=limits)
+    window = AD.Window(met, intervals, limits=limits)
     init = [D.upload(x) for x in clipped_init(met)]
     final = window.forward(init, D.upload(met.emis))
     for k, (g, w) in enumerate(zip(final, W.final)):
@@ -159,6 +165,36 @@ def test_mixed_dot_product_numpy_tangent_device_adjoint(dev):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
 
 
+def test_replay_after_a_second_forward_uses_its_own_trajectory(dev):
+    """Two forwards of one window before the first graph's backward: that backward replays the
+    first forward's trajectory (its own checkpoints), giving the gradient the first forward
+    alone gives."""
+    import torch
+
+    D, AD = dev
+    from pydyn_amd import meteo
+
+    met = meteo.Meteo(meteo.grid("small"), seed=3)
+    window = AD.Window(met, 2)
+    i, j, obs = AD.samples(met.g, n=40)
+    init = clipped_init(met)
+
+    def grads(second=None):
+        x = [D.upload(v).requires_grad_(True) for v in init]
+        e = D.upload(met.emis).requires_grad_(True)
+        out = AD.transport(window, *x, e)
+        if second is not None:  # another forward from other tracers, its graph left alone
+            AD.transport(window, *[D.upload(v * second) for v in init], D.upload(met.emis))
+        AD.loss(out[0], window.m_final, i, j, obs).backward()
+        return [D.download(v.grad) for v in (*x, e)]
+
+    alone = grads()
+    interleaved = grads(second=1.5)
+    for a, b in zip(alone, interleaved):
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    assert torch.cuda.is_available()
+
+
 @pytest.fixture(scope="module")
 def c4(dev):
     """The 120 x 90 x 34 grid, one tracer, 24 hourly intervals."""
@@ -186,6 +222,34 @@ def test_dot_product_24h_glb3x2(dev, c4):
     lhs = sum(float(np.sum(D.download(a) * b)) for a, b in zip(out, lam))
     rhs = sum(float(np.sum(a * D.download(b))) for a, b in zip(d, ad)) + float(np.sum(e * D.download(ae)))
     print(f"dot product: {lhs!r} {rhs!r} rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_dot_product_limiter_on_24h_glb3x2_oracle_tangent(dev, c4):
+    """Limiter on, from slopes the limiter clips: the NumPy tangent-linear model run alongside
+    the oracle's forward over 24 hourly intervals at 120 x 90 x 34 against the device adjoint of
+    the same (bit-identical) trajectory: <TLM (d, e), l> = <d, ADM_d l> + <e, ADM_e l> to 1e-12."""
+    from oracle import adjoint as A
+
+    D, AD = dev
+    met = c4
+    init = clipped_init(met)
+    rng = np.random.default_rng(18)
+    shape = init[0].shape
+    scale = float(np.abs(init[0]).mean())
+    dx = [1e-3 * scale * rng.standard_normal(shape) for _ in range(4)]
+    de = 1e-3 * float(np.abs(met.emis).mean()) * rng.standard_normal(met.emis.shape)
+    lam = [rng.standard_normal(shape) for _ in range(4)]
+    print("oracle tangent over 24 h of glb3x2 (a few minutes of CPU)", flush=True)
+    tl, w = A.Window.tangent(met, 24, dx, de, init=init, progress=lambda k: print(f"  interval {k}", flush=True))
+    window = AD.Window(met, 24)
+    final = window.forward([D.upload(x) for x in init], D.upload(met.emis))
+    for k, (g, f) in enumerate(zip(final, w.final)):
+        assert np.array_equal(D.download(g), f), f"forward {FIELDS[k]}"
+    ad, ae = window.backward([D.upload(x) for x in lam])
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(tl, lam))
+    rhs = sum(float(np.sum(a * D.download(b))) for a, b in zip(dx, ad)) + float(np.sum(de * D.download(ae)))
+    print(f"dot product, limiter on: {lhs!r} {rhs!r} rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
