@@ -7,6 +7,7 @@ Run in the build container only (the reference is not on the GPU machines):
     python tests/golden/make_golden.py run glb6x4 24
     python tests/golden/make_golden.py run glb3x2 720 # long: written interval by interval
     python tests/golden/make_golden.py zoom tiny2 48  # a global region with one child (zoom tree)
+    python tests/golden/make_golden.py zoom c2e 40    # 2 members x 4 tracers on the production tree
     python tests/golden/make_golden.py chem           # CH4 loss rates and decay (npz)
     python tests/golden/make_golden.py kzz            # Calc_Kzz: dkg and its intermediates (npz)
 
@@ -33,7 +34,8 @@ REPO = pathlib.Path(__file__).resolve().parents[2]
 GOLDEN = REPO / "tests" / "golden"
 REFERENCE = pathlib.Path("/root/reference")
 
-os.environ.setdefault("XDG_CACHE_HOME", str(REPO / "_work" / "cache"))  # the reference's C loop
+# the reference's libm route compiles a C loop into its cache: the oracle's (git-ignored) build folder
+os.environ.setdefault("XDG_CACHE_HOME", str(REPO / "oracle" / "_build" / "reference_cache"))
 sys.path.insert(0, str(REPO))
 sys.path.insert(1, str(REFERENCE))
 
@@ -325,7 +327,7 @@ def reference_zoom_run(pmet, cmet, n_intervals, every=None, redgrid=None, ensemb
 
 
 # ensembles: (zoom configuration, tracers per member, members)
-ENSEMBLES = {"tiny2e": ("tiny2", 4, 4), "small2e": ("small2", 2, 3)}
+ENSEMBLES = {"tiny2e": ("tiny2", 4, 4), "small2e": ("small2", 2, 3), "c2e": ("c2", 4, 2)}
 
 
 def zoom_golden(name, n_intervals):

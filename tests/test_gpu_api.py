@@ -287,6 +287,47 @@ def test_convec_many_tracers(dev, masked):
             assert_bitwise(D.download(one[k])[..., 0], D.download(dst[k])[..., t], f"{k} tracer {t}")
 
 
+def test_convec_split_form_full_grid_40_tracers(dev):
+    """The split form over the whole 120 x 90 x 34 grid with 40 tracers (tiles of 16 columns over
+    every row, the tracers in ranges): every tracer against its own one-tracer call (the one-kernel
+    form) and a block of columns against the oracle."""
+    import torch
+
+    from oracle import convection, physics
+    from pydyn_amd import meteo
+
+    D, K = dev
+    met = meteo.Meteo(meteo.grid("glb3x2"), seed=9)
+    lmax, nt = met.g.lmax, 40
+    m = np.ascontiguousarray(met.m)
+    im, jm, lm = m.shape
+    dkg = np.ascontiguousarray(physics.dkg(met.kvh, m, met.gph, lmax))
+    rng = np.random.default_rng(40)
+    rm = rng.uniform(0.5, 1.5, (im, jm, lm, nt)) * m[..., None] * 1.8e-6
+    st = {"m": m, "rm": rm, **{k: rm * rng.uniform(-1.0, 1.0, rm.shape) for k in ("rxm", "rym", "rzm")}}
+    conv = {k: D.upload(np.ascontiguousarray(getattr(met, k))) for k in ("entu", "detu", "entd", "detd")}
+    conv["dkg"] = D.upload(dkg)
+    for k in ("cloud_top", "cloud_lfs"):
+        conv[k] = D.upload(getattr(met, k).astype(np.int32)).to(torch.int32)
+    dst = {k: D.upload(st[k]) for k in FIELDS}
+    print(f"split form {im} x {jm} x {lm}, {nt} tracers", flush=True)
+    K.convec(*(dst[k] for k in FIELDS), conv, 1800.0, lmax)
+    D.context().raise_errors()
+    dm = D.upload(m)
+    for t in range(nt):
+        one = {k: D.upload(np.ascontiguousarray(st[k][..., t : t + 1])) for k in FIELDS[1:]}
+        K.convec(dm, *(one[k] for k in FIELDS[1:]), conv, 1800.0, lmax)
+        for k in FIELDS[1:]:
+            assert_bitwise(D.download(one[k])[..., 0], D.download(dst[k])[..., t], f"{k} tracer {t}")
+    sl = (slice(52, 71), slice(30, 42))  # the block the oracle runs (the operator is column-local)
+    part = {k: np.ascontiguousarray(st[k][sl]) for k in FIELDS}
+    c = {k: np.ascontiguousarray(getattr(met, k)[sl]) for k in ("entu", "detu", "entd", "detd", "cloud_top", "cloud_lfs")}
+    convection.convec(*(part[k] for k in FIELDS), c["entu"], c["detu"], c["entd"], c["detd"], c["cloud_top"],
+                      c["cloud_lfs"], np.ascontiguousarray(dkg[sl]), 1800.0, lmax)  # fmt: skip
+    for k in FIELDS[1:]:
+        assert_bitwise(D.download(dst[k])[sl], part[k], f"{k} against the oracle")
+
+
 def test_advect_y_on_region1_halo(dev):
     """dynamv on the interior slice of region 1's halo'd bm (0:im+1, 0:jm+1, 0:lm+1): the
     faces outside it through the whole-layer scaling when a layer sub-cycles."""

@@ -42,7 +42,7 @@ def _queued(name, chunk, limit=None, graphs=True, ensemble=None, mode=None):
     from pydyn_amd import _lib, meteo
     from pydyn_amd.zoom.run import ZoomRun
 
-    golden = f"zoom_{name}_sha1.json" if ensemble is None else "zoom_tiny2e_sha1.json"
+    golden = f"zoom_{name}_sha1.json" if ensemble is None else f"zoom_{name}e_sha1.json"
     ref = json.loads((GOLDEN / golden).read_text())["intervals"][:limit]
     if ensemble is None:
         pmet, cmet = meteo.zoom(name)
@@ -100,6 +100,14 @@ def test_queued_ensemble_tree_matches_reference(dev):
     _queued("tiny2", 8, ensemble=(4, 4))
 
 
+def test_queued_production_ensemble_tree_matches_reference(dev):
+    """2 members x 4 tracers on the production tree, queued in runs of 12 across the cuts of the
+    first 40 hours, against the reference's ensemble run (member 2's emission scaled, its
+    tracers started from its own seed)."""
+    run = _queued("c2", 12, ensemble=(4, 2))
+    assert run.native().tally()["redone"] >= 1
+
+
 @pytest.mark.parametrize("name", ["small2", "c2r"])
 def test_queued_tree_rehearsed_plans_match_reference(dev, name):
     """The queue's other mode (the one many tracers take): every interval's planned trial rehearsed
@@ -115,3 +123,43 @@ def test_queued_ensemble_rehearsed_plans_match_reference(dev):
     from pydyn_amd import _lib
 
     _queued("tiny2", 12, ensemble=(4, 4), mode=_lib.TREE_REHEARSED)
+
+
+def test_queued_64x4_members_equal_their_plain_queued_runs(dev):
+    """The batched form at full size: 64 members x 4 tracers (nt = 256) on the production tree
+    for 40 intervals (the last seven cut by Check_CFL), queued in runs of 12 (the mode without a
+    backup); every member's tracers and the tree's mass and fluxes against the member's own
+    plain queued run (nt = 4, the checked mode with its backup)."""
+    import torch
+
+    from pydyn_amd import meteo
+    from pydyn_amd.zoom.run import ZoomRun
+
+    pmet, cmet = meteo.zoom("c2", nt=4)
+    ens = meteo.Ensemble(pmet, 64)
+    batched = ZoomRun(pmet, cmet, ensemble=ens)
+    log = []
+    for i in range(0, 40, 12):
+        print(f"64 x 4: intervals {i}..{min(i + 12, 40) - 1}", flush=True)
+        log += batched.run(min(12, 40 - i))
+    assert sum(1 for nd, n in log if n > 1) >= 7, log
+    for m in range(64):
+        plain = ZoomRun(pmet, cmet, ensemble=ens.subset([m]))
+        got = []
+        for i in range(0, 40, 20):
+            got += plain.run(20)
+        assert got == log, f"member {m}: (ndyn, n) {got}"
+        for r, fields in batched.st.fields.items():
+            for k in ("rm", "rxm", "rym", "rzm"):
+                a = fields[k].t[..., 4 * m : 4 * m + 4].contiguous()
+                b = plain.st.fields[r][k].t.contiguous()
+                assert torch.equal(a.view(torch.int64), b.view(torch.int64)), (m, r, k)
+            if m in (0, 63):
+                for k in ("m", "am", "bm", "cm", "sp", "phlb"):
+                    a, b = fields[k].t.contiguous(), plain.st.fields[r][k].t.contiguous()
+                    assert torch.equal(a.view(torch.int64), b.view(torch.int64)), (m, r, k)
+        plain.close()
+        del plain
+        if m % 16 == 15:
+            print(f"  members 0..{m} equal their plain runs", flush=True)
+    batched.close()
