@@ -1,5 +1,5 @@
-"""The LAPACK-kind inverse on the device (k_lapack_inverse: DGETRF2 + DGETRI per matrix, one wave
-a matrix) and convec with it, bit for bit against the oracle (oracle/lapack.c, itself pinned to
+"""The LAPACK-kind inverse on the device (k_lapack_inverse: DGETRF2 + DGETRI per matrix, two
+matrices a wave) and convec with it, bit for bit against the oracle (oracle/lapack.c, itself pinned to
 the reference's dgetrf_ / dgetri_ twin by tests/test_oracle_lapack.py)."""
 
 import numpy as np
@@ -38,6 +38,8 @@ def test_inverse_rows_matches_oracle(dev, n):
     a = rng.standard_normal((203, n, n))  # pivoting throughout; a partial last workgroup
     a[:50] = np.eye(n) + 0.1 * a[:50]
     a[50:60] *= 1e-300  # pivots below DLAMCH('S'): the division branch
+    keep = (rng.random((40, n, n)) < 0.5) | np.eye(n, dtype=bool)  # a nonzero diagonal: nonsingular
+    a[61:140:2] *= keep  # exact zeros (the skips) in every other matrix: the halves of a wave diverge
     y, info = K.inverse_rows(torch.as_tensor(a, device="cuda"))
     assert int(info.cpu()[0]) == 0
     assert np.array_equal(_bits(y.cpu().numpy()), _bits(lapack.inverse_rows(a)))
