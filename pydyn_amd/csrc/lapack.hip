@@ -8,11 +8,11 @@
 // column-major in its half's LDS: every LDS instruction then serves two matrices, and a half
 // never reads the other half's matrix, so the halves may take different branches (pivots, zero
 // skips) without meeting.  The recursion of DGETRF2 depends only on the shape, so it is unrolled
-// at compile time (lu2<M, N>).  Every entry
-// receives its updates in the reference's order: the loops run across lanes only where the
-// reference's loop iterations touch different entries -- rows in DGEMM / DGEMV / DSCAL / the
-// column interchanges, columns in DLASWP and DTRSM (whose columns are independent) -- and every
-// loop that carries an entry's sequence of updates stays inside one lane.
+// at compile time (lu2).  Every entry receives its updates in the reference's order: the loops
+// run across lanes only where the reference's loop iterations touch different entries -- rows in
+// DGEMM / DGEMV / DSCAL, columns in DTRSM (whose columns are independent) -- and every loop that
+// carries an entry's sequence of updates stays inside one lane.  Interchanges (DLASWP's rows,
+// DGETRI's columns) move no data until the result is stored.
 #include "tm5_lib.h"
 
 namespace tm5 {
