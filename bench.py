@@ -19,7 +19,8 @@ years/hour (pydyn/gpu/bench.py:26, 262) with the ndyn Check_CFL actually chose.
 Also reported, in the same line: the step's algorithmic HBM bytes over its time ("hbm"), the zoom
 tree's rate ("zoom", 7 days), the reference's own benchmarked tree ("published_tree"), the
 64-member x 4-tracer batched form on the zoom tree against one member's plain run ("batched"),
-the cost of one gradient over 24 hourly intervals through the adjoint ("adjoint"), the dominant
+the cost of one gradient over 24 hourly intervals through the adjoint ("adjoint"), convec with the
+LAPACK kind's inverse against the Crout kind ("lapack"), the dominant
 kernel's roofline and the step's other kernels (their algorithmic bytes or FP64 operations per
 launch over their average launch time, from HIP events the library records around each launch on
 its stream, over intervals run after the timed ones) and the oracle (NumPy, one core) timed on a
@@ -52,7 +53,7 @@ def parse():
     p.add_argument("--no-batched", action="store_true", help="skip the 64-member batched rate")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window's cost")
     p.add_argument("--members", type=int, default=64)
-    p.add_argument("--only", choices=("zoom", "published_tree", "batched", "adjoint"),
+    p.add_argument("--only", choices=("zoom", "published_tree", "batched", "adjoint", "lapack"),
                    help="run that section alone and print its object (for profiling one workload)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="end with three plain device copies of MIB MiB (a known size for rocprofv3 --pmc)")
@@ -376,6 +377,40 @@ def adjoint_section(met):
     }
 
 
+def lapack_section(met, repeats=20):
+    """convec over every column of glb3x2x34 (one tracer, dt 1800 s) with the LAPACK kind's
+    inverse (DGETRF2 + DGETRI per column's 26 x 26 matrix, pydyn/lapack.py:43-66) against the
+    Crout kind: each kind's kernels per launch, from the library's events."""
+    import numpy as np
+    import torch
+
+    from pydyn_amd import device as D
+    from pydyn_amd.gpu import kernels as K
+
+    st = met.state()
+    m = D.upload(st["m"])
+    conv = {k: D.upload(np.ascontiguousarray(getattr(met, k))) for k in ("entu", "detu", "entd", "detd")}
+    conv["dkg"] = K.dkg(D.upload(met.kvh), m, D.upload(met.gph), met.g.lmax)
+    for k in ("cloud_top", "cloud_lfs"):
+        conv[k] = D.upload(getattr(met, k).astype(np.int32)).to(torch.int32)
+    q = [D.upload(st[k]) for k in ("rm", "rxm", "rym", "rzm")]
+    ctx = D.context()
+    out = {"workload": f"convec over {met.g.im} x {met.g.jm} columns of {met.g.lm} levels (lmax {met.g.lmax}), "
+                       "1 tracer, FP64, per launch"}  # fmt: skip
+    for kind in ("lapack", "crout"):
+        for _ in range(3):
+            K.convec(m, *q, conv, 1800.0, met.g.lmax, inverse=kind)
+        torch.cuda.synchronize()
+        ctx.profile_begin()
+        for _ in range(repeats):
+            K.convec(m, *q, conv, 1800.0, met.g.lmax, inverse=kind)
+        prof = ctx.profile_end()
+        out[kind] = {k: round(v[1] / v[0] * 1e6, 2) for k, v in sorted(prof.items())}
+        out[kind]["total_us"] = round(sum(v[1] for v in prof.values()) / repeats * 1e6, 2)
+    out["unit"] = "us per launch"
+    return out
+
+
 def cpu_baseline(met, intervals):
     from oracle.driver import Run
 
@@ -428,7 +463,8 @@ def main():
     met = meteo.Meteo(meteo.grid(a.grid), seed=1)
     if a.only:
         out = {"zoom": zoom_section, "published_tree": published_tree_section,
-               "batched": lambda: batched_section(a.members), "adjoint": lambda: adjoint_section(met)}[a.only]()  # fmt: skip
+               "batched": lambda: batched_section(a.members), "adjoint": lambda: adjoint_section(met),
+               "lapack": lambda: lapack_section(met)}[a.only]()  # fmt: skip
         print(json.dumps({a.only: out}), flush=True)
         return 0
     warm = Region(met)
@@ -491,6 +527,7 @@ def main():
         line["batched"] = batched_section(a.members)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_section(met)
+        line["lapack"] = lapack_section(met)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(met, a.cpu_intervals)
     print(json.dumps(line), flush=True)

@@ -7,6 +7,10 @@
         WRITE_SIZE` run of the same command, each scaled by what the counter reads for a plain
         device copy of COPY_BYTES bytes in that run (the largest __amd_rocclr_copyBuffer
         dispatch): bytes = FETCH_KiB * 1024 * (COPY_BYTES / copy FETCH bytes) + likewise WRITE.
+    python -m pydyn_amd.profiling counters DIR [DIR ...] --kernel REGEX
+        the mean per dispatch of every counter of the `--pmc` runs in the DIRs for the kernels
+        whose name matches REGEX, and the ratios the SQ counters give (fractions of the waves'
+        cycles spent issuing VALU or LDS instructions or waiting, FP64 instructions per wave).
 """
 
 import argparse
@@ -73,6 +77,36 @@ def traffic(fetch_dir, write_dir, copy_bytes):
     return out
 
 
+def counters(dirs, kernel):
+    """{counter: (dispatches, mean per dispatch)} over the --pmc runs in dirs, for the kernels whose
+    name matches the regex kernel."""
+    pat = re.compile(kernel)
+    acc = collections.defaultdict(list)
+    for d in dirs:
+        for r in _csv(d, "counter_collection.csv"):
+            if pat.search(r["Kernel_Name"]):
+                acc[r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: (len(v), sum(v) / len(v)) for k, v in sorted(acc.items())}
+
+
+RATIOS = [  # (label, numerator, denominator)
+    ("VALU issue / wave cycles", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+    ("LDS issue / wave cycles", "SQ_ACTIVE_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("any issue / wave cycles", "SQ_ACTIVE_INST_ANY", "SQ_WAVE_CYCLES"),
+    ("waiting (waitcnt) / wave cycles", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+    ("waiting to issue / wave cycles", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+    ("waiting to issue LDS / wave cycles", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("VALU instructions per wave", "SQ_INSTS_VALU", "SQ_WAVES"),
+    ("FP64 add per wave", "SQ_INSTS_VALU_ADD_F64", "SQ_WAVES"),
+    ("FP64 mul per wave", "SQ_INSTS_VALU_MUL_F64", "SQ_WAVES"),
+    ("FP64 fma per wave", "SQ_INSTS_VALU_FMA_F64", "SQ_WAVES"),
+    ("FP64 transcendental per wave", "SQ_INSTS_VALU_TRANS_F64", "SQ_WAVES"),
+    ("LDS instructions per wave", "SQ_INSTS_LDS", "SQ_WAVES"),
+    ("LDS bank conflict cycles / LDS issue", "SQ_LDS_BANK_CONFLICT", "SQ_ACTIVE_INST_LDS"),
+    ("busy / GPU active cycles", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE"),
+]
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     sub = p.add_subparsers(dest="cmd", required=True)
@@ -83,7 +117,19 @@ def main(argv=None):
     t.add_argument("write_dir")
     t.add_argument("copy_bytes", type=int)
     t.add_argument("--json")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--kernel", required=True)
     a = p.parse_args(argv)
+    if a.cmd == "counters":
+        got = counters(a.dirs, a.kernel)
+        print(f"{'counter':32s} {'dispatches':>10s} {'mean per dispatch':>20s}")
+        for k, (n, mean) in got.items():
+            print(f"{k:32s} {n:10d} {mean:20.1f}")
+        for label, num, den in RATIOS:
+            if num in got and den in got and got[den][1] > 0:
+                print(f"{label:40s} {got[num][1] / got[den][1]:10.3f}")
+        return 0
     if a.cmd == "stats":
         print(f"{'kernel':64s} {'calls':>6s} {'avg_us':>9s} {'total_us':>10s} {'pct':>6s}")
         for name, calls, tot, avg, pct in stats(a.dir):
