@@ -238,7 +238,7 @@ __device__ __forceinline__ void column_seq(const ColumnArgs &a, const ColumnSeq 
 }
 
 template <int N, bool ADJ>
-__global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int skip_mask)
+__global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
     constexpr int LD = ColumnLds<N>::LD, S = ColumnLds<N>::S;
@@ -261,11 +261,39 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
 
     // zfused: the first dynamw (n_advz 1, c = cm / 1) moves m before convection and m2phlb1 see it
     const bool zf = !ADJ && a.zfused;
-    double m_l = li < lm ? a.m(i, j, li) : 0.0, m_h = l1 < lm ? a.m(i, j, l1) : 0.0;
+    // Every input the column reads from HBM is loaded here, together, so that their latencies
+    // overlap (the exchanges through LDS below would hold each later group of loads back): the
+    // masses, the faces of the lane's two cells, the convection inputs, the cloud levels, and for
+    // a sequence on tracers the loss rates and the emission (decay factors made at once).
+    const double m0_l = li < lm ? a.m(i, j, li) : 0.0, m0_h = l1 < lm ? a.m(i, j, l1) : 0.0;
+    double zc_lo = 0.0, zc_hi = 0.0, zc_lo1 = 0.0, zc_hi1 = 0.0;  // cm below and above each cell
     if (zf) {
-        const double m0_l = m_l, m0_h = m_h;
-        if (li < lm) m_l = (m0_l + a.cm(i, j, li) / 1.0) - a.cm(i, j, li + 1) / 1.0;
-        if (l1 < lm) m_h = (m0_h + a.cm(i, j, l1) / 1.0) - a.cm(i, j, l1 + 1) / 1.0;
+        if (li < lm) {
+            zc_lo = a.cm(i, j, li);
+            zc_hi = a.cm(i, j, li + 1);
+        }
+        if (l1 < lm) {
+            zc_lo1 = a.cm(i, j, l1);
+            zc_hi1 = a.cm(i, j, l1 + 1);
+        }
+    }
+    double e_u = 0.0, d_u = 0.0, e_d = 0.0, d_d = 0.0, dk_l = 0.0;
+    if (li < N) {
+        e_u = a.entu(i, j, li);
+        d_u = a.detu(i, j, li);
+        e_d = a.entd(i, j, li);
+        d_d = a.detd(i, j, li);
+        dk_l = a.dkg(i, j, li);
+    }
+    const int ctop = a.top(i, j), clfs = a.lfs(i, j);
+    // a column the region does not own is computed like the others but keeps its tracers; the
+    // middle m2phlb1 is every column's
+    const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
+    const ColumnSeq cs = a.cbuf ? ColumnSeq{1.0, 1.0, 0.0, 0.0} : column_seq_setup(a, i, j, li);
+    double m_l = m0_l, m_h = m0_h;
+    if (zf) {
+        if (li < lm) m_l = (m0_l + zc_lo / 1.0) - zc_hi / 1.0;
+        if (l1 < lm) m_h = (m0_h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
         if (checked(a.spec_k)) {
             // a checked step: Check_CFL's dynamwm at cap 1 on both dynamw's masses (m before and
             // after the first), every interior face k tested by lane k and k + 32 of its column
@@ -279,7 +307,7 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
             for (int h2 = 0; h2 < 2; h2++) {
                 const int k = h2 ? l1 : li;
                 if (k >= 1 && k < lm) {
-                    const double ck = a.cm(i, j, k);
+                    const double ck = h2 ? zc_lo1 : zc_lo;  // cm(k)
                     const int u = ck >= 0.0 ? k - 1 : k;
                     ok = ok && fabs(ck / H[u]) < 1.0 && fabs(ck / mc[u]) < 1.0;
                 }
@@ -295,24 +323,19 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     }
     mc[li] = m_l;
     mc[l1] = m_h;
-    // a column the region does not own is computed like the others but keeps its tracers; the
-    // middle m2phlb1 is every column's
-    const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
     amu[li] = amd[li] = 0.0;
     if (li < N) {
-        const double e_u = a.entu(i, j, li), e_d = a.entd(i, j, li);
         eu[li] = e_u;
-        du[li] = a.detu(i, j, li);
+        du[li] = d_u;
         ed[li] = e_d;
-        dd[li] = a.detd(i, j, li);
-        dk[li] = a.dkg(i, j, li);
+        dd[li] = d_d;
+        dk[li] = dk_l;
         eum[li] = e_u / m_l;  // the fu(k, k-1) and fd(k-1, k-1) quotients, every level at once
         edm[li] = -e_d / m_l;
     }
     wsync();
 
     // ---- build conv1: lane c owns column c (pydyn/kernels.py:366-395)
-    const int ctop = a.top(i, j), clfs = a.lfs(i, j);
     // levels above the cloud top (updraft) or the level of free sinking (downdraft) contribute
     // exact zeros (the reference's where(act, .., 0.0)); a wave runs to the larger of its two
     // columns' levels, the other column's extra levels giving the same zeros
@@ -499,12 +522,12 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
     double zm0 = 0.0, zm0h = 0.0, zm2 = 0.0, zm2h = 0.0;
     if (zf) {
         if (li < lm) {
-            zm0 = a.m(i, j, li);
-            zm2 = (mc[li] + a.cm(i, j, li) / 1.0) - a.cm(i, j, li + 1) / 1.0;
+            zm0 = m0_l;
+            zm2 = (m_l + zc_lo / 1.0) - zc_hi / 1.0;
         }
         if (l1 < lm) {
-            zm0h = a.m(i, j, l1);
-            zm2h = (mc[l1] + a.cm(i, j, l1) / 1.0) - a.cm(i, j, l1 + 1) / 1.0;
+            zm0h = m0_h;
+            zm2h = (m_h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
         }
     }
     const bool zlim = a.zlimits != 0;
@@ -518,9 +541,9 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         const double rzu = zlim ? limit(Z[256 + u], rmu) : Z[256 + u];
         return face_flux(cf, gamma, rmu, rzu, Z[128 + u], Z[192 + u]);
     };
-    auto z_cell = [&](int l, const double *Mz, double m_old, double m_new, double &rm, double &rx, double &ry,
-                      double &rz) {
-        const double c_lo = a.cm(i, j, l) / 1.0, c_hi = a.cm(i, j, l + 1) / 1.0;
+    auto z_cell = [&](int l, double cm_lo, double cm_hi, const double *Mz, double m_old, double m_new, double &rm,
+                      double &rx, double &ry, double &rz) {
+        const double c_lo = cm_lo / 1.0, c_hi = cm_hi / 1.0;
         const FaceFlux lo = z_face(l, c_lo, Mz), hi = z_face(l + 1, c_hi, Mz);
         const double rzm = zlim ? limit(rz, rm) : rz;
         const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
@@ -549,13 +572,14 @@ __global__ void __launch_bounds__(256) k_column(ColumnArgs a, int *words, int sk
         }
         wsync();
         const double *Mz = second ? mc : Z;
-        if (li < lm) z_cell(li, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
-        if (l1 < lm) z_cell(l1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
+        if (li < lm)
+            z_cell(li, zc_lo, zc_hi, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
+        if (l1 < lm)
+            z_cell(l1, zc_lo1, zc_hi1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
     };
 
     // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
     // (the factor pass of the split form stops here: k_column_apply takes the tracers)
-    const ColumnSeq cs = column_seq_setup(a, i, j, li);
     const int ntl = a.cbuf ? 0 : a.q.nt;
     double lam_e = 0.0;  // ADJ: the emission's lambda from this lane's level
     for (int t = 0; t < ntl; t++) {
