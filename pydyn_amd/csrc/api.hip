@@ -887,8 +887,9 @@ static int middle(tm5_region *r, int ndyn, bool with_z = false, int spec_k = -1)
 }
 
 // one step of ndyn after Check_CFL's rehearsal of it passed, or (spec_k >= 0) a checked step of
-// queued interval spec_k, the first of its interval with a backup
-static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullptr)
+// queued interval spec_k, the first of its interval with a backup; open: the step's closing
+// m2phlb1 is left to close_step
+static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullptr, bool open = false)
 {
     const tm5_region_desc &d = r->d;
     TRY(operator_(r, 'x', ndyn, true, spec_k, backup));
@@ -896,9 +897,27 @@ static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullp
     TRY(middle(r, ndyn, true, spec_k));  // "z v s c m2phlb1 c s v z"
     TRY(operator_(r, 'y', ndyn, true, spec_k));
     TRY(operator_(r, 'x', ndyn, true, spec_k));
+    if (open) return 0;
     TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
                        r->copy_ms ? &r->ms : nullptr));
     r->ms_fresh = r->copy_ms;
+    return 0;
+}
+
+// An open queued step's m2phlb1 (with the copy of m for the next rehearsal); when next_ndyn > 0,
+// in the same launch the next step's dynam0 at the middle of that step, next_t (the two touch
+// disjoint arrays, and nothing the dynam0 writes is read before the next step).
+static int close_step(tm5_region *r, double next_t, int next_ndyn)
+{
+    const tm5_region_desc &d = r->d;
+    if (next_ndyn > 0)
+        TRY(launch_m2phlb1_dynam0(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp),
+                                  view3(&d.phlb), &r->ms, view3(&d.pu0), view3(&d.pv0), amplitude(d, next_t),
+                                  (double)next_ndyn / 2.0, view3(&d.am), view3(&d.bm), view3(&d.cm), r->skip, 1));
+    else
+        TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
+                           &r->ms));
+    r->ms_fresh = true;
     return 0;
 }
 
@@ -1011,10 +1030,15 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         const int end = i + batch < count ? i + batch : count;
         r->skip = TM5_ERRORS | TM5_SPEC;
         int rc = 0;
+        bool open = false;  // a queued step whose m2phlb1 is yet to be launched
         for (int k = i; k < end && rc == 0; k++) {
             const double t = t0 + (double)k * (double)nread;
             // the trials the plan expects to fail are rehearsed; the last one is the steps
             // themselves, checked
+            if (trials > 1 && open) {
+                rc = close_step(r, 0.0, 0);
+                open = false;
+            }
             for (int q = 0; q + 1 < trials && rc == 0; q++) {
                 rc = rehearse_queue(r, t, nd[q], nn[q], false);  // k_spec_check clears the bit
                 if (rc == 0) {
@@ -1025,12 +1049,18 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
             const int ndyn = nd[trials - 1], n = nn[trials - 1];
             r->copy_ms = true;
             for (int s = 0; s < n && rc == 0; s++) {
-                rc = dynam0_at(r, (t + (double)(s * ndyn)) + (double)ndyn / 2.0, ndyn, view3(&d.am), view3(&d.bm),
-                               view3(&d.cm));
-                if (rc == 0) rc = step(r, ndyn, k, s == 0 ? r->backup : nullptr);
+                const double ts = (t + (double)(s * ndyn)) + (double)ndyn / 2.0;
+                if (open)  // the previous step's m2phlb1 with this step's dynam0
+                    rc = close_step(r, ts, ndyn);
+                else
+                    rc = dynam0_at(r, ts, ndyn, view3(&d.am), view3(&d.bm), view3(&d.cm));
+                open = false;
+                if (rc == 0) rc = step(r, ndyn, k, s == 0 ? r->backup : nullptr, true);
+                open = rc == 0;
             }
             r->copy_ms = false;
         }
+        if (rc == 0 && open) rc = close_step(r, 0.0, 0);
         r->skip = TM5_ERRORS;
         TRY(rc);
         TRY(read_words(c));

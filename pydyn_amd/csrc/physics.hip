@@ -15,14 +15,14 @@ __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 // interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
 // convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
 // pit summed from the surface and sd accumulated from the top, in the reference's order.
-__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, TimeSrc ts,
-                                                const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
-                                                int *words, int skip_mask, View3 pu_out, View3 pv_out, int am_halo)
+// tile (bx, j) of dynam0 (the workgroup's share; conv: its LDS, lm * TI doubles)
+__device__ __forceinline__ void dynam0_tile(int bx, int j, double *conv, int im, int jm, int lm, int TI, View3 pu,
+                                            View3 pv, const TimeSrc &ts, const double *bt, double dt, View3 am,
+                                            View3 bm, View3 cm, int cyclic, const int *words, View3 pu_out,
+                                            View3 pv_out, int am_halo)
 {
     // TI columns i per workgroup (the context's TM5_TILE_DYNAM0)
-    if (skipped_p(words, skip_mask)) return;
-    extern __shared__ double conv[];  // [lm][TI]
-    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    const int i0 = bx * TI;
     const int ni = min(TI, im - i0);
     const int tid = threadIdx.x, nth = blockDim.x;
     const double a = time_amplitude(ts, words);
@@ -68,18 +68,27 @@ __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, 
     }
 }
 
+__global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, TimeSrc ts,
+                                                const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
+                                                int *words, int skip_mask, View3 pu_out, View3 pv_out, int am_halo)
+{
+    if (skipped_p(words, skip_mask)) return;
+    extern __shared__ double conv[];  // [lm][TI]
+    dynam0_tile(blockIdx.x, blockIdx.y, conv, im, jm, lm, TI, pu, pv, ts, bt, dt, am, bm, cm, cyclic, words, pu_out,
+                pv_out, am_halo);
+}
+
 // ------------------------------------------------------------------ m2phlb1
 // pydyn/zoom/massflow.py:282-290 on a tile of TI columns: the column masses staged in LDS,
 // sp = seqsum(m over l) g / dxyp(j) per column, then phlb = at + bt sp in parallel.
-__global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, int TI, View3 m, const double *dxyp,
-                                                 const double *at, const double *bt, View3 sp, View3 phlb, int *words,
-                                                 int skip_mask, View3 mcopy)
+// tile (bx, j) of m2phlb1 (sh: its LDS, (lm + 1) * TI doubles); mcopy (when mcopy.p is set): a
+// copy of m on the way
+__device__ __forceinline__ void m2phlb1_tile(int bx, int j, double *sh, int im, int lm, int TI, View3 m,
+                                             const double *dxyp, const double *at, const double *bt, View3 sp,
+                                             View3 phlb, View3 mcopy)
 {
-    // TI columns i per workgroup (TM5_TILE_M2PHLB); mcopy (when mcopy.p is set): a copy of m on the way
-    if (skipped_p(words, skip_mask)) return;
-    extern __shared__ double sh[];  // M [lm][TI], P [TI]
-    double *M = sh, *P = sh + lm * TI;
-    const int j = blockIdx.y, i0 = blockIdx.x * TI;
+    double *M = sh, *P = sh + lm * TI;  // M [lm][TI], P [TI]
+    const int i0 = bx * TI;
     const int ni = min(TI, im - i0);
     const int tid = threadIdx.x, nth = blockDim.x;
 #pragma unroll 8
@@ -103,6 +112,37 @@ __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, int TI,
     for (int idx = tid; idx < (lm + 1) * TI; idx += nth) {
         const int ti = idx % TI, l = idx / TI;
         if (ti < ni) phlb(i0 + ti, j, l) = at[l] + bt[l] * P[ti];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, int TI, View3 m, const double *dxyp,
+                                                 const double *at, const double *bt, View3 sp, View3 phlb, int *words,
+                                                 int skip_mask, View3 mcopy)
+{
+    // TI columns i per workgroup (TM5_TILE_M2PHLB)
+    if (skipped_p(words, skip_mask)) return;
+    extern __shared__ double sh[];
+    m2phlb1_tile(blockIdx.x, blockIdx.y, sh, im, lm, TI, m, dxyp, at, bt, sp, phlb, mcopy);
+}
+
+// A step's closing m2phlb1 and the next step's dynam0 in one launch: they touch disjoint arrays
+// (m2phlb1 reads m and writes sp, phlb and the copy of m; dynam0 reads pu, pv and writes am, bm,
+// cm), so their workgroups may run in any order.  Workgroups x < nbm are m2phlb1's tiles, the
+// rest dynam0's; each part keeps its own skip mask.
+__global__ void __launch_bounds__(256) k_m2phlb1_dynam0(int nbm, int im, int jm, int lm, int TI_m, View3 m,
+                                                        const double *dxyp, const double *at, View3 sp, View3 phlb,
+                                                        View3 mcopy, int skip_m, int TI_d, View3 pu, View3 pv,
+                                                        TimeSrc ts, const double *bt, double dt, View3 am, View3 bm,
+                                                        View3 cm, int cyclic, int skip_d, int *words)
+{
+    extern __shared__ double sh[];
+    if ((int)blockIdx.x < nbm) {
+        if (skipped_p(words, skip_m)) return;
+        m2phlb1_tile(blockIdx.x, blockIdx.y, sh, im, lm, TI_m, m, dxyp, at, bt, sp, phlb, mcopy);
+    } else {
+        if (skipped_p(words, skip_d)) return;
+        dynam0_tile(blockIdx.x - nbm, blockIdx.y, sh, im, jm, lm, TI_d, pu, pv, ts, bt, dt, am, bm, cm, cyclic, words,
+                    View3{}, View3{}, 0);
     }
 }
 
@@ -216,6 +256,24 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
     else x.ts.a = a;
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI, pu, pv,
                x.ts, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask, x.pu_out, x.pv_out, x.am_halo);
+    return status(hipGetLastError());
+}
+
+int launch_m2phlb1_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
+                          const double *bt, View3 sp, View3 phlb, const View3 *mcopy, View3 pu, View3 pv, double a,
+                          double dt, View3 am, View3 bm, View3 cm, int skip_d, int cyclic)
+{
+    const int TI_m = ctx->tile[TM5_TILE_M2PHLB], TI_d = ctx->tile[TM5_TILE_DYNAM0];
+    if (im < 1 || jm < 1 || lm < 1 || (lm + 1) * TI_m > 8192 || lm * TI_d > 8192 || !dxyp || !at || !bt)
+        return TM5_EINVAL;
+    const int nbm = (im + TI_m - 1) / TI_m, nbd = (im + TI_d - 1) / TI_d;
+    const size_t lds_m = (size_t)(lm + 1) * TI_m, lds_d = (size_t)lm * TI_d;
+    const View3 none{};
+    TimeSrc ts{};
+    ts.a = a;
+    TM5_LAUNCH(ctx, k_m2phlb1_dynam0, dim3(nbm + nbd, jm), dim3(256), (lds_m > lds_d ? lds_m : lds_d) * sizeof(double),
+               ctx->stream, nbm, im, jm, lm, TI_m, m, dxyp, at, sp, phlb, mcopy ? *mcopy : none, TM5_ERRORS, TI_d, pu,
+               pv, ts, bt, dt, am, bm, cm, cyclic, skip_d, ctx->words);
     return status(hipGetLastError());
 }
 

@@ -297,6 +297,11 @@ struct Dynam0Extra {
 };
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const Dynam0Extra *ex = nullptr);
+// m2phlb1 (skip mask TM5_ERRORS, as launch_m2phlb1) and a dynam0 at amplitude a (skip mask skip_d)
+// in one launch
+int launch_m2phlb1_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
+                          const double *bt, View3 sp, View3 phlb, const View3 *mcopy, View3 pu, View3 pv, double a,
+                          double dt, View3 am, View3 bm, View3 cm, int skip_d, int cyclic);
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);

@@ -353,18 +353,23 @@ def test_advect_y_on_region1_halo(dev):
     assert_bitwise(D.download(dfull), full, "full bm")
 
 
-def test_region_middle_split_form_equals_the_fused_step(dev):
-    """12 tracers on the small grid: an interval through the region driver (z fused into the
-    one-kernel column form) and the same steps operator by operator ('V', the column middle
-    without z, in the split form: the inverses through the context's buffer, then the tracers
-    in 16-column workgroups), bit for bit."""
+@pytest.mark.parametrize(("grid", "nt"), [
+    ("small", 12),   # 10 levels
+    ("glb3x2", 40),  # 34 levels: the apply's levels 32 and up, and two tracer ranges
+])  # fmt: skip
+def test_region_middle_split_form_equals_the_fused_step(dev, grid, nt):
+    """Many tracers: an interval through the region driver (z fused into the one-kernel column
+    form) and the same steps operator by operator ('V', the column middle without z, in the split
+    form: the inverses through the context's buffer, then the tracers in 16-column workgroups,
+    in tracer ranges), bit for bit."""
     import ctypes
 
     from pydyn_amd import _lib, meteo
     from pydyn_amd.region import Region
 
     D, K = dev
-    met = meteo.Meteo(meteo.grid("small"), seed=3, nt=12)
+    print(f"split form against the fused step: {grid}, {nt} tracers", flush=True)
+    met = meteo.Meteo(meteo.grid(grid), seed=3, nt=nt)
     fused, ops = Region(met), Region(met)
     ndyn, n = fused.interval()
     for s in range(n):
