@@ -514,8 +514,8 @@ __global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask
 }
 
 // ------------------------------------------------------------------ a child's x rows
-// pydyn/zoom/advect.py:301-366 (and check_cfl.py:121-145 mass-only): one 64-lane workgroup per
-// (j, l) row of cells xref-1..im-xref+2 (n cells; the two ends are the parent's, held) and its
+// pydyn/zoom/advect.py:301-366 (and check_cfl.py:121-145 mass-only): one workgroup of XC_THREADS
+// lanes (a cell a lane for rows of up to 128 cells) per (j, l) row of cells xref-1..im-xref+2 (n cells; the two ends are the parent's, held) and its
 // n-1 faces am(xref-1..im-xref+1).  The row's sub-cycle search scales its faces by (n-1)/n in
 // place per retry (kernels.py:169-199 with the ends held), the limiter runs on every cell of the
 // row first, the sub-steps update the interior cells, and the faces are multiplied back.
@@ -526,6 +526,11 @@ __global__ void __launch_bounds__(256) k_am_halo(ZR R, int *words, int skip_mask
 // and nothing reads in between.  X_OUTER: outer_x_faces_zeroed (pydyn/zoom/advect.py:121-136): the
 // row's first and last faces taken as zero and left in memory as they are (zeroed, then restored).
 constexpr int X_COMPRESS = FUSE_COMPRESS, X_OUTER = FUSE_OUTER;
+// Lanes a row: with many tracers each row's workgroup loops over its tracer range, loading the
+// next tracer while it steps the current one; two waves a row keep twice the loads in flight per
+// CU as one (the LDS a row needs, not the waves, bounds the rows a CU holds), which is what the
+// loop's memory latency asks for.
+constexpr int XC_THREADS = 128;
 
 // cell c of a child's x row (Fortran i0 + c, i0 = xref - 1) as compress_xedges leaves it
 __device__ __forceinline__ double xrow_cell(const HV &v, int c, int n, int i0, int r, int im, int j, int l, int t,
@@ -567,7 +572,7 @@ __device__ __forceinline__ void xrow_store(const HV &v, int c, int n, int i0, in
 }
 
 template <bool MASS_ONLY, int FUSE>
-__global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
+__global__ void __launch_bounds__(XC_THREADS) k_x_child(ZR C, int js, int je, int ls, int le, int limits, int nmax,
                                                 int fail_bit, int *words, int skip_mask, int tchunk, int spec_k)
 {
     // tchunk > 0: tracer ranges (grid y), several workgroups a row, each searching and moving the
@@ -589,14 +594,14 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
     const int xr = C.xr, im = C.im;
     double *M = lds, *M0 = M + n, *A = M0 + n, *R = A + nf, *F = R + 4 * n;
 
-    for (int c = lane; c < n; c += 64) M0[c] = M[c] = xrow_cell(C.m, c, n, i0, xr, im, j, l, 1, cz, true);
-    for (int k = lane; k < nf; k += 64) A[k] = oz && (k == 0 || k == nf - 1) ? 0.0 : C.am(i0 + k, j, l);
+    for (int c = lane; c < n; c += XC_THREADS) M0[c] = M[c] = xrow_cell(C.m, c, n, i0, xr, im, j, l, 1, cz, true);
+    for (int k = lane; k < nf; k += XC_THREADS) A[k] = oz && (k == 0 || k == nf - 1) ? 0.0 : C.am(i0 + k, j, l);
     __syncthreads();
 
     int nl = 1;
     for (;;) {
         bool ok = true;
-        for (int k = lane; k < nf; k += 64) {
+        for (int k = lane; k < nf; k += XC_THREADS) {
             const double ak = A[k];
             const int u = ak >= 0.0 ? k : k + 1;
             const bool held = u == 0 || u == n - 1;
@@ -610,10 +615,10 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                 if (!held) mx = (mx + alo) - ahi;
             }
         }
-        if (__all(ok)) break;
+        if (__syncthreads_and(ok)) break;
         nl += 1;
         __syncthreads();
-        for (int k = lane; k < nf; k += 64) A[k] = shrink(A[k], nl);
+        for (int k = lane; k < nf; k += XC_THREADS) A[k] = shrink(A[k], nl);
         __syncthreads();
         if (nl > nmax) {
             if (lane == 0) {
@@ -628,20 +633,20 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
 
     if (MASS_ONLY) {
         for (int s = 0; s < nl; s++) {
-            for (int c = lane + 1; c < n - 1; c += 64) M[c] = (M[c] + A[c - 1]) - A[c];
+            for (int c = lane + 1; c < n - 1; c += XC_THREADS) M[c] = (M[c] + A[c - 1]) - A[c];
             __syncthreads();
         }
-        for (int c = lane + 1; c < n - 1; c += 64) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
+        for (int c = lane + 1; c < n - 1; c += XC_THREADS) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
     } else {
         // each tracer's moments of the lane's (at most two) cells are loaded during the sub-steps
         // of the tracer before
         constexpr int PRE = 2;
-        const bool pre = n <= 64 * PRE;
+        const bool pre = n <= XC_THREADS * PRE;
         double qp[PRE][4];
         auto fetch = [&](int t) {
 #pragma unroll
             for (int k = 0; k < PRE; k++) {
-                const int c = lane + 64 * k;
+                const int c = lane + XC_THREADS * k;
                 if (c < n)
 #pragma unroll
                     for (int a = 0; a < 4; a++) qp[k][a] = xrow_cell(C.q[a], c, n, i0, xr, im, j, l, t, cz, a == 0);
@@ -650,12 +655,12 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
         if (pre && t0 <= t1) fetch(t0);
         for (int t = t0; t <= t1; t++) {
             if (t > t0) {
-                for (int c = lane; c < n; c += 64) M[c] = M0[c];
+                for (int c = lane; c < n; c += XC_THREADS) M[c] = M0[c];
             }
-            for (int c = lane; c < n; c += 64) {
+            for (int c = lane; c < n; c += XC_THREADS) {
                 double rm, rx, ry, rz;
                 if (pre) {
-                    const int k = (c - lane) / 64;
+                    const int k = (c - lane) / XC_THREADS;
                     rm = k == 0 ? qp[0][0] : qp[1][0];
                     rx = k == 0 ? qp[0][1] : qp[1][1];
                     ry = k == 0 ? qp[0][2] : qp[1][2];
@@ -674,7 +679,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
             if (pre && t < t1) fetch(t + 1);
             __syncthreads();
             for (int s = 0; s < nl; s++) {
-                for (int k = lane; k < nf; k += 64) {
+                for (int k = lane; k < nf; k += XC_THREADS) {
                     const double ak = A[k];
                     const int up = ak >= 0.0 ? k : k + 1;
                     const double alpha = ak * (1.0 / M[up]);
@@ -685,7 +690,7 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                     F[3 * nf + k] = ff.fw;
                 }
                 __syncthreads();
-                for (int c = lane + 1; c < n - 1; c += 64) {
+                for (int c = lane + 1; c < n - 1; c += XC_THREADS) {
                     const double m_old = M[c], a_lo = A[c - 1], a_hi = A[c];
                     const double m_new = (m_old + a_lo) - a_hi;
                     const double *f = F, *pf = F + nf, *fy = F + 2 * nf, *fz = F + 3 * nf;
@@ -700,16 +705,16 @@ __global__ void __launch_bounds__(64) k_x_child(ZR C, int js, int je, int ls, in
                 }
                 __syncthreads();
             }
-            for (int c = lane; c < n; c += 64) {
+            for (int c = lane; c < n; c += XC_THREADS) {
 #pragma unroll
                 for (int a = 0; a < 4; a++) xrow_store(C.q[a], c, n, i0, xr, im, j, l, t, cz, true, R[a * n + c]);
             }
             __syncthreads();
         }
         if (tchunk > 0) return;
-        for (int c = lane + 1; c < n - 1; c += 64) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
+        for (int c = lane + 1; c < n - 1; c += XC_THREADS) xrow_store(C.m, c, n, i0, xr, im, j, l, 1, cz, false, M[c]);
     }
-    for (int k = lane; k < nf; k += 64)
+    for (int k = lane; k < nf; k += XC_THREADS)
         if (!(oz && (k == 0 || k == nf - 1))) C.am(i0 + k, j, l) = A[k] * (double)nl;  // "restore old am"
 }
 
@@ -1213,18 +1218,18 @@ static int x_child_f(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, 
     int ranges = 1, tchunk = C.nt;
     if (!mass_only && C.nt >= 32) {  // many tracers: tracer ranges over the grid, then the masses
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false, FUSE>, 64, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false, FUSE>, XC_THREADS, lds) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), C.nt, 16);
         ranges = (C.nt + tchunk - 1) / tchunk;
     }
     if (!mass_only)
-        TM5_LAUNCH_NAMED(ctx, "k_x_child<false>", (k_x_child<false, FUSE>), dim3(grid.x, ranges), dim3(64), lds,
+        TM5_LAUNCH_NAMED(ctx, "k_x_child<false>", (k_x_child<false, FUSE>), dim3(grid.x, ranges), dim3(XC_THREADS), lds,
                          ctx->stream, C, js, je, ls, le, limits, nmax, fail_bit, ctx->words, zm(ctx, skip_mask),
                          ranges > 1 ? tchunk : 0, spec_k);
     if (mass_only || ranges > 1)
-        TM5_LAUNCH_NAMED(ctx, "k_x_child<true>", (k_x_child<true, FUSE>), grid, dim3(64), lds, ctx->stream, C, js, je,
+        TM5_LAUNCH_NAMED(ctx, "k_x_child<true>", (k_x_child<true, FUSE>), grid, dim3(XC_THREADS), lds, ctx->stream, C, js, je,
                          ls, le, 0, nmax, fail_bit, ctx->words, zm(ctx, skip_mask), 0, spec_k);
     return status(hipGetLastError());
 }
