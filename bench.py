@@ -28,6 +28,8 @@ bounded sample of the same workload on this host.
 """
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -207,6 +209,21 @@ def roofline(reg, intervals):
 
 
 # ------------------------------------------------------------------ sections
+@contextlib.contextmanager
+def _no_gc():
+    """A timed region without Python's cyclic garbage collector: it is collected before and held
+    off until after (as timeit does).  A full collection with torch loaded takes tens of ms, and
+    one landing inside a 0.13 s window would be timed as GPU work."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
+
+
 def _years(sim, wall):
     return sim / wall * 3600.0 / SECONDS_A_YEAR
 
@@ -215,11 +232,13 @@ def _queued(run, count):
     """count intervals of a ZoomRun queued on the native driver: (wall s, [(ndyn, n)])."""
     import torch
 
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    out = run.run(count)
-    torch.cuda.synchronize()
-    return time.perf_counter() - t, out
+    with _no_gc():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = run.run(count)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+    return wall, out
 
 
 def _launches(run, intervals):
@@ -359,15 +378,17 @@ def adjoint_section(met):
     for _ in range(2):  # the first pass warms up
         x = [t.clone().requires_grad_(True) for t in init]
         e = emis.clone().requires_grad_(True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = AD.transport(window, *x, e)
-        L = AD.loss(out[0], window.m_final, i, j, obs)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        L.backward()
-        torch.cuda.synchronize()
-        times.append((t1 - t0, time.perf_counter() - t1))
+        with _no_gc():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = AD.transport(window, *x, e)
+            L = AD.loss(out[0], window.m_final, i, j, obs)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
     fwd, bwd = times[-1]
     return {
         "workload": "24 x 3600 s of glb3x2x34, loss over 500 seeded columns, d/d(emission 120x90, initial tracers)",
@@ -415,12 +436,13 @@ def cpu_baseline(met, intervals):
     from oracle.driver import Run
 
     run = Run(met)
-    t = time.perf_counter()
     sim = 0
-    for _ in range(intervals):
-        ndyn, n = run.interval()
-        sim += ndyn * n
-    wall = time.perf_counter() - t
+    with _no_gc():  # as the device's timed regions
+        t = time.perf_counter()
+        for _ in range(intervals):
+            ndyn, n = run.interval()
+            sim += ndyn * n
+        wall = time.perf_counter() - t
     return {
         "value": round(_years(sim, wall), 6),
         "unit": "simulated years/hour",
@@ -440,11 +462,12 @@ def _region_run(met, count):
     from pydyn_amd.region import Region
 
     reg = Region(met)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    out = reg.run(count)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t
+    with _no_gc():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = reg.run(count)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
     reg.check()
     return wall, out, reg
 

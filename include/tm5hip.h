@@ -94,7 +94,11 @@ int tm5_ctx_clear(tm5_ctx *ctx, int mask);
 #define TM5_TILE_Z_LINES 1 /* columns i per z-line workgroup (dynamw, dynamwm), 1..32, at most 1024 / lm; 32 */
 #define TM5_TILE_DYNAM0 2  /* columns i per dynam0 workgroup, 1..64, lm * tile <= 8192; 32 */
 #define TM5_TILE_M2PHLB 3  /* columns i per m2phlb1 workgroup, 1..64, (lm + 1) * tile <= 8192; 32 */
-#define TM5_TILES 4
+/* the many-tracer forms (nt >= 8: the batched members x tracers) */
+#define TM5_TILE_Y_MANY 4     /* columns i per y-line workgroup with nt >= 8, 1..16, at most 1024 / jm; 16 */
+#define TM5_TILE_RANGE_MIN 5  /* fewest tracers in a tracer range of the sweeps and the column apply, 1..64; 16 */
+#define TM5_TILE_APPLY 6      /* columns per workgroup of the split column form's apply: 8, 16 or 32; 16 */
+#define TM5_TILES 7
 int tm5_ctx_set_tile(tm5_ctx *ctx, int which, int value);
 int tm5_ctx_get_tile(tm5_ctx *ctx, int which, int *value);
 /* Per-kernel GPU time: from tm5_profile_begin every launch on the context is bracketed by

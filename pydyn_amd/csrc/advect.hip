@@ -235,7 +235,8 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x<false>, 64, lds) != hipSuccess || per_cu < 1)
             per_cu = 1;
-        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), q.nt, 16);
+        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), q.nt,
+                              ctx->tile[TM5_TILE_RANGE_MIN]);
         ranges = (q.nt + tchunk - 1) / tchunk;
     }
     if (q.nt > 0)
@@ -504,7 +505,7 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
     const bool mass_only = q.nt == 0;
     // many tracers: the widest tile, each tracer's rows read in the longest runs along i (the
     // context's tile is tuned for the one-tracer step)
-    const int TI = lines_per_block(jm, q.nt >= 8 ? 16 : ctx->tile[TM5_TILE_Y_LINES]);
+    const int TI = lines_per_block(jm, ctx->tile[q.nt >= 8 ? TM5_TILE_Y_MANY : TM5_TILE_Y_LINES]);
     if (TI < 1) return TM5_EINVAL;
     const int chunks = (im * (jm - 1) + Y_NTH * Y_FPT - 1) / (Y_NTH * Y_FPT);
     if (!known_one)
@@ -519,7 +520,8 @@ int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm,
         return status(hipGetLastError());
     }
     // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
-    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt, 16)
+    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt,
+                                                 ctx->tile[TM5_TILE_RANGE_MIN])
                                   : q.nt;
     const int ranges = (q.nt + tchunk - 1) / tchunk;
     TM5_LAUNCH(ctx, k_y_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)5 * jm * TI * sizeof(double),
@@ -897,7 +899,8 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
         return status(hipGetLastError());
     }
     // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
-    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt, 16)
+    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt,
+                                                 ctx->tile[TM5_TILE_RANGE_MIN])
                                   : q.nt;
     const int ranges = (q.nt + tchunk - 1) / tchunk;
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)(4 * lm * TI + lm * TI) * sizeof(double),

@@ -186,6 +186,9 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
     c->tile[TM5_TILE_Z_LINES] = 32;
     c->tile[TM5_TILE_DYNAM0] = 32;
     c->tile[TM5_TILE_M2PHLB] = 32;
+    c->tile[TM5_TILE_Y_MANY] = 16;
+    c->tile[TM5_TILE_RANGE_MIN] = 16;
+    c->tile[TM5_TILE_APPLY] = 16;
     if (e == hipSuccess) e = hipMalloc((void **)&c->words, W_WORDS * sizeof(int));
     // per layer: nloop, the y check's tickets and verdicts (zero between launches)
     if (e == hipSuccess) e = hipMalloc((void **)&c->y_nloop, 3 * c->y_cap * sizeof(int));
@@ -304,8 +307,9 @@ extern "C" int tm5_profile_entry(tm5_ctx *c, int idx, const char **kernel, long 
 
 extern "C" int tm5_ctx_set_tile(tm5_ctx *c, int which, int value)
 {
-    static const int hi[TM5_TILES] = {16, 32, 64, 64};
+    static const int hi[TM5_TILES] = {16, 32, 64, 64, 16, 64, 32};
     if (!c || which < 0 || which >= TM5_TILES || value < 1 || value > hi[which]) return TM5_EINVAL;
+    if (which == TM5_TILE_APPLY && value != 8 && value != 16 && value != 32) return TM5_EINVAL;
     c->tile[which] = value;
     return 0;
 }

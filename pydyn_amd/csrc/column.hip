@@ -656,21 +656,23 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
 }
 
 // The apply of the split form, for many tracers: k_column's sequence on every tracer, from the
-// rows of C its factor pass stored in cbuf.  A workgroup takes APPLY_COLS neighbouring columns of
-// a row (two columns a wave, lanes over levels as in k_column) and a range of tracers (grid z),
+// rows of C its factor pass stored in cbuf.  A workgroup takes blockDim.x / 32 (8, 16 or 32: the
+// context's TM5_TILE_APPLY) neighbouring columns of a row (two columns a wave, lanes over levels
+// as in k_column) and a range of tracers (grid z),
 // so each level's cache line of a tracer is read and written by one workgroup at about the same
 // time: its waves meet at a barrier after every tracer (a rendezvous only; nothing is exchanged
 // between waves), and each wave loads the tracers ahead of the one it works on.  Levels 32 and
 // up (only sources and chemistry act there) follow for the range's tracers at once, cell by cell.
-constexpr int APPLY_COLS = 16;
+constexpr int APPLY_COLS_MAX = 32;
 template <int N>
-__global__ void __launch_bounds__(APPLY_COLS * 32, 4)
+__global__ void __launch_bounds__(APPLY_COLS_MAX * 32)
     k_column_apply(ColumnArgs a, const double *cbuf, int tchunk, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
-    constexpr int AHEAD = 2;                     // tracers loaded ahead
-    __shared__ double rvs[APPLY_COLS * 32];      // each column's rm by level, for v
-    __shared__ double dec_hi[APPLY_COLS * 32];   // each column's decay factors at levels 32 and up
+    constexpr int AHEAD = 2;                         // tracers loaded ahead
+    __shared__ double rvs[APPLY_COLS_MAX * 32];      // each column's rm by level, for v
+    __shared__ double dec_hi[APPLY_COLS_MAX * 32];   // each column's decay factors at levels 32 and up
+    const int cols = blockDim.x >> 5;
     // tiles renumbered XCD-contiguously (xcd_tile), the tracer range slowest
     const int nbx = gridDim.x, nby = gridDim.y;
     const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
@@ -683,7 +685,7 @@ __global__ void __launch_bounds__(APPLY_COLS * 32, 4)
     // neighbouring tiles do not share the sectors at their edges
     const int j = by;
     const int e = (a.q.sl % 8 == 0 && a.q.st % 8 == 0) ? (int)(((uintptr_t)(a.q.q[0] + a.q.off(0, j, 0, 0)) >> 3) & 7) : 0;
-    int i = bx * APPLY_COLS - e + wave * 2 + h;
+    int i = bx * cols - e + wave * 2 + h;
     const bool active = i >= 0 && i < a.im;
     if (!active) i = i < 0 ? 0 : a.im - 1;
     const bool store = active && (a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0);
@@ -833,10 +835,12 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
     // tracer ranges enough for the tiles to fill the device's workgroup slots (16 waves a CU)
     const int nt = args.q.nt;
     if (nt < 1) return 0;  // no tracers: the matrices only
-    const int tiles = (args.im + 7 + APPLY_COLS - 1) / APPLY_COLS * args.jm;
-    const int tchunk = tracer_chunk(tiles, 32 / APPLY_COLS * (ctx->ncu > 0 ? ctx->ncu : 256), nt, 16);
-    const dim3 agrid((args.im + 7 + APPLY_COLS - 1) / APPLY_COLS, args.jm, (nt + tchunk - 1) / tchunk);
-    const dim3 ablock(APPLY_COLS * 32);
+    const int cols = ctx->tile[TM5_TILE_APPLY];  // 8, 16 or 32 (tm5_ctx_set_tile)
+    const int tiles = (args.im + 7 + cols - 1) / cols * args.jm;
+    const int tchunk = tracer_chunk(tiles, 32 / cols * (ctx->ncu > 0 ? ctx->ncu : 256), nt,
+                                    ctx->tile[TM5_TILE_RANGE_MIN]);
+    const dim3 agrid((args.im + 7 + cols - 1) / cols, args.jm, (nt + tchunk - 1) / tchunk);
+    const dim3 ablock(cols * 32);
     TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, buf, tchunk,
                      ctx->words, skip_mask);
     return status(hipGetLastError());

@@ -1221,7 +1221,8 @@ static int x_child_f(tm5_ctx *ctx, const ZR &C, int js, int je, int ls, int le, 
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_child<false, FUSE>, XC_THREADS, lds) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
-        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), C.nt, 16);
+        tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), C.nt,
+                              ctx->tile[TM5_TILE_RANGE_MIN]);
         ranges = (C.nt + tchunk - 1) / tchunk;
     }
     if (!mass_only)
@@ -1266,12 +1267,14 @@ int zoom_y_child(tm5_ctx *ctx, const ZR &C, int is, int ie, int ls, int le, int 
             TM5_LAUNCH(ctx, k_y_child_search<false>, dim3(nlay), dim3(1024), 0, ctx->stream, C, is, ie, ls, nmax,
                        nloop, fail_bit, ctx->words, zm(ctx, skip_mask));
     }
-    const int TI = (1024 / n) < 16 ? (1024 / n) : 16;
+    const int cap = C.nt >= 8 ? ctx->tile[TM5_TILE_Y_MANY] : 16;
+    const int TI = (1024 / n) < cap ? (1024 / n) : cap;
     const int ni = ie - is + 1;
     const dim3 grid((unsigned)((ni + TI - 1) / TI), (unsigned)nlay), block((unsigned)((TI * n + 63) / 64 * 64));
     // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
     const int tchunk = !mass_only && C.nt >= 32
-                           ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, C.nt, 16)
+                           ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, C.nt,
+                                          ctx->tile[TM5_TILE_RANGE_MIN])
                            : C.nt;
     const int ranges = C.nt > 0 ? (C.nt + tchunk - 1) / tchunk : 1;
     auto steps = [&](auto fz) {

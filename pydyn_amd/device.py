@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 
 # tm5_ctx_set_tile's knobs (include/tm5hip.h), by name
-TILES = {"y_lines": 0, "z_lines": 1, "dynam0": 2, "m2phlb": 3}
+TILES = {"y_lines": 0, "z_lines": 1, "dynam0": 2, "m2phlb": 3, "y_many": 4, "range_min": 5, "apply": 6}
 # the shapes pydyn_amd.tune measured best, applied to every new context (absent: the defaults)
 TILES_FILE = pathlib.Path(__file__).resolve().parent / "tiles.json"
 TILES_FILE_ENABLED = True

@@ -1,68 +1,107 @@
-"""Tile shapes of the line and column kernels, measured on the GPU.
+"""Tile shapes of the line, column and many-tracer kernels, measured on the GPU.
 
-    python -m pydyn_amd.tune [--grid glb3x2] [--intervals 6] [--write]
+    python -m pydyn_amd.tune [--workload region|batched|all] [--intervals 6] [--write]
 
-For each knob of tm5_ctx_set_tile (device.TILES), in turn, every candidate value is set on the
-context and the region's queued intervals are run with per-kernel timing (the library's events
-around each launch); the value whose kernels take the least GPU time is kept before the next
-knob is tried.  --write stores the result in pydyn_amd/tiles.json, which every new context
-applies.  A tile shape changes how the work is split into workgroups, never an operation or its
-order, so the results are the same bits for every value (the GPU tests run with the file's).
+Each knob of tm5_ctx_set_tile (device.TILES) is tuned on the workload whose kernels it shapes:
+
+  region   the plain step on one global region (glb3x2 x 34, one tracer): the y-line tile, the
+           dynam0 and m2phlb1 tiles;
+  batched  64 members x 4 tracers on the production zoom tree (nt = 256): the z-line tile (z runs
+           on its own in a tree), the y tile of the many-tracer sweeps, the fewest tracers a
+           tracer range holds, and the apply tile of the split column form.
+
+For each knob in turn every candidate value is set on the context and the workload's queued
+intervals are run with per-kernel timing (the library's events around each launch); the value
+whose kernels take the least GPU time per dynamic step is kept before the next knob is tried.
+--write stores the result in pydyn_amd/tiles.json, which every new context applies.  A tile shape
+changes how the work is split into workgroups, never an operation or its order, so the results
+are the same bits for every value (the GPU tests run with the file's, and with other shapes).
 """
 
 import argparse
 import json
 import sys
 
-# knob -> (candidate values, the kernels it shapes)
+# workload -> knob -> (candidate values, the kernels it shapes; None: every kernel of the step)
 CANDIDATES = {
-    "y_lines": ((2, 4, 6, 8, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
-    "z_lines": ((4, 8, 16, 30, 32), ("k_z_steps<false>", "k_z_steps<true>", "k_z_mass1")),
-    "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0",)),
-    "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1",)),
+    "region": {
+        "y_lines": ((2, 4, 6, 8, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
+        "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_m2phlb1_dynam0")),
+        "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1", "k_m2phlb1_dynam0")),
+    },
+    "batched": {
+        "z_lines": ((4, 8, 16, 30, 32), None),
+        "y_many": ((4, 6, 8, 11, 16), None),
+        "range_min": ((4, 8, 16, 32, 64), None),
+        "apply": ((8, 16, 32), None),
+    },
 }
 
 
-def _time(reg, ctx, kernels, intervals):
-    ctx.profile_begin()
-    reg.run(intervals)
-    prof = ctx.profile_end()
-    return sum(prof[k][1] for k in kernels if k in prof)
-
-
-def tune(grid="glb3x2", intervals=6, log=print):
-    from pydyn_amd import device as D
+def _region(members):
     from pydyn_amd import meteo
     from pydyn_amd.region import Region
 
-    met = meteo.Meteo(meteo.grid(grid), seed=1)
-    reg = Region(met)
+    reg = Region(meteo.Meteo(meteo.grid("glb3x2"), seed=1))
+    return reg.run, reg.close
+
+
+def _batched(members):
+    from pydyn_amd import meteo
+    from pydyn_amd.zoom.run import ZoomRun
+
+    p, c = meteo.zoom("c2", nt=4)
+    run = ZoomRun(p, c, ensemble=meteo.Ensemble(p, members))
+    return run.run, run.close
+
+
+WORKLOADS = {"region": _region, "batched": _batched}
+
+
+def _time(run, ctx, kernels, intervals):
+    """GPU time per dynamic step of `intervals` more intervals (the kernels listed, or all)."""
+    ctx.profile_begin()
+    out = run(intervals)
+    prof = ctx.profile_end()
+    steps = sum(n for _, n in out)
+    return sum(s for k, (_, s) in prof.items() if kernels is None or k in kernels) / steps
+
+
+def tune(workload="region", intervals=6, members=64, log=print):
+    from pydyn_amd import device as D
+
+    run, close = WORKLOADS[workload](members)
     ctx = D.context()
-    reg.run(2)  # warm up
+    run(2)  # warm up
     best = {}
-    for knob, (values, kernels) in CANDIDATES.items():
-        times = {}
-        for v in values:
-            try:
-                ctx.set_tile(knob, v)
-            except Exception:  # outside the knob's range for this grid
+    try:
+        for knob, (values, kernels) in CANDIDATES[workload].items():
+            times = {}
+            for v in values:
+                try:
+                    ctx.set_tile(knob, v)
+                except Exception:  # outside the knob's range for this grid
+                    continue
+                run(1)
+                times[v] = min(_time(run, ctx, kernels, intervals) for _ in range(2))
+                if times[v] == 0.0:  # the workload launches none of these kernels: keep the default
+                    times.pop(v)
+            if not times:
                 continue
-            reg.run(1)
-            times[v] = min(_time(reg, ctx, kernels, intervals) for _ in range(2))
-            if times[v] == 0.0:  # the grid's run launches none of these kernels: keep the default
-                times.pop(v)
-        if not times:
-            continue
-        best[knob] = min(times, key=times.get)
-        ctx.set_tile(knob, best[knob])
-        log(f"{knob:8s} " + "  ".join(f"{v}: {t / intervals * 1e6:8.2f} us" for v, t in times.items()) + f"  -> {best[knob]}")
+            best[knob] = min(times, key=times.get)
+            ctx.set_tile(knob, best[knob])
+            log(f"{knob:9s} " + "  ".join(f"{v}: {t * 1e6:9.2f} us" for v, t in times.items()) + f"  -> {best[knob]}"
+                "  (GPU time a step)")
+    finally:
+        close()
     return best
 
 
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    p.add_argument("--grid", default="glb3x2")
+    p.add_argument("--workload", choices=("region", "batched", "all"), default="all")
     p.add_argument("--intervals", type=int, default=6)
+    p.add_argument("--members", type=int, default=64, help="members of the batched workload (4 tracers each)")
     p.add_argument("--write", action="store_true", help="store the result in pydyn_amd/tiles.json")
     p.add_argument("--out", help="store the result in this file instead")
     a = p.parse_args(argv)
@@ -71,11 +110,20 @@ def main(argv=None):
     from pydyn_amd import device as D
 
     D.TILES_FILE_ENABLED = False  # measure from the library's defaults
-    best = tune(a.grid, a.intervals, log=lambda s: print(s, flush=True))
-    out = {"grid": a.grid, "device": "MI355X (gfx950)", "tiles": best}
+    best, names = {}, {"region": "glb3x2 x 34, 1 tracer", "batched": f"c2 tree, {a.members} members x 4 tracers"}
+    done = {}
+    for w in ("region", "batched") if a.workload == "all" else (a.workload,):
+        print(f"== {w}: {names[w]}", flush=True)
+        best.update(tune(w, a.intervals, a.members, log=lambda s: print(s, flush=True)))
+        done[w] = names[w]
+    out = {"workloads": done, "device": "MI355X (gfx950)", "tiles": best}
     print(json.dumps(out))
     path = pathlib.Path(a.out) if a.out else D.TILES_FILE if a.write else None
     if path is not None:
+        if a.workload != "all" and path.exists():  # keep the other workload's knobs
+            old = json.loads(path.read_text())
+            out["workloads"] = {**old.get("workloads", {}), **done}
+            out["tiles"] = {**old.get("tiles", {}), **best}
         path.write_text(json.dumps(out, indent=1) + "\n")
     return 0
 

@@ -165,6 +165,40 @@ def test_zoom_ensemble_members_equal_their_plain_runs(dev, base, ntr, members, i
         _member_slices_equal(batched, run, m, ntr)
 
 
+@pytest.mark.parametrize("shape", [
+    {"y_many": 3, "range_min": 5, "apply": 8},
+    {"y_many": 16, "range_min": 1, "apply": 32},
+    {"y_many": 8, "range_min": 64, "apply": 16},
+])  # fmt: skip
+def test_many_tracer_tile_shapes_change_no_bits(dev, shape):
+    """The many-tracer knobs of tm5_ctx_set_tile (y tile, fewest tracers a range, apply tile)
+    split the same operations differently: 16 members x 4 tracers (nt = 64, tracer ranges in
+    every sweep and the split column form) on the tiny tree with shapes unlike the tuned ones,
+    every checked member against its plain run (nt = 4, which these knobs do not shape)."""
+    from pydyn_amd.zoom.run import ZoomRun
+
+    D = dev
+    ctx = D.context()
+    saved = {k: ctx.tile(k) for k in ("y_many", "range_min", "apply")}
+    try:
+        for k, v in shape.items():
+            ctx.set_tile(k, v)
+        pmet, cmet, ens = _ensemble("tiny2", 4, 16)
+        batched = ZoomRun(pmet, cmet, ensemble=ens)
+        plains = {m: ZoomRun(pmet, cmet, ensemble=ens.subset([m])) for m in (0, 9, 15)}
+        for it in range(24):
+            nd = batched.interval()
+            for m, run in plains.items():
+                assert run.interval() == nd, (it, m)
+        for m, run in plains.items():
+            _member_slices_equal(batched, run, m, 4)
+    finally:
+        for k, v in saved.items():
+            ctx.set_tile(k, v)
+    with pytest.raises(RuntimeError):
+        ctx.set_tile("apply", 12)
+
+
 # ------------------------------------------------------------------ the physics bodies alone
 def test_zoom_physics_bodies_match_oracle(dev):
     """convec_region, sources_region, chemistry_region and assign of pydyn_amd.zoom.lib (the
