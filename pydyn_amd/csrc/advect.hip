@@ -31,7 +31,7 @@ template <bool MASS_ONLY>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
                                           const int *imred, double *amax_out, int spec_k, double *bk, int tchunk,
-                                          int hj0, int hj1)
+                                          int hj0, int hj1, FluxSrc fs, int nd0)
 {
     // rows j in hj0..hj1 (uniform rows only) also get their periodic halo, index -1 = index n - 1 and
     // index n = index 0, for m and every moment (a zoom tree's region 1, pydyn/zoom/advect.py:283-288)
@@ -39,11 +39,24 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
     // bk (whole rows only): the row's m and moments as read, i fastest, m first and then every
     // tracer's rm, rxm, rym, rzm (each im * jm * lm)
+    // fs.u.p (whole periodic rows): the faces made from pu, am(k) = dt (a pu(k)) with face 0 the
+    // same as face im, as dynam0 makes them, instead of read from am; the first nd0 workgroups (one
+    // tracer range only) are dynam0's tiles making bm and cm (fs.bm.p): dispatched first, their
+    // column sums (a chain of dependent adds a column) run while the rows keep the device busy
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
+    if ((int)blockIdx.x < nd0) {  // a dynam0 tile: bm and cm of the step, am left to the rows
+        const int ntx = (im + fs.TI - 1) / fs.TI;
+        TimeSrc ts{};
+        ts.a = fs.a;
+        dynam0_tile(blockIdx.x % ntx, blockIdx.x / ntx, lds, im, jm, lm, fs.TI, fs.u, fs.v, ts, fs.bt, fs.dt, View3{},
+                    fs.bm, fs.cm, 1, words, View3{}, View3{}, 0);
+        return;
+    }
     // rows renumbered XCD-contiguously, the tracer range slowest
-    const int b = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int row = b % gridDim.x, bz = b / gridDim.x;
+    const int nrx = gridDim.x - nd0;
+    const int b = xcd_tile(blockIdx.y * nrx + (blockIdx.x - nd0), nrx * gridDim.y);
+    const int row = b % nrx, bz = b / nrx;
     const int j = row % jm, l = row / jm;
     const int t0 = tchunk > 0 ? bz * tchunk : 0;
     const int t1 = tchunk > 0 && t0 + tchunk < q.nt ? t0 + tchunk : q.nt;
@@ -62,7 +75,11 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         M0[i] = M[i] = v;
         if (bk) bk[brow + i] = v;
     }
-    for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
+    if (fs.u.p) {
+        for (int k = lane; k <= n; k += 64) A[k] = fs.dt * (fs.a * fs.u(k == 0 ? im : k, j, l));
+    } else {
+        for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
+    }
     // the first tracer's moments of the lane's (at most two) cells, in flight during the search
     constexpr int PRE = 2;
     const bool pre = !MASS_ONLY && t0 < t1 && n <= 64 * PRE;
@@ -219,14 +236,220 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     for (int k = lane; k <= n; k += 64) am(k, j, l) = A[k] * (double)nl;  // "restore old am"
 }
 
+// ------------------------------------------------------------------ x twice
+// A checked step's last dynamu and the next checked step's first in one launch: they act on the
+// same row one after the other (the step's closing m2phlb1, which only reads m, and the next step's
+// Setup_MassFlow, which only makes fluxes, come between them in the reference's order and touch
+// nothing either reads), so a workgroup takes its row through both.  Pass A takes the faces from
+// am, pass B makes them from pu (FluxSrc) and stores them restored; the first nd0 workgroups make
+// the next step's bm and cm (dynam0's tiles), as k_x's.
+
+// k_x's search of a periodic row of n cells whose mass is M: nloop = 1, 2, ... with every face's
+// upwind cell carried through nloop mass-only sub-steps, the faces A scaled by (nloop-1)/nloop in
+// place after a failure; the accepted nloop, or nmax + 1 when none up to nmax passes
+__device__ int x_row_search(int n, double *A, const double *M, int nmax, int lane)
+{
+    int nl = 1;
+    for (;;) {
+        bool ok = true;
+        for (int k = lane; k <= n; k += 64) {
+            const double ak = A[k];
+            const int u = ak >= 0.0 ? (k == 0 ? n - 1 : k - 1) : (k == n ? 0 : k);
+            const double alo = A[u], ahi = A[u + 1];
+            double mx = M[u];
+            for (int s = 0; s < nl; s++) {
+                if (!(fabs(ak / mx) < 1.0)) {
+                    ok = false;
+                    break;
+                }
+                mx = (mx + alo) - ahi;
+            }
+        }
+        if (__all(ok)) return nl;
+        nl += 1;
+        __syncthreads();
+        for (int k = lane; k <= n; k += 64) A[k] = shrink(A[k], nl);
+        __syncthreads();
+        if (nl > nmax) return nl;
+    }
+}
+
+// k_x's nl sub-steps of one tracer on the row: faces 1..n (face 0 takes face n's fluxes), then
+// every cell; M, R (rm, rxm, rym, rzm) in place, F the faces' fluxes
+__device__ void x_row_substeps(int n, int nl, const double *A, double *M, double *R, double *F, int limits, int lane)
+{
+    for (int s = 0; s < nl; s++) {
+        for (int k = lane + 1; k <= n; k += 64) {
+            const double ak = A[k];
+            const int up = ak >= 0.0 ? k - 1 : (k == n ? 0 : k);
+            const double alpha = ak * (1.0 / M[up]);
+            const FaceFlux ff = face_flux(ak, alpha, R[up], R[n + up], R[2 * n + up], R[3 * n + up]);
+            F[k] = ff.f;
+            F[(n + 1) + k] = ff.pf;
+            F[2 * (n + 1) + k] = ff.ft;
+            F[3 * (n + 1) + k] = ff.fw;
+        }
+        __syncthreads();
+        if (lane < 4) F[lane * (n + 1)] = F[lane * (n + 1) + n];
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) {
+            const double m_old = M[i], a_lo = A[i], a_hi = A[i + 1];
+            const double m_new = (m_old + a_lo) - a_hi;
+            const double *f = F, *pf = F + (n + 1), *fy = F + 2 * (n + 1), *fz = F + 3 * (n + 1);
+            const double rm_new = R[i] - (f[i + 1] - f[i]);
+            double rx = slope_update(R[n + i], pf[i], pf[i + 1], a_lo, a_hi, rm_new, f[i], f[i + 1], m_old, m_new);
+            if (limits) rx = limit(rx, rm_new);
+            R[i] = rm_new;
+            R[n + i] = rx;
+            R[2 * n + i] = R[2 * n + i] - (fy[i + 1] - fy[i]);
+            R[3 * n + i] = R[3 * n + i] - (fz[i + 1] - fz[i]);
+            M[i] = m_new;
+        }
+        __syncthreads();
+    }
+}
+
+// spec_a / spec_b: the checked steps' intervals (a search past nmax is a failed check, spec_fail_first);
+// bk (null: none): pass B's input -- the row's m and moments after pass A, k_x's backup layout.  A
+// row whose pass B fails its check still writes its part of bk (the host restores it whole).
+__global__ void __launch_bounds__(64) k_xx(int im, int jm, int lm, View3 m, Tracers q, View3 am, FluxSrc fs, int nd0,
+                                           int limits, int nmax, int *words, int *rows_out, int skip_mask,
+                                           int spec_a, int spec_b, double *bk)
+{
+    if (skipped(words, skip_mask)) return;
+    extern __shared__ double lds[];
+    if ((int)blockIdx.x < nd0) {  // a dynam0 tile: the next step's bm and cm
+        const int ntx = (im + fs.TI - 1) / fs.TI;
+        TimeSrc ts{};
+        ts.a = fs.a;
+        dynam0_tile(blockIdx.x % ntx, blockIdx.x / ntx, lds, im, jm, lm, fs.TI, fs.u, fs.v, ts, fs.bt, fs.dt, View3{},
+                    fs.bm, fs.cm, 1, words, View3{}, View3{}, 0);
+        return;
+    }
+    const int row = xcd_tile(blockIdx.x - nd0, gridDim.x - nd0);  // rows renumbered XCD-contiguously
+    const int j = row % jm, l = row / jm;
+    const int n = im, lane = threadIdx.x;
+    double *M = lds;            // n   current mass
+    double *M0 = M + n;         // n   the row's mass at the start
+    double *M1 = M0 + n;        // n   after pass A
+    double *AA = M1 + n;        // n+1 pass A's faces
+    double *AB = AA + n + 1;    // n+1 pass B's faces
+    double *R = AB + n + 1;     // 4n  rm, rxm, rym, rzm of one tracer
+    double *F = R + 4 * n;      // 4(n+1) f, pf, ft (=fy), fw (=fz)
+    for (int i = lane; i < n; i += 64) M0[i] = M1[i] = m(i, j, l);
+    for (int k = lane; k <= n; k += 64) {
+        AA[k] = am(k, j, l);
+        AB[k] = fs.dt * (fs.a * fs.u(k == 0 ? im : k, j, l));
+    }
+    // the first tracer's moments of the lane's (at most two) cells, in flight during the searches
+    constexpr int PRE = 2;
+    const bool pre = q.nt > 0 && n <= 64 * PRE;
+    double qp[PRE][4];
+#pragma unroll
+    for (int c = 0; c < PRE; c++) {
+        const int i = lane + 64 * c;
+        if (pre && i < n) {
+            const long long o = q.off(i, j, l, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++) qp[c][a] = q.q[a][o];
+        }
+    }
+    __syncthreads();
+    const int nla = x_row_search(n, AA, M0, nmax, lane);
+    if (nla > nmax) {
+        if (lane == 0) spec_fail_first(words, spec_a);
+        return;
+    }
+    for (int s = 0; s < nla; s++)  // each cell reads only itself and its two faces: in place
+        for (int i = lane; i < n; i += 64) M1[i] = (M1[i] + AA[i]) - AA[i + 1];
+    __syncthreads();
+    const int nlb = x_row_search(n, AB, M1, nmax, lane);
+    const bool fail_b = nlb > nmax;
+    if (lane == 0) {
+        if (fail_b) spec_fail_first(words, spec_b);
+        rows_out[row] = nlb;
+    }
+    if (fail_b && !bk) return;
+    const long long ncell = (long long)im * jm * lm, brow = (long long)im * (j + (long long)jm * l);
+    if (bk)
+        for (int i = lane; i < n; i += 64) bk[brow + i] = M1[i];
+    for (int t = 0; t < q.nt; t++) {
+        for (int i = lane; i < n; i += 64) {
+            M[i] = M0[i];
+            const long long o = q.off(i, j, l, t);
+            double rm, rx, ry, rz;
+            if (pre && t == 0) {  // i = lane + 64 c with c < PRE
+                const int c = (i - lane) / 64;
+                rm = c == 0 ? qp[0][0] : qp[1][0];
+                rx = c == 0 ? qp[0][1] : qp[1][1];
+                ry = c == 0 ? qp[0][2] : qp[1][2];
+                rz = c == 0 ? qp[0][3] : qp[1][3];
+            } else {
+                rm = q.q[0][o];
+                rx = q.q[1][o];
+                ry = q.q[2][o];
+                rz = q.q[3][o];
+            }
+            R[i] = rm;
+            R[n + i] = limits ? limit(rx, rm) : rx;
+            R[2 * n + i] = ry;
+            R[3 * n + i] = rz;
+        }
+        __syncthreads();
+        x_row_substeps(n, nla, AA, M, R, F, limits, lane);
+        // the tracer at the next step's start: into the backup, pass B's initial limiter
+        for (int i = lane; i < n; i += 64) {
+            if (bk) {
+#pragma unroll
+                for (int a = 0; a < 4; a++) bk[(1 + 4 * t + a) * ncell + brow + i] = R[a * n + i];
+            }
+            if (limits) R[n + i] = limit(R[n + i], R[i]);
+        }
+        __syncthreads();
+        if (fail_b) continue;
+        x_row_substeps(n, nlb, AB, M, R, F, limits, lane);
+        for (int i = lane; i < n; i += 64) {
+            const long long o = q.off(i, j, l, t);
+#pragma unroll
+            for (int a = 0; a < 4; a++) q.q[a][o] = R[a * n + i];
+        }
+        __syncthreads();
+    }
+    if (fail_b) return;
+    if (q.nt == 0) {  // the mass alone through pass B
+        for (int s = 0; s < nlb; s++)
+            for (int i = lane; i < n; i += 64) M1[i] = (M1[i] + AB[i]) - AB[i + 1];
+        __syncthreads();
+    }
+    const double *Mend = q.nt > 0 ? M : M1;
+    for (int i = lane; i < n; i += 64) m(i, j, l) = Mend[i];
+    for (int k = lane; k <= n; k += 64) am(k, j, l) = AB[k] * (double)nlb;  // "restore old am"
+}
+
+int launch_xx(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int skip_mask,
+              int spec_a, int spec_b, double *backup, const FluxSrc &fs)
+{
+    if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap || !checked(spec_a) || !checked(spec_b))
+        return TM5_EINVAL;
+    if (!fs.u.p || !fs.v.p || !fs.bm.p || !fs.cm.p || !fs.bt || fs.TI < 1 || lm * fs.TI > 8192) return TM5_EINVAL;
+    size_t lds = (size_t)(13 * im + 6) * sizeof(double);
+    if ((size_t)lm * fs.TI * sizeof(double) > lds) lds = (size_t)lm * fs.TI * sizeof(double);
+    const int nd0 = ((im + fs.TI - 1) / fs.TI) * jm;
+    ctx->x_nrows = jm * lm;
+    TM5_LAUNCH(ctx, k_xx, dim3((unsigned)(nd0 + jm * lm)), dim3(64), lds, ctx->stream, im, jm, lm, m, q, am, fs, nd0,
+               limits, nmax, ctx->words, ctx->x_rows, skip_mask, spec_a, spec_b, backup);
+    return status(hipGetLastError());
+}
+
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask, const int *imred, int *rows_out, double *amax_out, int spec_k, double *backup, int halo_j0,
-             int halo_j1)
+             int halo_j1, const FluxSrc *flux)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
-    if (backup && imred) return TM5_EINVAL;  // the backup holds whole rows
+    if ((backup || flux) && imred) return TM5_EINVAL;  // the backup holds whole rows; faces made from pu too
+    const FluxSrc fs = flux ? *flux : FluxSrc{};
     if (imred && halo_j1 >= halo_j0) return TM5_EINVAL;  // a reduced row's halo is the reduced grid's
-    const size_t lds = (size_t)(11 * im + 5) * sizeof(double);
+    size_t lds = (size_t)(11 * im + 5) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
     int *rows = rows_out ? rows_out : ctx->x_rows;
@@ -239,13 +462,28 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
                               ctx->tile[TM5_TILE_RANGE_MIN]);
         ranges = (q.nt + tchunk - 1) / tchunk;
     }
+    // the step's bm and cm: dynam0's tiles as extra workgroups of the launch that takes every
+    // tracer (one range), else a dynam0 launch of their own before it
+    unsigned nd0 = 0;
+    if (fs.bm.p) {
+        if (!fs.u.p || !fs.v.p || !fs.cm.p || !fs.bt || fs.TI < 1 || lm * fs.TI > 8192) return TM5_EINVAL;
+        if (ranges > 1) {
+            const int rc = launch_dynam0(ctx, im, jm, lm, fs.u, fs.v, fs.a, fs.bt, fs.dt, View3{}, fs.bm, fs.cm,
+                                         skip_mask, 1);
+            if (rc) return rc;
+        } else {
+            nd0 = (unsigned)(((im + fs.TI - 1) / fs.TI) * jm);
+            if ((size_t)lm * fs.TI * sizeof(double) > lds) lds = (size_t)lm * fs.TI * sizeof(double);
+        }
+    }
     if (q.nt > 0)
-        TM5_LAUNCH(ctx, k_x<false>, dim3(grid.x, ranges), block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax,
-                   ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, ranges > 1 ? tchunk : 0,
-                   halo_j0, halo_j1);
+        TM5_LAUNCH(ctx, k_x<false>, dim3(grid.x + nd0, ranges), block, lds, ctx->stream, im, jm, lm, m, q, am, limits,
+                   nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup,
+                   ranges > 1 ? tchunk : 0, halo_j0, halo_j1, fs, (int)nd0);
     if (q.nt == 0 || ranges > 1)
-        TM5_LAUNCH(ctx, k_x<true>, grid, block, lds, ctx->stream, im, jm, lm, m, q, am, limits, nmax, ctx->words,
-                   rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0, halo_j0, halo_j1);
+        TM5_LAUNCH(ctx, k_x<true>, dim3(grid.x + (q.nt == 0 ? nd0 : 0)), block, lds, ctx->stream, im, jm, lm, m, q,
+                   am, limits, nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0,
+                   halo_j0, halo_j1, fs, q.nt == 0 ? (int)nd0 : 0);
     return status(hipGetLastError());
 }
 

@@ -1,5 +1,6 @@
 // The extern "C" boundary of libtm5hip (include/tm5hip.h): the context, one entry point per
 // operator, and the native driver of a single global region.
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -687,14 +688,14 @@ struct tm5_region {
     tm5_ctx *ctx;
     tm5_region_desc d;
     int skip;         // the step kernels' skip mask: TM5_ERRORS, with TM5_SPEC while queuing ahead
-    bool copy_ms;     // while queuing: the step's last m2phlb1 also copies m into the rehearsal's ms
     bool ms_fresh;    // ms holds m (the next rehearsal need not copy it)
     int plan;         // tm5_region_run: the step the last interval settled on (0: none yet)
     int batch;        // tm5_region_run: intervals it queues per read of the device
     double *scratch;  // the rehearsal's m, am, bm, cm (i-fastest, contiguous)
     View3 ms, ams, bms, cms;
-    double *backup;   // tm5_region_run: the queued interval's m and moments at its start (k_x's layout)
-    size_t backup_n;  // its length in doubles
+    double *backup;   // tm5_region_run: a queued interval's m and moments at its start (k_x's layout),
+                      // two of them (even and odd intervals)
+    size_t backup_n;  // the length of each in doubles
 };
 
 static View3 packed(double *p, int n0, int n1) { return View3{p, 1, n0, (long long)n0 * n1}; }
@@ -806,8 +807,10 @@ static int rehearse(tm5_region *r, double t0, int ndyn, int n, bool *ok)
 // rehearsal every y layer and the z region pass at 1, and a step whose x rows pass at nloop <= 9
 // and whose y and z faces pass at 1 is exactly a step whose rehearsal passes.
 // backup (checked x only): where the x kernel leaves the m and moments it reads.
+// flux (x): the step's fluxes made by the launch (am by its rows, bm and cm by extra workgroups)
+// instead of a dynam0 launch before it
 static int operator_(tm5_region *r, char op, int ndyn, bool known_one = false, int spec_k = -1,
-                     double *backup = nullptr)
+                     double *backup = nullptr, const FluxSrc *flux = nullptr)
 {
     tm5_ctx *c = r->ctx;
     const tm5_region_desc &d = r->d;
@@ -817,8 +820,9 @@ static int operator_(tm5_region *r, char op, int ndyn, bool known_one = false, i
     case 'x':
         if (spec_k >= 0)  // the rehearsal's cap (its search stops past 10 - 1)
             return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 10 - 1, TM5_FAIL_X, r->skip,
-                            nullptr, nullptr, nullptr, spec_k, backup);
-        return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, r->skip);
+                            nullptr, nullptr, nullptr, spec_k, backup, 0, -1, flux);
+        return launch_x(c, im, jm, lm, view3(&d.m), q, view3(&d.am), d.limits, 50 - 1, TM5_FAIL_X, r->skip,
+                        nullptr, nullptr, nullptr, -1, nullptr, 0, -1, flux);
     case 'y':
         return launch_y(c, im, jm, lm, view3(&d.m), q, view3(&d.bm), d.limits, 10 - 1, TM5_FAIL_Y, r->skip,
                         known_one, spec_k);
@@ -890,37 +894,75 @@ static int middle(tm5_region *r, int ndyn, bool with_z = false, int spec_k = -1)
     return launch_column(r->ctx, a, d.lmax, r->skip);
 }
 
-// one step of ndyn after Check_CFL's rehearsal of it passed, or (spec_k >= 0) a checked step of
-// queued interval spec_k, the first of its interval with a backup; open: the step's closing
-// m2phlb1 is left to close_step
-static int step(tm5_region *r, int ndyn, int spec_k = -1, double *backup = nullptr, bool open = false)
+// the fluxes of a step of ndyn whose middle is at t, made by its first x launch (Setup_MassFlow's
+// dynam0 at t with pydyn_amd.meteo's amplitude: am by the rows, bm and cm by extra workgroups)
+static FluxSrc step_flux(const tm5_region *r, double t, int ndyn)
 {
     const tm5_region_desc &d = r->d;
-    TRY(operator_(r, 'x', ndyn, true, spec_k, backup));
-    TRY(operator_(r, 'y', ndyn, true, spec_k));
-    TRY(middle(r, ndyn, true, spec_k));  // "z v s c m2phlb1 c s v z"
-    TRY(operator_(r, 'y', ndyn, true, spec_k));
-    TRY(operator_(r, 'x', ndyn, true, spec_k));
-    if (open) return 0;
-    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
-                       r->copy_ms ? &r->ms : nullptr));
-    r->ms_fresh = r->copy_ms;
+    FluxSrc f{};
+    f.u = view3(&d.pu0);
+    f.v = view3(&d.pv0);
+    f.a = amplitude(d, t);
+    f.dt = (double)ndyn / 2.0;
+    f.bt = d.bt;
+    f.bm = view3(&d.bm);
+    f.cm = view3(&d.cm);
+    f.TI = r->ctx->tile[TM5_TILE_DYNAM0];
+    return f;
+}
+
+// one step of ndyn after Check_CFL's rehearsal of it passed
+static int step(tm5_region *r, int ndyn)
+{
+    const tm5_region_desc &d = r->d;
+    TRY(operator_(r, 'x', ndyn, true));
+    TRY(operator_(r, 'y', ndyn, true));
+    TRY(middle(r, ndyn, true));  // "z v s c m2phlb1 c s v z"
+    TRY(operator_(r, 'y', ndyn, true));
+    TRY(operator_(r, 'x', ndyn, true));
+    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb)));
+    r->ms_fresh = false;
     return 0;
 }
 
-// An open queued step's m2phlb1 (with the copy of m for the next rehearsal); when next_ndyn > 0,
-// in the same launch the next step's dynam0 at the middle of that step, next_t (the two touch
-// disjoint arrays, and nothing the dynam0 writes is read before the next step).
-static int close_step(tm5_region *r, double next_t, int next_ndyn)
+// A checked step of queued interval k (the rehearsal's tests made by its own kernels; backup: the
+// first of its interval writes its input there) whose fluxes its first x launch makes, no dynam0
+// launched.  Its last x is left pending (*pend: the interval it checks): a pending x and the next
+// step's first run as one launch (k_xx), and flush_x launches one alone.  Its closing m2phlb1 is
+// left to close_step: the column kernel's middle m2phlb1 overwrites sp and phlb before anything
+// reads them, so only the last step before the state is seen (a trial's rehearsal, the end of the
+// batch) is closed.
+static int queued_step(tm5_region *r, int ndyn, int k, double *backup, const FluxSrc &fl, int *pend)
 {
     const tm5_region_desc &d = r->d;
-    if (next_ndyn > 0)
-        TRY(launch_m2phlb1_dynam0(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp),
-                                  view3(&d.phlb), &r->ms, view3(&d.pu0), view3(&d.pv0), amplitude(d, next_t),
-                                  (double)next_ndyn / 2.0, view3(&d.am), view3(&d.bm), view3(&d.cm), r->skip, 1));
+    if (*pend >= 0)
+        TRY(launch_xx(r->ctx, d.im, d.jm, d.lm, view3(&d.m), tracers(&d.q), view3(&d.am), d.limits, 10 - 1, r->skip,
+                      *pend, k, backup, fl));
     else
-        TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
-                           &r->ms));
+        TRY(operator_(r, 'x', ndyn, true, k, backup, &fl));
+    *pend = -1;
+    TRY(operator_(r, 'y', ndyn, true, k));
+    TRY(middle(r, ndyn, true, k));  // "z v s c m2phlb1 c s v z"
+    TRY(operator_(r, 'y', ndyn, true, k));
+    *pend = k;
+    return 0;
+}
+
+// the pending last x of a queued step, alone
+static int flush_x(tm5_region *r, int *pend)
+{
+    if (*pend < 0) return 0;
+    const int k = *pend;
+    *pend = -1;
+    return operator_(r, 'x', 0, true, k);
+}
+
+// An open queued step's m2phlb1, with the copy of m for the next rehearsal
+static int close_step(tm5_region *r)
+{
+    const tm5_region_desc &d = r->d;
+    TRY(launch_m2phlb1(r->ctx, d.im, d.jm, d.lm, view3(&d.m), d.dxyp, d.at, d.bt, view3(&d.sp), view3(&d.phlb),
+                       &r->ms));
     r->ms_fresh = true;
     return 0;
 }
@@ -1007,16 +1049,20 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
             TRY(tm5_region_interval(r, t0 + (double)k * (double)nread, nread, ndyn_max, ndyn_out + k, n_out + k));
         return 0;
     }
+    // two backups, by the parity of the interval: a k_xx launch writes interval k's while it still
+    // checks interval k - 1, whose backup a failure there restores
     const size_t ncell = (size_t)d.im * d.jm * d.lm, nb = ncell * (1 + 4 * (size_t)d.q.nt);
     if (r->backup_n < nb) {
         if (r->backup) TRY(status(hipFree(r->backup)));
         r->backup = nullptr;
         r->backup_n = 0;
-        TRY(status(hipMalloc((void **)&r->backup, nb * sizeof(double))));
+        TRY(status(hipMalloc((void **)&r->backup, 2 * nb * sizeof(double))));
         r->backup_n = nb;
     }
+    auto backup_of = [&](int k) { return r->backup + (size_t)(k & 1) * nb; };
     TRY(set_word(c, W_FLAGS, 0, TM5_REHEARSAL | TM5_SPEC));
     TRY(set_word(c, W_SPEC_KIND, 0, -1));
+    TRY(set_word(c, W_SPEC_AT, INT_MAX, -1));  // k_xx keeps the smallest failing interval
     // intervals queued per read of the device: doubled after a batch that held to its plan, back
     // to 2 after one that did not (the kernels queued after a miss only skip, but each costs a
     // dispatch)
@@ -1035,12 +1081,14 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         r->skip = TM5_ERRORS | TM5_SPEC;
         int rc = 0;
         bool open = false;  // a queued step whose m2phlb1 is yet to be launched
+        int pend = -1;      // the interval of the queued step whose last x is yet to be launched
         for (int k = i; k < end && rc == 0; k++) {
             const double t = t0 + (double)k * (double)nread;
             // the trials the plan expects to fail are rehearsed; the last one is the steps
             // themselves, checked
             if (trials > 1 && open) {
-                rc = close_step(r, 0.0, 0);
+                rc = flush_x(r, &pend);
+                if (rc == 0) rc = close_step(r);
                 open = false;
             }
             for (int q = 0; q + 1 < trials && rc == 0; q++) {
@@ -1051,20 +1099,14 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
                 }
             }
             const int ndyn = nd[trials - 1], n = nn[trials - 1];
-            r->copy_ms = true;
             for (int s = 0; s < n && rc == 0; s++) {
                 const double ts = (t + (double)(s * ndyn)) + (double)ndyn / 2.0;
-                if (open)  // the previous step's m2phlb1 with this step's dynam0
-                    rc = close_step(r, ts, ndyn);
-                else
-                    rc = dynam0_at(r, ts, ndyn, view3(&d.am), view3(&d.bm), view3(&d.cm));
-                open = false;
-                if (rc == 0) rc = step(r, ndyn, k, s == 0 ? r->backup : nullptr, true);
+                rc = queued_step(r, ndyn, k, s == 0 ? backup_of(k) : nullptr, step_flux(r, ts, ndyn), &pend);
                 open = rc == 0;
             }
-            r->copy_ms = false;
         }
-        if (rc == 0 && open) rc = close_step(r, 0.0, 0);
+        if (rc == 0) rc = flush_x(r, &pend);
+        if (rc == 0 && open) rc = close_step(r);
         r->skip = TM5_ERRORS;
         TRY(rc);
         TRY(read_words(c));
@@ -1086,11 +1128,12 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         if (c->hwords[W_SPEC_KIND]) {
             const long long nc = (long long)ncell;
             TM5_LAUNCH(c, k_restore, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, d.im, d.jm, d.lm,
-                       r->backup, view3(&d.m), tracers(&d.q));
+                       backup_of(f), view3(&d.m), tracers(&d.q));
             TRY(status(hipGetLastError()));
         }
         TRY(set_word(c, W_FLAGS, 0, TM5_SPEC | TM5_REHEARSAL));
         TRY(set_word(c, W_SPEC_KIND, 0, -1));
+        TRY(set_word(c, W_SPEC_AT, INT_MAX, -1));
         r->ms_fresh = false;
         TRY(tm5_region_interval(r, t0 + (double)f * (double)nread, nread, ndyn_max, ndyn_out + f, n_out + f));
         plan = ndyn_out[f];

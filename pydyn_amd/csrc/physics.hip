@@ -10,64 +10,7 @@ __device__ __forceinline__ bool skipped_p(const int *words, int mask)
 }
 
 // ------------------------------------------------------------------ dynam0
-// pydyn/kernels.py:102-125 + pydyn/zoom/massflow.py:446-461 on a tile of TI columns (i0.., j =
-// blockIdx.y), all levels: the fluxes are the meteo's times the amplitude a (rounded, as the
-// interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
-// convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
-// pit summed from the surface and sd accumulated from the top, in the reference's order.
-// tile (bx, j) of dynam0 (the workgroup's share; conv: its LDS, lm * TI doubles)
-__device__ __forceinline__ void dynam0_tile(int bx, int j, double *conv, int im, int jm, int lm, int TI, View3 pu,
-                                            View3 pv, const TimeSrc &ts, const double *bt, double dt, View3 am,
-                                            View3 bm, View3 cm, int cyclic, const int *words, View3 pu_out,
-                                            View3 pv_out, int am_halo)
-{
-    // TI columns i per workgroup (the context's TM5_TILE_DYNAM0)
-    const int i0 = bx * TI;
-    const int ni = min(TI, im - i0);
-    const int tid = threadIdx.x, nth = blockDim.x;
-    const double a = time_amplitude(ts, words);
-#pragma unroll 4
-    for (int idx = tid; idx < lm * TI; idx += nth) {
-        const int ti = idx % TI, l = idx / TI;
-        if (ti >= ni) continue;
-        const int i = i0 + ti;
-        const double p0 = a * pu(i, j, l), p1 = a * pu(i + 1, j, l);
-        const double v0 = a * pv(i, j, l), v1 = a * pv(i, j + 1, l);
-        conv[idx] = ((p0 - p1) + v0) - v1;
-        const double a0 = dt * (i == 0 && cyclic ? a * pu(im, j, l) : p0);  // a cyclic region: am(0) = am(im)
-        const double a1 = dt * p1, b0 = dt * v0, b1 = dt * v1;
-        am(i, j, l) = a0;
-        if (i == im - 1) am(im, j, l) = a1;
-        if (am_halo && i == 1) am(im + 1, j, l) = a0;
-        bm(i, j, l) = b0;
-        if (j == jm - 1) bm(i, jm, l) = b1;
-        if (pu_out.p) {
-            pu_out(i, j, l) = p0;
-            if (i == im - 1) pu_out(im, j, l) = p1;
-        }
-        if (pv_out.p) {
-            pv_out(i, j, l) = v0;
-            if (j == jm - 1) pv_out(i, jm, l) = v1;
-        }
-    }
-    __syncthreads();
-    for (int ti = tid; ti < ni; ti += nth) {
-        const int i = i0 + ti;
-        double pit = conv[ti];
-        for (int l = 1; l < lm; l++) pit = pit + conv[l * TI + ti];
-        cm(i, j, 0) = 0.0;
-        cm(i, j, lm) = 0.0;
-        if (lm > 1) {
-            double sd = conv[(lm - 1) * TI + ti] - (bt[lm - 1] - bt[lm]) * pit;
-            cm(i, j, lm - 1) = (-dt) * sd;
-            for (int l = lm - 2; l >= 1; l--) {
-                sd = (sd + conv[l * TI + ti]) - (bt[l] - bt[l + 1]) * pit;
-                cm(i, j, l) = (-dt) * sd;
-            }
-        }
-    }
-}
-
+// (the tile of dynam0 is tm5_lib.h's dynam0_tile, shared with the x kernel's fused form)
 __global__ void __launch_bounds__(256) k_dynam0(int im, int jm, int lm, int TI, View3 pu, View3 pv, TimeSrc ts,
                                                 const double *bt, double dt, View3 am, View3 bm, View3 cm, int cyclic,
                                                 int *words, int skip_mask, View3 pu_out, View3 pv_out, int am_halo)
@@ -123,27 +66,6 @@ __global__ void __launch_bounds__(256) k_m2phlb1(int im, int jm, int lm, int TI,
     if (skipped_p(words, skip_mask)) return;
     extern __shared__ double sh[];
     m2phlb1_tile(blockIdx.x, blockIdx.y, sh, im, lm, TI, m, dxyp, at, bt, sp, phlb, mcopy);
-}
-
-// A step's closing m2phlb1 and the next step's dynam0 in one launch: they touch disjoint arrays
-// (m2phlb1 reads m and writes sp, phlb and the copy of m; dynam0 reads pu, pv and writes am, bm,
-// cm), so their workgroups may run in any order.  Workgroups x < nbm are m2phlb1's tiles, the
-// rest dynam0's; each part keeps its own skip mask.
-__global__ void __launch_bounds__(256) k_m2phlb1_dynam0(int nbm, int im, int jm, int lm, int TI_m, View3 m,
-                                                        const double *dxyp, const double *at, View3 sp, View3 phlb,
-                                                        View3 mcopy, int skip_m, int TI_d, View3 pu, View3 pv,
-                                                        TimeSrc ts, const double *bt, double dt, View3 am, View3 bm,
-                                                        View3 cm, int cyclic, int skip_d, int *words)
-{
-    extern __shared__ double sh[];
-    if ((int)blockIdx.x < nbm) {
-        if (skipped_p(words, skip_m)) return;
-        m2phlb1_tile(blockIdx.x, blockIdx.y, sh, im, lm, TI_m, m, dxyp, at, bt, sp, phlb, mcopy);
-    } else {
-        if (skipped_p(words, skip_d)) return;
-        dynam0_tile(blockIdx.x - nbm, blockIdx.y, sh, im, jm, lm, TI_d, pu, pv, ts, bt, dt, am, bm, cm, cyclic, words,
-                    View3{}, View3{}, 0);
-    }
 }
 
 // ------------------------------------------------------------------ cluster sums
@@ -256,24 +178,6 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
     else x.ts.a = a;
     TM5_LAUNCH(ctx, k_dynam0, grid, dim3(256), (size_t)lm * TI * sizeof(double), ctx->stream, im, jm, lm, TI, pu, pv,
                x.ts, bt, dt, am, bm, cm, cyclic, ctx->words, skip_mask, x.pu_out, x.pv_out, x.am_halo);
-    return status(hipGetLastError());
-}
-
-int launch_m2phlb1_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
-                          const double *bt, View3 sp, View3 phlb, const View3 *mcopy, View3 pu, View3 pv, double a,
-                          double dt, View3 am, View3 bm, View3 cm, int skip_d, int cyclic)
-{
-    const int TI_m = ctx->tile[TM5_TILE_M2PHLB], TI_d = ctx->tile[TM5_TILE_DYNAM0];
-    if (im < 1 || jm < 1 || lm < 1 || (lm + 1) * TI_m > 8192 || lm * TI_d > 8192 || !dxyp || !at || !bt)
-        return TM5_EINVAL;
-    const int nbm = (im + TI_m - 1) / TI_m, nbd = (im + TI_d - 1) / TI_d;
-    const size_t lds_m = (size_t)(lm + 1) * TI_m, lds_d = (size_t)lm * TI_d;
-    const View3 none{};
-    TimeSrc ts{};
-    ts.a = a;
-    TM5_LAUNCH(ctx, k_m2phlb1_dynam0, dim3(nbm + nbd, jm), dim3(256), (lds_m > lds_d ? lds_m : lds_d) * sizeof(double),
-               ctx->stream, nbm, im, jm, lm, TI_m, m, dxyp, at, sp, phlb, mcopy ? *mcopy : none, TM5_ERRORS, TI_d, pu,
-               pv, ts, bt, dt, am, bm, cm, cyclic, skip_d, ctx->words);
     return status(hipGetLastError());
 }
 

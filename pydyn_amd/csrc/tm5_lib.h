@@ -217,13 +217,43 @@ __device__ __forceinline__ void spec_fail(int *words, int spec_k)
     atomicOr(words + W_FLAGS, TM5_SPEC);
 }
 
+// Setup_MassFlow's fluxes made by a step's first x launch instead of a dynam0 launch (a region's
+// queued steps), with dynam0's operations: each row makes the am it uses, am = dt (a pu) with
+// am(0) = am(im) (a cyclic row), and stores it; with bm.p set, extra workgroups of the launch make
+// bm and cm (dynam0's tiles of TI columns without am) for the step's later kernels.  u.p null: am
+// is read from its array.
+struct FluxSrc {
+    View3 u, v;        // pu, pv (the meteo's, before the amplitude)
+    double a, dt;      // the amplitude at the step's middle; ndyn / 2
+    const double *bt;  // the hybrid b at the interfaces (cm)
+    View3 bm, cm;      // where the extra workgroups store bm and cm (bm.p null: none)
+    int TI;            // columns of a dynam0 tile
+};
+
+// spec_fail for a launch that checks two queued intervals (k_xx: a step's last x and the next
+// step's first): the earlier failing interval is the one the host redoes, so the smallest is kept
+// (the host sets W_SPEC_AT to INT_MAX before it queues)
+__device__ __forceinline__ void spec_fail_first(int *words, int spec_k)
+{
+    atomicMin(words + W_SPEC_AT, spec_k);
+    words[W_SPEC_KIND] = 1;
+    __threadfence();
+    atomicOr(words + W_FLAGS, TM5_SPEC);
+}
+
 // launchers (advect.hip)
 // rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
 // halo_j0..halo_j1 (0-based rows, uniform rows only): those rows also get their periodic halo cells,
 // index -1 and im of the views (a zoom tree's region 1)
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr,
-             int spec_k = -1, double *backup = nullptr, int halo_j0 = 0, int halo_j1 = -1);
+             int spec_k = -1, double *backup = nullptr, int halo_j0 = 0, int halo_j1 = -1,
+             const FluxSrc *flux = nullptr);
+// A checked step's last x (interval spec_a, faces from am) and the next checked step's first
+// (interval spec_b, faces made from pu with fs, its bm and cm by extra workgroups) in one launch, a
+// row taking both; backup (null: none): the next step's input, as launch_x's backup
+int launch_xx(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int skip_mask,
+              int spec_a, int spec_b, double *backup, const FluxSrc &fs);
 // known_one: the caller knows the sub-cycle count is 1 everywhere (no search launched); spec_k >= 0:
 // a checked step of queued interval spec_k, which tests Check_CFL's cap 1 itself
 int launch_y(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 bm, int limits, int nmax, int fail_bit,
@@ -295,13 +325,71 @@ struct Dynam0Extra {
     View3 pu_out, pv_out;
     int am_halo;
 };
+// pydyn/kernels.py:102-125 + pydyn/zoom/massflow.py:446-461 on a tile of TI columns (i0.., j =
+// blockIdx.y), all levels: the fluxes are the meteo's times the amplitude a (rounded, as the
+// interpolated meteo the reference stores), am = dt pu, bm = dt pv with am(0) = am(im), the
+// convergence of every (i, l) computed in parallel into LDS; then per column cm = -dt sd with
+// pit summed from the surface and sd accumulated from the top, in the reference's order.
+// tile (bx, j) of dynam0 (the workgroup's share; conv: its LDS, lm * TI doubles)
+__device__ __forceinline__ void dynam0_tile(int bx, int j, double *conv, int im, int jm, int lm, int TI, View3 pu,
+                                            View3 pv, const TimeSrc &ts, const double *bt, double dt, View3 am,
+                                            View3 bm, View3 cm, int cyclic, const int *words, View3 pu_out,
+                                            View3 pv_out, int am_halo)
+{
+    // TI columns i per workgroup (the context's TM5_TILE_DYNAM0)
+    const int i0 = bx * TI;
+    const int ni = min(TI, im - i0);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const double a = time_amplitude(ts, words);
+#pragma unroll 4
+    for (int idx = tid; idx < lm * TI; idx += nth) {
+        const int ti = idx % TI, l = idx / TI;
+        if (ti >= ni) continue;
+        const int i = i0 + ti;
+        const double p0 = a * pu(i, j, l), p1 = a * pu(i + 1, j, l);
+        const double v0 = a * pv(i, j, l), v1 = a * pv(i, j + 1, l);
+        conv[idx] = ((p0 - p1) + v0) - v1;
+        const double a0 = dt * (i == 0 && cyclic ? a * pu(im, j, l) : p0);  // a cyclic region: am(0) = am(im)
+        const double a1 = dt * p1, b0 = dt * v0, b1 = dt * v1;
+        if (am.p) {  // null: am is made by the x kernel that uses it
+            am(i, j, l) = a0;
+            if (i == im - 1) am(im, j, l) = a1;
+            if (am_halo && i == 1) am(im + 1, j, l) = a0;
+        }
+        bm(i, j, l) = b0;
+        if (j == jm - 1) bm(i, jm, l) = b1;
+        if (pu_out.p) {
+            pu_out(i, j, l) = p0;
+            if (i == im - 1) pu_out(im, j, l) = p1;
+        }
+        if (pv_out.p) {
+            pv_out(i, j, l) = v0;
+            if (j == jm - 1) pv_out(i, jm, l) = v1;
+        }
+    }
+    __syncthreads();
+    for (int ti = tid; ti < ni; ti += nth) {
+        const int i = i0 + ti;
+        // (unrolled so that the LDS and bt loads of the next levels issue ahead of the chain of sums)
+        double pit = conv[ti];
+#pragma unroll 8
+        for (int l = 1; l < lm; l++) pit = pit + conv[l * TI + ti];
+        cm(i, j, 0) = 0.0;
+        cm(i, j, lm) = 0.0;
+        if (lm > 1) {
+            double sd = conv[(lm - 1) * TI + ti] - (bt[lm - 1] - bt[lm]) * pit;
+            cm(i, j, lm - 1) = (-dt) * sd;
+#pragma unroll 8
+            for (int l = lm - 2; l >= 1; l--) {
+                sd = (sd + conv[l * TI + ti]) - (bt[l] - bt[l + 1]) * pit;
+                cm(i, j, l) = (-dt) * sd;
+            }
+        }
+    }
+}
+
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const Dynam0Extra *ex = nullptr);
-// m2phlb1 (skip mask TM5_ERRORS, as launch_m2phlb1) and a dynam0 at amplitude a (skip mask skip_d)
-// in one launch
-int launch_m2phlb1_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
-                          const double *bt, View3 sp, View3 phlb, const View3 *mcopy, View3 pu, View3 pv, double a,
-                          double dt, View3 am, View3 bm, View3 cm, int skip_d, int cyclic);
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);

@@ -26,8 +26,8 @@ import sys
 CANDIDATES = {
     "region": {
         "y_lines": ((2, 4, 6, 8, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
-        "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_m2phlb1_dynam0")),
-        "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1", "k_m2phlb1_dynam0")),
+        "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_x<false>")),  # a step's first x makes its fluxes
+        "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1",)),
     },
     "batched": {
         "z_lines": ((4, 8, 16, 30, 32), None),

@@ -280,13 +280,13 @@ def test_profile_times_every_launch(dev):
     ctx.profile_begin()
     reg.interval()
     prof = ctx.profile_end()
-    # two x and two y kernels and one Setup_MassFlow a step; a checked step searches nothing, and
-    # its z sub-steps run inside the column kernel
-    for k, least in (("k_x<false>", 2), ("k_y_steps<false>", 2), ("k_dynam0", 1)):
+    # two x and two y kernels a step; a checked step searches nothing, its z sub-steps run inside
+    # the column kernel, and its fluxes are made by its kernels (no Setup_MassFlow launch)
+    for k, least in (("k_x<false>", 2), ("k_y_steps<false>", 2), ("k_m2phlb1", 1)):
         assert k in prof, (k, sorted(prof))
         count, secs = prof[k]
         assert count >= least and 0.0 < secs < 1.0, (k, prof[k])
-    assert not {"k_y_search", "k_z_search", "k_z_steps<false>"} & set(prof), sorted(prof)
+    assert not {"k_y_search", "k_z_search", "k_z_steps<false>", "k_dynam0"} & set(prof), sorted(prof)
     assert any(k.startswith("k_column<") for k in prof), sorted(prof)
     reg.interval()  # not recorded
     assert ctx.profile_entries() == prof
