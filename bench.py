@@ -121,10 +121,16 @@ def kernel_units(g, nt):
     q = 4 * nt
     return {
         "k_column<%d>" % g.lmax: ("fp64_valu", cols * column_flops(g.lmax, nt), "flop"),
-        "k_x<false>": ("hbm", 8 * (2 * cells * (1 + q) + 2 * fx), "B"),
+        "k_x<false, false>": ("hbm", 8 * (2 * cells * (1 + q) + 2 * fx), "B"),
+        # a step's first x making its fluxes: am from pu (read in place of am), and dynam0's tiles
+        # reading pu, pv and writing bm, cm
+        "k_x<false, true>": ("hbm", 8 * (2 * cells * (1 + q) + 2 * fx + (fx + fy) + (fy + fz)), "B"),
+        # a step's last x and the next step's first: the row's m and moments read and written once,
+        # am read, pu read, am written, and the dynam0 tiles as above
+        "k_xx": ("hbm", 8 * (2 * cells * (1 + q) + 3 * fx + (fx + fy) + (fy + fz)), "B"),
         "k_y_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fy), "B"),
         "k_z_steps<false>": ("hbm", 8 * (2 * cells * (1 + q) + fz), "B"),
-        "k_x<true>": ("hbm", 8 * (2 * cells + 2 * fx), "B"),
+        "k_x<true, false>": ("hbm", 8 * (2 * cells + 2 * fx), "B"),
         "k_y_steps<true>": ("hbm", 8 * (2 * cells + fy), "B"),
         "k_z_steps<true>": ("hbm", 8 * (2 * cells + fz), "B"),
         "k_z_search": ("hbm", 8 * (cells + fz), "B"),

@@ -10,6 +10,8 @@
 //
 // Tracers are processed one at a time over the same mass trajectory (recomputed per tracer,
 // the same bits), so LDS does not grow with the number of tracers.
+#include <type_traits>
+
 #include "tm5_lib.h"
 
 namespace tm5 {
@@ -26,8 +28,8 @@ __device__ __forceinline__ bool skipped(const int *words, int mask)
 // every face is scaled by (nloop-1)/nloop in place.
 // With tracer ranges (tchunk > 0, grid y) several workgroups take the same row: each searches and
 // moves the row's mass on its own, none writes m, am, rows_out or amax_out, and the caller runs
-// the mass-only form afterwards.
-template <bool MASS_ONLY>
+// the mass-only form afterwards.  MAKES: the step's fluxes made by the launch (fs, nd0), else read.
+template <bool MASS_ONLY, bool MAKES>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
                                           const int *imred, double *amax_out, int spec_k, double *bk, int tchunk,
@@ -45,7 +47,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     // column sums (a chain of dependent adds a column) run while the rows keep the device busy
     if (skipped(words, skip_mask)) return;
     extern __shared__ double lds[];
-    if ((int)blockIdx.x < nd0) {  // a dynam0 tile: bm and cm of the step, am left to the rows
+    if (MAKES && (int)blockIdx.x < nd0) {  // a dynam0 tile: bm and cm of the step, am left to the rows
         const int ntx = (im + fs.TI - 1) / fs.TI;
         TimeSrc ts{};
         ts.a = fs.a;
@@ -54,8 +56,8 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         return;
     }
     // rows renumbered XCD-contiguously, the tracer range slowest
-    const int nrx = gridDim.x - nd0;
-    const int b = xcd_tile(blockIdx.y * nrx + (blockIdx.x - nd0), nrx * gridDim.y);
+    const int skip0 = MAKES ? nd0 : 0, nrx = gridDim.x - skip0;
+    const int b = xcd_tile(blockIdx.y * nrx + (blockIdx.x - skip0), nrx * gridDim.y);
     const int row = b % nrx, bz = b / nrx;
     const int j = row % jm, l = row / jm;
     const int t0 = tchunk > 0 ? bz * tchunk : 0;
@@ -75,7 +77,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         M0[i] = M[i] = v;
         if (bk) bk[brow + i] = v;
     }
-    if (fs.u.p) {
+    if (MAKES) {
         for (int k = lane; k <= n; k += 64) A[k] = fs.dt * (fs.a * fs.u(k == 0 ? im : k, j, l));
     } else {
         for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
@@ -456,7 +458,8 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
     int ranges = 1, tchunk = q.nt;
     if (q.nt >= 32 && !backup) {  // many tracers: tracer ranges over the grid, then the masses
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x<false>, 64, lds) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x<false, false>, 64, lds) != hipSuccess ||
+            per_cu < 1)
             per_cu = 1;
         tchunk = tracer_chunk(grid.x, (long long)per_cu * (ctx->ncu > 0 ? ctx->ncu : 256), q.nt,
                               ctx->tile[TM5_TILE_RANGE_MIN]);
@@ -476,14 +479,28 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
             if ((size_t)lm * fs.TI * sizeof(double) > lds) lds = (size_t)lm * fs.TI * sizeof(double);
         }
     }
-    if (q.nt > 0)
-        TM5_LAUNCH(ctx, k_x<false>, dim3(grid.x + nd0, ranges), block, lds, ctx->stream, im, jm, lm, m, q, am, limits,
-                   nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup,
-                   ranges > 1 ? tchunk : 0, halo_j0, halo_j1, fs, (int)nd0);
-    if (q.nt == 0 || ranges > 1)
-        TM5_LAUNCH(ctx, k_x<true>, dim3(grid.x + (q.nt == 0 ? nd0 : 0)), block, lds, ctx->stream, im, jm, lm, m, q,
-                   am, limits, nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, 0,
-                   halo_j0, halo_j1, fs, q.nt == 0 ? (int)nd0 : 0);
+    // the tracers' launch, then (no tracers, or after tracer ranges) the masses' launch; the one
+    // launch with every tracer or without tracers makes the fluxes when asked to
+    auto go = [&](auto mass_only, auto makes, dim3 g, int tc, int nd) {
+        constexpr bool MO = decltype(mass_only)::value, MK = decltype(makes)::value;
+        static const char *const names[2][2] = {{"k_x<false, false>", "k_x<false, true>"},
+                                                {"k_x<true, false>", "k_x<true, true>"}};
+        TM5_LAUNCH_NAMED(ctx, names[MO][MK], (k_x<MO, MK>), g, block, lds, ctx->stream, im, jm, lm, m, q, am, limits,
+                         nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, tc, halo_j0,
+                         halo_j1, fs, nd);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    if (q.nt > 0) {
+        if (nd0 && ranges == 1)
+            go(F{}, T{}, dim3(grid.x + nd0), 0, (int)nd0);
+        else
+            go(F{}, F{}, dim3(grid.x, ranges), ranges > 1 ? tchunk : 0, 0);
+    }
+    if (q.nt == 0 && nd0)
+        go(T{}, T{}, dim3(grid.x + nd0), 0, (int)nd0);
+    else if (q.nt == 0 || ranges > 1)
+        go(T{}, F{}, grid, 0, 0);
     return status(hipGetLastError());
 }
 

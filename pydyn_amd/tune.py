@@ -26,7 +26,8 @@ import sys
 CANDIDATES = {
     "region": {
         "y_lines": ((2, 4, 6, 8, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
-        "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_x<false>")),  # a step's first x makes its fluxes
+        # a step's first x (or the launch with the step before's last) makes the step's fluxes
+        "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_x<false, true>", "k_xx")),
         "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1",)),
     },
     "batched": {

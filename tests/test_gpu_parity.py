@@ -281,8 +281,8 @@ def test_profile_times_every_launch(dev):
     reg.interval()
     prof = ctx.profile_end()
     # two x and two y kernels a step; a checked step searches nothing, its z sub-steps run inside
-    # the column kernel, and its fluxes are made by its kernels (no Setup_MassFlow launch)
-    for k, least in (("k_x<false>", 2), ("k_y_steps<false>", 2), ("k_m2phlb1", 1)):
+    # the column kernel, and its first x makes its fluxes (no Setup_MassFlow launch)
+    for k, least in (("k_x<false, true>", 1), ("k_x<false, false>", 1), ("k_y_steps<false>", 2), ("k_m2phlb1", 1)):
         assert k in prof, (k, sorted(prof))
         count, secs = prof[k]
         assert count >= least and 0.0 < secs < 1.0, (k, prof[k])
