@@ -216,17 +216,17 @@ extern "C" int tm5_ctx_create(uint64_t stream, tm5_ctx **out)
 extern "C" int tm5_ctx_destroy(tm5_ctx *c)
 {
     if (!c) return 0;
-    if (c->stream || c->words) hipStreamSynchronize(c->stream);
+    if (c->stream || c->words) (void)hipStreamSynchronize(c->stream);
     if (c->prof_store) {
-        for (hipEvent_t e : c->prof_store->events) hipEventDestroy(e);
+        for (hipEvent_t e : c->prof_store->events) (void)hipEventDestroy(e);
         delete c->prof_store;
     }
-    if (c->words) hipFree(c->words);
-    if (c->y_nloop) hipFree(c->y_nloop);
-    if (c->x_rows) hipFree(c->x_rows);
-    if (c->z_ok) hipFree(c->z_ok);
-    if (c->cbuf) hipFree(c->cbuf);
-    if (c->hwords) hipHostFree(c->hwords);
+    if (c->words) (void)hipFree(c->words);
+    if (c->y_nloop) (void)hipFree(c->y_nloop);
+    if (c->x_rows) (void)hipFree(c->x_rows);
+    if (c->z_ok) (void)hipFree(c->z_ok);
+    if (c->cbuf) (void)hipFree(c->cbuf);
+    if (c->hwords) (void)hipHostFree(c->hwords);
     free(c);
     return 0;
 }
@@ -727,9 +727,9 @@ extern "C" int tm5_region_create(tm5_ctx *ctx, const tm5_region_desc *d, tm5_reg
 extern "C" int tm5_region_destroy(tm5_region *r)
 {
     if (!r) return 0;
-    hipStreamSynchronize(r->ctx->stream);
-    hipFree(r->scratch);
-    if (r->backup) hipFree(r->backup);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    (void)hipFree(r->scratch);
+    if (r->backup) (void)hipFree(r->backup);
     free(r);
     return 0;
 }

@@ -774,11 +774,11 @@ int graph_step(tm5_tree *t, int ndyn, int nread, bool chk)
     e = hipStreamEndCapture(t->cap, &graph);
     c->stream = keep;
     if (rc != 0 || e != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc != 0 ? rc : status(e);
     }
     e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
         g->exec = nullptr;
         return status(e);
@@ -920,16 +920,16 @@ bool contiguous(const tm5_hview &v, long long n0, long long n1, long long n2, in
 void free_tree(tm5_tree *t)
 {
     if (!t) return;
-    if (t->ctx) hipStreamSynchronize(t->ctx->stream);
+    if (t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
     for (Graph &g : t->graphs)
-        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (TReg &g : t->reg)
         for (double *s : g.slab)
-            if (s) hipFree(s);
-    if (t->save) hipFree(t->save);
-    if (t->backup) hipFree(t->backup);
-    if (t->t0dev) hipFree(t->t0dev);
-    if (t->cap) hipStreamDestroy(t->cap);
+            if (s) (void)hipFree(s);
+    if (t->save) (void)hipFree(t->save);
+    if (t->backup) (void)hipFree(t->backup);
+    if (t->t0dev) (void)hipFree(t->t0dev);
+    if (t->cap) (void)hipStreamDestroy(t->cap);
     delete t;
 }
 
