@@ -181,6 +181,52 @@ int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, doub
     return status(hipGetLastError());
 }
 
+// the regions of a k_dynam0_regions launch: region k's tiles are workgroups first[k] .. first[k+1]-1
+struct Dynam0Set {
+    int n, TI;
+    int first[D0_REGIONS + 1];
+    const double *bt;
+    Dynam0Region r[D0_REGIONS];
+};
+
+__global__ void __launch_bounds__(256) k_dynam0_regions(Dynam0Set set, int *words, int skip_mask)
+{
+    if (skipped_p(words, skip_mask)) return;
+    extern __shared__ double conv[];  // [lm][TI] of the workgroup's region
+    int k = set.n - 1;
+    while (k > 0 && (int)blockIdx.x < set.first[k]) k--;
+    const Dynam0Region &g = set.r[k];
+    const int t = (int)blockIdx.x - set.first[k], ntx = (g.im + set.TI - 1) / set.TI;
+    dynam0_tile(t % ntx, t / ntx, conv, g.im, g.jm, g.lm, set.TI, g.pu, g.pv, g.ex.ts, set.bt, g.dt, g.am, g.bm, g.cm,
+                g.cyclic, words, g.ex.pu_out, g.ex.pv_out, g.ex.am_halo);
+}
+
+int launch_dynam0_regions(tm5_ctx *ctx, int n, const Dynam0Region *regions, const double *bt, int skip_mask)
+{
+    const int TI = ctx->tile[TM5_TILE_DYNAM0];
+    if (n < 0 || !bt || (n > 0 && !regions)) return TM5_EINVAL;
+    for (int k0 = 0; k0 < n; k0 += D0_REGIONS) {  // D0_REGIONS regions a launch
+        Dynam0Set set{};
+        set.n = n - k0 < D0_REGIONS ? n - k0 : D0_REGIONS;
+        set.TI = TI;
+        set.bt = bt;
+        int lm_max = 1;
+        for (int k = 0; k < set.n; k++) {
+            const Dynam0Region &g = regions[k0 + k];
+            if (g.im < 1 || g.jm < 1 || g.lm < 1 || g.lm * TI > 8192) return TM5_EINVAL;
+            set.r[k] = g;
+            if (!g.ex.ts.t0) set.r[k].ex.ts.a = g.a;
+            set.first[k + 1] = set.first[k] + ((g.im + TI - 1) / TI) * g.jm;
+            lm_max = g.lm > lm_max ? g.lm : lm_max;
+        }
+        TM5_LAUNCH(ctx, k_dynam0_regions, dim3((unsigned)set.first[set.n]), dim3(256), (size_t)lm_max * TI * sizeof(double),
+                   ctx->stream, set, ctx->words, skip_mask);
+        const int rc = status(hipGetLastError());
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy)
 {

@@ -390,6 +390,16 @@ __device__ __forceinline__ void dynam0_tile(int bx, int j, double *conv, int im,
 
 int launch_dynam0(tm5_ctx *ctx, int im, int jm, int lm, View3 pu, View3 pv, double a, const double *bt, double dt,
                   View3 am, View3 bm, View3 cm, int skip_mask, int cyclic = 1, const Dynam0Extra *ex = nullptr);
+// dynam0 of several regions in one launch (a zoom tree's Setup_MassFlow: no region's fluxes depend
+// on another's): each region's tiles as launch_dynam0's, the regions' workgroups one after another
+constexpr int D0_REGIONS = 8;
+struct Dynam0Region {
+    int im, jm, lm, cyclic;
+    View3 pu, pv, am, bm, cm;
+    double a, dt;  // a: the amplitude when ex.ts has no clock
+    Dynam0Extra ex;
+};
+int launch_dynam0_regions(tm5_ctx *ctx, int n, const Dynam0Region *regions, const double *bt, int skip_mask);
 // mcopy: receives a copy of m
 int launch_m2phlb1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, const double *dxyp, const double *at,
                    const double *bt, View3 sp, View3 phlb, const View3 *mcopy = nullptr);

@@ -490,8 +490,11 @@ int column_pass(tm5_tree *t, int r, int ndyn)
     return 0;
 }
 
-// do_steps (pydyn/zoom/steps.py:31-69): the next operators of region r up to a block boundary
-// (edges put to the children unless the last was x) or a z right after y
+// do_steps (pydyn/zoom/steps.py:31-69): the next operators of region r up to a block boundary or a
+// z right after y.  The reference also puts the edges to the children at a block boundary unless
+// the last operator was x; those puts are not launched: the children run only after a z that
+// follows y or at the end of a block ending in x, so the x and y of that same block have put
+// their edges again since, over the same cells, before any child reads them.
 int do_steps(tm5_tree *t, int r, int ndyn)
 {
     const std::string &order = t->reg[r].order;
@@ -512,13 +515,7 @@ int do_steps(tm5_tree *t, int r, int ndyn)
         }
         t->status[r] += 1;
         const int now = t->status[r];
-        if (now % NOPS == 0) {
-            if (op != 'x') {
-                TTRY(put_edges(t, r, 0, 1));
-                TTRY(put_edges(t, r, 1, 1));
-            }
-            return 0;
-        }
+        if (now % NOPS == 0) return 0;
         if (op == 'z' && order[now - 2] == 'y') return 0;
     }
     return 0;
@@ -558,20 +555,27 @@ int step(tm5_tree *t, int ndyn)
 }
 
 // ------------------------------------------------------------------ Setup_MassFlow
-int dynam0(tm5_tree *t, int r, int ndyn, bool rehearsal)
+// region r's dynam0 (pydyn/zoom/massflow.py:181-186 through the meteo interpolation): the meteo's
+// fluxes at the step's amplitude into its pu, pv, and am, bm, cm from them
+Dynam0Region dynam0_of(tm5_tree *t, int r, int ndyn)
 {
     TReg &g = t->reg[r];
     const ZR &Rr = g.R;
-    Dynam0Extra ex{};
-    ex.ts = t->ts;
-    ex.ts.ndyn = (double)ndyn;
-    ex.pu_out = sub3(Rr.pu, 0, 1, 1);
-    ex.pv_out = sub3(Rr.pv, 1, 1, 1);
-    ex.am_halo = Rr.xcyc == 1;
-    const double dt = (double)ndyn / (2.0 * (double)g.d.tref_abs);
-    return launch_dynam0(t->ctx, Rr.im, Rr.jm, Rr.lm, view3(&g.pu0), view3(&g.pv0), t->ts.a, t->bt, dt,
-                         sub3(Rr.am, 0, 1, 1), sub3(Rr.bm, 1, 1, 1), sub3(Rr.cm, 1, 1, 0), mask(t, rehearsal), Rr.xcyc,
-                         &ex);
+    Dynam0Region d{};
+    d.im = Rr.im, d.jm = Rr.jm, d.lm = Rr.lm, d.cyclic = Rr.xcyc;
+    d.pu = view3(&g.pu0);
+    d.pv = view3(&g.pv0);
+    d.am = sub3(Rr.am, 0, 1, 1);
+    d.bm = sub3(Rr.bm, 1, 1, 1);
+    d.cm = sub3(Rr.cm, 1, 1, 0);
+    d.a = t->ts.a;
+    d.dt = (double)ndyn / (2.0 * (double)g.d.tref_abs);
+    d.ex.ts = t->ts;
+    d.ex.ts.ndyn = (double)ndyn;
+    d.ex.pu_out = sub3(Rr.pu, 0, 1, 1);
+    d.ex.pv_out = sub3(Rr.pv, 1, 1, 1);
+    d.ex.am_halo = Rr.xcyc == 1;
+    return d;
 }
 
 int coarsen_region(tm5_tree *t, int r)
@@ -586,13 +590,21 @@ int coarsen_region(tm5_tree *t, int r)
     return coarsen_region(t, p);
 }
 
-// pydyn/zoom/massflow.py:181-193: dynam0 and coarsen_region region by region, then coarsen_ambm
+// pydyn/zoom/massflow.py:181-193: dynam0 and coarsen_region region by region, then coarsen_ambm.
+// Every region's dynam0 runs first, in one launch: coarsen_region(r) touches only r and its
+// ancestors, which come before r, so no dynam0 after it in the reference's order is one it reads
+// or writes.
 int setup_massflow(tm5_tree *t, int ndyn, bool rehearsal)
 {
-    for (int r = 1; r <= t->n; r++) {
-        TTRY(dynam0(t, r, ndyn, rehearsal));
-        TTRY(coarsen_region(t, r));
+    std::vector<Dynam0Region> regs;
+    try {
+        regs.reserve((size_t)t->n);
+    } catch (...) {
+        return TM5_ENOMEM;
     }
+    for (int r = 1; r <= t->n; r++) regs.push_back(dynam0_of(t, r, ndyn));
+    TTRY(launch_dynam0_regions(t->ctx, t->n, regs.data(), t->bt, mask(t, rehearsal)));
+    for (int r = 1; r <= t->n; r++) TTRY(coarsen_region(t, r));
     for (int r = 1; r <= t->n; r++)
         for (int ch : t->reg[r].children) TTRY(zoom_coarsen_ambm(t->ctx, R(t, r), R(t, ch), mask(t, false)));
     return 0;
