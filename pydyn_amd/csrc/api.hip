@@ -1070,7 +1070,7 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
     if (batch < 2) batch = 2;
     int &plan = r->plan;  // the step Check_CFL is predicted to settle on: the previous interval's
     if (plan < 1 || plan > ndyn_max) plan = ndyn_max;
-    int i = 0;
+    int i = 0, miss_at = -1;  // miss_at: the interval of the last miss
     while (i < count) {
         int nd[100], nn[100], trials = 0;
         if (!trial_chain(nread, ndyn_max, plan, nd, nn, &trials)) {
@@ -1123,9 +1123,9 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         }
         batch = 2;
         // interval f's trials or checks disagreed with the plan: nothing after them ran.  A failed
-        // check left the interval's state part-stepped: it comes back from the backup.  Then the
-        // interval is redone the reference's way, and the next ones planned on its step.
-        if (c->hwords[W_SPEC_KIND]) {
+        // check left the interval's state part-stepped: it comes back from the backup.
+        const bool check_failed = c->hwords[W_SPEC_KIND] != 0;
+        if (check_failed) {
             const long long nc = (long long)ncell;
             TM5_LAUNCH(c, k_restore, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, d.im, d.jm, d.lm,
                        backup_of(f), view3(&d.m), tracers(&d.q));
@@ -1135,6 +1135,19 @@ extern "C" int tm5_region_run(tm5_region *r, double t0, int nread, int ndyn_max,
         TRY(set_word(c, W_SPEC_KIND, 0, -1));
         TRY(set_word(c, W_SPEC_AT, INT_MAX, -1));
         r->ms_fresh = false;
+        // The interval is queued again on the step its miss points to -- after a failed check of
+        // the planned step the next shorter one of new_valid_timestep's chain, after a trial that
+        // passed against the plan the longest -- as checked steps again, so a wrong guess is only
+        // another miss.  A second miss on the same interval redoes it the reference's way, and the
+        // next ones are planned on its step.
+        if (f != miss_at) {
+            miss_at = f;
+            plan = check_failed ? new_valid_timestep(nd[trials - 1], nread, nread, 1) : ndyn_max;
+            if (plan >= 1) {
+                i = f;
+                continue;
+            }
+        }
         TRY(tm5_region_interval(r, t0 + (double)f * (double)nread, nread, ndyn_max, ndyn_out + f, n_out + f));
         plan = ndyn_out[f];
         i = f + 1;
