@@ -33,14 +33,17 @@ template <bool MASS_ONLY, bool MAKES>
 __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits,
                                           int nmax, int *words, int *rows_out, int fail_bit, int skip_mask,
                                           const int *imred, double *amax_out, int spec_k, double *bk, int tchunk,
-                                          int hj0, int hj1, FluxSrc fs, int nd0)
+                                          int hj0, int hj1, FluxSrc fs, int nd0, int keep_am)
 {
-    // rows j in hj0..hj1 (uniform rows only) also get their periodic halo, index -1 = index n - 1 and
-    // index n = index 0, for m and every moment (a zoom tree's region 1, pydyn/zoom/advect.py:283-288)
+    // rows j in hj0..hj1 also get their periodic halo, index -1 = index n - 1 and index n = index 0,
+    // for m and every moment (a zoom tree's region 1, pydyn/zoom/advect.py:283-288; on a reduced
+    // row the same cells of its reduced form, which the split back overwrites)
     // amax_out (optional): per row the largest |Courant| of its accepted trial (the reference's xi)
     // checked(spec_k): a search past nmax is a failed check of a checked step (spec_fail), not fail_bit
     // bk (whole rows only): the row's m and moments as read, i fastest, m first and then every
     // tracer's rm, rxm, rym, rzm (each im * jm * lm)
+    // keep_am: am written back as it was read, not "restored" by nloop (a reduced-grid x, where
+    // the reference puts back a copy of the faces taken before)
     // fs.u.p (whole periodic rows): the faces made from pu, am(k) = dt (a pu(k)) with face 0 the
     // same as face im, as dynam0 makes them, instead of read from am; the first nd0 workgroups (one
     // tracer range only) are dynam0's tiles making bm and cm (fs.bm.p): dispatched first, their
@@ -70,6 +73,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     double *A = M0 + n;    // n+1 faces
     double *R = A + n + 1; // 4n  rm, rxm, rym, rzm of one tracer
     double *F = R + 4 * n; // 4(n+1) f, pf, ft (=fy), fw (=fz)
+    double *A0 = F + 4 * (n + 1);  // n+1 (keep_am) the faces as read
 
     const long long ncell = (long long)im * jm * lm, brow = (long long)im * (j + (long long)jm * l);
     for (int i = lane; i < n; i += 64) {
@@ -82,6 +86,8 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
     } else {
         for (int k = lane; k <= n; k += 64) A[k] = am(k, j, l);
     }
+    if (keep_am)
+        for (int k = lane; k <= n; k += 64) A0[k] = A[k];
     // the first tracer's moments of the lane's (at most two) cells, in flight during the search
     constexpr int PRE = 2;
     const bool pre = !MASS_ONLY && t0 < t1 && n <= 64 * PRE;
@@ -235,7 +241,7 @@ __global__ void __launch_bounds__(64) k_x(int im, int jm, int lm, View3 m, Trace
         m(-1, j, l) = M[n - 1];
         m(n, j, l) = M[0];
     }
-    for (int k = lane; k <= n; k += 64) am(k, j, l) = A[k] * (double)nl;  // "restore old am"
+    for (int k = lane; k <= n; k += 64) am(k, j, l) = keep_am ? A0[k] : A[k] * (double)nl;  // "restore old am"
 }
 
 // ------------------------------------------------------------------ x twice
@@ -445,13 +451,12 @@ int launch_xx(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am
 
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask, const int *imred, int *rows_out, double *amax_out, int spec_k, double *backup, int halo_j0,
-             int halo_j1, const FluxSrc *flux)
+             int halo_j1, const FluxSrc *flux, int keep_am)
 {
     if (im < 1 || im > 512 || jm < 1 || lm < 1 || jm * lm > ctx->x_cap) return TM5_EINVAL;
     if ((backup || flux) && imred) return TM5_EINVAL;  // the backup holds whole rows; faces made from pu too
     const FluxSrc fs = flux ? *flux : FluxSrc{};
-    if (imred && halo_j1 >= halo_j0) return TM5_EINVAL;  // a reduced row's halo is the reduced grid's
-    size_t lds = (size_t)(11 * im + 5) * sizeof(double);
+    size_t lds = (size_t)((keep_am ? 12 : 11) * im + (keep_am ? 6 : 5)) * sizeof(double);
     const dim3 grid(jm * lm), block(64);
     ctx->x_nrows = jm * lm;
     int *rows = rows_out ? rows_out : ctx->x_rows;
@@ -487,7 +492,7 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
                                                 {"k_x<true, false>", "k_x<true, true>"}};
         TM5_LAUNCH_NAMED(ctx, names[MO][MK], (k_x<MO, MK>), g, block, lds, ctx->stream, im, jm, lm, m, q, am, limits,
                          nmax, ctx->words, rows, fail_bit, skip_mask, imred, amax_out, spec_k, backup, tc, halo_j0,
-                         halo_j1, fs, nd);
+                         halo_j1, fs, nd, keep_am);
     };
     using T = std::true_type;
     using F = std::false_type;

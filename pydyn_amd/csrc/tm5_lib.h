@@ -243,12 +243,12 @@ __device__ __forceinline__ void spec_fail_first(int *words, int spec_k)
 
 // launchers (advect.hip)
 // rows_out (default: the context's): per row nloop; amax_out (optional): per row the largest Courant number
-// halo_j0..halo_j1 (0-based rows, uniform rows only): those rows also get their periodic halo cells,
-// index -1 and im of the views (a zoom tree's region 1)
+// halo_j0..halo_j1 (0-based rows): those rows also get their periodic halo cells, index -1 and n of
+// the views (a zoom tree's region 1; a reduced row's n is its imred); keep_am: am written back as read
 int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am, int limits, int nmax, int fail_bit,
              int skip_mask, const int *imred = nullptr, int *rows_out = nullptr, double *amax_out = nullptr,
              int spec_k = -1, double *backup = nullptr, int halo_j0 = 0, int halo_j1 = -1,
-             const FluxSrc *flux = nullptr);
+             const FluxSrc *flux = nullptr, int keep_am = 0);
 // A checked step's last x (interval spec_a, faces from am) and the next checked step's first
 // (interval spec_b, faces made from pu with fs, its bm and cm by extra workgroups) in one launch, a
 // row taking both; backup (null: none): the next step's input, as launch_x's backup

@@ -974,20 +974,38 @@ __global__ void __launch_bounds__(256) k_y_child_restore(ZR C, int ls, const int
 
 
 // ------------------------------------------------------------------ region 1's reduced grid
-// pydyn/redgrid.py:57-92 on the rows jred[b] (1-based) with c = im / imred cells per cluster.
-// One 64-lane workgroup per (reduced row, level): the row staged in LDS, since the clusters are
-// summed (and later split) in place.  f = 0 is m; then the tracer arrays, rm..rzm of each tracer.
-__global__ void __launch_bounds__(64) k_uni2red(ZR R, int tracers, int *words, int skip_mask)
+// pydyn/redgrid.py:57-92 on the rows jred[b] (1-based) with c = im / imred cells per cluster, around
+// region 1's dynamu (zoom_x_reduced).  One 64-lane workgroup per (reduced row, level), the row
+// staged in LDS, since the clusters are summed (and later split) in place.
+//
+// To the reduced grid: the row's uniform m (into mu) and am (into au) kept for the split back; am
+// subsampled, am(i) = am(i c) for i = 1..imred, am(imred+1..im) = -1, am(0) = am(imred); m and
+// every tracer array summed over each cluster from its first cell, the tracers beyond the reduced
+// row zeroed (m is not).
+__global__ void __launch_bounds__(64) k_uni2red(ZR R, HV mu, HV au, int tracers, int *words, int skip_mask)
 {
     if (zskip(words, skip_mask)) return;
-    extern __shared__ double row[];
+    extern __shared__ double row[];  // im + 2
     const int j = R.jred[blockIdx.x], l = blockIdx.y + 1, lane = threadIdx.x;
     const int n = R.imred[j - 1], c = R.im / n;
+    for (int i = lane; i <= R.im + 1; i += 64) {
+        const double v = R.am(i, j, l);
+        row[i] = v;
+        au(i, j, l) = v;
+    }
+    __syncthreads();
+    for (int i = 1 + lane; i <= R.im; i += 64) R.am(i, j, l) = i <= n ? row[i * c] : -1.0;
+    if (lane == 0) R.am(0, j, l) = row[n * c];
+    __syncthreads();
     const int nf = tracers ? 1 + 4 * R.nt : 1;
     for (int f = 0; f < nf; f++) {
         const HV &v = f == 0 ? R.m : R.q[(f - 1) % 4];
         const int t = f == 0 ? 1 : (f - 1) / 4 + 1;
-        for (int i = lane; i < R.im; i += 64) row[i] = v(i + 1, j, l, t);
+        for (int i = lane; i < R.im; i += 64) {
+            const double x = v(i + 1, j, l, t);
+            row[i] = x;
+            if (f == 0) mu(i + 1, j, l) = x;
+        }
         __syncthreads();
         for (int k = lane; k < n; k += 64) {
             double acc = row[k * c];
@@ -1000,24 +1018,14 @@ __global__ void __launch_bounds__(64) k_uni2red(ZR R, int tracers, int *words, i
     }
 }
 
-// am(i) = am(i c) for i = 1..imred, am(imred+1..im) = -1, am(0) = am(imred), in index order
-__global__ void __launch_bounds__(64) k_uni2red_mf(ZR R, int *words, int skip_mask)
-{
-    if (zskip(words, skip_mask)) return;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= R.nred * R.lm) return;
-    const int j = R.jred[idx % R.nred], l = idx / R.nred + 1;
-    const int n = R.imred[j - 1], c = R.im / n;
-    for (int i = 1; i <= n; i++) R.am(i, j, l) = R.am(i * c, j, l);
-    for (int i = n + 1; i <= R.im; i++) R.am(i, j, l) = -1.0;
-    R.am(0, j, l) = R.am(n, j, l);
-}
-
-// The clusters split again with the uniform-grid masses m_uni(i) + am_uni(i-1) - am_uni(i) (the
-// row's mass advanced on the uniform grid): q(i) = m_uni(i) / m(cluster) q(cluster) for every
-// tracer array, then m = m_uni on the row.  mu, au: the uniform m and am before the x step
-// (copies laid out as m and am).
-__global__ void __launch_bounds__(64) k_red2uni(ZR R, HV mu, HV au, int tracers, int *words, int skip_mask)
+// Back to the uniform grid: the clusters split with the row's uniform masses advanced by its
+// uniform faces, m_uni(i) = (mu(i) + au(i-1)) - au(i): q(i) = (m_uni(i) / m(cluster)) q(cluster) for
+// every tracer array, m = m_uni; am put back as it was before the subsampling (the reference's copy
+// of the faces); then the halo (pydyn/zoom/advect.py:237-277, the reference's quirk): with tracers
+// q(0) = q(im) for m and every tracer array on a row with fewer cells than im; mass-only (Check_CFL's
+// rehearsal) on the rows js..je its x advects m(0) = m(im), and m(im + 1) = m(1) on a row of im.
+__global__ void __launch_bounds__(64) k_red2uni(ZR R, HV mu, HV au, int tracers, int js, int je, int *words,
+                                                int skip_mask)
 {
     if (zskip(words, skip_mask)) return;
     extern __shared__ double sh[];
@@ -1026,44 +1034,32 @@ __global__ void __launch_bounds__(64) k_red2uni(ZR R, HV mu, HV au, int tracers,
     const int n = R.imred[j - 1], c = R.im / n;
     for (int k = lane; k < n; k += 64) mred[k] = R.m(k + 1, j, l);
     __syncthreads();
-    if (tracers) {
-        for (int f = 0; f < 4 * R.nt; f++) {
-            const HV &v = R.q[f % 4];
-            const int t = f / 4 + 1;
-            for (int k = lane; k < n; k += 64) red[k] = v(k + 1, j, l, t);
-            __syncthreads();
-            for (int i = lane; i < n * c; i += 64) {
-                const int k = i / c;
-                const double m_new = (mu(i + 1, j, l) + au(i, j, l)) - au(i + 1, j, l);
-                v(i + 1, j, l, t) = (m_new / mred[k]) * red[k];
-            }
-            __syncthreads();
+    const int nq = tracers ? 4 * R.nt : 0;
+    for (int f = 0; f < nq; f++) {
+        const HV &v = R.q[f % 4];
+        const int t = f / 4 + 1;
+        for (int k = lane; k < n; k += 64) red[k] = v(k + 1, j, l, t);
+        __syncthreads();
+        for (int i = lane; i < n * c; i += 64) {
+            const int k = i / c;
+            const double m_new = (mu(i + 1, j, l) + au(i, j, l)) - au(i + 1, j, l);
+            v(i + 1, j, l, t) = (m_new / mred[k]) * red[k];
         }
+        __syncthreads();
     }
     for (int i = lane; i < R.im; i += 64) R.m(i + 1, j, l) = (mu(i + 1, j, l) + au(i, j, l)) - au(i + 1, j, l);
-}
-
-// region 1's x halos of rows js..je on the reduced grid: main form (before the split) q(0) =
-// q(imred), q(imred+1) = q(1); after the split the reduced rows get q(0) = q(im) (quirk = 1).
-// The mass-only form after its split: m(0) = m(im), and m(im+1) = m(1) on uniform rows only.
-__global__ void __launch_bounds__(256) k_x_halo_red(ZR R, int js, int je, int nf, int stage, int *words, int skip_mask)
-{
-    if (zskip(words, skip_mask)) return;
-    const int nj = je - js + 1;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)nj * R.lm * nf) return;
-    const int j = js + (int)(idx % nj), l = (int)((idx / nj) % R.lm) + 1, f = (int)(idx / ((long long)nj * R.lm));
-    const HV &v = f == 0 ? R.m : R.q[(f - 1) / R.nt];
-    const int t = f == 0 ? 1 : (f - 1) % R.nt + 1;
-    const int n = R.imred[j - 1];
-    if (stage == 0) {
-        v(0, j, l, t) = v(n, j, l, t);
-        v(n + 1, j, l, t) = v(1, j, l, t);
-    } else if (stage == 1) {
-        if (n != R.im) v(0, j, l, t) = v(R.im, j, l, t);
-    } else {
-        v(0, j, l, t) = v(R.im, j, l, t);
-        if (n == R.im) v(R.im + 1, j, l, t) = v(1, j, l, t);
+    for (int i = lane; i <= R.im + 1; i += 64) R.am(i, j, l) = au(i, j, l);
+    __syncthreads();
+    if (tracers) {
+        if (n == R.im) return;
+        for (int f = lane; f < 1 + nq; f += 64) {
+            const HV &v = f == 0 ? R.m : R.q[(f - 1) % 4];
+            const int t = f == 0 ? 1 : (f - 1) / 4 + 1;
+            v(0, j, l, t) = v(R.im, j, l, t);
+        }
+    } else if (lane == 0 && j >= js && j <= je) {
+        R.m(0, j, l) = R.m(R.im, j, l);
+        if (n == R.im) R.m(R.im + 1, j, l) = R.m(1, j, l);
     }
 }
 
@@ -1317,43 +1313,29 @@ using namespace tm5;
 // uniform m and am kept, the clusters summed and the faces subsampled, every row advected with
 // its own cell count, the halos, the clusters split with the uniformly advanced mass, am put back.
 // spec_k: a checked step's (the search capped at max_nloop - 1, a failure spec_fail)
+// Region 1's dynamu on its reduced grid (pydyn/zoom/advect.py:237-277): the reduced rows to their
+// clusters, dynamu on every row (a reduced row with its imred cells; the periodic halo of rows
+// js..je written by the x kernel, which puts am back as it read it), the reduced rows split back.
 int zoom_x_reduced(tm5_ctx *c, const tm5_zregion &d, int js, int je, int limits, int max_nloop, int mass_only, int fail,
                    int skip, int spec_k)
 {
     const ZR R = zr(d);
     if (!d.jred || !d.imred || !d.scratch || R.im > 4096) return TM5_EINVAL;
-    const long long nm = (long long)(R.im + 4) * (R.jm + 4) * R.lm, na = (long long)(R.im + 2) * (R.jm + 2) * (R.lm + 2);
-    HV mu = R.m, au = R.am;
+    const long long nm = (long long)(R.im + 4) * (R.jm + 4) * R.lm;
+    HV mu = R.m, au = R.am;  // the reduced rows' uniform m and am, laid out as m and am
     mu.p = d.scratch;
     au.p = d.scratch + nm;
-    ZTRY(zoom_copy(c, R.m.p, mu.p, nm));
-    ZTRY(zoom_copy(c, R.am.p, au.p, na));
     const dim3 rows((unsigned)R.nred, (unsigned)R.lm);
     const int tracers = mass_only ? 0 : 1;
-    TM5_LAUNCH(c, k_uni2red, rows, dim3(64), (size_t)R.im * sizeof(double), c->stream, R, tracers, c->words, skip);
-    TM5_LAUNCH(c, k_uni2red_mf, zgrid((long long)R.nred * R.lm, 64), dim3(64), 0, c->stream, R, c->words, skip);
+    TM5_LAUNCH(c, k_uni2red, rows, dim3(64), (size_t)(R.im + 2) * sizeof(double), c->stream, R, mu, au, tracers,
+               c->words, skip);
     ZTRY(launch_x(c, R.im, R.jm, R.lm, sub3(R.m, 1, 1, 1), mass_only ? no_tracers() : subq(R, 1, 1, 1),
                   sub3(R.am, 0, 1, 1), mass_only ? 0 : limits, max_nloop - 1, fail, skip, R.imred, nullptr, nullptr,
-                  spec_k));
-    const int nf = mass_only ? 1 : 1 + 4 * R.nt;
-    const long long nh = (long long)(je - js + 1) * R.lm * nf;
-    if (!mass_only && nh > 0)
-        TM5_LAUNCH(c, k_x_halo_red, zgrid(nh), dim3(256), 0, c->stream, R, js, je, nf, 0, c->words, skip);
-    TM5_LAUNCH(c, k_red2uni, rows, dim3(64), (size_t)2 * R.im * sizeof(double), c->stream, R, mu, au, tracers,
+                  spec_k, nullptr, js - 1, je - 1, nullptr, 1));
+    TM5_LAUNCH(c, k_red2uni, rows, dim3(64), (size_t)2 * R.im * sizeof(double), c->stream, R, mu, au, tracers, js, je,
                c->words, skip);
-    ZTRY(zoom_copy(c, au.p, R.am.p, na));
-    if (nh > 0) {
-        if (mass_only) {
-            TM5_LAUNCH(c, k_x_halo_red, zgrid(nh), dim3(256), 0, c->stream, R, js, je, nf, 2, c->words, skip);
-        }
-    }
-    if (!mass_only) {  // every reduced row: q(0) = q(im) after the split
-        const long long nq = (long long)R.jm * R.lm * nf;
-        TM5_LAUNCH(c, k_x_halo_red, zgrid(nq), dim3(256), 0, c->stream, R, 1, R.jm, nf, 1, c->words, skip);
-    }
     return status(hipGetLastError());
 }
-
 
 static bool zbad(const tm5_zregion *r) { return r == nullptr || r->m.data == nullptr || r->im < 1 || r->jm < 1 || r->lm < 1; }
 static bool zglobal(const tm5_zregion *r) { return r->xcyc == 1 && r->touch_sp == 1 && r->touch_np == 1; }
