@@ -10,9 +10,11 @@ Each knob of tm5_ctx_set_tile (device.TILES) is tuned on the workload whose kern
            on its own in a tree), the y tile of the many-tracer sweeps, the fewest tracers a
            tracer range holds, and the apply tile of the split column form.
 
-For each knob in turn every candidate value is set on the context and the workload's queued
-intervals are run with per-kernel timing (the library's events around each launch); the value
-whose kernels take the least GPU time per dynamic step is kept before the next knob is tried.
+For each knob in turn every candidate value is set on the context and the workload is built
+afresh and run over the same queued intervals (two to warm up, then the timed ones, so that every
+candidate meets the same Check_CFL cuts) with per-kernel timing (the library's events around each
+launch); the value whose kernels take the least GPU time per dynamic step is kept before the next
+knob is tried.
 --write stores the result in pydyn_amd/tiles.json, which every new context applies.  A tile shape
 changes how the work is split into workgroups, never an operation or its order, so the results
 are the same bits for every value (the GPU tests run with the file's, and with other shapes).
@@ -25,7 +27,7 @@ import sys
 # workload -> knob -> (candidate values, the kernels it shapes; None: every kernel of the step)
 CANDIDATES = {
     "region": {
-        "y_lines": ((2, 4, 6, 8, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
+        "y_lines": ((2, 3, 4, 5, 6, 7, 8, 10, 11, 16), ("k_y_steps<false>", "k_y_steps<true>", "k_y_mass1")),
         # a step's first x (or the launch with the step before's last) makes the step's fluxes
         "dynam0": ((4, 8, 16, 32, 64), ("k_dynam0", "k_x<false, true>", "k_xx")),
         "m2phlb": ((4, 8, 16, 32, 64), ("k_m2phlb1",)),
@@ -43,8 +45,13 @@ def _region(members):
     from pydyn_amd import meteo
     from pydyn_amd.region import Region
 
-    reg = Region(meteo.Meteo(meteo.grid("glb3x2"), seed=1))
-    return reg.run, reg.close
+    met = meteo.Meteo(meteo.grid("glb3x2"), seed=1)
+
+    def make():
+        reg = Region(met)
+        return reg.run, reg.close
+
+    return make
 
 
 def _batched(members):
@@ -52,18 +59,29 @@ def _batched(members):
     from pydyn_amd.zoom.run import ZoomRun
 
     p, c = meteo.zoom("c2", nt=4)
-    run = ZoomRun(p, c, ensemble=meteo.Ensemble(p, members))
-    return run.run, run.close
+    ens = meteo.Ensemble(p, members)
+
+    def make():
+        run = ZoomRun(p, c, ensemble=ens)
+        return run.run, run.close
+
+    return make
 
 
 WORKLOADS = {"region": _region, "batched": _batched}
 
 
-def _time(run, ctx, kernels, intervals):
-    """GPU time per dynamic step of `intervals` more intervals (the kernels listed, or all)."""
-    ctx.profile_begin()
-    out = run(intervals)
-    prof = ctx.profile_end()
+def _time(make, ctx, kernels, intervals):
+    """GPU time per dynamic step of intervals 2 .. intervals + 1 of a fresh run (the kernels
+    listed, or all)."""
+    run, close = make()
+    try:
+        run(2)  # warm up (and capture the step graphs)
+        ctx.profile_begin()
+        out = run(intervals)
+        prof = ctx.profile_end()
+    finally:
+        close()
     steps = sum(n for _, n in out)
     return sum(s for k, (_, s) in prof.items() if kernels is None or k in kernels) / steps
 
@@ -71,30 +89,25 @@ def _time(run, ctx, kernels, intervals):
 def tune(workload="region", intervals=6, members=64, log=print):
     from pydyn_amd import device as D
 
-    run, close = WORKLOADS[workload](members)
+    make = WORKLOADS[workload](members)
     ctx = D.context()
-    run(2)  # warm up
     best = {}
-    try:
-        for knob, (values, kernels) in CANDIDATES[workload].items():
-            times = {}
-            for v in values:
-                try:
-                    ctx.set_tile(knob, v)
-                except Exception:  # outside the knob's range for this grid
-                    continue
-                run(1)
-                times[v] = min(_time(run, ctx, kernels, intervals) for _ in range(2))
-                if times[v] == 0.0:  # the workload launches none of these kernels: keep the default
-                    times.pop(v)
-            if not times:
+    for knob, (values, kernels) in CANDIDATES[workload].items():
+        times = {}
+        for v in values:
+            try:
+                ctx.set_tile(knob, v)
+            except Exception:  # outside the knob's range for this grid
                 continue
-            best[knob] = min(times, key=times.get)
-            ctx.set_tile(knob, best[knob])
-            log(f"{knob:9s} " + "  ".join(f"{v}: {t * 1e6:9.2f} us" for v, t in times.items()) + f"  -> {best[knob]}"
-                "  (GPU time a step)")
-    finally:
-        close()
+            times[v] = min(_time(make, ctx, kernels, intervals) for _ in range(2))
+            if times[v] == 0.0:  # the workload launches none of these kernels: keep the default
+                times.pop(v)
+        if not times:
+            continue
+        best[knob] = min(times, key=times.get)
+        ctx.set_tile(knob, best[knob])
+        log(f"{knob:9s} " + "  ".join(f"{v}: {t * 1e6:9.2f} us" for v, t in times.items()) + f"  -> {best[knob]}"
+            "  (GPU time a step)")
     return best
 
 
