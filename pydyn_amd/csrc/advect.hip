@@ -470,14 +470,14 @@ int launch_x(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 am,
                               ctx->tile[TM5_TILE_RANGE_MIN]);
         ranges = (q.nt + tchunk - 1) / tchunk;
     }
-    // the step's bm and cm: dynam0's tiles as extra workgroups of the launch that takes every
-    // tracer (one range), else a dynam0 launch of their own before it
+    // the step's fluxes: am made by the rows and bm and cm by dynam0's tiles as extra workgroups of
+    // the launch that takes every tracer (one range); with tracer ranges, a whole dynam0 launch of
+    // their own before the x launches, which then read am
     unsigned nd0 = 0;
     if (fs.bm.p) {
         if (!fs.u.p || !fs.v.p || !fs.cm.p || !fs.bt || fs.TI < 1 || lm * fs.TI > 8192) return TM5_EINVAL;
         if (ranges > 1) {
-            const int rc = launch_dynam0(ctx, im, jm, lm, fs.u, fs.v, fs.a, fs.bt, fs.dt, View3{}, fs.bm, fs.cm,
-                                         skip_mask, 1);
+            const int rc = launch_dynam0(ctx, im, jm, lm, fs.u, fs.v, fs.a, fs.bt, fs.dt, am, fs.bm, fs.cm, skip_mask, 1);
             if (rc) return rc;
         } else {
             nd0 = (unsigned)(((im + fs.TI - 1) / fs.TI) * jm);
