@@ -81,8 +81,10 @@ __device__ __forceinline__ int wave_max2(int v)
 
 template <int N> struct ColumnLds {
     static constexpr int LD = (N % 2 == 0) ? N + 1 : N;  // odd: a row read by every lane spreads over the banks
-    static constexpr int BUILD = 11 * 32;                  // the build's per-level vectors (share the matrix's space)
-    static constexpr int R0 = N * LD > BUILD ? N * LD : BUILD;
+    static constexpr int BUILD = 15 * 32;                  // the build's per-level vectors (share the matrix's space)
+    static constexpr int ZSTAGE = 5 * 64 + 4 * 65;          // the fused dynamw's staging and face fluxes
+    static constexpr int R1 = BUILD > ZSTAGE ? BUILD : ZSTAGE;
+    static constexpr int R0 = N * LD > R1 ? N * LD : R1;
     // per half: matrix / build vectors, the column's m at every level, the broadcast slot of rm;
     // a multiple of 32 plus 8 doubles, so the two halves' broadcast reads fall in different banks
     static constexpr int S = ((R0 + 96 + 31) / 32) * 32 + 8;
@@ -256,6 +258,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     double *Mt = H;  // N x LD: conv1, then L\U, then C (column c written by lane c)
     double *eu = H, *du = H + 32, *ed = H + 64, *dd = H + 96, *dk = H + 128, *eum = H + 160, *edm = H + 192;
     double *amu = H + 224, *amd = H + 256, *cd = H + 288, *cdl = H + 320;
+    double *ind = H + 352, *kpd = H + 384, *inu = H + 416, *kpu = H + 448;  // the recursions' inflows, keeps
     double *mc = H + ColumnLds<N>::R0;  // m at all lm levels (m2phlb1)
     double *rv = mc + 64;               // the column's rm by level, for v
 
@@ -342,21 +345,68 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     const int kup = ctop < N ? ctop : N, kdn = clfs < N ? clfs : N;
     const int db = (kup > kdn ? kup : kdn) + 1 < N ? (kup > kdn ? kup : kdn) + 1 : N;
     const int KUP = wave_max2(kup), KDN = wave_max2(kdn), DB = wave_max2(db);
+    // Each draft's recursion in three passes: the column's mass flux level by level (the same
+    // in every lane of the half: adds and compares only), then every level's keep factor in the
+    // level's own lane (one division for all levels), then the lane's matrix column, run down
+    // the levels with those factors.  Every value is the same expression on the same operands.
     double colv[N + 1];  // rows 0..N of this lane's column of the flux matrix; rows 0 and N stay zero
 #pragma unroll
     for (int k = 0; k <= N; k++) colv[k] = 0.0;
-    {  // downdraft rows k-1 for k = kdn..2
-        double run = 0.0, prev = 0.0;
+    {  // downdraft levels k = kdn..2: inflow into level k-1 and its mass flux
+        double prev = 0.0;
 #pragma unroll
         for (int k = N; k >= 2; k--) {
             if (k <= KDN) {
                 const bool on = clfs >= k;
-                const double edk = ed[k - 1], ddk = dd[k - 1];
-                const double inflow = prev - edk;
-                const double aa = inflow + ddk;
-                const bool sinking = aa < 0.0;
-                const double keep = sinking ? max0(1.0 + ddk / inflow) : 0.0;
-                const double amd_k = (on && sinking) ? aa : 0.0;
+                const double inflow = prev - ed[k - 1];
+                const double aa = inflow + dd[k - 1];
+                const double amd_k = (on && aa < 0.0) ? aa : 0.0;
+                if (li == 0) {
+                    ind[k - 1] = inflow;
+                    amd[k - 1] = amd_k;
+                }
+                prev = amd_k;
+            }
+        }
+    }
+    {  // updraft levels k = 1..kup
+        double prev = 0.0;
+#pragma unroll
+        for (int k = 1; k <= N; k++) {
+            if (k <= KUP) {
+                const bool on = ctop >= k;
+                const double inflow = prev + eu[k - 1];
+                const double aa = inflow - du[k - 1];
+                const double amu_k = (on && aa > 0.0) ? aa : 0.0;
+                if (li == 0) {
+                    inu[k - 1] = inflow;
+                    amu[k] = amu_k;
+                }
+                prev = amu_k;
+            }
+        }
+    }
+    wsync();
+    // lane j: the downdraft's keep at level k = j + 1 (ed, dd of row j), the updraft's at k = j + 1
+    // (eu, du of row j)
+    if (li >= 1 && li + 1 <= KDN) {
+        const double inflow = ind[li];
+        const double aa = inflow + d_d;
+        kpd[li] = aa < 0.0 ? max0(1.0 + d_d / inflow) : 0.0;
+    }
+    if (li + 1 <= KUP) {
+        const double inflow = inu[li];
+        const double aa = inflow - d_u;
+        kpu[li] = aa > 0.0 ? max0(1.0 - d_u / inflow) : 0.0;
+    }
+    wsync();
+    {  // downdraft rows k-1 for k = kdn..2
+        double run = 0.0;
+#pragma unroll
+        for (int k = N; k >= 2; k--) {
+            if (k <= KDN) {
+                const bool on = clfs >= k;
+                const double keep = kpd[k - 1];
                 double v = 0.0;
                 if (li >= k)
                     v = on ? run * keep : 0.0;
@@ -364,31 +414,22 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
                     v = on ? edm[k - 1] * keep : 0.0;
                 colv[k - 1] = v;
                 run = v;
-                if (li == 0) amd[k - 1] = amd_k;
-                prev = amd_k;
             }
         }
     }
     {  // updraft rows k = 1..kup, added in front: row k is fu + fd
-        double run = 0.0, prev = 0.0;
+        double run = 0.0;
 #pragma unroll
         for (int k = 1; k <= N; k++) {
             double v = 0.0;
             if (k <= KUP) {
                 const bool on = ctop >= k;
-                const double euk = eu[k - 1], duk = du[k - 1];
-                const double inflow = prev + euk;
-                const double aa = inflow - duk;
-                const bool rising = aa > 0.0;
-                const double keep = rising ? max0(1.0 - duk / inflow) : 0.0;
-                const double amu_k = (on && rising) ? aa : 0.0;
+                const double keep = kpu[k - 1];
                 if (li < k - 1)
                     v = on ? run * keep : 0.0;
                 else if (li == k - 1)
                     v = on ? eum[k - 1] * keep : 0.0;
                 run = v;
-                if (li == 0) amu[k] = amu_k;
-                prev = amu_k;
             }
             if (k < N) colv[k] = v + colv[k];
         }
@@ -517,7 +558,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     // zfused: dynamw's sub-step through LDS staging Z (the matrix's space, free once C is read):
     // Z[0, 64) the masses at the first sub-step's start, then rm, rxm, rym, rzm by level; the
     // second sub-step starts from mc.  Each lane takes the two faces of each of its cells.
-    static_assert(ColumnLds<N>::R0 >= 5 * 64, "the staging fits the matrix's space");
+    static_assert(ColumnLds<N>::R0 >= ColumnLds<N>::ZSTAGE, "the staging fits the matrix's space");
     double *Z = H;
     double zm0 = 0.0, zm0h = 0.0, zm2 = 0.0, zm2h = 0.0;
     if (zf) {
@@ -541,10 +582,16 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
         const double rzu = zlim ? limit(Z[256 + u], rmu) : Z[256 + u];
         return face_flux(cf, gamma, rmu, rzu, Z[128 + u], Z[192 + u]);
     };
-    auto z_cell = [&](int l, double cm_lo, double cm_hi, const double *Mz, double m_old, double m_new, double &rm,
-                      double &rx, double &ry, double &rz) {
+    // the sub-step's face fluxes, face k's f, pf, ft, fw at F[k], F[ZF + k], F[2 ZF + k], F[3 ZF + k]:
+    // each interior face is made once, by the lane of the cell above it, and read by both cells
+    constexpr int ZF = 65;
+    double *F = Z + 5 * 64;
+    auto z_cell = [&](int l, double cm_lo, double cm_hi, double m_old, double m_new, double &rm, double &rx,
+                      double &ry, double &rz) {
         const double c_lo = cm_lo / 1.0, c_hi = cm_hi / 1.0;
-        const FaceFlux lo = z_face(l, c_lo, Mz), hi = z_face(l + 1, c_hi, Mz);
+        FaceFlux lo{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};  // the closed ends carry nothing
+        if (l >= 1) lo = FaceFlux{F[l], F[ZF + l], F[2 * ZF + l], F[3 * ZF + l]};
+        if (l + 1 < lm) hi = FaceFlux{F[l + 1], F[ZF + l + 1], F[2 * ZF + l + 1], F[3 * ZF + l + 1]};
         const double rzm = zlim ? limit(rz, rm) : rz;
         const double rm_new = (rm + lo.f) - hi.f;  // advance: rm + f(l-1) - f(l)
         double rzn = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
@@ -572,10 +619,19 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
         }
         wsync();
         const double *Mz = second ? mc : Z;
-        if (li < lm)
-            z_cell(li, zc_lo, zc_hi, Mz, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
+        auto put = [&](int k, double cm_k) {  // face k, whose cm is the lower face's of cell k
+            const FaceFlux ff = z_face(k, cm_k / 1.0, Mz);
+            F[k] = ff.f;
+            F[ZF + k] = ff.pf;
+            F[2 * ZF + k] = ff.ft;
+            F[3 * ZF + k] = ff.fw;
+        };
+        if (li >= 1 && li < lm) put(li, zc_lo);
+        if (l1 < lm) put(l1, zc_lo1);
+        wsync();
+        if (li < lm) z_cell(li, zc_lo, zc_hi, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
         if (l1 < lm)
-            z_cell(l1, zc_lo1, zc_hi1, Mz, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
+            z_cell(l1, zc_lo1, zc_hi1, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
     };
 
     // ---- apply: the flagged sequence around v for each tracer of the column, levels li and l1
