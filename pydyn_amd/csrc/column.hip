@@ -239,6 +239,20 @@ __device__ __forceinline__ void column_seq(const ColumnArgs &a, const ColumnSeq 
     }
 }
 
+// column_seq with v on the lane's first cell, applied only where `on` (a column the region owns):
+// elsewhere q is left as it is, but the lanes still pass the half's LDS exchanges
+template <int N>
+__device__ __forceinline__ void column_seq_if(bool on, const ColumnArgs &a, const ColumnSeq &s, const double (&R)[N],
+                                              double diag, int li, int lev1, int t, double *rv, double (&q)[8])
+{
+    double w[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) w[m] = q[m];
+    column_seq<N, true>(a, s, R, diag, li, li, lev1, t, rv, w);
+#pragma unroll
+    for (int m = 0; m < 8; m++) q[m] = on ? w[m] : q[m];
+}
+
 template <int N, bool ADJ>
 __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int skip_mask)
 {
@@ -813,6 +827,162 @@ __global__ void __launch_bounds__(APPLY_COLS_MAX * 32)
     }
 }
 
+// The apply with the dynamw that follows the sequence in the step fused (zpost): every tracer
+// of the tile's columns takes k_column_apply's sequence (on the columns the region owns), then,
+// on the columns of the z scope, dynamw at n_advz 1 (c = cm / 1, pydyn/kernels.py:324-346, as
+// k_z_steps' sub-step) from the masses as they stand.  Each lane holds the tracer's cells at
+// levels li and li + 32, so the column's every level is in its half's registers.  A face's
+// Courant number and upwind cell depend on the masses only and are found once; per tracer the
+// moments pass through the half's LDS, lane k makes face k's fluxes (and face k + 32's), and both
+// cells of each face read them back.  No workgroup writes m (the tracer ranges of a column all
+// read it): k_z_mass_post moves it afterwards.  A checked step's dynamw is tested here, every
+// column's faces at cap 1, by the first tracer range.
+constexpr int APPLY_Z_COLS = 8;
+constexpr int APPLY_Z_F = 65;                               // a flux plane: faces 0 .. 64
+constexpr int APPLY_Z_LDS = 5 * 64 + 4 * APPLY_Z_F + 32 + 3;  // masses, moments, fluxes, rm for v
+template <int N>
+__global__ void __launch_bounds__(APPLY_Z_COLS * 32)
+    k_column_apply_z(ColumnArgs a, const double *cbuf, int tchunk, int *words, int skip_mask)
+{
+    if (skipped_k(words, skip_mask)) return;
+    extern __shared__ double zsh[];
+    const int cols = blockDim.x >> 5;
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int b = xcd_tile((blockIdx.z * nby + blockIdx.y) * nbx + blockIdx.x, nbx * nby * gridDim.z);
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int h = lane >> 5, li = lane & 31, l1 = li + 32, lm = a.lm;
+    const int t0 = bz * tchunk, nt = a.q.nt - t0 < tchunk ? a.q.nt - t0 : tchunk;
+    const int j = by;
+    const int e = (a.q.sl % 8 == 0 && a.q.st % 8 == 0) ? (int)(((uintptr_t)(a.q.q[0] + a.q.off(0, j, 0, 0)) >> 3) & 7) : 0;
+    int i = bx * cols - e + wave * 2 + h;
+    const bool active = i >= 0 && i < a.im;
+    if (!active) i = i < 0 ? 0 : a.im - 1;
+    const bool own = active && (a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0);
+    const bool zin = active && i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1;
+    double *Z = zsh + (wave * 2 + h) * APPLY_Z_LDS;  // Z[0, 64) masses, then rm, rxm, rym, rzm by level
+    double *F = Z + 5 * 64;                          // f, pf, ft, fw by face
+    double *rv = F + 4 * APPLY_Z_F;
+    const double *Cr = cbuf + ((long long)j * a.im + i) * (N * N) + (long long)(li < N ? li : N - 1) * N;
+    double crow[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) crow[k] = Cr[k];
+    const double diag = li < N ? Cr[li] : 0.0;
+    const ColumnSeq cs = column_seq_setup(a, i, j, li);
+    const bool in0 = li < lm, in1 = l1 < lm;
+    // the masses before dynamw and the faces below and above each of the lane's cells
+    const double m0 = in0 ? a.m(i, j, li) : 0.0, m0h = in1 ? a.m(i, j, l1) : 0.0;
+    const double zc_lo = in0 ? a.cm(i, j, li) : 0.0, zc_hi = in0 ? a.cm(i, j, li + 1) : 0.0;
+    const double zc_lo1 = in1 ? a.cm(i, j, l1) : 0.0, zc_hi1 = in1 ? a.cm(i, j, l1 + 1) : 0.0;
+    const double m2 = (m0 + zc_lo / 1.0) - zc_hi / 1.0, m2h = (m0h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
+    if (in0) Z[li] = m0;
+    if (in1) Z[l1] = m0h;
+    wsync();
+    if (checked(a.spec_k) && bz == 0) {
+        // Check_CFL's dynamwm at cap 1: every interior face k by lane k and k + 32 of its column
+        bool ok = true, bad = false;
+        if (active) {
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+                const int k = h2 ? l1 : li;
+                if (k >= 1 && k < lm) {
+                    const double ck = h2 ? zc_lo1 : zc_lo;  // cm(k)
+                    ok = ok && fabs(ck / Z[ck >= 0.0 ? k - 1 : k]) < 1.0;
+                }
+            }
+            if (li == 0) bad = a.cm(i, j, 0) != 0.0 || a.cm(i, j, lm) != 0.0;
+        }
+        const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+        if (lane == 0) {
+            if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+            if (any_fail) spec_fail(words, a.spec_k);
+        }
+    }
+    const bool zlim = a.zlimits != 0;
+    // face k = li (interior: 1 <= k < lm) and face k1 = l1 (k1 < lm): c = cm / 1, upwind cell and
+    // Courant number (the same for every tracer)
+    const bool fa = li >= 1 && in0, fb = in1;
+    const double ca = zc_lo / 1.0, cb = zc_lo1 / 1.0;
+    const int ua = ca >= 0.0 ? li - 1 : li, ub = cb >= 0.0 ? l1 - 1 : l1;
+    const double ga = fa ? ca / Z[ua] : 0.0, gb = fb ? cb / Z[ub] : 0.0;
+    auto put_face = [&](int k, int u, double cf, double g) {
+        const double rmu = Z[64 + u];
+        const double rzu = zlim ? limit(Z[256 + u], rmu) : Z[256 + u];
+        const FaceFlux ff = face_flux(cf, g, rmu, rzu, Z[128 + u], Z[192 + u]);
+        F[k] = ff.f;
+        F[APPLY_Z_F + k] = ff.pf;
+        F[2 * APPLY_Z_F + k] = ff.ft;
+        F[3 * APPLY_Z_F + k] = ff.fw;
+    };
+    auto z_cell = [&](int l, double cm_lo, double cm_hi, double m_old, double m_new, double &rm, double &rx,
+                      double &ry, double &rz) {
+        const double c_lo = cm_lo / 1.0, c_hi = cm_hi / 1.0;
+        FaceFlux lo{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};  // the closed ends carry nothing
+        if (l >= 1) lo = FaceFlux{F[l], F[APPLY_Z_F + l], F[2 * APPLY_Z_F + l], F[3 * APPLY_Z_F + l]};
+        if (l + 1 < lm) hi = FaceFlux{F[l + 1], F[APPLY_Z_F + l + 1], F[2 * APPLY_Z_F + l + 1], F[3 * APPLY_Z_F + l + 1]};
+        const double rzm = zlim ? limit(rz, rm) : rz;
+        const double rm_new = (rm + lo.f) - hi.f;
+        double rzn = slope_update(rzm, lo.pf, hi.pf, c_lo, c_hi, rm_new, lo.f, hi.f, m_old, m_new);
+        if (zlim) rzn = limit(rzn, rm_new);
+        rx = (rx + lo.ft) - hi.ft;
+        ry = (ry + lo.fw) - hi.fw;
+        rm = rm_new;
+        rz = rzn;
+    };
+    const long long o = a.q.off(i, j, in0 ? li : 0, t0), o1 = a.q.off(i, j, in1 ? l1 : 0, t0), st = a.q.st;
+    for (int t = 0; t < nt; t++) {
+        double q[8];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            q[m] = in0 ? a.q.q[m][o + t * st] : 0.0;
+            q[4 + m] = in1 ? a.q.q[m][o1 + t * st] : 0.0;
+        }
+        // the sequence (a column the region does not own keeps its tracers through it; its lanes
+        // still take part in the half's exchanges)
+        column_seq_if<N>(own, a, cs, crow, diag, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
+        wsync();  // the previous tracer's staged values and fluxes are read
+        if (in0) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) Z[64 * (m + 1) + li] = q[m];
+        }
+        if (in1) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) Z[64 * (m + 1) + l1] = q[4 + m];
+        }
+        wsync();
+        if (fa) put_face(li, ua, ca, ga);
+        if (fb) put_face(l1, ub, cb, gb);
+        wsync();
+        if (zin) {
+            if (in0) z_cell(li, zc_lo, zc_hi, m0, m2, q[0], q[1], q[2], q[3]);
+            if (in1) z_cell(l1, zc_lo1, zc_hi1, m0h, m2h, q[4], q[5], q[6], q[7]);
+        }
+        if (own || zin) {
+            if (in0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) a.q.q[m][o + t * st] = q[m];
+            }
+            if (in1) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) a.q.q[m][o1 + t * st] = q[4 + m];
+            }
+        }
+        __builtin_amdgcn_s_barrier();
+    }
+}
+
+// zpost's masses: m <- (m + cm(l) / 1) - cm(l + 1) / 1 on the scope's columns (k_z_steps' mass at
+// n_advz 1), one thread a cell
+__global__ void __launch_bounds__(256) k_z_mass_post(ColumnArgs a, int *words, int skip_mask)
+{
+    if (skipped_k(words, skip_mask)) return;
+    const long long n = (long long)a.im * a.jm * a.lm, c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int i = (int)(c % a.im), j = (int)((c / a.im) % a.jm), l = (int)(c / ((long long)a.im * a.jm));
+    if (i < a.zi0 || i > a.zi1 || j < a.zj0 || j > a.zj1) return;
+    a.m(i, j, l) = (a.m(i, j, l) + a.cm(i, j, l) / 1.0) - a.cm(i, j, l + 1) / 1.0;
+}
+
 // the split form from this many tracers on (a factor pass, then k_column_apply)
 constexpr int SPLIT_NT = 8;
 
@@ -855,6 +1025,10 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
         return status(hipGetLastError());
     }
     if (args.lapack && (args.zfused || args.mid || args.pre || args.twice || args.post)) return TM5_EINVAL;
+    if (args.zpost && (args.zfused || args.lapack || args.q.nt < SPLIT_NT || !args.cm.p || args.zi0 < 0 ||
+                       args.zj0 < 0 || args.zi1 >= args.im || args.zj1 >= args.jm || args.zi1 < args.zi0 ||
+                       args.zj1 < args.zj0))
+        return TM5_EINVAL;  // the split form's, on a scope inside the region
     if (!args.lapack && (args.zfused || args.q.nt < SPLIT_NT)) {
         TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), grid, block, lds, ctx->stream, args, ctx->words,
                          skip_mask);
@@ -897,6 +1071,22 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
                                     ctx->tile[TM5_TILE_RANGE_MIN]);
     const dim3 agrid((args.im + 7 + cols - 1) / cols, args.jm, (nt + tchunk - 1) / tchunk);
     const dim3 ablock(cols * 32);
+    if (args.zpost) {  // the step's next dynamw too, then its masses
+        const int zcols = cols < APPLY_Z_COLS ? cols : APPLY_Z_COLS;
+        const int ztiles = (args.im + 7 + zcols - 1) / zcols * args.jm;
+        const int zchunk = tracer_chunk(ztiles, 32 / zcols * (ctx->ncu > 0 ? ctx->ncu : 256), nt,
+                                        ctx->tile[TM5_TILE_RANGE_MIN]);
+        const dim3 zgrid((args.im + 7 + zcols - 1) / zcols, args.jm, (nt + zchunk - 1) / zchunk);
+        TM5_LAUNCH_NAMED(ctx, "k_column_apply_z", k_column_apply_z<N>, zgrid, dim3(zcols * 32),
+                         (size_t)zcols * APPLY_Z_LDS * sizeof(double), ctx->stream, args, buf, zchunk, ctx->words,
+                         skip_mask);
+        int rz = status(hipGetLastError());
+        if (rz) return rz;
+        const long long ncell = (long long)args.im * args.jm * args.lm;
+        TM5_LAUNCH(ctx, k_z_mass_post, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream, args,
+                   ctx->words, skip_mask);
+        return status(hipGetLastError());
+    }
     TM5_LAUNCH_NAMED(ctx, "k_column_apply", k_column_apply<N>, agrid, ablock, 0, ctx->stream, args, buf, tchunk,
                      ctx->words, skip_mask);
     return status(hipGetLastError());

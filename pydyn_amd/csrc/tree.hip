@@ -449,8 +449,9 @@ int advect_z(tm5_tree *t, int r)
     return 0;
 }
 
-// convection, sources and chemistry of a region in one column pass (tm5_zoom_physics)
-int column(tm5_tree *t, int r, int ndyn, const char *ops)
+// convection, sources and chemistry of a region in one column pass (tm5_zoom_physics); zp: with the
+// step's next dynamw fused
+int column(tm5_tree *t, int r, int ndyn, const char *ops, const ZPost *zp)
 {
     TReg &g = t->reg[r];
     tm5_zphysics p = g.phys;
@@ -460,7 +461,7 @@ int column(tm5_tree *t, int r, int ndyn, const char *ops)
     p.dt_chem = half;
     p.at = t->at;
     p.bt = t->bt;
-    return tm5_zoom_physics(t->ctx, &g.d, &p, ops);
+    return zoom_physics(t->ctx, &g.d, &p, ops, zp);
 }
 
 // The column pass starting at region r's current status: the longest sequence the order allows
@@ -485,7 +486,26 @@ int column_pass(tm5_tree *t, int r, int ndyn)
     } else {
         return TM5_EINVAL;  // sources or chemistry apart from convection
     }
-    TTRY(column(t, r, ndyn, ops));
+    // The operator after the pass a dynamw at n_advz 1 (a checked step's, or one its rehearsal
+    // passed at 1) on the split form's many tracers: it runs in the pass's apply, on the scope of
+    // its block.  Nothing comes between them in the order (the children run after a z only when
+    // y precedes it, and region 1's "vscMcsv" ends its first half with nothing launched); a
+    // child's mix_edges follows, as after its z.
+    const int zs = s + n;
+    if (zs < (int)order.size() && order[zs] == 'z' && (t->known_one || checked(t->spec)) &&
+        t->reg[r].d.nt >= column_split_nt()) {
+        std::string blk;
+        bool second;
+        block_of(t, r, zs, order, NOPS, blk, second);
+        int is, ie, js, je;
+        z_scope(t, r, blk, is, ie, js, je);
+        const ZPost zp{is - 1, ie - 1, js - 1, je - 1, t->limits, t->spec};
+        TTRY(column(t, r, ndyn, ops, &zp));
+        t->done[r] = zs + 1;
+        if (r != 1) TTRY(tm5_zoom_mix_edges(t->ctx, &t->reg[r].d, 1));
+        return 0;
+    }
+    TTRY(column(t, r, ndyn, ops, nullptr));
     t->done[r] = s + n;
     return 0;
 }

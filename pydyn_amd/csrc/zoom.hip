@@ -1563,7 +1563,9 @@ extern "C" int tm5_zoom_decay(tm5_ctx *c, const tm5_zregion *r, const tm5_view *
     return status(hipGetLastError());
 }
 
-extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops)
+// The column pass of a region; zp (non-null): with the step's next dynamw fused into it (the split
+// form's zpost) on cm over the scope zp->i0..i1, j0..j1 (0-based), limits and checked step spec_k.
+int tm5::zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops, const ZPost *zp)
 {
     if (!c || zbad(r) || !p || !ops || !column_supported(p->lmax) || r->lm > 64 || p->lmax > r->lm) return TM5_EINVAL;
     ColumnArgs a{};
@@ -1612,7 +1614,19 @@ extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphy
     a.phlb = sub3(R.phlb, 1, 1, 1);
     if ((a.pre || a.post || a.twice) && (!p->emis.data || !p->loss.data)) return TM5_EINVAL;
     if (a.mid && (!p->at || !p->bt || !r->dxyp)) return TM5_EINVAL;
+    if (zp) {
+        a.zpost = 1;
+        a.cm = sub3(R.cm, 1, 1, 0);
+        a.zlimits = zp->limits;
+        a.spec_k = zp->spec_k;
+        a.zi0 = zp->i0, a.zi1 = zp->i1, a.zj0 = zp->j0, a.zj1 = zp->j1;
+    }
     return launch_column(c, a, p->lmax, zm(c, TM5_ERRORS));
+}
+
+extern "C" int tm5_zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops)
+{
+    return tm5::zoom_physics(c, r, p, ops, nullptr);
 }
 
 extern "C" int tm5_zoom_copy(tm5_ctx *c, const double *src, double *dst, int64_t n)
