@@ -278,13 +278,17 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
 
     // zfused: the first dynamw (n_advz 1, c = cm / 1) moves m before convection and m2phlb1 see it
     const bool zf = !ADJ && a.zfused;
+    // the dynamw after the sequence: zfused's second, or zpost's on the z scope's columns (not in
+    // the split form's factor pass, whose apply takes it)
+    const bool zq = !ADJ && !a.cbuf && (a.zfused || a.zpost);
+    const bool zin = a.zfused || (i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1);
     // Every input the column reads from HBM is loaded here, together, so that their latencies
     // overlap (the exchanges through LDS below would hold each later group of loads back): the
     // masses, the faces of the lane's two cells, the convection inputs, the cloud levels, and for
     // a sequence on tracers the loss rates and the emission (decay factors made at once).
     const double m0_l = li < lm ? a.m(i, j, li) : 0.0, m0_h = l1 < lm ? a.m(i, j, l1) : 0.0;
     double zc_lo = 0.0, zc_hi = 0.0, zc_lo1 = 0.0, zc_hi1 = 0.0;  // cm below and above each cell
-    if (zf) {
+    if (zq) {
         if (li < lm) {
             zc_lo = a.cm(i, j, li);
             zc_hi = a.cm(i, j, li + 1);
@@ -336,6 +340,27 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
                 if (any_fail) spec_fail(words, a.spec_k);
             }
             wsync();  // H is read: the build vectors take its space
+        }
+    } else if (zq && checked(a.spec_k)) {
+        // zpost in a checked step: Check_CFL's dynamwm at cap 1 on the masses as they stand, every
+        // column of the region (a column outside the z scope is only tested)
+        mc[li] = m_l;
+        mc[l1] = m_h;
+        wsync();
+        bool ok = true, bad = false;
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int k = h2 ? l1 : li;
+            if (active && k >= 1 && k < lm) {
+                const double ck = h2 ? zc_lo1 : zc_lo;  // cm(k)
+                ok = ok && fabs(ck / mc[ck >= 0.0 ? k - 1 : k]) < 1.0;
+            }
+        }
+        if (active && li == 0) bad = a.cm(i, j, 0) != 0.0 || a.cm(i, j, lm) != 0.0;
+        const bool any_bad = __any(bad), any_fail = __any(!ok);  // every lane votes
+        if (lane == 0) {
+            if (any_bad) atomicOr(words + W_FLAGS, TM5_BAD_Z);
+            if (any_fail) spec_fail(words, a.spec_k);
         }
     }
     mc[li] = m_l;
@@ -575,7 +600,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     static_assert(ColumnLds<N>::R0 >= ColumnLds<N>::ZSTAGE, "the staging fits the matrix's space");
     double *Z = H;
     double zm0 = 0.0, zm0h = 0.0, zm2 = 0.0, zm2h = 0.0;
-    if (zf) {
+    if (zq) {
         if (li < lm) {
             zm0 = m0_l;
             zm2 = (m_l + zc_lo / 1.0) - zc_hi / 1.0;
@@ -643,6 +668,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
         if (li >= 1 && li < lm) put(li, zc_lo);
         if (l1 < lm) put(l1, zc_lo1);
         wsync();
+        if (!zin) return;  // a column outside the z scope is not moved
         if (li < lm) z_cell(li, zc_lo, zc_hi, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
         if (l1 < lm)
             z_cell(l1, zc_lo1, zc_hi1, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
@@ -692,10 +718,13 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
             }
         } else {
             if (zf) z_op(false, q);
-            column_seq<N, true>(a, cs, crow, diag, li, li, l1, t, rv, q);
-            if (zf) z_op(true, q);
+            if (zq && a.owned)  // a column the region does not own keeps its tracers through the sequence
+                column_seq_if<N>(own, a, cs, crow, diag, li, l1, t, rv, q);
+            else
+                column_seq<N, true>(a, cs, crow, diag, li, li, l1, t, rv, q);
+            if (zq) z_op(true, q);
         }
-        if (active && own) {
+        if (active && (own || (zq && zin))) {
             if (li < lm) {
 #pragma unroll
                 for (int m = 0; m < 4; m++) a.q.q[m][o] = q[m];
@@ -710,7 +739,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
         if (active && own && (li == a.level || l1 == a.level)) a.lam_emis(i, j, 0) = a.lam_emis(i, j, 0) + lam_e;
         return;
     }
-    if (zf && active) {  // the second dynamw's mass
+    if (zq && zin && active) {  // the second dynamw's mass
         if (li < lm) a.m(i, j, li) = zm2;
         if (l1 < lm) a.m(i, j, l1) = zm2h;
     }
@@ -1009,6 +1038,9 @@ template <int N>
 static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, double *ext, size_t ext_n)
 {
     if (args.zfused && (args.adjoint || args.owned || !args.cm.p)) return TM5_EINVAL;  // a region's step only
+    if (args.zpost && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.zi0 < 0 || args.zj0 < 0 ||
+                       args.zi1 >= args.im || args.zj1 >= args.jm || args.zi1 < args.zi0 || args.zj1 < args.zj0))
+        return TM5_EINVAL;  // a forward step's, on a scope inside the region
     const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
     const int ncol = args.im * args.jm;
     const dim3 grid((ncol + 7) / 8), block(256);
@@ -1025,10 +1057,6 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
         return status(hipGetLastError());
     }
     if (args.lapack && (args.zfused || args.mid || args.pre || args.twice || args.post)) return TM5_EINVAL;
-    if (args.zpost && (args.zfused || args.lapack || args.q.nt < SPLIT_NT || !args.cm.p || args.zi0 < 0 ||
-                       args.zj0 < 0 || args.zi1 >= args.im || args.zj1 >= args.jm || args.zi1 < args.zi0 ||
-                       args.zj1 < args.zj0))
-        return TM5_EINVAL;  // the split form's, on a scope inside the region
     if (!args.lapack && (args.zfused || args.q.nt < SPLIT_NT)) {
         TM5_LAUNCH_NAMED(ctx, name[N], (k_column<N, false>), grid, block, lds, ctx->stream, args, ctx->words,
                          skip_mask);

@@ -287,9 +287,10 @@ struct ColumnArgs {
     // the limiter flag zlimits; spec_k >= 0: a checked step's, testing both at cap 1 first
     int zfused, zlimits, spec_k;
     View3 cm;
-    // zpost (the split form): the dynamw that follows the sequence in the step, at n_advz 1 on cm
-    // with zlimits, moving the columns i in [zi0, zi1], j in [zj0, zj1] (0-based); spec_k >= 0: a
-    // checked step's, testing every column's faces at cap 1 first
+    // zpost: the dynamw that follows the sequence in the step, at n_advz 1 on cm with zlimits,
+    // moving the columns i in [zi0, zi1], j in [zj0, zj1] (0-based; a column the region does not
+    // own is moved but skips the sequence); spec_k >= 0: a checked step's, testing every column's
+    // faces at cap 1 first
     int zpost, zi0, zi1, zj0, zj1;
     // non-null: the factor pass of the split form (many tracers), which stores each column's rows
     // of C there ((im, jm) columns, i fastest, of N x N row-major) and leaves the tracers alone
@@ -454,9 +455,9 @@ int zoom_put_x(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
 int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
 int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, int l1, int tracers, bool un);
 int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers);
-// a zoom region's column pass with the step's next dynamw fused into the split form's apply (the
-// z scope's columns i0..i1, j0..j1, 0-based; at n_advz 1 with the limiter flag; spec_k >= 0: a
-// checked step's, tested at cap 1)
+// a zoom region's column pass with the step's next dynamw fused into it (the z scope's columns
+// i0..i1, j0..j1, 0-based; at n_advz 1 with the limiter flag; spec_k >= 0: a checked step's,
+// tested at cap 1)
 struct ZPost {
     int i0, i1, j0, j1, limits, spec_k;
 };

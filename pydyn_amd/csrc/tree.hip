@@ -487,13 +487,12 @@ int column_pass(tm5_tree *t, int r, int ndyn)
         return TM5_EINVAL;  // sources or chemistry apart from convection
     }
     // The operator after the pass a dynamw at n_advz 1 (a checked step's, or one its rehearsal
-    // passed at 1) on the split form's many tracers: it runs in the pass's apply, on the scope of
-    // its block.  Nothing comes between them in the order (the children run after a z only when
+    // passed at 1): it runs in the pass (the column kernel, or the split form's apply for many
+    // tracers), on the scope of its block.  Nothing comes between them in the order (the children run after a z only when
     // y precedes it, and region 1's "vscMcsv" ends its first half with nothing launched); a
     // child's mix_edges follows, as after its z.
     const int zs = s + n;
-    if (zs < (int)order.size() && order[zs] == 'z' && (t->known_one || checked(t->spec)) &&
-        t->reg[r].d.nt >= column_split_nt()) {
+    if (zs < (int)order.size() && order[zs] == 'z' && (t->known_one || checked(t->spec))) {
         std::string blk;
         bool second;
         block_of(t, r, zs, order, NOPS, blk, second);
