@@ -968,7 +968,10 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
         }
         // the sequence (a column the region does not own keeps its tracers through it; its lanes
         // still take part in the half's exchanges)
-        column_seq_if<N>(own, a, cs, crow, diag, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
+        if (a.owned)
+            column_seq_if<N>(own, a, cs, crow, diag, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
+        else
+            column_seq<N, true>(a, cs, crow, diag, li, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
         wsync();  // the previous tracer's staged values and fluxes are read
         if (in0) {
 #pragma unroll
