@@ -825,10 +825,15 @@ __device__ int z_column_next(const View3 &m, const View3 &cm, int i, int j, int 
     return nmax + 1;
 }
 
-// the scope the trial advances, 0-based inclusive within the region's interior
+// the scope the trial advances, 0-based inclusive within the region's interior, less the columns
+// of a hole (hi0..hi1, hj0..hj1; none by default) whose dynamw the caller runs elsewhere
 struct ZScope {
     int i0, i1, j0, j1;
-    __device__ __forceinline__ bool has(int i, int j) const { return i >= i0 && i <= i1 && j >= j0 && j <= j1; }
+    int hi0 = 0, hi1 = -1, hj0 = 0, hj1 = -1;
+    __device__ __forceinline__ bool has(int i, int j) const
+    {
+        return i >= i0 && i <= i1 && j >= j0 && j <= j1 && !(i >= hi0 && i <= hi1 && j >= hj0 && j <= hj1);
+    }
 };
 
 constexpr int Z_FPT = 8;  // interior faces per thread in the z search
@@ -954,7 +959,8 @@ __global__ void __launch_bounds__(1024) k_z_steps(int im, int jm, int lm, int TI
         }
     }
     if (j < sc.j0 || j > sc.j1 || i0 > sc.i1 || i0 + TI - 1 < sc.i0) return;  // a workgroup only tested
-    const bool act = i0 + ti >= sc.i0 && i0 + ti <= sc.i1;
+    if (j >= sc.hj0 && j <= sc.hj1 && i0 >= sc.hi0 && i0 + TI - 1 <= sc.hi1) return;  // inside the hole
+    const bool act = sc.has(i0 + ti, j);
     const double dn = (double)nz;
     const double c_lo = cm(i, j, ll) / dn, c_hi = cm(i, j, ll + 1) / dn;
     // each face's upwind cell (the closed ends carry nothing)
@@ -1125,20 +1131,24 @@ int launch_z_mass1(tm5_ctx *ctx, int im, int jm, int lm, View3 m, View3 cm, int 
 }
 
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
-                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one, int spec_k)
+                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one, int spec_k, int hi0, int hi1,
+                    int hj0, int hj1)
 {
     if (checked(spec_k)) known_one = true;
     if (im < 1 || jm < 1 || lm < 2 || (long long)im * jm * (lm + 1) >= (1LL << 31)) return TM5_EINVAL;
     if (i0 < 0 || j0 < 0 || i1 >= im || j1 >= jm || i1 < i0 || j1 < j0) return TM5_EINVAL;
     const bool mass_only = q.nt == 0;
-    const int TI = lines_per_block(lm, ctx->tile[TM5_TILE_Z_LINES]);
+    // with a hole, tiles of three columns: the workgroups inside it return at once
+    const bool holed = hi0 <= hi1 && hj0 <= hj1;
+    const int TI = lines_per_block(lm, holed ? 3 : ctx->tile[TM5_TILE_Z_LINES]);
     if (TI < 1) return TM5_EINVAL;
     const long long nface = (long long)im * jm * (lm - 1);
     long long nb = (nface + 256LL * Z_FPT - 1) / (256LL * Z_FPT);
     const long long nb_cols = ((long long)im * jm + 255) / 256;
     if (nb < nb_cols) nb = nb_cols;
     if (nb > ctx->z_cap) return TM5_EINVAL;
-    const ZScope scope{i0, i1, j0, j1};
+    if (holed && !known_one) return TM5_EINVAL;  // a hole: n_advz 1 known
+    const ZScope scope{i0, i1, j0, j1, hi0, hi1, hj0, hj1};
     if (!known_one)  // as launch_y's known_one: n_advz 1 without a search
         TM5_LAUNCH(ctx, k_z_search, dim3((unsigned)nb), dim3(256), 0, ctx->stream, im, jm, lm, m, cm, scope, nmax,
                    ctx->z_ok, ctx->words, fail_bit, skip_mask);
@@ -1151,16 +1161,27 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     View3 ms{m.p + o0 * m.si + o1 * m.sj, m.si, m.sj, m.sl}, cs{cm.p + o0 * cm.si + o1 * cm.sj, cm.si, cm.sj, cm.sl};
     Tracers qs = q;
     for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (o0 * q.si + o1 * q.sj);
-    const ZScope moved = whole ? scope : ZScope{0, si - 1, 0, sj - 1};
+    const ZScope moved = whole ? scope : ZScope{0, si - 1, 0, sj - 1, hi0 - o0, hi1 - o0, hj0 - o1, hj1 - o1};
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
     if (mass_only) {
         TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
                    ctx->words, skip_mask, fixed_n, spec_k, 0, moved);
         return status(hipGetLastError());
     }
-    // many tracers: tracer ranges over the grid (one workgroup of these a CU), then the masses
-    const int tchunk = q.nt >= 32 ? tracer_chunk((long long)grid.x * grid.y, ctx->ncu > 0 ? ctx->ncu : 256, q.nt,
-                                                 ctx->tile[TM5_TILE_RANGE_MIN])
+    // many tracers: tracer ranges over the grid (one workgroup of these a CU; of a hole's
+    // three-column tiles eight), then the masses.  The ranges are sized for the tiles that move
+    // columns: the others (only tested, or inside the hole) return at once.
+    long long tiles = 0;
+    for (int by = moved.j0; by <= moved.j1 && by < (int)grid.y; by++) {
+        for (int bx = 0; bx < (int)grid.x; bx++) {
+            const int t0 = bx * TI, t1 = t0 + TI - 1;
+            if (t0 > moved.i1 || t1 < moved.i0) continue;
+            if (by >= moved.hj0 && by <= moved.hj1 && t0 >= moved.hi0 && t1 <= moved.hi1) continue;
+            tiles++;
+        }
+    }
+    const long long slots = (long long)(ctx->ncu > 0 ? ctx->ncu : 256) * (holed ? 8 : 1);
+    const int tchunk = q.nt >= 32 ? tracer_chunk(tiles > 0 ? tiles : 1, slots, q.nt, ctx->tile[TM5_TILE_RANGE_MIN])
                                   : q.nt;
     const int ranges = (q.nt + tchunk - 1) / tchunk;
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)(4 * lm * TI + lm * TI) * sizeof(double),

@@ -239,6 +239,13 @@ __device__ __forceinline__ void column_seq(const ColumnArgs &a, const ColumnSeq 
     }
 }
 
+// zpre's columns: its rectangle less its hole
+__device__ __forceinline__ bool zpre_has(const ColumnArgs &a, int i, int j)
+{
+    return i >= a.zpi0 && i <= a.zpi1 && j >= a.zpj0 && j <= a.zpj1 &&
+           !(i >= a.zhi0 && i <= a.zhi1 && j >= a.zhj0 && j <= a.zhj1);
+}
+
 // column_seq with v on the lane's first cell, applied only where `on` (a column the region owns):
 // elsewhere q is left as it is, but the lanes still pass the half's LDS exchanges
 template <int N>
@@ -282,13 +289,16 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     // the split form's factor pass, whose apply takes it)
     const bool zq = !ADJ && !a.cbuf && (a.zfused || a.zpost);
     const bool zin = a.zfused || (i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1);
+    // zpre in the split form's factor pass: the matrix (and m2phlb1) from the mass after the
+    // dynamw the pass takes on this column
+    const bool zpc = !ADJ && a.zpre && a.cbuf && zpre_has(a, i, j);
     // Every input the column reads from HBM is loaded here, together, so that their latencies
     // overlap (the exchanges through LDS below would hold each later group of loads back): the
     // masses, the faces of the lane's two cells, the convection inputs, the cloud levels, and for
     // a sequence on tracers the loss rates and the emission (decay factors made at once).
     const double m0_l = li < lm ? a.m(i, j, li) : 0.0, m0_h = l1 < lm ? a.m(i, j, l1) : 0.0;
     double zc_lo = 0.0, zc_hi = 0.0, zc_lo1 = 0.0, zc_hi1 = 0.0;  // cm below and above each cell
-    if (zq) {
+    if (zq || zpc) {
         if (li < lm) {
             zc_lo = a.cm(i, j, li);
             zc_hi = a.cm(i, j, li + 1);
@@ -312,6 +322,10 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     const bool own = a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0;
     const ColumnSeq cs = a.cbuf ? ColumnSeq{1.0, 1.0, 0.0, 0.0} : column_seq_setup(a, i, j, li);
     double m_l = m0_l, m_h = m0_h;
+    if (zpc) {
+        if (li < lm) m_l = (m0_l + zc_lo / 1.0) - zc_hi / 1.0;
+        if (l1 < lm) m_h = (m0_h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
+    }
     if (zf) {
         if (li < lm) m_l = (m0_l + zc_lo / 1.0) - zc_hi / 1.0;
         if (l1 < lm) m_h = (m0_h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
@@ -856,16 +870,17 @@ __global__ void __launch_bounds__(APPLY_COLS_MAX * 32)
     }
 }
 
-// The apply with the dynamw that follows the sequence in the step fused (zpost): every tracer
-// of the tile's columns takes k_column_apply's sequence (on the columns the region owns), then,
-// on the columns of the z scope, dynamw at n_advz 1 (c = cm / 1, pydyn/kernels.py:324-346, as
-// k_z_steps' sub-step) from the masses as they stand.  Each lane holds the tracer's cells at
+// The apply with the step's dynamw around the sequence fused: every tracer of the tile's columns
+// takes k_column_apply's sequence (on the columns the region owns) and, at n_advz 1 (c = cm / 1,
+// pydyn/kernels.py:324-346, as k_z_steps' sub-step), zpre's dynamw before it on the columns it
+// names and zpost's after it on the columns of its scope.  Each lane holds the tracer's cells at
 // levels li and li + 32, so the column's every level is in its half's registers.  A face's
-// Courant number and upwind cell depend on the masses only and are found once; per tracer the
-// moments pass through the half's LDS, lane k makes face k's fluxes (and face k + 32's), and both
-// cells of each face read them back.  No workgroup writes m (the tracer ranges of a column all
-// read it): k_z_mass_post moves it afterwards.  A checked step's dynamw is tested here, every
-// column's faces at cap 1, by the first tracer range.
+// Courant number and upwind cell depend on the masses only and are found once for each dynamw;
+// per tracer the moments pass through the half's LDS, lane k makes face k's fluxes (and face
+// k + 32's), and both cells of each face read them back.  No workgroup writes m (the tracer
+// ranges of a column all read it): k_z_mass_post moves it afterwards.  A checked step's zpost is
+// tested here, every column's faces at cap 1, by the first tracer range (zpre's was tested where
+// the rest of its scope moved).
 constexpr int APPLY_Z_COLS = 8;
 constexpr int APPLY_Z_F = 65;                               // a flux plane: faces 0 .. 64
 constexpr int APPLY_Z_LDS = 5 * 64 + 4 * APPLY_Z_F + 32 + 3;  // masses, moments, fluxes, rm for v
@@ -888,7 +903,8 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
     const bool active = i >= 0 && i < a.im;
     if (!active) i = i < 0 ? 0 : a.im - 1;
     const bool own = active && (a.owned == nullptr || a.owned[(long long)j * a.im + i] != 0);
-    const bool zin = active && i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1;
+    const bool zin = active && a.zpost && i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1;
+    const bool zpc = active && a.zpre && zpre_has(a, i, j);
     double *Z = zsh + (wave * 2 + h) * APPLY_Z_LDS;  // Z[0, 64) masses, then rm, rxm, rym, rzm by level
     double *F = Z + 5 * 64;                          // f, pf, ft, fw by face
     double *rv = F + 4 * APPLY_Z_F;
@@ -899,15 +915,33 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
     const double diag = li < N ? Cr[li] : 0.0;
     const ColumnSeq cs = column_seq_setup(a, i, j, li);
     const bool in0 = li < lm, in1 = l1 < lm;
-    // the masses before dynamw and the faces below and above each of the lane's cells
+    // the masses as they stand, after zpre's dynamw (mp, on its columns) and after zpost's (m2),
+    // and the faces below and above each of the lane's cells
     const double m0 = in0 ? a.m(i, j, li) : 0.0, m0h = in1 ? a.m(i, j, l1) : 0.0;
     const double zc_lo = in0 ? a.cm(i, j, li) : 0.0, zc_hi = in0 ? a.cm(i, j, li + 1) : 0.0;
     const double zc_lo1 = in1 ? a.cm(i, j, l1) : 0.0, zc_hi1 = in1 ? a.cm(i, j, l1 + 1) : 0.0;
-    const double m2 = (m0 + zc_lo / 1.0) - zc_hi / 1.0, m2h = (m0h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
-    if (in0) Z[li] = m0;
-    if (in1) Z[l1] = m0h;
+    const double mp = (m0 + zc_lo / 1.0) - zc_hi / 1.0, mph = (m0h + zc_lo1 / 1.0) - zc_hi1 / 1.0;
+    const double mb = zpc ? mp : m0, mbh = zpc ? mph : m0h;  // before zpost's dynamw
+    const double m2 = (mb + zc_lo / 1.0) - zc_hi / 1.0, m2h = (mbh + zc_lo1 / 1.0) - zc_hi1 / 1.0;
+    const bool zlim = a.zlimits != 0;
+    // face k = li (interior: 1 <= k < lm) and face k1 = l1 (k1 < lm): c = cm / 1, upwind cell and,
+    // from the masses staged in Z, the Courant number (the same for every tracer)
+    const bool fa = li >= 1 && in0, fb = in1;
+    const double ca = zc_lo / 1.0, cb = zc_lo1 / 1.0;
+    const int ua = ca >= 0.0 ? li - 1 : li, ub = cb >= 0.0 ? l1 - 1 : l1;
+    double gpa = 0.0, gpb = 0.0;  // zpre's
+    if (a.zpre) {
+        if (in0) Z[li] = m0;
+        if (in1) Z[l1] = m0h;
+        wsync();
+        gpa = fa ? ca / Z[ua] : 0.0;
+        gpb = fb ? cb / Z[ub] : 0.0;
+        wsync();
+    }
+    if (in0) Z[li] = mb;
+    if (in1) Z[l1] = mbh;
     wsync();
-    if (checked(a.spec_k) && bz == 0) {
+    if (a.zpost && checked(a.spec_k) && bz == 0) {
         // Check_CFL's dynamwm at cap 1: every interior face k by lane k and k + 32 of its column
         bool ok = true, bad = false;
         if (active) {
@@ -927,13 +961,7 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
             if (any_fail) spec_fail(words, a.spec_k);
         }
     }
-    const bool zlim = a.zlimits != 0;
-    // face k = li (interior: 1 <= k < lm) and face k1 = l1 (k1 < lm): c = cm / 1, upwind cell and
-    // Courant number (the same for every tracer)
-    const bool fa = li >= 1 && in0, fb = in1;
-    const double ca = zc_lo / 1.0, cb = zc_lo1 / 1.0;
-    const int ua = ca >= 0.0 ? li - 1 : li, ub = cb >= 0.0 ? l1 - 1 : l1;
-    const double ga = fa ? ca / Z[ua] : 0.0, gb = fb ? cb / Z[ub] : 0.0;
+    const double ga = fa ? ca / Z[ua] : 0.0, gb = fb ? cb / Z[ub] : 0.0;  // zpost's
     auto put_face = [&](int k, int u, double cf, double g) {
         const double rmu = Z[64 + u];
         const double rzu = zlim ? limit(Z[256 + u], rmu) : Z[256 + u];
@@ -958,21 +986,10 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
         rm = rm_new;
         rz = rzn;
     };
-    const long long o = a.q.off(i, j, in0 ? li : 0, t0), o1 = a.q.off(i, j, in1 ? l1 : 0, t0), st = a.q.st;
-    for (int t = 0; t < nt; t++) {
-        double q[8];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            q[m] = in0 ? a.q.q[m][o + t * st] : 0.0;
-            q[4 + m] = in1 ? a.q.q[m][o1 + t * st] : 0.0;
-        }
-        // the sequence (a column the region does not own keeps its tracers through it; its lanes
-        // still take part in the half's exchanges)
-        if (a.owned)
-            column_seq_if<N>(own, a, cs, crow, diag, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
-        else
-            column_seq<N, true>(a, cs, crow, diag, li, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
-        wsync();  // the previous tracer's staged values and fluxes are read
+    // one dynamw on the tracer's cells: the moments staged, the faces made, the cells moved where
+    // `moves` (the half's lanes all pass the exchanges)
+    auto z_op = [&](double g0, double g1, bool moves, double mo, double mn, double moh, double mnh, double (&q)[8]) {
+        wsync();  // the previous staged values and fluxes are read
         if (in0) {
 #pragma unroll
             for (int m = 0; m < 4; m++) Z[64 * (m + 1) + li] = q[m];
@@ -982,14 +999,31 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
             for (int m = 0; m < 4; m++) Z[64 * (m + 1) + l1] = q[4 + m];
         }
         wsync();
-        if (fa) put_face(li, ua, ca, ga);
-        if (fb) put_face(l1, ub, cb, gb);
+        if (fa) put_face(li, ua, ca, g0);
+        if (fb) put_face(l1, ub, cb, g1);
         wsync();
-        if (zin) {
-            if (in0) z_cell(li, zc_lo, zc_hi, m0, m2, q[0], q[1], q[2], q[3]);
-            if (in1) z_cell(l1, zc_lo1, zc_hi1, m0h, m2h, q[4], q[5], q[6], q[7]);
+        if (moves) {
+            if (in0) z_cell(li, zc_lo, zc_hi, mo, mn, q[0], q[1], q[2], q[3]);
+            if (in1) z_cell(l1, zc_lo1, zc_hi1, moh, mnh, q[4], q[5], q[6], q[7]);
         }
-        if (own || zin) {
+    };
+    const long long o = a.q.off(i, j, in0 ? li : 0, t0), o1 = a.q.off(i, j, in1 ? l1 : 0, t0), st = a.q.st;
+    for (int t = 0; t < nt; t++) {
+        double q[8];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            q[m] = in0 ? a.q.q[m][o + t * st] : 0.0;
+            q[4 + m] = in1 ? a.q.q[m][o1 + t * st] : 0.0;
+        }
+        if (a.zpre) z_op(gpa, gpb, zpc, m0, mp, m0h, mph, q);
+        // the sequence (a column the region does not own keeps its tracers through it; its lanes
+        // still take part in the half's exchanges)
+        if (a.owned)
+            column_seq_if<N>(own, a, cs, crow, diag, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
+        else
+            column_seq<N, true>(a, cs, crow, diag, li, li, in1 ? l1 : NO_LEVEL, t0 + t, rv, q);
+        if (a.zpost) z_op(ga, gb, zin, mb, m2, mbh, m2h, q);
+        if (own || zin || zpc) {
             if (in0) {
 #pragma unroll
                 for (int m = 0; m < 4; m++) a.q.q[m][o + t * st] = q[m];
@@ -1003,16 +1037,21 @@ __global__ void __launch_bounds__(APPLY_Z_COLS * 32)
     }
 }
 
-// zpost's masses: m <- (m + cm(l) / 1) - cm(l + 1) / 1 on the scope's columns (k_z_steps' mass at
-// n_advz 1), one thread a cell
+// The fused dynamw's masses: m <- (m + cm(l) / 1) - cm(l + 1) / 1 for zpre's (on its columns) and
+// then zpost's (on its scope) -- k_z_steps' mass at n_advz 1 -- one thread a cell
 __global__ void __launch_bounds__(256) k_z_mass_post(ColumnArgs a, int *words, int skip_mask)
 {
     if (skipped_k(words, skip_mask)) return;
     const long long n = (long long)a.im * a.jm * a.lm, c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int i = (int)(c % a.im), j = (int)((c / a.im) % a.jm), l = (int)(c / ((long long)a.im * a.jm));
-    if (i < a.zi0 || i > a.zi1 || j < a.zj0 || j > a.zj1) return;
-    a.m(i, j, l) = (a.m(i, j, l) + a.cm(i, j, l) / 1.0) - a.cm(i, j, l + 1) / 1.0;
+    const bool pre = a.zpre && zpre_has(a, i, j);
+    const bool post = a.zpost && i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1;
+    if (!pre && !post) return;
+    double mm = a.m(i, j, l);
+    if (pre) mm = (mm + a.cm(i, j, l) / 1.0) - a.cm(i, j, l + 1) / 1.0;
+    if (post) mm = (mm + a.cm(i, j, l) / 1.0) - a.cm(i, j, l + 1) / 1.0;
+    a.m(i, j, l) = mm;
 }
 
 // the split form from this many tracers on (a factor pass, then k_column_apply)
@@ -1044,6 +1083,9 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
     if (args.zpost && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.zi0 < 0 || args.zj0 < 0 ||
                        args.zi1 >= args.im || args.zj1 >= args.jm || args.zi1 < args.zi0 || args.zj1 < args.zj0))
         return TM5_EINVAL;  // a forward step's, on a scope inside the region
+    if (args.zpre && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.q.nt < SPLIT_NT ||
+                      args.zpi0 < 0 || args.zpj0 < 0 || args.zpi1 >= args.im || args.zpj1 >= args.jm))
+        return TM5_EINVAL;  // the split form's, on a rectangle inside the region
     const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
     const int ncol = args.im * args.jm;
     const dim3 grid((ncol + 7) / 8), block(256);
@@ -1102,7 +1144,7 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
                                     ctx->tile[TM5_TILE_RANGE_MIN]);
     const dim3 agrid((args.im + 7 + cols - 1) / cols, args.jm, (nt + tchunk - 1) / tchunk);
     const dim3 ablock(cols * 32);
-    if (args.zpost) {  // the step's next dynamw too, then its masses
+    if (args.zpost || args.zpre) {  // the step's dynamw around the sequence too, then their masses
         const int zcols = cols < APPLY_Z_COLS ? cols : APPLY_Z_COLS;
         const int ztiles = (args.im + 7 + zcols - 1) / zcols * args.jm;
         const int zchunk = tracer_chunk(ztiles, 32 / zcols * (ctx->ncu > 0 ? ctx->ncu : 256), nt,

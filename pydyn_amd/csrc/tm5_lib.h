@@ -292,6 +292,10 @@ struct ColumnArgs {
     // own is moved but skips the sequence); spec_k >= 0: a checked step's, testing every column's
     // faces at cap 1 first
     int zpost, zi0, zi1, zj0, zj1;
+    // zpre (the split form): the dynamw before the sequence on the columns in [zpi0, zpi1] x
+    // [zpj0, zpj1] outside the hole [zhi0, zhi1] x [zhj0, zhj1] (0-based), at n_advz 1 on cm with
+    // zlimits; the caller has moved the rest of its scope and tested every column
+    int zpre, zpi0, zpi1, zpj0, zpj1, zhi0, zhi1, zhj0, zhj1;
     // non-null: the factor pass of the split form (many tracers), which stores each column's rows
     // of C there ((im, jm) columns, i fastest, of N x N row-major) and leaves the tracers alone
     double *cbuf;
@@ -445,8 +449,11 @@ struct KzzArgs {
 };
 int launch_calc_kzz(tm5_ctx *ctx, const KzzArgs &a);
 
+// dynamw on the columns i0..i1, j0..j1 (0-based) less the hole hi0..hi1, hj0..hj1 (none by default;
+// a hole needs n_advz 1 known: known_one or a checked step)
 int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, View3 cm, int i0, int i1, int j0, int j1,
-                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1);
+                    int limits, int nmax, int fail_bit, int skip_mask, bool known_one = false, int spec_k = -1,
+                    int hi0 = 0, int hi1 = -1, int hj0 = 0, int hj1 = -1);
 int launch_cluster_sum(tm5_ctx *ctx, int n, int c, int r1, int r2, View3 q, View3 out);
 
 // the zoom tree's launchers (zoom.hip); every one adds ctx->skip_or to its skip mask.  tracers = 0:
@@ -455,11 +462,14 @@ int zoom_put_x(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
 int zoom_put_y(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);
 int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, int l1, int tracers, bool un);
 int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers);
-// a zoom region's column pass with the step's next dynamw fused into it (the z scope's columns
-// i0..i1, j0..j1, 0-based; at n_advz 1 with the limiter flag; spec_k >= 0: a checked step's,
-// tested at cap 1)
+// a zoom region's column pass with the step's dynamw around it fused, at n_advz 1 with the limiter
+// flag (0-based columns): post, the one after it on its scope i0..i1, j0..j1 (spec_k >= 0: a
+// checked step's, tested at cap 1); pre (the split form), the one before it on pi0..pi1, pj0..pj1
+// less the hole hi0..hi1, hj0..hj1, the rest of whose scope has moved and been tested
 struct ZPost {
-    int i0, i1, j0, j1, limits, spec_k;
+    int post = 0, i0 = 0, i1 = -1, j0 = 0, j1 = -1;
+    int pre = 0, pi0 = 0, pi1 = -1, pj0 = 0, pj1 = -1, hi0 = 0, hi1 = -1, hj0 = 0, hj1 = -1;
+    int limits = 1, spec_k = -1;
 };
 int zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, const char *ops, const ZPost *zp);
 int zoom_update_parent(tm5_ctx *ctx, const ZR &P, const ZR &C, int tracers);

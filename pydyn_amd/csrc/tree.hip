@@ -25,6 +25,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -156,6 +157,7 @@ struct tm5_tree {
     int clock_n = 0;         // > 0: the step's last kernel moves the clock on (intervals of clock_n steps)
     TimeSrc ts{};            // dynam0's amplitude
     std::vector<int> status, done, cstatus;
+    std::vector<ZPost> zdefer;  // a region's dynamw deferred to its next column pass (pre set)
     bool m2phlb1_done = false;
     int half = 0;
     // buffers
@@ -364,6 +366,19 @@ int put_edges(tm5_tree *t, int r, int axis, int tracers)
     return 0;
 }
 
+// A region's deferred dynamw, moved now on the columns its column pass was to take (before any
+// other operator of the region, or the end of its Proces_Region, reads them).
+int flush_z(tm5_tree *t, int r)
+{
+    ZPost &zd = t->zdefer[r];
+    if (!zd.pre) return 0;
+    zd.pre = 0;
+    const ZR &Rr = t->reg[r].R;
+    return launch_z_scoped(t->ctx, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.cm, 1, 1, 0),
+                           zd.pi0, zd.pi1, zd.pj0, zd.pj1, t->limits, 15, TM5_FAIL_Z, mask(t, false), true, t->spec,
+                           zd.hi0, zd.hi1, zd.hj0, zd.hj1);
+}
+
 // advect_x_zoom (pydyn/zoom/advect.py:160-176): put_xedges, dynamu on the block's scope with the
 // outer faces zeroed in the second half
 int advect_x(tm5_tree *t, int r)
@@ -371,6 +386,7 @@ int advect_x(tm5_tree *t, int r)
     tm5_ctx *c = t->ctx;
     TReg &g = t->reg[r];
     const ZR &Rr = g.R;
+    TTRY(flush_z(t, r));
     TTRY(put_edges(t, r, 0, 1));
     std::string blk;
     bool second;
@@ -404,6 +420,7 @@ int advect_y(tm5_tree *t, int r)
     tm5_ctx *c = t->ctx;
     TReg &g = t->reg[r];
     const ZR &Rr = g.R;
+    TTRY(flush_z(t, r));
     TTRY(put_edges(t, r, 1, 1));
     std::string blk;
     bool second;
@@ -433,18 +450,59 @@ int advect_y(tm5_tree *t, int r)
 }
 
 // advect_z_zoom (pydyn/zoom/advect.py:197-206)
+//
+// The dynamw before a column pass (z after y, the pass its block's next operator), at n_advz 1 on
+// many tracers, moves now only the columns something reads or rewrites before that pass, and the
+// pass (the split form's factor pass and apply) moves the rest: for region 1, whose child runs in
+// between and reads none of its columns (the edge puts come in x and y), the child's footprint and
+// a column around it (update_parent rewrites the footprint's m and rm); for a child without
+// children, its ring of xref columns and yref rows at every wall (mix_edges, next, mixes blocks
+// of it).  The launch now tests every column in a checked step, as the whole dynamw's would.
 int advect_z(tm5_tree *t, int r)
 {
     tm5_ctx *c = t->ctx;
     TReg &g = t->reg[r];
     const ZR &Rr = g.R;
+    TTRY(flush_z(t, r));
     std::string blk;
     bool second;
-    block_of(t, r, t->status[r], g.order, NOPS, blk, second);
+    const int s = t->status[r];
+    block_of(t, r, s, g.order, NOPS, blk, second);
     int is, ie, js, je;
     z_scope(t, r, blk, is, ie, js, je);
-    TTRY(launch_z_scoped(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.cm, 1, 1, 0), is - 1,
-                         ie - 1, js - 1, je - 1, t->limits, 15, TM5_FAIL_Z, mask(t, false), t->known_one, t->spec));
+    const bool defer = (t->known_one || checked(t->spec)) && g.d.nt >= column_split_nt() &&
+                       s + 1 < (int)g.order.size() && g.order[s + 1] == 'v' && s >= 1 && g.order[s - 1] == 'y' &&
+                       (r == 1 ? g.children.size() == 1 : g.children.empty());
+    auto scoped = [&](int i0, int i1, int j0, int j1, int hi0, int hi1, int hj0, int hj1) {
+        return launch_z_scoped(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.cm, 1, 1, 0), i0,
+                               i1, j0, j1, t->limits, 15, TM5_FAIL_Z, mask(t, false), t->known_one, t->spec, hi0, hi1,
+                               hj0, hj1);
+    };
+    ZPost zd;
+    zd.pre = 1;
+    if (defer && r == 1) {
+        const tm5_zregion &ch = t->reg[g.children[0]].d;  // footprint ibeg..iend, jbeg..jend (1-based)
+        const int fi0 = std::max(ch.ibeg - 2, is - 1), fi1 = std::min(ch.iend, ie - 1);
+        const int fj0 = std::max(ch.jbeg - 2, js - 1), fj1 = std::min(ch.jend, je - 1);
+        if (fi0 <= fi1 && fj0 <= fj1) {
+            TTRY(scoped(fi0, fi1, fj0, fj1, 0, -1, 0, -1));
+            zd.pi0 = is - 1, zd.pi1 = ie - 1, zd.pj0 = js - 1, zd.pj1 = je - 1;
+            zd.hi0 = fi0, zd.hi1 = fi1, zd.hj0 = fj0, zd.hj1 = fj1;
+            t->zdefer[r] = zd;
+            return 0;
+        }
+    } else if (defer) {
+        // the inner rectangle: every column at least xref from an x wall and yref from a y wall
+        const int ii0 = std::max(g.d.xref, is - 1), ii1 = std::min(Rr.im - 1 - g.d.xref, ie - 1);
+        const int ij0 = std::max(g.d.yref, js - 1), ij1 = std::min(Rr.jm - 1 - g.d.yref, je - 1);
+        if (ii0 <= ii1 && ij0 <= ij1) {
+            TTRY(scoped(is - 1, ie - 1, js - 1, je - 1, ii0, ii1, ij0, ij1));
+            zd.pi0 = ii0, zd.pi1 = ii1, zd.pj0 = ij0, zd.pj1 = ij1;
+            t->zdefer[r] = zd;
+            return tm5_zoom_mix_edges(c, &g.d, 1);
+        }
+    }
+    TTRY(scoped(is - 1, ie - 1, js - 1, je - 1, 0, -1, 0, -1));
     if (r != 1) TTRY(tm5_zoom_mix_edges(c, &g.d, 1));
     return 0;
 }
@@ -491,21 +549,24 @@ int column_pass(tm5_tree *t, int r, int ndyn)
     // tracers), on the scope of its block.  Nothing comes between them in the order (the children run after a z only when
     // y precedes it, and region 1's "vscMcsv" ends its first half with nothing launched); a
     // child's mix_edges follows, as after its z.
+    ZPost zp = t->zdefer[r];  // the dynamw before the pass, deferred to it (pre set)
+    t->zdefer[r].pre = 0;
+    zp.limits = t->limits;
+    zp.spec_k = t->spec;
     const int zs = s + n;
-    if (zs < (int)order.size() && order[zs] == 'z' && (t->known_one || checked(t->spec))) {
+    const bool post = zs < (int)order.size() && order[zs] == 'z' && (t->known_one || checked(t->spec));
+    if (post) {
         std::string blk;
         bool second;
         block_of(t, r, zs, order, NOPS, blk, second);
         int is, ie, js, je;
         z_scope(t, r, blk, is, ie, js, je);
-        const ZPost zp{is - 1, ie - 1, js - 1, je - 1, t->limits, t->spec};
-        TTRY(column(t, r, ndyn, ops, &zp));
-        t->done[r] = zs + 1;
-        if (r != 1) TTRY(tm5_zoom_mix_edges(t->ctx, &t->reg[r].d, 1));
-        return 0;
+        zp.post = 1;
+        zp.i0 = is - 1, zp.i1 = ie - 1, zp.j0 = js - 1, zp.j1 = je - 1;
     }
-    TTRY(column(t, r, ndyn, ops, nullptr));
-    t->done[r] = s + n;
+    TTRY(column(t, r, ndyn, ops, zp.pre || zp.post ? &zp : nullptr));
+    t->done[r] = post ? zs + 1 : s + n;
+    if (post && r != 1) TTRY(tm5_zoom_mix_edges(t->ctx, &t->reg[r].d, 1));
     return 0;
 }
 
@@ -551,6 +612,7 @@ int proces_region(tm5_tree *t, int r, int ndyn)
         for (int ch : g.children) TTRY(proces_region(t, ch, ndyn));
         if (t->status[r] % NOPS != 0) TTRY(do_steps(t, r, ndyn));
     }
+    TTRY(flush_z(t, r));
     const int mk = mask(t, false);
     if (r != 1) {  // update_parent and m2phlb: disjoint arrays, one launch
         TTRY(zoom_update_m2phlb(c, R(t, g.parent), g.R, t->at, t->bt, mk));
@@ -566,9 +628,13 @@ int proces_region(tm5_tree *t, int r, int ndyn)
 // one ndyn step (fluxes set): every status reset, Proces_Region(1) for each half
 int step(tm5_tree *t, int ndyn)
 {
-    for (int r = 1; r <= t->n; r++) t->status[r] = t->done[r] = 0;
+    for (int r = 1; r <= t->n; r++) {
+        t->status[r] = t->done[r] = 0;
+        t->zdefer[r].pre = 0;
+    }
     t->m2phlb1_done = false;
     for (t->half = 0; t->half < 2; t->half++) TTRY(proces_region(t, 1, ndyn));
+    for (int r = 1; r <= t->n; r++) TTRY(flush_z(t, r));  // none is left: every one has its pass
     if (t->m2phlb1_done) return TM5_EINVAL;  // the step's second half must end with region 1's m2phlb1
     return 0;
 }
@@ -982,6 +1048,7 @@ int tree_create(tm5_ctx *ctx, const tm5_tree_desc *d, tm5_tree **out)
     t->status.assign(t->n + 1, 0);
     t->done.assign(t->n + 1, 0);
     t->cstatus.assign(t->n + 1, 0);
+    t->zdefer.assign(t->n + 1, ZPost{});
     int rc = 0;
     size_t save_n = 0;
     for (int r = 1; r <= t->n && rc == 0; r++) {

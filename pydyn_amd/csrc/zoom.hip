@@ -1614,12 +1614,15 @@ int tm5::zoom_physics(tm5_ctx *c, const tm5_zregion *r, const tm5_zphysics *p, c
     a.phlb = sub3(R.phlb, 1, 1, 1);
     if ((a.pre || a.post || a.twice) && (!p->emis.data || !p->loss.data)) return TM5_EINVAL;
     if (a.mid && (!p->at || !p->bt || !r->dxyp)) return TM5_EINVAL;
-    if (zp) {
-        a.zpost = 1;
+    if (zp && (zp->post || zp->pre)) {
         a.cm = sub3(R.cm, 1, 1, 0);
         a.zlimits = zp->limits;
         a.spec_k = zp->spec_k;
+        a.zpost = zp->post;
         a.zi0 = zp->i0, a.zi1 = zp->i1, a.zj0 = zp->j0, a.zj1 = zp->j1;
+        a.zpre = zp->pre;
+        a.zpi0 = zp->pi0, a.zpi1 = zp->pi1, a.zpj0 = zp->pj0, a.zpj1 = zp->pj1;
+        a.zhi0 = zp->hi0, a.zhi1 = zp->hi1, a.zhj0 = zp->hj0, a.zhj1 = zp->hj1;
     }
     return launch_column(c, a, p->lmax, zm(c, TM5_ERRORS));
 }
