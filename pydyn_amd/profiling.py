@@ -50,6 +50,24 @@ def stats(d):
     return sorted(rows, key=lambda x: -x[2])
 
 
+def medians(d):
+    """{kernel: median duration ns} from kernel_trace.csv, when the run kept its trace: the launch
+    a kernel makes when it works (a queued run's launches after a missed plan skip in about a
+    microsecond and pull the average down)."""
+    durs = collections.defaultdict(list)
+    try:
+        rows = _csv(d, "kernel_trace.csv")
+    except FileNotFoundError:
+        return {}
+    for r in rows:
+        durs[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    out = {}
+    for k, v in durs.items():
+        v.sort()
+        out[k] = float(v[len(v) // 2])
+    return out
+
+
 def counter(d, name):
     """{kernel: [dispatches, mean value]} of one counter from counter_collection.csv."""
     acc = collections.defaultdict(list)
@@ -131,10 +149,12 @@ def main(argv=None):
                 print(f"{label:40s} {got[num][1] / got[den][1]:10.3f}")
         return 0
     if a.cmd == "stats":
-        print(f"{'kernel':64s} {'calls':>6s} {'avg_us':>9s} {'total_us':>10s} {'pct':>6s}")
+        med = medians(a.dir)
+        print(f"{'kernel':64s} {'calls':>6s} {'avg_us':>9s} {'median_us':>9s} {'total_us':>10s} {'pct':>6s}")
         for name, calls, tot, avg, pct in stats(a.dir):
             short = name.split("(")[0].replace("void ", "")
-            print(f"{short[:64]:64s} {calls:6d} {avg / 1e3:9.2f} {tot / 1e3:10.1f} {pct:6.2f}")
+            mu = f"{med[name] / 1e3:9.2f}" if name in med else f"{'-':>9s}"
+            print(f"{short[:64]:64s} {calls:6d} {avg / 1e3:9.2f} {mu} {tot / 1e3:10.1f} {pct:6.2f}")
     else:
         out = traffic(a.fetch_dir, a.write_dir, a.copy_bytes)
         if a.json:
