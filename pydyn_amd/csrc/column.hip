@@ -289,9 +289,9 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     // the split form's factor pass, whose apply takes it)
     const bool zq = !ADJ && !a.cbuf && (a.zfused || a.zpost);
     const bool zin = a.zfused || (i >= a.zi0 && i <= a.zi1 && j >= a.zj0 && j <= a.zj1);
-    // zpre in the split form's factor pass: the matrix (and m2phlb1) from the mass after the
-    // dynamw the pass takes on this column
-    const bool zpc = !ADJ && a.zpre && a.cbuf && zpre_has(a, i, j);
+    // zpre: the matrix (and m2phlb1) from the mass after the dynamw the pass takes on this column
+    // (in the split form's factor pass, whose apply moves the tracers)
+    const bool zpc = !ADJ && a.zpre && zpre_has(a, i, j);
     // Every input the column reads from HBM is loaded here, together, so that their latencies
     // overlap (the exchanges through LDS below would hold each later group of loads back): the
     // masses, the faces of the lane's two cells, the convection inputs, the cloud levels, and for
@@ -614,7 +614,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     static_assert(ColumnLds<N>::R0 >= ColumnLds<N>::ZSTAGE, "the staging fits the matrix's space");
     double *Z = H;
     double zm0 = 0.0, zm0h = 0.0, zm2 = 0.0, zm2h = 0.0;
-    if (zq) {
+    if (zq || a.zpre) {
         if (li < lm) {
             zm0 = m0_l;
             zm2 = (m_l + zc_lo / 1.0) - zc_hi / 1.0;
@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     };
 
     // dynamw at n_advz 1 on one tracer: m0 -> m1 (first), m1 -> m2 (second)
-    auto z_op = [&](bool second, double (&q)[8]) {
+    auto z_op = [&](bool second, bool moves, double (&q)[8]) {
         wsync();  // earlier reads of Z and of the matrix precede these writes
         if (!second) {
             if (li < lm) Z[li] = zm0;
@@ -682,7 +682,7 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
         if (li >= 1 && li < lm) put(li, zc_lo);
         if (l1 < lm) put(l1, zc_lo1);
         wsync();
-        if (!zin) return;  // a column outside the z scope is not moved
+        if (!moves) return;  // a column outside the dynamw's scope is not moved
         if (li < lm) z_cell(li, zc_lo, zc_hi, second ? mc[li] : zm0, second ? zm2 : mc[li], q[0], q[1], q[2], q[3]);
         if (l1 < lm)
             z_cell(l1, zc_lo1, zc_hi1, second ? mc[l1] : zm0h, second ? zm2h : mc[l1], q[4], q[5], q[6], q[7]);
@@ -731,14 +731,14 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
                 column_c(cs, q);
             }
         } else {
-            if (zf) z_op(false, q);
-            if (zq && a.owned)  // a column the region does not own keeps its tracers through the sequence
+            if (zf || a.zpre) z_op(false, zf || zpc, q);
+            if ((zq || a.zpre) && a.owned)  // a column the region does not own keeps its tracers through the sequence
                 column_seq_if<N>(own, a, cs, crow, diag, li, l1, t, rv, q);
             else
                 column_seq<N, true>(a, cs, crow, diag, li, li, l1, t, rv, q);
-            if (zq) z_op(true, q);
+            if (zq) z_op(true, zin, q);
         }
-        if (active && (own || (zq && zin))) {
+        if (active && (own || (zq && zin) || zpc)) {
             if (li < lm) {
 #pragma unroll
                 for (int m = 0; m < 4; m++) a.q.q[m][o] = q[m];
@@ -756,6 +756,9 @@ __global__ void __launch_bounds__(256, 3) k_column(ColumnArgs a, int *words, int
     if (zq && zin && active) {  // the second dynamw's mass
         if (li < lm) a.m(i, j, li) = zm2;
         if (l1 < lm) a.m(i, j, l1) = zm2h;
+    } else if (zpc && active && !a.cbuf) {  // zpre's alone (the split form's: k_z_mass_post)
+        if (li < lm) a.m(i, j, li) = m_l;
+        if (l1 < lm) a.m(i, j, l1) = m_h;
     }
     // m2phlb1 between the two chemistry calls (pydyn/zoom/massflow.py:282-290, 24-32): sp from
     // the column's mass summed upward, phlb = at + bt sp
@@ -1083,9 +1086,9 @@ static int launch_column_n(tm5_ctx *ctx, const ColumnArgs &args, int skip_mask, 
     if (args.zpost && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.zi0 < 0 || args.zj0 < 0 ||
                        args.zi1 >= args.im || args.zj1 >= args.jm || args.zi1 < args.zi0 || args.zj1 < args.zj0))
         return TM5_EINVAL;  // a forward step's, on a scope inside the region
-    if (args.zpre && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.q.nt < SPLIT_NT ||
-                      args.zpi0 < 0 || args.zpj0 < 0 || args.zpi1 >= args.im || args.zpj1 >= args.jm))
-        return TM5_EINVAL;  // the split form's, on a rectangle inside the region
+    if (args.zpre && (args.zfused || args.adjoint || args.lapack || !args.cm.p || args.zpi0 < 0 || args.zpj0 < 0 ||
+                      args.zpi1 >= args.im || args.zpj1 >= args.jm))
+        return TM5_EINVAL;  // a forward step's, on a rectangle inside the region
     const size_t lds = (size_t)8 * ColumnLds<N>::S * sizeof(double);  // four waves, two columns each
     const int ncol = args.im * args.jm;
     const dim3 grid((ncol + 7) / 8), block(256);

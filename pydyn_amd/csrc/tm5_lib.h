@@ -292,7 +292,7 @@ struct ColumnArgs {
     // own is moved but skips the sequence); spec_k >= 0: a checked step's, testing every column's
     // faces at cap 1 first
     int zpost, zi0, zi1, zj0, zj1;
-    // zpre (the split form): the dynamw before the sequence on the columns in [zpi0, zpi1] x
+    // zpre: the dynamw before the sequence on the columns in [zpi0, zpi1] x
     // [zpj0, zpj1] outside the hole [zhi0, zhi1] x [zhj0, zhj1] (0-based), at n_advz 1 on cm with
     // zlimits; the caller has moved the rest of its scope and tested every column
     int zpre, zpi0, zpi1, zpj0, zpj1, zhi0, zhi1, zhj0, zhj1;
@@ -464,7 +464,7 @@ int zoom_compress(tm5_ctx *ctx, const ZR &C, int axis, int a0, int a1, int l0, i
 int zoom_mix_edges(tm5_ctx *ctx, const ZR &C, int xedges, int tracers);
 // a zoom region's column pass with the step's dynamw around it fused, at n_advz 1 with the limiter
 // flag (0-based columns): post, the one after it on its scope i0..i1, j0..j1 (spec_k >= 0: a
-// checked step's, tested at cap 1); pre (the split form), the one before it on pi0..pi1, pj0..pj1
+// checked step's, tested at cap 1); pre, the one before it on pi0..pi1, pj0..pj1
 // less the hole hi0..hi1, hj0..hj1, the rest of whose scope has moved and been tested
 struct ZPost {
     int post = 0, i0 = 0, i1 = -1, j0 = 0, j1 = -1;

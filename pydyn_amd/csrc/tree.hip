@@ -451,13 +451,14 @@ int advect_y(tm5_tree *t, int r)
 
 // advect_z_zoom (pydyn/zoom/advect.py:197-206)
 //
-// The dynamw before a column pass (z after y, the pass its block's next operator), at n_advz 1 on
-// many tracers, moves now only the columns something reads or rewrites before that pass, and the
-// pass (the split form's factor pass and apply) moves the rest: for region 1, whose child runs in
-// between and reads none of its columns (the edge puts come in x and y), the child's footprint and
-// a column around it (update_parent rewrites the footprint's m and rm); for a child without
-// children, its ring of xref columns and yref rows at every wall (mix_edges, next, mixes blocks
-// of it).  The launch now tests every column in a checked step, as the whole dynamw's would.
+// The dynamw before a column pass (z after y, the pass its block's next operator), at n_advz 1,
+// moves now only the columns something reads or rewrites before that pass, and the pass (the
+// column kernel, or the split form's factor pass and apply) moves the rest: for region 1, whose
+// child runs in between and reads none of its columns (the edge puts come in x and y), the
+// child's footprint and a column around it (update_parent rewrites the footprint's m and rm); for
+// a child without children, its ring of xref columns and yref rows at every wall (mix_edges,
+// next, mixes blocks of it).  The launch now tests every column in a checked step, as the whole
+// dynamw's would.
 int advect_z(tm5_tree *t, int r)
 {
     tm5_ctx *c = t->ctx;
@@ -470,8 +471,7 @@ int advect_z(tm5_tree *t, int r)
     block_of(t, r, s, g.order, NOPS, blk, second);
     int is, ie, js, je;
     z_scope(t, r, blk, is, ie, js, je);
-    const bool defer = (t->known_one || checked(t->spec)) && g.d.nt >= column_split_nt() &&
-                       s + 1 < (int)g.order.size() && g.order[s + 1] == 'v' && s >= 1 && g.order[s - 1] == 'y' &&
+    const bool defer = (t->known_one || checked(t->spec)) && s + 1 < (int)g.order.size() && g.order[s + 1] == 'v' && s >= 1 && g.order[s - 1] == 'y' &&
                        (r == 1 ? g.children.size() == 1 : g.children.empty());
     auto scoped = [&](int i0, int i1, int j0, int j1, int hi0, int hi1, int hj0, int hj1) {
         return launch_z_scoped(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1), sub3(Rr.cm, 1, 1, 0), i0,
