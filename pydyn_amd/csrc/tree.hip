@@ -496,7 +496,29 @@ int advect_z(tm5_tree *t, int r)
         const int ii0 = std::max(g.d.xref, is - 1), ii1 = std::min(Rr.im - 1 - g.d.xref, ie - 1);
         const int ij0 = std::max(g.d.yref, js - 1), ij1 = std::min(Rr.jm - 1 - g.d.yref, je - 1);
         if (ii0 <= ii1 && ij0 <= ij1) {
-            TTRY(scoped(is - 1, ie - 1, js - 1, je - 1, ii0, ii1, ij0, ij1));
+            // many tracers: the ring in up to three launches, the scope's rows below and above the
+            // inner rectangle (whole rows, the context's tiles), then its rows beside it (a hole,
+            // three-column tiles); the first launch tests every column in a checked step.  Few
+            // tracers: one holed launch over the scope (each launch is mostly its fixed cost).
+            if (subq(Rr, 1, 1, 1).nt < column_split_nt()) {
+                TTRY(scoped(is - 1, ie - 1, js - 1, je - 1, ii0, ii1, ij0, ij1));
+                zd.pi0 = ii0, zd.pi1 = ii1, zd.pj0 = ij0, zd.pj1 = ij1;
+                t->zdefer[r] = zd;
+                return tm5_zoom_mix_edges(c, &g.d, 1);
+            }
+            bool tested = false;
+            auto part = [&](int j0, int j1, bool hole) {
+                if (j0 > j1) return 0;
+                const int spec = tested ? SPEC_NONE : t->spec;
+                tested = true;
+                return launch_z_scoped(c, Rr.im, Rr.jm, Rr.lm, sub3(Rr.m, 1, 1, 1), subq(Rr, 1, 1, 1),
+                                       sub3(Rr.cm, 1, 1, 0), is - 1, ie - 1, j0, j1, t->limits, 15, TM5_FAIL_Z,
+                                       mask(t, false), true, spec, hole ? ii0 : 0, hole ? ii1 : -1,
+                                       hole ? ij0 : 0, hole ? ij1 : -1);
+            };
+            TTRY(part(js - 1, ij0 - 1, false));
+            TTRY(part(ij1 + 1, je - 1, false));
+            TTRY(part(ij0, ij1, true));
             zd.pi0 = ii0, zd.pi1 = ii1, zd.pj0 = ij0, zd.pj1 = ij1;
             t->zdefer[r] = zd;
             return tm5_zoom_mix_edges(c, &g.d, 1);
