@@ -1163,14 +1163,8 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
     for (int a = 0; a < 4 && q.nt > 0; a++) qs.q[a] = q.q[a] + (o0 * q.si + o1 * q.sj);
     const ZScope moved = whole ? scope : ZScope{0, si - 1, 0, sj - 1, hi0 - o0, hi1 - o0, hj0 - o1, hj1 - o1};
     const dim3 grid((si + TI - 1) / TI, sj), block = line_block(TI, lm);
-    if (mass_only) {
-        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
-                   ctx->words, skip_mask, fixed_n, spec_k, 0, moved);
-        return status(hipGetLastError());
-    }
-    // many tracers: tracer ranges over the grid (one workgroup of these a CU; of a hole's
-    // three-column tiles eight), then the masses.  The ranges are sized for the tiles that move
-    // columns: the others (only tested, or inside the hole) return at once.
+    // the tiles that move columns (the kernel's others, only tested or inside the hole, return
+    // at once)
     long long tiles = 0;
     for (int by = moved.j0; by <= moved.j1 && by < (int)grid.y; by++) {
         for (int bx = 0; bx < (int)grid.x; bx++) {
@@ -1180,9 +1174,20 @@ int launch_z_scoped(tm5_ctx *ctx, int im, int jm, int lm, View3 m, Tracers q, Vi
             tiles++;
         }
     }
-    const long long slots = (long long)(ctx->ncu > 0 ? ctx->ncu : 256) * (holed ? 8 : 1);
-    const int tchunk = q.nt >= 32 ? tracer_chunk(tiles > 0 ? tiles : 1, slots, q.nt, ctx->tile[TM5_TILE_RANGE_MIN])
-                                  : q.nt;
+    // a hole over the whole scope moves nothing: no launch unless it carries a checked step's test
+    if (tiles == 0 && !checked(spec_k)) return status(hipGetLastError());
+    if (mass_only) {
+        TM5_LAUNCH(ctx, k_z_steps<true>, grid, block, 0, ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits,
+                   ctx->words, skip_mask, fixed_n, spec_k, 0, moved);
+        return status(hipGetLastError());
+    }
+    // many tracers: tracer ranges over the grid (one workgroup of these a CU; of a hole's
+    // three-column tiles twelve: two waves each at the kernel's six waves a SIMD), then the
+    // masses.  The ranges are sized for the tiles that move columns; with none (a checked step's
+    // test alone) one range.
+    const long long slots = (long long)(ctx->ncu > 0 ? ctx->ncu : 256) * (holed ? 12 : 1);
+    const int tchunk = q.nt >= 32 && tiles > 0 ? tracer_chunk(tiles, slots, q.nt, ctx->tile[TM5_TILE_RANGE_MIN])
+                                               : q.nt;
     const int ranges = (q.nt + tchunk - 1) / tchunk;
     TM5_LAUNCH(ctx, k_z_steps<false>, dim3(grid.x, grid.y, ranges), block, (size_t)(4 * lm * TI + lm * TI) * sizeof(double),
                ctx->stream, si, sj, lm, TI, ms, qs, cs, limits, limits, ctx->words, skip_mask, fixed_n, spec_k,
